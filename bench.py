@@ -1,0 +1,233 @@
+#!/usr/bin/env python3
+"""Benchmark: AV1 post-filter chain (deblock + CDEF + LR + film grain) on
+one MI355X, 4K 10-bit 4:2:0, bit-exact.
+
+A step is one pass of the whole chain (rvpf_postfilter_frame: deblock in
+place, CDEF, loop restoration into the reference frame, crop + film grain
+into the display frame) over a batch of 4 synthetic frames of a GOP
+(qindex 64 / 96 / 160 / 200, global pan 1.5 px/frame), every filter on,
+restoration units of 64 mixing Wiener / self-guided / none. Inputs are
+resident in HBM before the timed region (one fresh reconstruction per
+step and frame, since deblocking works in place).
+
+Prints one JSON line: Mpix/s (luma pixels of the displayed frames per
+second), the dominant kernel's HBM roofline fraction (algorithmic bytes
+per launch / its average duration, timed with HIP events on the launch
+stream), and the CPU oracle's throughput on the host as the baseline.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "rivet-av1_amd"))
+
+HBM_PEAK_GBS = 8000.0
+QINDEX = [64, 96, 160, 200]
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_frames(n, w, h, bd, seed):
+    from rvpf import workload
+    frames = []
+    for i in range(n):
+        frames.append(workload.Frame(w, h, bit_depth=bd, ssx=1, ssy=1, sb_size=128, qindex=QINDEX[i % 4],
+                                     seed=seed, frame_index=i, pan_px=1.5, lf=True, cdef=True, cdef_bits=3,
+                                     lr=2, lr_unit_size=64, grain=True, grain_lag=3, grain_y_points=8,
+                                     with_source=False))
+    return frames
+
+
+def samples_per_frame(f):
+    return sum(f.crop_shape(p)[0] * f.crop_shape(p)[1] for p in range(f.num_planes))
+
+
+def cpu_baseline(frames, seconds):
+    """The oracle chain on the host, threaded like the reference's decoder."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
+    threads = min(threads, 16)
+    pyoracle.postfilter_frame(frames[0], threads=threads)  # warm-up
+    n, t0 = 0, time.perf_counter()
+    while True:
+        pyoracle.postfilter_frame(frames[n % len(frames)], threads=threads)
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= seconds or n >= 400:
+            break
+    px = sum(frames[i % len(frames)].luma_pixels() for i in range(n))
+    return {"value": round(px / el / 1e6, 2), "unit": "Mpix/s", "cores": threads, "kind": "port",
+            "sample": "%d frames of the same 4K 10-bit 4:2:0 chain (%.1f s), C oracle, %d threads" % (n, el, threads)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--frames", type=int, default=4)
+    ap.add_argument("--width", type=int, default=3840)
+    ap.add_argument("--height", type=int, default=2160)
+    ap.add_argument("--bit-depth", type=int, default=10)
+    ap.add_argument("--seed", type=int, default=2160)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-stages", action="store_true")
+    ap.add_argument("--no-check", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the post-filter chain shards by stream/frame with no exchange; "
+              "multi-GPU runs are not part of this build (use --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from rvpf import api
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    log("generating %d frames %dx%d %d-bit" % (args.frames, args.width, args.height, args.bit_depth))
+    frames = make_frames(args.frames, args.width, args.height, args.bit_depth, args.seed)
+    ctx = api.Context()
+    dfs = [api.DeviceFrame(f) for f in frames]
+    nsteps = args.steps + args.warmup
+    # fresh reconstructions for every step (deblocking works in place)
+    pristine = [df.upload_planes(f.rec) for df, f in zip(dfs, frames)]
+    recs = [[[None if t is None else t.clone() for t in pristine[i]] for i in range(len(frames))]
+            for _ in range(nsteps)]
+    refs = [df.alloc_like_recon() for df in dfs]
+    disps = [df.alloc_display() for df in dfs]
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+
+    def step(s):
+        for i, df in enumerate(dfs):
+            api.postfilter_frame(ctx, df, recs[s][i], refs[i], disps[i], stream)
+
+    log("warmup %d steps" % args.warmup)
+    for s in range(args.warmup):
+        step(s)
+    torch.cuda.synchronize()
+    log("timing %d steps" % args.steps)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for s in range(args.warmup, nsteps):
+        step(s)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    gpu_ms = ev0.elapsed_time(ev1)
+    ms_per_step = gpu_ms / args.steps
+    luma_px = sum(f.luma_pixels() for f in frames)
+    mpix = luma_px * args.steps / (gpu_ms / 1e3) / 1e6
+    log("chain: %.3f ms/step (%d frames), %.1f Mpix/s, wall %.3f s" % (ms_per_step, len(frames), mpix, wall))
+
+    bitexact = None
+    if not args.no_check:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+        f = frames[0]
+        deb, ref, disp = pyoracle.postfilter_frame(f, threads=min(16, os.cpu_count() or 1))
+        got_ref, got_disp = api.to_host(refs[0]), api.to_host(disps[0])
+        bitexact = True
+        for p in range(f.num_planes):
+            r, c = f.mi_area(p)
+            bitexact &= bool(np.array_equal(got_ref[p][:r, :c], ref[p][:r, :c]))
+            bitexact &= bool(np.array_equal(got_disp[p], disp[p]))
+        log("bit-exact vs oracle (frame 0):", bitexact)
+
+    # per-stage timing with HIP events on the launch stream
+    b = 1 if args.bit_depth == 8 else 2
+    stages = {}
+    if not args.no_stages:
+        spf = samples_per_frame(frames[0])
+        alg = {"deblock": 2 * b, "cdef": 2 * b, "lr": 2 * b + b * 4 / 64, "grain": 2 * b}
+        cdefs = [df.alloc_like_recon() for df in dfs]
+        reps = max(1, args.steps // 2)
+        fresh = [[None if t is None else t.clone() for t in pristine[i]] for i in range(len(frames))]
+        times = {k: 0.0 for k in alg}
+        counts = {k: 0 for k in alg}
+        for r in range(reps + 1):
+            for i, df in enumerate(dfs):
+                for t, src in zip(fresh[i], pristine[i]):
+                    if t is not None:
+                        t.copy_(src)
+                e = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
+                e[0].record(stream)
+                api.deblock(ctx, df, fresh[i], stream)
+                e[1].record(stream)
+                api.cdef(ctx, df, fresh[i], cdefs[i], stream)
+                e[2].record(stream)
+                api.lr(ctx, df, fresh[i], cdefs[i], refs[i], stream)
+                e[3].record(stream)
+                api.grain(ctx, df, refs[i], disps[i], stream)
+                e[4].record(stream)
+                torch.cuda.synchronize()
+                if r == 0:
+                    continue  # warm-up
+                for k, (a, z) in zip(alg, zip(e[:-1], e[1:])):
+                    times[k] += a.elapsed_time(z)
+                    counts[k] += 1
+        for k in alg:
+            ms = times[k] / counts[k]
+            gbs = spf * alg[k] / (ms / 1e3) / 1e9
+            stages[k] = {"ms": round(ms, 4), "alg_bytes": int(spf * alg[k]), "GB_s": round(gbs, 1),
+                         "frac": round(gbs / HBM_PEAK_GBS, 4)}
+            log("stage %-8s %.4f ms  %.1f GB/s algorithmic (%.1f%% of HBM peak)" % (k, ms, gbs, 100 * gbs / HBM_PEAK_GBS))
+
+    roofline = None
+    if stages:
+        dom = max(stages, key=lambda k: stages[k]["ms"])
+        s = stages[dom]
+        roofline = {"bound": "hbm", "achieved": s["GB_s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": s["frac"], "traffic": None, "kernel": dom,
+                    "alg_bytes_per_launch": s["alg_bytes"], "ms_per_launch": s["ms"]}
+    chain_alg = samples_per_frame(frames[0]) * (8 * b + b * 4 / 64) * len(frames)
+    chain_gbs = chain_alg / (ms_per_step / 1e3) / 1e9
+
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline (%.0f s budget)" % args.cpu_seconds)
+        cpu = cpu_baseline(frames, args.cpu_seconds)
+        log("cpu:", cpu)
+
+    out = {
+        "metric": "post-filter Mpix/s (deblock+CDEF+LR+grain) 4K 10-bit, bit-exact; % HBM roofline",
+        "value": round(mpix, 1),
+        "unit": "Mpix/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u16" if b == 2 else "u8",
+        "data": "synthetic",
+        "config": {
+            "workload": "%dx%d %d-bit 4:2:0, %d-frame GOP (qindex %s), deblock+CDEF+LR(Wiener/SGR, 64-px units)"
+                        "+film grain, per-frame drop-in" % (args.width, args.height, args.bit_depth, len(frames),
+                                                           "/".join(str(q) for q in QINDEX[:len(frames)])),
+            "frames_per_step": len(frames),
+            "bitexact_frame0": bitexact,
+            "chain_alg_GB_s": round(chain_gbs, 1),
+            "chain_hbm_frac": round(chain_gbs / HBM_PEAK_GBS, 4),
+            "stages": stages,
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

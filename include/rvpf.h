@@ -1,0 +1,244 @@
+/*
+ * rvpf.h — C ABI of the MI355X (gfx950) AV1 decoder post-filter chain.
+ *
+ * The chain is the AV1 in-loop and output filters run once a frame is
+ * reconstructed: the deblocking loop filter (7.14), CDEF (7.15), loop
+ * restoration (7.17, Wiener and self-guided) and film grain synthesis
+ * (7.18.3). Every entry point here replaces a call inside rivet-av1's
+ * decoder; each declaration names the reference function it stands in for
+ * (paths relative to the rivet-av1 crate root).
+ *
+ * Conventions
+ *   - Plain C: fixed-width integers, plain pointers and sizes, no HIP or
+ *     torch types. A HIP stream is passed as `void *` (NULL = default
+ *     stream). Sample planes and side-info arrays handed to the rvpf_*
+ *     compute calls are DEVICE pointers (hipMalloc'd or torch tensors).
+ *   - Every call returns an int status (RVPF_OK or a negative RVPF_E_*),
+ *     never aborts. Work is enqueued on the stream; the call returns once
+ *     it is enqueued (rvpf_ctx_sync waits).
+ *   - Samples are uint16_t at every bit depth (the reference's PlaneBuf,
+ *     src/decoder/state.rs:11-42), or uint8_t at 8 bits when
+ *     rvpf_planes.bytes_per_sample == 1. Plane p sample (x, y) is at
+ *     data[p][y * stride[p] + x]; there is no negative offset.
+ *
+ * Bit-exactness contract: for legal side info (transform blocks aligned to
+ * their size inside blocks aligned to theirs, as every AV1 bitstream has)
+ * the outputs equal rivet-av1's CPU filters sample for sample over
+ *   - deblocking: the whole plane buffer;
+ *   - CDEF / LR / chain reference planes: the mode-info-aligned area
+ *     ((mi_cols*4) >> ssx) x ((mi_rows*4) >> ssy) of each plane;
+ *   - display planes: the cropped upscaled_width x frame_height picture.
+ */
+#ifndef RVPF_H
+#define RVPF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes (the reference's Error kinds, src/lib.rs:60-73). */
+#define RVPF_OK 0
+#define RVPF_E_INVALID (-1)     /* geometry, strides or parameters out of range */
+#define RVPF_E_UNSUPPORTED (-2) /* e.g. use_superres, bytes_per_sample 1 above 8 bits */
+#define RVPF_E_DEVICE (-3)      /* a HIP runtime error */
+#define RVPF_E_NOMEM (-4)       /* device allocation failed */
+
+/* Restoration types (src/consts.rs RESTORE_*). */
+#define RVPF_RESTORE_NONE 0
+#define RVPF_RESTORE_WIENER 1
+#define RVPF_RESTORE_SGRPROJ 2
+
+/*
+ * Mode info of one 4x4 luma block: the fields of the reference's `Mi`
+ * (src/decoder/state.rs:56-75) that the loop filter (postfilter.rs:107-264)
+ * and CDEF's skip test (postfilter.rs:306-307) read. 8 bytes.
+ */
+typedef struct rvpf_mi {
+  uint8_t mi_size;    /* BLOCK_SIZE, 0..21 (BLOCK_4X4 .. BLOCK_16X64) */
+  uint8_t y_mode;     /* YMode; inter modes are >= NEARESTMV (14) */
+  int8_t ref_frame0;  /* RefFrame[0]: -1 none, 0 intra, 1..7 inter */
+  uint8_t seg_skip;   /* bit 0: skip (no residual); bits 1..3: segment_id */
+  int8_t delta_lf[4]; /* DeltaLF[0..3] */
+} rvpf_mi;
+
+/*
+ * One loop restoration unit (LrType / LrWiener / LrSgrSet / LrSgrXqd of
+ * the reference's LrPlane, src/decoder/mod.rs:31-41). 12 bytes.
+ */
+typedef struct rvpf_lr_unit {
+  uint8_t type;         /* RVPF_RESTORE_* */
+  uint8_t sgr_set;      /* 0..15 */
+  int8_t sgr_xqd[2];    /* [-96,31], [-32,95] */
+  int8_t wiener[2][3];  /* [0] vertical, [1] horizontal taps c0..c2 */
+  uint8_t pad[2];
+} rvpf_lr_unit;
+
+/* Film grain parameters (the reference's FilmGrainParams, src/header.rs:14-43). */
+typedef struct rvpf_grain_params {
+  int32_t apply_grain;
+  uint32_t grain_seed;
+  int32_t num_y_points; /* 0..14 */
+  int32_t point_y_value[16];
+  int32_t point_y_scaling[16];
+  int32_t chroma_scaling_from_luma;
+  int32_t num_cb_points; /* 0..10 */
+  int32_t point_cb_value[16];
+  int32_t point_cb_scaling[16];
+  int32_t num_cr_points; /* 0..10 */
+  int32_t point_cr_value[16];
+  int32_t point_cr_scaling[16];
+  int32_t grain_scaling_minus_8; /* 0..3 */
+  int32_t ar_coeff_lag;          /* 0..3 */
+  int32_t ar_coeffs_y_plus_128[24];
+  int32_t ar_coeffs_cb_plus_128[25];
+  int32_t ar_coeffs_cr_plus_128[25];
+  int32_t ar_coeff_shift_minus_6; /* 0..3 */
+  int32_t grain_scale_shift;      /* 0..3 */
+  int32_t cb_mult, cb_luma_mult, cb_offset;
+  int32_t cr_mult, cr_luma_mult, cr_offset;
+  int32_t overlap_flag;
+  int32_t clip_to_restricted_range;
+} rvpf_grain_params;
+
+/*
+ * Everything scalar the chain reads from the sequence and frame headers
+ * (the reference's FrameCtx / FrameHeader / SequenceHeader fields,
+ * src/decoder/mod.rs:44-80, src/header.rs:89-177).
+ */
+typedef struct rvpf_frame_params {
+  /* sequence */
+  int32_t bit_depth;   /* 8, 10, 12 */
+  int32_t ssx, ssy;    /* chroma subsampling: 4:2:0 = 1,1; 4:2:2 = 1,0; 4:4:4 = 0,0 */
+  int32_t mono_chrome; /* 1 plane */
+  int32_t matrix_coefficients;
+  int32_t enable_cdef;
+  int32_t film_grain_params_present;
+  /* frame geometry */
+  int32_t frame_width, frame_height;
+  int32_t upscaled_width; /* must equal frame_width (super-res unsupported) */
+  int32_t use_superres;
+  int32_t mi_rows, mi_cols;
+  int32_t coded_lossless, allow_intrabc;
+  /* loop filter (7.14) */
+  int32_t loop_filter_level[4];
+  int32_t loop_filter_sharpness;
+  int32_t loop_filter_delta_enabled;
+  int32_t loop_filter_ref_deltas[8];
+  int32_t loop_filter_mode_deltas[2];
+  int32_t delta_lf_multi;
+  int32_t segmentation_enabled;
+  int32_t seg_feature_enabled[8][8]; /* [segment][SEG_LVL_*] */
+  int32_t seg_feature_data[8][8];
+  /* CDEF (7.15) */
+  int32_t cdef_damping; /* 3..6 */
+  int32_t cdef_y_pri_strength[8], cdef_y_sec_strength[8];   /* sec in {0,1,2,4} */
+  int32_t cdef_uv_pri_strength[8], cdef_uv_sec_strength[8];
+  /* loop restoration (7.17) */
+  int32_t frame_restoration_type[3]; /* RVPF_RESTORE_* per plane */
+  int32_t loop_restoration_size[3];  /* 32..256 */
+  int32_t lr_unit_rows[3], lr_unit_cols[3]; /* count_units_in_frame(), src/decoder/mod.rs:1229 */
+  /* film grain (7.18.3) */
+  rvpf_grain_params grain;
+} rvpf_frame_params;
+
+/*
+ * Per-block side info (device pointers). The 4x4 arrays have row stride
+ * mi_stride >= mi_cols + 1 and at least mi_rows + 1 rows, like the
+ * reference's `ms = mi_cols + 32` arrays (src/decoder/mod.rs:547-548):
+ * blocks overhanging the frame write the extra row and column, and the
+ * chroma filters read them.
+ */
+typedef struct rvpf_side_info {
+  const rvpf_mi *mi;
+  int32_t mi_stride;
+  /* TX_SIZE per 4x4 of each plane (luma-MI stride for all planes; chroma
+   * entry (row >> ssy) * mi_stride + (col >> ssx)), as lf_tx_sizes. */
+  const uint8_t *tx_size[3];
+  /* cdef_idx per 64x64 (-1: not filtered), stride cdef_stride >= (mi_cols+15)/16. */
+  const int8_t *cdef_idx;
+  int32_t cdef_stride;
+  /* lr_unit_rows[p] * lr_unit_cols[p] units per plane with restoration. */
+  const rvpf_lr_unit *lr_units[3];
+} rvpf_side_info;
+
+/* Sample planes (device pointers). */
+typedef struct rvpf_planes {
+  void *data[3];
+  int64_t stride[3]; /* in samples */
+  int32_t width[3];  /* allocated columns (<= stride) */
+  int32_t height[3]; /* allocated rows */
+  int32_t bytes_per_sample; /* 2, or 1 at 8 bits */
+} rvpf_planes;
+
+typedef struct rvpf_ctx rvpf_ctx;
+
+/* A context owns device scratch, uploaded tables and graph instances, on
+ * the device current at creation. Calls on one context are not reentrant
+ * across host threads; use one context per thread. */
+int rvpf_ctx_create(rvpf_ctx **out);
+void rvpf_ctx_destroy(rvpf_ctx *ctx);
+int rvpf_ctx_sync(rvpf_ctx *ctx, void *stream);
+/* Message of the last error on this context (static storage, never NULL). */
+const char *rvpf_ctx_last_error(const rvpf_ctx *ctx);
+
+/* Validates params + geometry exactly as the compute calls do (host only). */
+int rvpf_validate(const rvpf_frame_params *p, const rvpf_side_info *s);
+
+/*
+ * Deblocking loop filter, in place on `frame` (replaces
+ * loop_filter_threads, src/decoder/postfilter.rs:67-104, with
+ * edge_loop_filter :107-203, filter_strength :207-264 and
+ * dsp::lf::filter_lanes, src/dsp/lf.rs:58-211). Vertical edges of a plane
+ * are all filtered before its horizontal ones.
+ */
+int rvpf_deblock(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_side_info *s,
+                 rvpf_planes *frame, void *stream);
+
+/*
+ * CDEF from the deblocked frame into `out` (replaces cdef,
+ * src/decoder/postfilter.rs:273-301, with cdef_block :304-342,
+ * cdef_direction :345-401, cdef_filter_block :449-513 and
+ * dsp::cdef::filter_scalar, src/dsp/cdef.rs:48-90). Samples of blocks that
+ * are not filtered are copied.
+ */
+int rvpf_cdef(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_side_info *s,
+              const rvpf_planes *deblocked, rvpf_planes *out, void *stream);
+
+/*
+ * Loop restoration of the CDEF output into `out`, reading the deblocked
+ * frame above and below each 64-row stripe (replaces loop_restoration,
+ * src/decoder/postfilter.rs:608-636, with lr_stripe :640-661,
+ * restore_stripe :677-716, restore_rect_from_window :781-853 and
+ * box_filter_rect :901-991). Samples outside restored units are copied.
+ */
+int rvpf_lr(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_side_info *s,
+            const rvpf_planes *deblocked, const rvpf_planes *cdef, rvpf_planes *out,
+            void *stream);
+
+/*
+ * Output stage: the upscaled_width x frame_height crop of `ref` into
+ * `display`, with film grain added when film_grain_params_present &&
+ * grain.apply_grain (replaces the grain branch of Decoder::output,
+ * src/decoder/mod.rs:846-917, and grain::apply, src/decoder/grain.rs:24-354).
+ */
+int rvpf_grain(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_planes *ref,
+               rvpf_planes *display, void *stream);
+
+/*
+ * The whole chain of Decoder::decode_frame_wrapup (src/decoder/mod.rs:747-816)
+ * and Decoder::output: deblock in place on `recon`, CDEF, loop
+ * restoration into `ref_out` (the reference frame), then crop + grain into
+ * `display` (may be NULL when the frame is not shown).
+ */
+int rvpf_postfilter_frame(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_side_info *s,
+                          rvpf_planes *recon, rvpf_planes *ref_out, rvpf_planes *display,
+                          void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RVPF_H */

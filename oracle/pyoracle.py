@@ -1,0 +1,142 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, never by the product package.
+"""
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+sys.path.insert(0, os.path.join(ROOT, "rivet-av1_amd"))
+from rvpf import abi  # noqa: E402
+
+_lib = None
+
+
+class OrcPlanes(C.Structure):
+    _fields_ = [("data", C.c_void_p * 3), ("stride", C.c_int64 * 3)]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = os.path.join(HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built (run __graft_entry__.build())")
+        L = C.CDLL(path)
+        P = C.POINTER
+        L.orc_lf_filter_lanes.argtypes = [C.c_void_p] + [C.c_int] * 6
+        L.orc_filter_strength.argtypes = [P(abi.RvpfFrameParams), P(abi.RvpfSideInfo), C.c_int, C.c_int,
+                                          C.c_int, C.c_int, C.c_void_p]
+        L.orc_loop_filter.argtypes = [P(abi.RvpfFrameParams), P(abi.RvpfSideInfo), P(OrcPlanes), C.c_int]
+        L.orc_cdef_direction.argtypes = [C.c_void_p, C.c_int64, C.c_int, C.c_int, C.c_int, C.c_void_p]
+        L.orc_cdef_direction.restype = C.c_int
+        L.orc_cdef.argtypes = [P(abi.RvpfFrameParams), P(abi.RvpfSideInfo), P(OrcPlanes), P(OrcPlanes), C.c_int]
+        L.orc_lr.argtypes = [P(abi.RvpfFrameParams), P(abi.RvpfSideInfo), P(OrcPlanes), P(OrcPlanes),
+                             P(OrcPlanes), C.c_int]
+        L.orc_grain_templates.argtypes = [P(abi.RvpfFrameParams), C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_grain_scaling_luts.argtypes = [P(abi.RvpfFrameParams), C.c_void_p]
+        L.orc_grain_apply.argtypes = [P(abi.RvpfFrameParams), C.c_int, C.c_int, C.c_void_p * 3]
+        L.orc_postfilter_frame.argtypes = [P(abi.RvpfFrameParams), P(abi.RvpfSideInfo), P(OrcPlanes),
+                                           P(OrcPlanes), C.c_void_p]
+        L.orc_grain_rng_step.argtypes = [C.c_uint32]
+        L.orc_grain_rng_step.restype = C.c_uint32
+        _lib = L
+    return _lib
+
+
+def planes_struct(arrs):
+    s = OrcPlanes()
+    for p, a in enumerate(arrs):
+        if a is None:
+            continue
+        assert a.dtype == np.uint16 and a.flags.c_contiguous
+        s.data[p] = a.ctypes.data
+        s.stride[p] = a.shape[1]
+    return s
+
+
+def lf_filter_lanes(v, limit, blimit, thresh, filter_size, chroma, bd):
+    """v: int32 array [14, 4]; filtered in place."""
+    assert v.dtype == np.int32 and v.shape == (14, 4) and v.flags.c_contiguous
+    lib().orc_lf_filter_lanes(v.ctypes.data, limit, blimit, thresh, filter_size, int(chroma), bd)
+    return v
+
+
+def cdef_direction(luma, x0, y0, bd):
+    var = C.c_int32()
+    d = lib().orc_cdef_direction(luma.ctypes.data, luma.shape[1], x0, y0, bd, C.byref(var))
+    return d, var.value
+
+
+def loop_filter(frame, planes, threads=1):
+    s = frame.host_side_info()
+    ps = planes_struct(planes)
+    lib().orc_loop_filter(C.byref(frame.params), C.byref(s), C.byref(ps), threads)
+    return planes
+
+
+def cdef(frame, deblocked, threads=1):
+    out = [None if a is None else np.zeros_like(a) for a in deblocked]
+    s = frame.host_side_info()
+    pi, po = planes_struct(deblocked), planes_struct(out)
+    lib().orc_cdef(C.byref(frame.params), C.byref(s), C.byref(pi), C.byref(po), threads)
+    return out
+
+
+def lr(frame, deblocked, cdef_planes, threads=1):
+    out = [None if a is None else np.zeros_like(a) for a in cdef_planes]
+    s = frame.host_side_info()
+    pd, pc, po = planes_struct(deblocked), planes_struct(cdef_planes), planes_struct(out)
+    lib().orc_lr(C.byref(frame.params), C.byref(s), C.byref(pd), C.byref(pc), C.byref(po), threads)
+    return out
+
+
+def grain_templates(params):
+    t = [np.zeros(73 * 82, dtype=np.int32) for _ in range(3)]
+    lib().orc_grain_templates(C.byref(params), t[0].ctypes.data, t[1].ctypes.data, t[2].ctypes.data)
+    return [a.reshape(73, 82) for a in t]
+
+
+def grain_scaling_luts(params):
+    lut = np.zeros((3, 256), dtype=np.int32)
+    lib().orc_grain_scaling_luts(C.byref(params), lut.ctypes.data)
+    return lut
+
+
+def grain_apply(params, w, h, planes):
+    """planes: tight uint16 arrays (luma h x w, chroma rounded up); in place."""
+    arr = (C.c_void_p * 3)()
+    for p, a in enumerate(planes):
+        if a is not None:
+            assert a.dtype == np.uint16 and a.flags.c_contiguous
+            arr[p] = a.ctypes.data
+    lib().orc_grain_apply(C.byref(params), w, h, arr)
+    return planes
+
+
+def rng_step(state):
+    return lib().orc_grain_rng_step(state)
+
+
+def postfilter_frame(frame, threads=1, recon=None):
+    """Runs the chain on copies of frame.rec. Returns (deblocked, ref, display)."""
+    rec = [None if a is None else a.copy() for a in (recon or frame.rec)]
+    ref = [None if a is None else np.zeros_like(a) for a in rec]
+    disp = []
+    arr = (C.c_void_p * 3)()
+    for p in range(3):
+        if p < frame.num_planes:
+            d = np.zeros(frame.crop_shape(p), dtype=np.uint16)
+            arr[p] = d.ctypes.data
+            disp.append(d)
+        else:
+            disp.append(None)
+    s = frame.host_side_info()
+    pr, po = planes_struct(rec), planes_struct(ref)
+    lib().orc_postfilter_frame(C.byref(frame.params), C.byref(s), C.byref(pr), C.byref(po), arr, threads)
+    return rec, ref, disp

@@ -1,0 +1,333 @@
+/*
+ * grain.hip — film grain synthesis (AV1 7.18.3) for gfx950.
+ *
+ * Follows rivet-av1's grain::apply (src/decoder/grain.rs:8-354).
+ *
+ * grain_prepare_kernel: one wave per frame.
+ *   - Gaussian templates (7.18.3.3): luma 73x82, cb/cr 38x44 (4:2:0) or
+ *     larger. The 16-bit LFSR is linear over GF(2), so lane t jumps to the
+ *     state after t*C steps with precomputed powers A^(2^k) of the step
+ *     matrix and then generates its C values.
+ *   - The auto-regressive filter is a recursion: sample (y, x) reads rows
+ *     y-lag..y and columns x-lag..x+lag. Processing it along the diagonals
+ *     T = (lag+1)*y + x visits every dependency at an earlier T, so all
+ *     samples of one T run in parallel (one lane per template row).
+ *   - Scaling LUTs (7.18.3.4) and the per-stripe random block offsets.
+ * grain_apply_kernel: one thread per chroma sample (and the 1, 2 or 4 luma
+ *   samples it covers). The noise of a sample is rebuilt directly from the
+ *   template: its 32x32 block's offset, the horizontal blend with the block
+ *   to the left, then the vertical blend with the stripe above (whose value
+ *   is itself horizontally blended), exactly the order in which the
+ *   reference fills its noise stripes and noise image. Chroma reads the
+ *   un-noised luma of the input, as the reference blends chroma first.
+ */
+#include "av1_tables.h"
+#include "rvpf_internal.h"
+
+namespace rvpf {
+
+namespace {
+
+/* --- LFSR jump tables ------------------------------------------------ */
+constexpr uint16_t lfsr_step(uint16_t r) {
+  return (uint16_t)((r >> 1) | ((((r) ^ (r >> 1) ^ (r >> 3) ^ (r >> 12)) & 1) << 15));
+}
+
+struct JumpTable {
+  uint16_t col[16][16]; /* col[k][j] = A^(2^k) e_j */
+};
+
+constexpr uint16_t apply_cols(const uint16_t *cols, uint16_t s) {
+  uint16_t r = 0;
+  for (int j = 0; j < 16; j++)
+    if ((s >> j) & 1) r ^= cols[j];
+  return r;
+}
+
+constexpr JumpTable make_jump_table() {
+  JumpTable t{};
+  for (int j = 0; j < 16; j++) t.col[0][j] = lfsr_step((uint16_t)(1u << j));
+  for (int k = 1; k < 16; k++)
+    for (int j = 0; j < 16; j++) t.col[k][j] = apply_cols(t.col[k - 1], t.col[k - 1][j]);
+  return t;
+}
+
+constexpr JumpTable JUMP = make_jump_table();
+
+__device__ __forceinline__ uint32_t jump(uint32_t s, uint32_t n) {
+  for (int k = 0; k < 16 && n; k++, n >>= 1)
+    if (n & 1) {
+      uint32_t r = 0;
+#pragma unroll
+      for (int j = 0; j < 16; j++) r ^= ((s >> j) & 1) ? JUMP.col[k][j] : 0u;
+      s = r;
+    }
+  return s;
+}
+
+__device__ __forceinline__ uint32_t step(uint32_t r) {
+  const uint32_t bit = (r ^ (r >> 1) ^ (r >> 3) ^ (r >> 12)) & 1;
+  return (r >> 1) | (bit << 15);
+}
+
+__device__ __forceinline__ int round2(int x, int n) { return (x + ((1 << n) >> 1)) >> n; }
+__device__ __forceinline__ int clip3(int lo, int hi, int x) { return min(max(x, lo), hi); }
+
+constexpr int TW = 82, TH = 73;
+
+/* Gaussian values of the LFSR sequence from `seed` into t (rows x cols, stride TW). */
+__device__ void gaussian_fill(int16_t *t, uint32_t seed, int rows, int cols, int shift, bool on) {
+  const int n = rows * cols;
+  const int per = (n + 63) / 64;
+  const int i0 = threadIdx.x * per;
+  if (!on) {
+    for (int i = i0; i < min(i0 + per, n); i++) t[(i / cols) * TW + i % cols] = 0;
+    return;
+  }
+  uint32_t s = jump(seed, (uint32_t)i0);
+  for (int i = i0; i < min(i0 + per, n); i++) {
+    s = step(s);
+    const int gv = AV1_GAUSSIAN_SEQUENCE[(s >> 5) & 2047];
+    t[(i / cols) * TW + i % cols] = (int16_t)round2(gv, shift);
+  }
+}
+
+__global__ __launch_bounds__(64) void grain_prepare_kernel(GrainHdr h, GrainScratch sc) {
+  __shared__ int16_t tl[TH * TW], tcb[TH * TW], tcr[TH * TW];
+  const int lane = threadIdx.x;
+  const int bd = h.bd;
+  const int gmin = -(128 << (bd - 8)), gmax = (256 << (bd - 8)) - 1 - (128 << (bd - 8));
+  const int shift = 12 - bd + h.grain_scale_shift;
+  const bool chroma = h.nplanes > 1;
+  const int cw = h.ssx ? 44 : 82, ch = h.ssy ? 38 : 73;
+  gaussian_fill(tl, h.seed, TH, TW, shift, h.num_y > 0);
+  if (chroma) {
+    gaussian_fill(tcb, h.seed ^ 0xb524, ch, cw, shift, h.num_cb > 0 || h.cfl);
+    gaussian_fill(tcr, h.seed ^ 0x49d8, ch, cw, shift, h.num_cr > 0 || h.cfl);
+  }
+  __syncthreads();
+  const int lag = h.ar_lag, k = lag + 1, ars = h.ar_shift;
+  /* luma AR (grain.rs:57-73) */
+  for (int T = 3 * k + 3; T <= 72 * k + 78; T++) {
+    for (int y = 3 + lane; y < 73; y += 64) {
+      const int x = T - k * y;
+      if (x < 3 || x > 78) continue;
+      int s = 0, pos = 0;
+      for (int dr = -lag; dr <= 0; dr++)
+        for (int dc = -lag; dc <= lag; dc++) {
+          if (dr == 0 && dc == 0) break;
+          s += tl[(y + dr) * TW + x + dc] * h.ar_y[pos];
+          pos++;
+        }
+      tl[y * TW + x] = (int16_t)clip3(gmin, gmax, tl[y * TW + x] + round2(s, ars));
+    }
+    __syncthreads();
+  }
+  /* chroma AR (grain.rs:101-134) */
+  if (chroma) {
+    for (int T = 3 * k + 3; T <= (ch - 1) * k + (cw - 4); T++) {
+      for (int y = 3 + lane; y < ch; y += 64) {
+        const int x = T - k * y;
+        if (x < 3 || x > cw - 4) continue;
+        int s0 = 0, s1 = 0, pos = 0;
+        for (int dr = -lag; dr <= 0; dr++)
+          for (int dc = -lag; dc <= lag; dc++) {
+            if (dr == 0 && dc == 0) break;
+            s0 += tcb[(y + dr) * TW + x + dc] * h.ar_cb[pos];
+            s1 += tcr[(y + dr) * TW + x + dc] * h.ar_cr[pos];
+            pos++;
+          }
+        if (h.num_y > 0) {
+          const int lx = ((x - 3) << h.ssx) + 3, ly = ((y - 3) << h.ssy) + 3;
+          int l = 0;
+          for (int i = 0; i <= h.ssy; i++)
+            for (int j = 0; j <= h.ssx; j++) l += tl[(ly + i) * TW + lx + j];
+          l = round2(l, h.ssx + h.ssy);
+          s0 += l * h.ar_cb[pos];
+          s1 += l * h.ar_cr[pos];
+        }
+        tcb[y * TW + x] = (int16_t)clip3(gmin, gmax, tcb[y * TW + x] + round2(s0, ars));
+        tcr[y * TW + x] = (int16_t)clip3(gmin, gmax, tcr[y * TW + x] + round2(s1, ars));
+      }
+      __syncthreads();
+    }
+  }
+  /* templates out */
+  for (int i = lane; i < TH * TW; i += 64) {
+    sc.tmpl[i] = tl[i];
+    sc.tmpl[TH * TW + i] = chroma ? tcb[i] : 0;
+    sc.tmpl[2 * TH * TW + i] = chroma ? tcr[i] : 0;
+  }
+  /* scaling LUTs (grain.rs:137-165) */
+  for (int p = 0; p < h.nplanes; p++) {
+    const int np = h.npoints[p];
+    const int *xs = h.points[p][0], *ys = h.points[p][1];
+    for (int x = lane; x < 256; x += 64) {
+      int v = 0;
+      if (np > 0) {
+        if (x < xs[0]) {
+          v = ys[0];
+        } else if (x >= xs[np - 1]) {
+          v = ys[np - 1];
+        } else {
+          int i = 0;
+          while (i + 1 < np && xs[i + 1] <= x) i++;
+          const int dy = ys[i + 1] - ys[i], dx = xs[i + 1] - xs[i];
+          const int delta = dy * ((65536 + (dx >> 1)) / dx);
+          v = ys[i] + (((x - xs[i]) * delta + 32768) >> 16);
+        }
+      }
+      sc.lut[p * 256 + x] = (int16_t)v;
+    }
+  }
+  /* block offsets per noise stripe (grain.rs:186-199) */
+  const int nstripes = ((h.h + 1) / 2 + 15) / 16;
+  const int nblocks = ((h.w + 1) / 2 + 15) / 16;
+  for (int ln = lane; ln < nstripes; ln += 64) {
+    uint32_t r = h.seed;
+    r ^= (uint32_t)(((ln * 37 + 178) & 255) << 8);
+    r ^= (uint32_t)((ln * 173 + 105) & 255);
+    for (int b = 0; b < nblocks; b++) {
+      r = step(r);
+      sc.offsets[ln * sc.max_blocks + b] = (uint8_t)((r >> 8) & 255);
+    }
+  }
+}
+
+/* Noise of plane sample (x, y): blocks of 32 (16 subsampled) with a 2 (1)
+ * sample overlap, horizontally blended, in noise-stripe row i of stripe ln. */
+__device__ __forceinline__ int stripe_noise(const int16_t *t, const uint8_t *offs, int mb, int ln,
+                                            int i, int x, int psx, int psy, bool overlap, int gmin,
+                                            int gmax) {
+  const int bs = 32 >> psx;
+  const int bx = x / bs, j = x - bx * bs;
+  const int o = offs[ln * mb + bx];
+  const int ox = o >> 4, oy = o & 15;
+  const int pox = psx ? 6 + ox : 9 + ox * 2, poy = psy ? 6 + oy : 9 + oy * 2;
+  int gv = t[(poy + i) * TW + pox + j];
+  if (overlap && bx > 0 && j < (psx ? 1 : 2)) {
+    const int o2 = offs[ln * mb + bx - 1];
+    const int ox2 = o2 >> 4, oy2 = o2 & 15;
+    const int pox2 = psx ? 6 + ox2 : 9 + ox2 * 2, poy2 = psy ? 6 + oy2 : 9 + oy2 * 2;
+    const int old = t[(poy2 + i) * TW + pox2 + bs + j];
+    if (psx)
+      gv = old * 23 + gv * 22;
+    else
+      gv = j == 0 ? old * 27 + gv * 17 : old * 17 + gv * 27;
+    gv = clip3(gmin, gmax, round2(gv, 5));
+  }
+  return gv;
+}
+
+__device__ __forceinline__ int noise_at(const int16_t *t, const uint8_t *offs, int mb, int x, int y,
+                                        int psx, int psy, bool overlap, int gmin, int gmax) {
+  const int sh = 5 - psy;
+  const int ln = y >> sh, i = y - (ln << sh);
+  int gv = stripe_noise(t, offs, mb, ln, i, x, psx, psy, overlap, gmin, gmax);
+  if (overlap && ln > 0 && i < (psy ? 1 : 2)) {
+    const int old = stripe_noise(t, offs, mb, ln - 1, i + (32 >> psy), x, psx, psy, overlap, gmin, gmax);
+    if (psy)
+      gv = old * 23 + gv * 22;
+    else
+      gv = i == 0 ? old * 27 + gv * 17 : old * 17 + gv * 27;
+    gv = clip3(gmin, gmax, round2(gv, 5));
+  }
+  return gv;
+}
+
+__device__ __forceinline__ int scale_lut(const int16_t *lut, int bd, int index) {
+  const int sh = bd - 8;
+  const int x = index >> sh;
+  const int rem = index - (x << sh);
+  if (bd == 8 || x == 255) return lut[x];
+  const int start = lut[x], end = lut[x + 1];
+  return start + round2((end - start) * rem, sh);
+}
+
+template <typename TI, typename TO>
+__global__ __launch_bounds__(256) void grain_apply_kernel(GrainHdr h, GrainScratch sc, PlanePtr i0,
+                                                          PlanePtr i1, PlanePtr i2, PlanePtr o0,
+                                                          PlanePtr o1, PlanePtr o2) {
+  __shared__ int16_t lut[3 * 256];
+  for (int i = threadIdx.x; i < h.nplanes * 256; i += blockDim.x) lut[i] = sc.lut[i];
+  __syncthreads();
+  const int ssx = h.nplanes > 1 ? h.ssx : 0, ssy = h.nplanes > 1 ? h.ssy : 0;
+  const int cw = (h.w + ssx) >> ssx, ch = (h.h + ssy) >> ssy;
+  const int cx = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int cy = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (cx >= cw || cy >= ch) return;
+  const int bd = h.bd;
+  const int gmin = -(128 << (bd - 8)), gmax = (256 << (bd - 8)) - 1 - (128 << (bd - 8));
+  const bool ov = h.overlap;
+  const int mb = sc.max_blocks;
+  const TI *lin = reinterpret_cast<const TI *>(i0.data);
+  TO *lout = reinterpret_cast<TO *>(o0.data);
+  if (h.nplanes > 1) {
+    const int lx = cx << ssx, ly = cy << ssy;
+    const int lnx = min(lx + 1, h.w - 1);
+    const int64_t lrow = (int64_t)ly * i0.stride;
+    const int avg = ssx ? round2((int)lin[lrow + lx] + (int)lin[lrow + lnx], 1) : (int)lin[lrow + lx];
+    const int maxv = (1 << bd) - 1;
+    for (int p = 1; p < 3; p++) {
+      const PlanePtr &ip = p == 1 ? i1 : i2;
+      const PlanePtr &op = p == 1 ? o1 : o2;
+      const int orig = reinterpret_cast<const TI *>(ip.data)[(int64_t)cy * ip.stride + cx];
+      int v = orig;
+      const int npts = p == 1 ? h.num_cb : h.num_cr;
+      if (npts > 0 || h.cfl) {
+        int merged;
+        if (h.cfl) {
+          merged = avg;
+        } else {
+          const int lm = p == 1 ? h.cb_luma_mult : h.cr_luma_mult;
+          const int mu = p == 1 ? h.cb_mult : h.cr_mult;
+          const int of = p == 1 ? h.cb_offset : h.cr_offset;
+          const int combined = avg * (lm - 128) + orig * (mu - 128);
+          merged = clip3(0, maxv, (combined >> 6) + (of - 256) * (1 << (bd - 8)));
+        }
+        const int noise = noise_at(sc.tmpl + p * TH * TW, sc.offsets, mb, cx, cy, ssx, ssy, ov, gmin, gmax);
+        const int sn = round2(scale_lut(lut + p * 256, bd, merged) * noise, h.scaling_shift);
+        v = clip3(h.min_value, h.max_chroma, orig + sn);
+      }
+      reinterpret_cast<TO *>(op.data)[(int64_t)cy * op.stride + cx] = (TO)v;
+    }
+  }
+  /* luma samples of this chroma position */
+  for (int dy = 0; dy <= ssy; dy++) {
+    const int ly = (cy << ssy) + dy;
+    if (ly >= h.h) break;
+    for (int dx = 0; dx <= ssx; dx++) {
+      const int lx = (cx << ssx) + dx;
+      if (lx >= h.w) break;
+      const int orig = lin[(int64_t)ly * i0.stride + lx];
+      int v = orig;
+      if (h.num_y > 0) {
+        const int noise = noise_at(sc.tmpl, sc.offsets, mb, lx, ly, 0, 0, ov, gmin, gmax);
+        const int sn = round2(scale_lut(lut, bd, orig) * noise, h.scaling_shift);
+        v = clip3(h.min_value, h.max_luma, orig + sn);
+      }
+      lout[(int64_t)ly * o0.stride + lx] = (TO)v;
+    }
+  }
+}
+
+}  // namespace
+
+int launch_grain_prepare(const GrainHdr &h, const GrainScratch &s, hipStream_t st) {
+  grain_prepare_kernel<<<1, 64, 0, st>>>(h, s);
+  return (int)hipGetLastError();
+}
+
+int launch_grain_apply(const GrainHdr &h, const GrainScratch &s, const PlanePtr in[3],
+                       PlanePtr out[3], int bps, hipStream_t st) {
+  const int ssx = h.nplanes > 1 ? h.ssx : 0, ssy = h.nplanes > 1 ? h.ssy : 0;
+  const int cw = (h.w + ssx) >> ssx, ch = (h.h + ssy) >> ssy;
+  dim3 grid((cw + 63) / 64, (ch + 3) / 4);
+  if (bps == 1)
+    grain_apply_kernel<uint8_t, uint8_t><<<grid, 256, 0, st>>>(h, s, in[0], in[1], in[2], out[0], out[1], out[2]);
+  else
+    grain_apply_kernel<uint16_t, uint16_t><<<grid, 256, 0, st>>>(h, s, in[0], in[1], in[2], out[0], out[1], out[2]);
+  return (int)hipGetLastError();
+}
+
+}  // namespace rvpf

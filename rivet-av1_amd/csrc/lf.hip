@@ -1,0 +1,373 @@
+/*
+ * lf.hip — deblocking loop filter (AV1 7.14) for gfx950.
+ *
+ * Follows rivet-av1's loop_filter_threads / edge_loop_filter /
+ * filter_strength (src/decoder/postfilter.rs:67-264) and the sample filter
+ * dsp::lf::filter_lanes (src/dsp/lf.rs:58-211).
+ *
+ * One thread per 4-sample edge of a plane; one launch per pass (vertical
+ * edges, then horizontal edges), all planes flattened into one grid.
+ * In place. The reference walks the edges of a row in order and writes
+ * back p2..q2 (p5..q5) whatever filter ran; a parallel pass must not, since
+ * a 4-wide edge's neighbour 4 samples on modifies those samples. Each
+ * thread therefore stores only its filter's footprint:
+ *   filter size 4            -> p1..q1
+ *   filter size 8 (6/8-tap)  -> p3..q3 (p3, q3, and for chroma p2, q2, are
+ *                               written back unchanged)
+ *   filter size 16           -> p7..q7 (p7, p6, q6, q7 unchanged)
+ * For side info whose transform blocks are aligned to their size (every
+ * AV1 stream), a size-s edge at x has no other edge in (x - s, x + s), so
+ * these footprints never overlap another thread's reads or writes and the
+ * result equals the sequential one sample for sample.
+ */
+#include "av1_tables.h"
+#include "rvpf_internal.h"
+
+namespace rvpf {
+
+namespace {
+
+__device__ __forceinline__ int clip3(int lo, int hi, int x) { return min(max(x, lo), hi); }
+
+/* filter_strength, src/decoder/postfilter.rs:207-264: lvl of the block. */
+__device__ __forceinline__ int lf_level(const LfHdr &h, const rvpf_mi &m, int i) {
+  const int seg = (m.seg_skip >> 1) & 7;
+  const int rf = m.ref_frame0;
+  const int mode = m.y_mode;
+  const int mode_type = (mode >= 14 && mode != 16 && mode != 24); /* >= NEARESTMV, not GLOBALMV / GLOBAL_GLOBALMV */
+  const int dlf = h.delta_lf_multi ? m.delta_lf[i] : m.delta_lf[0];
+  int lvl = clip3(0, 63, dlf + h.level[i]);
+  if (h.seg_enabled && ((h.seg_lf_enabled[seg] >> i) & 1)) lvl = clip3(0, 63, h.seg_lf_data[seg][i] + lvl);
+  if (h.delta_enabled) {
+    const int ns = 1 << (lvl >> 5);
+    if (rf == 0)
+      lvl += h.ref_deltas[0] * ns;
+    else if (rf > 0)
+      lvl += (h.ref_deltas[rf & 7] + h.mode_deltas[mode_type]) * ns;
+    lvl = clip3(0, 63, lvl);
+  }
+  return lvl;
+}
+
+/* 7.14.6 for one lane: v[k] is the k-th sample across the edge (q0 at 7).
+ * Returns the filter applied: 0 none, 1 narrow, 2 wide-8 (6 for chroma),
+ * 3 wide-16. Same arithmetic as filter_lanes, lane by lane. */
+__device__ __forceinline__ int filter_lane(int v[14], int limit_bd, int blimit_bd, int thresh_bd,
+                                           int filter_size, bool chroma, int bd) {
+  const int sh = bd - 8;
+  const int p0 = v[6], p1 = v[5], p2 = v[4], p3 = v[3];
+  const int q0 = v[7], q1 = v[8], q2 = v[9], q3 = v[10];
+  const int filter_len = filter_size == 4 ? 4 : (chroma ? 6 : (filter_size == 8 ? 8 : 16));
+  const bool hev = abs(p1 - p0) > thresh_bd || abs(q1 - q0) > thresh_bd;
+  bool mask = abs(p0 - q0) * 2 + (abs(p1 - q1) >> 1) > blimit_bd || abs(p1 - p0) > limit_bd ||
+              abs(q1 - q0) > limit_bd;
+  if (filter_len >= 6) mask = mask || abs(p2 - p1) > limit_bd || abs(q2 - q1) > limit_bd;
+  if (filter_len >= 8) mask = mask || abs(p3 - p2) > limit_bd || abs(q3 - q2) > limit_bd;
+  if (mask) return 0;
+  const int t = 1 << sh;
+  bool flat = false, flat2 = false;
+  if (filter_size >= 8) {
+    bool m = abs(p1 - p0) > t || abs(q1 - q0) > t || abs(p2 - p0) > t || abs(q2 - q0) > t;
+    if (filter_len >= 8) m = m || abs(p3 - p0) > t || abs(q3 - q0) > t;
+    flat = !m;
+  }
+  if (filter_size >= 16) {
+    flat2 = !(abs(v[0] - p0) > t || abs(v[13] - q0) > t || abs(v[1] - p0) > t ||
+              abs(v[12] - q0) > t || abs(v[2] - p0) > t || abs(v[11] - q0) > t);
+  }
+  if (filter_size == 4 || !flat) {
+    /* narrow filter (7.14.6.3) */
+    const int lo = -(1 << (bd - 1)), hi = (1 << (bd - 1)) - 1;
+    const int off = 0x80 << sh;
+    const int ps1 = p1 - off, ps0 = p0 - off, qs0 = q0 - off, qs1 = q1 - off;
+    int fl = hev ? clip3(lo, hi, ps1 - qs1) : 0;
+    fl = clip3(lo, hi, fl + 3 * (qs0 - ps0));
+    const int filter1 = clip3(lo, hi, fl + 4) >> 3;
+    const int filter2 = clip3(lo, hi, fl + 3) >> 3;
+    v[7] = clip3(lo, hi, qs0 - filter1) + off;
+    v[6] = clip3(lo, hi, ps0 + filter2) + off;
+    if (!hev) {
+      const int f = (filter1 + 1) >> 1;
+      v[8] = clip3(lo, hi, qs1 - f) + off;
+      v[5] = clip3(lo, hi, ps1 + f) + off;
+    }
+    return 1;
+  }
+  if (filter_size == 8 || !flat2) {
+    if (!chroma) {
+      /* log2 size 3, n = 3, centre tap doubled: 8-tap flat filter */
+      const int o4 = (3 * p3 + 2 * p2 + p1 + p0 + q0 + 4) >> 3;
+      const int o5 = (2 * p3 + p2 + 2 * p1 + p0 + q0 + q1 + 4) >> 3;
+      const int o6 = (p3 + p2 + p1 + 2 * p0 + q0 + q1 + q2 + 4) >> 3;
+      const int o7 = (p2 + p1 + p0 + 2 * q0 + q1 + q2 + q3 + 4) >> 3;
+      const int o8 = (p1 + p0 + q0 + 2 * q1 + q2 + 2 * q3 + 4) >> 3;
+      const int o9 = (p0 + q0 + q1 + 2 * q2 + 3 * q3 + 4) >> 3;
+      v[4] = o4;
+      v[5] = o5;
+      v[6] = o6;
+      v[7] = o7;
+      v[8] = o8;
+      v[9] = o9;
+    } else {
+      /* log2 size 3, n = 2, taps |j| <= 1 doubled: 6-tap chroma filter */
+      const int o5 = (3 * p2 + 2 * p1 + 2 * p0 + q0 + 4) >> 3;
+      const int o6 = (p2 + 2 * p1 + 2 * p0 + 2 * q0 + q1 + 4) >> 3;
+      const int o7 = (p1 + 2 * p0 + 2 * q0 + 2 * q1 + q2 + 4) >> 3;
+      const int o8 = (p0 + 2 * q0 + 2 * q1 + 3 * q2 + 4) >> 3;
+      v[5] = o5;
+      v[6] = o6;
+      v[7] = o7;
+      v[8] = o8;
+    }
+    return 2;
+  }
+  /* log2 size 4, n = 6, taps |j| <= 1 doubled: 16-wide filter; output i
+   * (-6..5) sums v[clamp(i + j, -7, 6)] over j = -6..6. */
+  int o[12];
+#pragma unroll
+  for (int i = -6; i < 6; i++) {
+    int acc = 0;
+#pragma unroll
+    for (int j = -6; j <= 6; j++) {
+      const int pp = min(max(i + j, -7), 6);
+      acc += v[7 + pp] * ((j >= -1 && j <= 1) ? 2 : 1);
+    }
+    o[i + 6] = (acc + 8) >> 4;
+  }
+#pragma unroll
+  for (int i = 0; i < 12; i++) v[1 + i] = o[i];
+  return 3;
+}
+
+template <typename T>
+struct Io;
+template <>
+struct Io<uint16_t> {
+  /* 4 consecutive samples at p (8-byte aligned) */
+  static __device__ __forceinline__ void ld4(const uint16_t *p, int *o) {
+    const uint2 u = *reinterpret_cast<const uint2 *>(p);
+    o[0] = u.x & 0xffff;
+    o[1] = u.x >> 16;
+    o[2] = u.y & 0xffff;
+    o[3] = u.y >> 16;
+  }
+  static __device__ __forceinline__ void st4(uint16_t *p, const int *o) {
+    uint2 u;
+    u.x = (uint32_t)o[0] | ((uint32_t)o[1] << 16);
+    u.y = (uint32_t)o[2] | ((uint32_t)o[3] << 16);
+    *reinterpret_cast<uint2 *>(p) = u;
+  }
+  /* 2 consecutive samples at p (4-byte aligned) */
+  static __device__ __forceinline__ void st2(uint16_t *p, int a, int b) {
+    *reinterpret_cast<uint32_t *>(p) = (uint32_t)a | ((uint32_t)b << 16);
+  }
+};
+template <>
+struct Io<uint8_t> {
+  static __device__ __forceinline__ void ld4(const uint8_t *p, int *o) {
+    const uint32_t u = *reinterpret_cast<const uint32_t *>(p);
+    o[0] = u & 0xff;
+    o[1] = (u >> 8) & 0xff;
+    o[2] = (u >> 16) & 0xff;
+    o[3] = u >> 24;
+  }
+  static __device__ __forceinline__ void st4(uint8_t *p, const int *o) {
+    *reinterpret_cast<uint32_t *>(p) =
+        (uint32_t)o[0] | ((uint32_t)o[1] << 8) | ((uint32_t)o[2] << 16) | ((uint32_t)o[3] << 24);
+  }
+  static __device__ __forceinline__ void st2(uint8_t *p, int a, int b) {
+    *reinterpret_cast<uint16_t *>(p) = (uint16_t)(a | (b << 8));
+  }
+};
+
+struct LfLaunch {
+  Geom g;
+  LfHdr h;
+  LfSide s;
+  PlanePtr planes[3];
+  int plane_id[3];   /* planes filtered, in grid order */
+  int blk_start[4];  /* first block of each grid plane, and the total */
+  int blk_x[3];      /* blocks across each grid plane */
+};
+
+/* Edge decision of edge_loop_filter (postfilter.rs:115-172). Returns the
+ * filter size (0: no filter) and the strength. */
+__device__ __forceinline__ int edge_decision(const LfLaunch &a, int plane, int pass, int ex, int ey,
+                                             int *limit_bd, int *blimit_bd, int *thresh_bd) {
+  const Geom &g = a.g;
+  const int ssx = plane ? g.ssx : 0, ssy = plane ? g.ssy : 0;
+  const int row = (ey << ssy), col = (ex << ssx);
+  const int x = col * 4, y = row * 4;
+  if (x >= g.width || y >= g.height) return 0;
+  if (pass == 0 ? x == 0 : y == 0) return 0;
+  const int xp = x >> ssx, yp = y >> ssy;
+  const int rowo = row | ssy, colo = col | ssx;
+  const int ms = a.s.mi_stride;
+  const rvpf_mi m = a.s.mi[(int64_t)rowo * ms + colo];
+  const uint8_t *tx = a.s.tx[plane];
+  const int tx_sz = min((int)tx[(int64_t)ey * ms + ex], 18);
+  const int prev_tx_sz = min((int)(pass == 0 ? tx[(int64_t)ey * ms + ex - 1] : tx[(int64_t)(ey - 1) * ms + ex]), 18);
+  const int plane_size = AV1_SUBSAMPLED_SIZE[min((int)m.mi_size, 21)][ssx][ssy];
+  bool is_block_edge = true; /* BLOCK_INVALID: a block edge everywhere */
+  if (plane_size < 22)
+    is_block_edge = pass == 0 ? (xp & (4 * AV1_NUM_4X4_BLOCKS_WIDE[plane_size] - 1)) == 0
+                              : (yp & (4 * AV1_NUM_4X4_BLOCKS_HIGH[plane_size] - 1)) == 0;
+  const bool is_tx_edge = pass == 0 ? (xp & (AV1_TX_WIDTH[tx_sz] - 1)) == 0
+                                    : (yp & (AV1_TX_HEIGHT[tx_sz] - 1)) == 0;
+  const bool skip = m.seg_skip & 1;
+  const bool is_intra = m.ref_frame0 <= 0;
+  if (!(is_tx_edge && (is_block_edge || !skip || is_intra))) return 0;
+  const int base_size = pass == 0 ? min(AV1_TX_WIDTH[prev_tx_sz], AV1_TX_WIDTH[tx_sz])
+                                  : min(AV1_TX_HEIGHT[prev_tx_sz], AV1_TX_HEIGHT[tx_sz]);
+  int filter_size = min(plane == 0 ? 16 : 8, base_size);
+  /* A 16-wide edge of legal side info is at least 16 samples from the
+   * plane's edge; keep the reads in bounds for side info that is not. */
+  if (filter_size == 16 && (pass == 0 ? xp : yp) < 8) filter_size = 8;
+  const int li = plane == 0 ? pass : plane + 1;
+  int lvl = lf_level(a.h, m, li);
+  if (lvl == 0) {
+    const int prow = pass == 0 ? rowo : rowo - (1 << ssy);
+    const int pcol = pass == 0 ? colo - (1 << ssx) : colo;
+    const rvpf_mi pm = a.s.mi[(int64_t)prow * ms + pcol];
+    lvl = lf_level(a.h, pm, li);
+  }
+  if (lvl == 0) return 0;
+  const int sharp = a.h.sharpness;
+  const int shift = sharp > 4 ? 2 : (sharp > 0 ? 1 : 0);
+  const int limit = sharp > 0 ? clip3(1, 9 - sharp, lvl >> shift) : max(lvl >> shift, 1);
+  const int sh = g.bd - 8;
+  *limit_bd = limit << sh;
+  *blimit_bd = (2 * (lvl + 2) + limit) << sh;
+  *thresh_bd = (lvl >> 4) << sh;
+  return filter_size;
+}
+
+template <typename T, int PASS>
+__global__ __launch_bounds__(256) void lf_pass_kernel(LfLaunch a) {
+  const int b = blockIdx.x;
+  int gp = 0;
+  if (b >= a.blk_start[1]) gp = 1;
+  if (b >= a.blk_start[2]) gp = 2;
+  const int plane = a.plane_id[gp];
+  const int lb = b - a.blk_start[gp];
+  const int ex = (lb % a.blk_x[gp]) * 64 + (threadIdx.x & 63);
+  const int ey = (lb / a.blk_x[gp]) * 4 + (threadIdx.x >> 6);
+  const Geom &g = a.g;
+  const int ssx = plane ? g.ssx : 0, ssy = plane ? g.ssy : 0;
+  if ((ey << ssy) >= g.mi_rows || (ex << ssx) >= g.mi_cols) return;
+  int limit_bd, blimit_bd, thresh_bd;
+  const int fs = edge_decision(a, plane, PASS, ex, ey, &limit_bd, &blimit_bd, &thresh_bd);
+  if (fs == 0) return;
+  const bool chroma = plane != 0;
+  T *base = reinterpret_cast<T *>(a.planes[plane].data);
+  const int64_t stride = a.planes[plane].stride;
+  const int xp = ex * 4, yp = ey * 4;
+  if (PASS == 0) {
+    /* vertical edge at column xp: each lane is a row */
+#pragma unroll 1
+    for (int l = 0; l < 4; l++) {
+      T *row = base + (int64_t)(yp + l) * stride + xp;
+      int s[16];
+      if (fs == 16) {
+        Io<T>::ld4(row - 8, s);
+        Io<T>::ld4(row - 4, s + 4);
+        Io<T>::ld4(row, s + 8);
+        Io<T>::ld4(row + 4, s + 12);
+      } else {
+        Io<T>::ld4(row - 4, s + 4);
+        Io<T>::ld4(row, s + 8);
+      }
+      int v[14];
+#pragma unroll
+      for (int k = 0; k < 14; k++) v[k] = s[k + 1]; /* v[k] = sample at xp - 7 + k */
+      const int r = filter_lane(v, limit_bd, blimit_bd, thresh_bd, fs, chroma, g.bd);
+      if (r == 0) continue;
+#pragma unroll
+      for (int k = 0; k < 14; k++) s[k + 1] = v[k];
+      if (fs == 4) {
+        Io<T>::st2(row - 2, s[6], s[7]);
+        Io<T>::st2(row, s[8], s[9]);
+      } else if (fs == 8) {
+        Io<T>::st4(row - 4, s + 4);
+        Io<T>::st4(row, s + 8);
+      } else {
+        Io<T>::st4(row - 8, s);
+        Io<T>::st4(row - 4, s + 4);
+        Io<T>::st4(row, s + 8);
+        Io<T>::st4(row + 4, s + 12);
+      }
+    }
+  } else {
+    /* horizontal edge at row yp: each lane is a column */
+    T *col = base + (int64_t)yp * stride + xp;
+    const int reach = fs == 16 ? 7 : 4;
+    int v[4][14];
+#pragma unroll
+    for (int k = 0; k < 14; k++) {
+      const int d = k - 7;
+      if (d >= -reach && d < reach) {
+        int q[4];
+        Io<T>::ld4(col + (int64_t)d * stride, q);
+#pragma unroll
+        for (int l = 0; l < 4; l++) v[l][k] = q[l];
+      } else {
+#pragma unroll
+        for (int l = 0; l < 4; l++) v[l][k] = 0;
+      }
+    }
+    int any = 0;
+#pragma unroll
+    for (int l = 0; l < 4; l++) any |= filter_lane(v[l], limit_bd, blimit_bd, thresh_bd, fs, chroma, g.bd);
+    if (!any) return;
+    const int lo = fs == 4 ? -2 : (fs == 8 ? -4 : -7);
+    const int hi = fs == 4 ? 2 : (fs == 8 ? 4 : 7);
+#pragma unroll
+    for (int k = 0; k < 14; k++) {
+      const int d = k - 7;
+      if (d >= lo && d < hi) {
+        int q[4] = {v[0][k], v[1][k], v[2][k], v[3][k]};
+        Io<T>::st4(col + (int64_t)d * stride, q);
+      }
+    }
+  }
+}
+
+}  // namespace
+
+int launch_deblock(const Geom &g, const LfHdr &h, const LfSide &s, PlanePtr planes[3], int bps,
+                   hipStream_t st) {
+  LfLaunch a;
+  a.g = g;
+  a.h = h;
+  a.s = s;
+  int n = 0, total = 0;
+  for (int p = 0; p < g.nplanes; p++) {
+    a.planes[p] = planes[p];
+    if (!(p == 0 || h.level[1 + p] != 0)) continue;
+    const int ssx = p ? g.ssx : 0, ssy = p ? g.ssy : 0;
+    const int cols4 = (g.mi_cols + (1 << ssx) - 1) >> ssx;
+    const int rows4 = (g.mi_rows + (1 << ssy) - 1) >> ssy;
+    a.plane_id[n] = p;
+    a.blk_x[n] = (cols4 + 63) / 64;
+    a.blk_start[n] = total;
+    total += a.blk_x[n] * ((rows4 + 3) / 4);
+    n++;
+  }
+  for (int i = n; i < 3; i++) {
+    a.plane_id[i] = 0;
+    a.blk_x[i] = 1;
+    a.blk_start[i] = total;
+  }
+  a.blk_start[3] = total;
+  if (total == 0) return 0;
+  if (bps == 1) {
+    lf_pass_kernel<uint8_t, 0><<<total, 256, 0, st>>>(a);
+    lf_pass_kernel<uint8_t, 1><<<total, 256, 0, st>>>(a);
+  } else {
+    lf_pass_kernel<uint16_t, 0><<<total, 256, 0, st>>>(a);
+    lf_pass_kernel<uint16_t, 1><<<total, 256, 0, st>>>(a);
+  }
+  return (int)hipGetLastError();
+}
+
+}  // namespace rvpf

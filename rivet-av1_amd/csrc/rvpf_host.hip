@@ -1,0 +1,659 @@
+/*
+ * rvpf_host.hip — the C ABI of include/rvpf.h: validation, device scratch,
+ * parameter packing and the stage order of the chain.
+ *
+ * The chain order is Decoder::decode_frame_wrapup's (src/decoder/mod.rs:747-816):
+ * loop filter only if loop_filter_level[0] or [1] is non-zero, CDEF
+ * (identity when coded_lossless, allow_intrabc or !enable_cdef), loop
+ * restoration (identity when no plane uses it), then Decoder::output's crop
+ * and film grain (src/decoder/mod.rs:846-917). Super-resolution is not
+ * supported (RVPF_E_UNSUPPORTED).
+ */
+#include <stdio.h>
+#include <string.h>
+
+#include <new>
+
+#include "av1_tables.h"
+#include "rvpf_internal.h"
+
+using namespace rvpf;
+
+struct rvpf_ctx {
+  int device;
+  hipStream_t side;
+  hipEvent_t ev_fork, ev_join;
+  void *cdef_buf;
+  size_t cdef_bytes;
+  GrainScratch grain;
+  char err[256];
+};
+
+namespace {
+
+thread_local char g_err[256] = "";
+
+int fail(rvpf_ctx *ctx, int code, const char *msg) {
+  char *dst = ctx ? ctx->err : g_err;
+  snprintf(dst, 256, "%s", msg);
+  return code;
+}
+
+int hip_fail(rvpf_ctx *ctx, hipError_t e, const char *where) {
+  char buf[256];
+  snprintf(buf, sizeof buf, "%s: %s", where, hipGetErrorString(e));
+  return fail(ctx, RVPF_E_DEVICE, buf);
+}
+
+int count_units(int unit, int size) {
+  int n = (size + (unit >> 1)) / unit;
+  return n < 1 ? 1 : n;
+}
+
+bool increasing(const int32_t *v, int n) {
+  for (int i = 1; i < n; i++)
+    if (v[i] <= v[i - 1]) return false;
+  return true;
+}
+
+bool in_range(const int32_t *v, int n, int lo, int hi) {
+  for (int i = 0; i < n; i++)
+    if (v[i] < lo || v[i] > hi) return false;
+  return true;
+}
+
+int nplanes_of(const rvpf_frame_params *p) { return p->mono_chrome ? 1 : 3; }
+
+bool lf_on(const rvpf_frame_params *p) { return p->loop_filter_level[0] != 0 || p->loop_filter_level[1] != 0; }
+
+bool cdef_on(const rvpf_frame_params *p) {
+  return !(p->coded_lossless || p->allow_intrabc || !p->enable_cdef);
+}
+
+bool lr_plane_on(const rvpf_frame_params *p, int pl) { return p->frame_restoration_type[pl] != 0; }
+
+bool lr_on(const rvpf_frame_params *p) {
+  for (int pl = 0; pl < nplanes_of(p); pl++)
+    if (lr_plane_on(p, pl)) return true;
+  return false;
+}
+
+bool grain_on(const rvpf_frame_params *p) { return p->film_grain_params_present && p->grain.apply_grain; }
+
+void plane_ss(const rvpf_frame_params *p, int pl, int *sx, int *sy) {
+  *sx = pl ? p->ssx : 0;
+  *sy = pl ? p->ssy : 0;
+}
+
+/* the mode-info area of a plane */
+void area(const rvpf_frame_params *p, int pl, int *w, int *h) {
+  int sx, sy;
+  plane_ss(p, pl, &sx, &sy);
+  *w = (p->mi_cols * 4) >> sx;
+  *h = (p->mi_rows * 4) >> sy;
+}
+
+void crop(const rvpf_frame_params *p, int pl, int *w, int *h) {
+  int sx, sy;
+  plane_ss(p, pl, &sx, &sy);
+  *w = (p->upscaled_width + sx) >> sx;
+  *h = (p->frame_height + sy) >> sy;
+}
+
+int validate_params(const rvpf_frame_params *p, char *msg) {
+#define BAD(...)                          \
+  do {                                    \
+    snprintf(msg, 256, __VA_ARGS__);      \
+    return RVPF_E_INVALID;                \
+  } while (0)
+  if (!p) BAD("params is NULL");
+  if (p->use_superres || p->upscaled_width != p->frame_width) {
+    snprintf(msg, 256, "super-resolution is not supported");
+    return RVPF_E_UNSUPPORTED;
+  }
+  if (p->bit_depth != 8 && p->bit_depth != 10 && p->bit_depth != 12) BAD("bit_depth %d", p->bit_depth);
+  if (p->ssx < 0 || p->ssx > 1 || p->ssy < 0 || p->ssy > 1 || (p->ssx == 0 && p->ssy == 1))
+    BAD("subsampling %d,%d", p->ssx, p->ssy);
+  if (p->frame_width < 1 || p->frame_height < 1 || p->frame_width > 65536 || p->frame_height > 65536)
+    BAD("frame size %dx%d", p->frame_width, p->frame_height);
+  if (p->mi_cols != 2 * ((p->frame_width + 7) >> 3) || p->mi_rows != 2 * ((p->frame_height + 7) >> 3))
+    BAD("mi_rows/mi_cols %d/%d do not match the frame size", p->mi_rows, p->mi_cols);
+  if (!in_range(p->loop_filter_level, 4, 0, 63)) BAD("loop_filter_level out of range");
+  if (p->loop_filter_sharpness < 0 || p->loop_filter_sharpness > 7) BAD("loop_filter_sharpness");
+  if (!in_range(p->loop_filter_ref_deltas, 8, -64, 63) || !in_range(p->loop_filter_mode_deltas, 2, -64, 63))
+    BAD("loop filter deltas out of range");
+  for (int s = 0; s < 8; s++)
+    if (!in_range(&p->seg_feature_data[s][1], 4, -63, 63)) BAD("segment loop filter data out of range");
+  if (cdef_on(p)) {
+    if (p->cdef_damping < 3 || p->cdef_damping > 6) BAD("cdef_damping %d", p->cdef_damping);
+    for (int i = 0; i < 8; i++) {
+      if (p->cdef_y_pri_strength[i] < 0 || p->cdef_y_pri_strength[i] > 15 || p->cdef_uv_pri_strength[i] < 0 ||
+          p->cdef_uv_pri_strength[i] > 15)
+        BAD("cdef primary strength out of range");
+      const int ys = p->cdef_y_sec_strength[i], us = p->cdef_uv_sec_strength[i];
+      if (!(ys == 0 || ys == 1 || ys == 2 || ys == 4) || !(us == 0 || us == 1 || us == 2 || us == 4))
+        BAD("cdef secondary strength must be 0, 1, 2 or 4");
+    }
+  }
+  for (int pl = 0; pl < nplanes_of(p); pl++) {
+    if (p->frame_restoration_type[pl] < 0 || p->frame_restoration_type[pl] > 3) BAD("frame_restoration_type");
+    if (!lr_plane_on(p, pl)) continue;
+    const int us = p->loop_restoration_size[pl];
+    if (us != 32 && us != 64 && us != 128 && us != 256) BAD("loop_restoration_size %d", us);
+    int cw, ch;
+    crop(p, pl, &cw, &ch);
+    if (p->lr_unit_rows[pl] != count_units(us, ch) || p->lr_unit_cols[pl] != count_units(us, cw))
+      BAD("lr_unit_rows/cols of plane %d do not match count_units_in_frame", pl);
+  }
+  if (grain_on(p)) {
+    const rvpf_grain_params *g = &p->grain;
+    if (g->num_y_points < 0 || g->num_y_points > 14) BAD("num_y_points > 14");
+    if (g->num_cb_points < 0 || g->num_cb_points > 10 || g->num_cr_points < 0 || g->num_cr_points > 10)
+      BAD("num_cb_points / num_cr_points > 10");
+    if (!in_range(g->point_y_value, g->num_y_points, 0, 255) || !in_range(g->point_y_scaling, g->num_y_points, 0, 255) ||
+        !in_range(g->point_cb_value, g->num_cb_points, 0, 255) || !in_range(g->point_cb_scaling, g->num_cb_points, 0, 255) ||
+        !in_range(g->point_cr_value, g->num_cr_points, 0, 255) || !in_range(g->point_cr_scaling, g->num_cr_points, 0, 255))
+      BAD("film grain points out of range");
+    if (!increasing(g->point_y_value, g->num_y_points) || !increasing(g->point_cb_value, g->num_cb_points) ||
+        !increasing(g->point_cr_value, g->num_cr_points))
+      BAD("film grain point values do not increase");
+    if (p->mono_chrome && (g->chroma_scaling_from_luma || g->num_cb_points || g->num_cr_points))
+      BAD("chroma grain on a monochrome frame");
+    if (g->grain_seed > 0xffff) BAD("grain_seed");
+    if (g->grain_scaling_minus_8 < 0 || g->grain_scaling_minus_8 > 3 || g->ar_coeff_lag < 0 || g->ar_coeff_lag > 3 ||
+        g->ar_coeff_shift_minus_6 < 0 || g->ar_coeff_shift_minus_6 > 3 || g->grain_scale_shift < 0 ||
+        g->grain_scale_shift > 3)
+      BAD("film grain shifts out of range");
+    if (!in_range(g->ar_coeffs_y_plus_128, 24, 0, 255) || !in_range(g->ar_coeffs_cb_plus_128, 25, 0, 255) ||
+        !in_range(g->ar_coeffs_cr_plus_128, 25, 0, 255))
+      BAD("film grain AR coefficients out of range");
+    if (g->cb_mult < 0 || g->cb_mult > 255 || g->cb_luma_mult < 0 || g->cb_luma_mult > 255 || g->cb_offset < 0 ||
+        g->cb_offset > 511 || g->cr_mult < 0 || g->cr_mult > 255 || g->cr_luma_mult < 0 || g->cr_luma_mult > 255 ||
+        g->cr_offset < 0 || g->cr_offset > 511)
+      BAD("film grain chroma multipliers out of range");
+  }
+  return RVPF_OK;
+#undef BAD
+}
+
+int validate_side(const rvpf_frame_params *p, const rvpf_side_info *s, char *msg) {
+  const bool need_mi = lf_on(p) || cdef_on(p);
+  if (need_mi) {
+    if (!s || !s->mi) {
+      snprintf(msg, 256, "side info: mi is NULL");
+      return RVPF_E_INVALID;
+    }
+    if (s->mi_stride < p->mi_cols + 1) {
+      snprintf(msg, 256, "side info: mi_stride %d < mi_cols + 1", s->mi_stride);
+      return RVPF_E_INVALID;
+    }
+  }
+  if (lf_on(p))
+    for (int pl = 0; pl < nplanes_of(p); pl++)
+      if ((pl == 0 || p->loop_filter_level[1 + pl]) && !s->tx_size[pl]) {
+        snprintf(msg, 256, "side info: tx_size[%d] is NULL", pl);
+        return RVPF_E_INVALID;
+      }
+  if (cdef_on(p) && (!s->cdef_idx || s->cdef_stride < (p->mi_cols + 15) / 16)) {
+    snprintf(msg, 256, "side info: cdef_idx is NULL or cdef_stride too small");
+    return RVPF_E_INVALID;
+  }
+  for (int pl = 0; pl < nplanes_of(p); pl++)
+    if (lr_plane_on(p, pl) && (!s || !s->lr_units[pl])) {
+      snprintf(msg, 256, "side info: lr_units[%d] is NULL", pl);
+      return RVPF_E_INVALID;
+    }
+  return RVPF_OK;
+}
+
+/* margin: extra rows/cols beyond `need` the stage reads */
+int validate_planes(const rvpf_frame_params *p, const rvpf_planes *pl, const char *name, int use_crop, int margin,
+                    char *msg) {
+  if (!pl) {
+    snprintf(msg, 256, "%s planes are NULL", name);
+    return RVPF_E_INVALID;
+  }
+  if (pl->bytes_per_sample != 2 && pl->bytes_per_sample != 1) {
+    snprintf(msg, 256, "%s: bytes_per_sample %d", name, pl->bytes_per_sample);
+    return RVPF_E_INVALID;
+  }
+  if (pl->bytes_per_sample == 1 && p->bit_depth != 8) {
+    snprintf(msg, 256, "%s: 8-bit samples above 8 bits per sample", name);
+    return RVPF_E_UNSUPPORTED;
+  }
+  for (int i = 0; i < nplanes_of(p); i++) {
+    int w, h;
+    if (use_crop)
+      crop(p, i, &w, &h);
+    else
+      area(p, i, &w, &h);
+    w += margin;
+    h += margin;
+    if (!pl->data[i] || pl->stride[i] < pl->width[i] || pl->width[i] < w || pl->height[i] < h) {
+      snprintf(msg, 256, "%s plane %d: needs %dx%d samples, has %dx%d (stride %lld)", name, i, w, h, pl->width[i],
+               pl->height[i], (long long)pl->stride[i]);
+      return RVPF_E_INVALID;
+    }
+    /* the loop filter moves 4 samples at a time */
+    if (margin && (pl->stride[i] % 4 != 0 || ((uintptr_t)pl->data[i] & 7) != 0)) {
+      snprintf(msg, 256, "%s plane %d: stride must be a multiple of 4 samples and data 8-byte aligned", name, i);
+      return RVPF_E_INVALID;
+    }
+    if (((uintptr_t)pl->data[i] & (pl->bytes_per_sample - 1)) != 0) {
+      snprintf(msg, 256, "%s plane %d: data not aligned to its sample size", name, i);
+      return RVPF_E_INVALID;
+    }
+  }
+  return RVPF_OK;
+}
+
+Geom make_geom(const rvpf_frame_params *p) {
+  Geom g;
+  g.bd = p->bit_depth;
+  g.ssx = p->mono_chrome ? 1 : p->ssx;
+  g.ssy = p->mono_chrome ? 1 : p->ssy;
+  g.nplanes = nplanes_of(p);
+  g.width = p->frame_width;
+  g.height = p->frame_height;
+  g.mi_rows = p->mi_rows;
+  g.mi_cols = p->mi_cols;
+  return g;
+}
+
+void to_ptrs(const rvpf_planes *pl, PlanePtr out[3]) {
+  for (int i = 0; i < 3; i++) {
+    out[i].data = pl ? pl->data[i] : nullptr;
+    out[i].stride = pl ? pl->stride[i] : 0;
+  }
+}
+
+LfHdr make_lf(const rvpf_frame_params *p) {
+  LfHdr h;
+  memset(&h, 0, sizeof h);
+  for (int i = 0; i < 4; i++) h.level[i] = p->loop_filter_level[i];
+  h.sharpness = p->loop_filter_sharpness;
+  h.delta_enabled = p->loop_filter_delta_enabled;
+  for (int i = 0; i < 8; i++) h.ref_deltas[i] = p->loop_filter_ref_deltas[i];
+  h.mode_deltas[0] = p->loop_filter_mode_deltas[0];
+  h.mode_deltas[1] = p->loop_filter_mode_deltas[1];
+  h.delta_lf_multi = p->delta_lf_multi;
+  h.seg_enabled = p->segmentation_enabled;
+  for (int s = 0; s < 8; s++)
+    for (int i = 0; i < 4; i++) {
+      if (p->seg_feature_enabled[s][1 + i]) h.seg_lf_enabled[s] |= 1 << i;
+      h.seg_lf_data[s][i] = p->seg_feature_data[s][1 + i];
+    }
+  return h;
+}
+
+CdefHdr make_cdef(const rvpf_frame_params *p) {
+  CdefHdr h;
+  h.damping = p->cdef_damping;
+  for (int i = 0; i < 8; i++) {
+    h.y_pri[i] = p->cdef_y_pri_strength[i];
+    h.y_sec[i] = p->cdef_y_sec_strength[i];
+    h.uv_pri[i] = p->cdef_uv_pri_strength[i];
+    h.uv_sec[i] = p->cdef_uv_sec_strength[i];
+  }
+  return h;
+}
+
+LrHdr make_lr(const rvpf_frame_params *p, const rvpf_side_info *s) {
+  LrHdr h;
+  memset(&h, 0, sizeof h);
+  for (int pl = 0; pl < nplanes_of(p); pl++) {
+    h.type[pl] = p->frame_restoration_type[pl];
+    h.unit_size[pl] = p->loop_restoration_size[pl];
+    h.unit_rows[pl] = p->lr_unit_rows[pl];
+    h.unit_cols[pl] = p->lr_unit_cols[pl];
+    h.units[pl] = s ? s->lr_units[pl] : nullptr;
+  }
+  return h;
+}
+
+GrainHdr make_grain(const rvpf_frame_params *p) {
+  const rvpf_grain_params *g = &p->grain;
+  GrainHdr h;
+  memset(&h, 0, sizeof h);
+  const int bd = p->bit_depth;
+  h.bd = bd;
+  h.ssx = p->ssx;
+  h.ssy = p->ssy;
+  h.nplanes = nplanes_of(p);
+  h.w = p->upscaled_width;
+  h.h = p->frame_height;
+  h.seed = g->grain_seed;
+  h.num_y = g->num_y_points;
+  h.num_cb = g->num_cb_points;
+  h.num_cr = g->num_cr_points;
+  h.cfl = g->chroma_scaling_from_luma;
+  h.scaling_shift = g->grain_scaling_minus_8 + 8;
+  h.ar_lag = g->ar_coeff_lag;
+  h.ar_shift = g->ar_coeff_shift_minus_6 + 6;
+  h.grain_scale_shift = g->grain_scale_shift;
+  for (int i = 0; i < 24; i++) h.ar_y[i] = g->ar_coeffs_y_plus_128[i] - 128;
+  for (int i = 0; i < 25; i++) {
+    h.ar_cb[i] = g->ar_coeffs_cb_plus_128[i] - 128;
+    h.ar_cr[i] = g->ar_coeffs_cr_plus_128[i] - 128;
+  }
+  h.cb_mult = g->cb_mult;
+  h.cb_luma_mult = g->cb_luma_mult;
+  h.cb_offset = g->cb_offset;
+  h.cr_mult = g->cr_mult;
+  h.cr_luma_mult = g->cr_luma_mult;
+  h.cr_offset = g->cr_offset;
+  h.overlap = g->overlap_flag;
+  if (g->clip_to_restricted_range) {
+    h.min_value = 16 << (bd - 8);
+    h.max_luma = 235 << (bd - 8);
+    h.max_chroma = p->matrix_coefficients == 0 ? h.max_luma : 240 << (bd - 8);
+  } else {
+    h.min_value = 0;
+    h.max_luma = h.max_chroma = (256 << (bd - 8)) - 1;
+  }
+  /* scaling points per plane (grain.rs:140-146) */
+  for (int pl = 0; pl < 3; pl++) {
+    int n;
+    const int32_t *xs, *ys;
+    if (pl == 0 || g->chroma_scaling_from_luma) {
+      n = g->num_y_points;
+      xs = g->point_y_value;
+      ys = g->point_y_scaling;
+    } else if (pl == 1) {
+      n = g->num_cb_points;
+      xs = g->point_cb_value;
+      ys = g->point_cb_scaling;
+    } else {
+      n = g->num_cr_points;
+      xs = g->point_cr_value;
+      ys = g->point_cr_scaling;
+    }
+    h.npoints[pl] = n;
+    for (int i = 0; i < n && i < 16; i++) {
+      h.points[pl][0][i] = xs[i];
+      h.points[pl][1][i] = ys[i];
+    }
+  }
+  return h;
+}
+
+int ensure_cdef_buf(rvpf_ctx *ctx, size_t bytes) {
+  if (ctx->cdef_bytes >= bytes) return RVPF_OK;
+  if (ctx->cdef_buf) (void)hipFree(ctx->cdef_buf);
+  ctx->cdef_buf = nullptr;
+  ctx->cdef_bytes = 0;
+  hipError_t e = hipMalloc(&ctx->cdef_buf, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(ctx, RVPF_E_NOMEM, "cannot allocate the CDEF scratch planes");
+  }
+  ctx->cdef_bytes = bytes;
+  return RVPF_OK;
+}
+
+int ensure_grain_scratch(rvpf_ctx *ctx, int w, int h) {
+  const int stripes = ((h + 1) / 2 + 15) / 16 + 1;
+  const int blocks = ((w + 1) / 2 + 15) / 16 + 1;
+  GrainScratch &g = ctx->grain;
+  if (g.tmpl && stripes <= g.max_stripes && blocks <= g.max_blocks) return RVPF_OK;
+  if (g.offsets) (void)hipFree(g.offsets);
+  g.offsets = nullptr;
+  const int ms = stripes > g.max_stripes ? stripes : g.max_stripes;
+  const int mb = blocks > g.max_blocks ? blocks : g.max_blocks;
+  hipError_t e = hipSuccess;
+  if (!g.tmpl) e = hipMalloc(&g.tmpl, 3 * 73 * 82 * sizeof(int16_t));
+  if (e == hipSuccess && !g.lut) e = hipMalloc(&g.lut, 3 * 256 * sizeof(int16_t));
+  if (e == hipSuccess) e = hipMalloc(&g.offsets, (size_t)ms * mb);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(ctx, RVPF_E_NOMEM, "cannot allocate the film grain scratch");
+  }
+  g.max_stripes = ms;
+  g.max_blocks = mb;
+  return RVPF_OK;
+}
+
+int check_launch(rvpf_ctx *ctx, int rc, const char *what) {
+  if (rc != 0) return hip_fail(ctx, (hipError_t)rc, what);
+  return RVPF_OK;
+}
+
+/* display output: crop (+ grain) of `ref` */
+int run_output(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_planes *ref, rvpf_planes *disp,
+               hipStream_t st, bool prepared) {
+  const Geom g = make_geom(p);
+  PlanePtr in[3], out[3];
+  to_ptrs(ref, in);
+  to_ptrs(disp, out);
+  if (grain_on(p)) {
+    const GrainHdr gh = make_grain(p);
+    int rc;
+    if (!prepared) {
+      if ((rc = ensure_grain_scratch(ctx, p->upscaled_width, p->frame_height))) return rc;
+      if ((rc = check_launch(ctx, launch_grain_prepare(gh, ctx->grain, st), "grain prepare"))) return rc;
+    }
+    return check_launch(ctx, launch_grain_apply(gh, ctx->grain, in, out, ref->bytes_per_sample, st), "grain apply");
+  }
+  int w[3] = {0, 0, 0}, h[3] = {0, 0, 0};
+  for (int i = 0; i < g.nplanes; i++) crop(p, i, &w[i], &h[i]);
+  return check_launch(ctx, launch_copy_planes(g, in, out, w, h, ref->bytes_per_sample, st), "display copy");
+}
+
+}  // namespace
+
+extern "C" {
+
+int rvpf_ctx_create(rvpf_ctx **out) {
+  if (!out) return fail(nullptr, RVPF_E_INVALID, "out is NULL");
+  *out = nullptr;
+  rvpf_ctx *ctx = new (std::nothrow) rvpf_ctx();
+  if (!ctx) return fail(nullptr, RVPF_E_NOMEM, "cannot allocate the context");
+  memset(ctx, 0, sizeof *ctx);
+  hipError_t e = hipGetDevice(&ctx->device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming);
+  if (e != hipSuccess) {
+    int rc = hip_fail(nullptr, e, "rvpf_ctx_create");
+    (void)hipGetLastError();
+    rvpf_ctx_destroy(ctx);
+    return rc;
+  }
+  *out = ctx;
+  return RVPF_OK;
+}
+
+void rvpf_ctx_destroy(rvpf_ctx *ctx) {
+  if (!ctx) return;
+  if (ctx->side) (void)hipStreamSynchronize(ctx->side);
+  if (ctx->cdef_buf) (void)hipFree(ctx->cdef_buf);
+  if (ctx->grain.tmpl) (void)hipFree(ctx->grain.tmpl);
+  if (ctx->grain.lut) (void)hipFree(ctx->grain.lut);
+  if (ctx->grain.offsets) (void)hipFree(ctx->grain.offsets);
+  if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+  if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
+  if (ctx->side) (void)hipStreamDestroy(ctx->side);
+  delete ctx;
+}
+
+int rvpf_ctx_sync(rvpf_ctx *ctx, void *stream) {
+  if (!ctx) return fail(nullptr, RVPF_E_INVALID, "ctx is NULL");
+  hipError_t e = hipStreamSynchronize((hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(ctx, e, "rvpf_ctx_sync");
+  return RVPF_OK;
+}
+
+const char *rvpf_ctx_last_error(const rvpf_ctx *ctx) { return ctx ? ctx->err : g_err; }
+
+int rvpf_validate(const rvpf_frame_params *p, const rvpf_side_info *s) {
+  int rc = validate_params(p, g_err);
+  if (rc) return rc;
+  return validate_side(p, s, g_err);
+}
+
+int rvpf_deblock(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_side_info *s, rvpf_planes *frame,
+                 void *stream) {
+  if (!ctx) return fail(nullptr, RVPF_E_INVALID, "ctx is NULL");
+  int rc;
+  if ((rc = validate_params(p, ctx->err)) || (rc = validate_side(p, s, ctx->err)) ||
+      (rc = validate_planes(p, frame, "frame", 0, 8, ctx->err)))
+    return rc;
+  if (!lf_on(p)) return RVPF_OK;
+  const Geom g = make_geom(p);
+  PlanePtr pl[3];
+  to_ptrs(frame, pl);
+  LfSide side;
+  side.mi = s->mi;
+  side.mi_stride = s->mi_stride;
+  for (int i = 0; i < 3; i++) side.tx[i] = s->tx_size[i];
+  return check_launch(ctx, launch_deblock(g, make_lf(p), side, pl, frame->bytes_per_sample, (hipStream_t)stream),
+                      "deblock");
+}
+
+int rvpf_cdef(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_side_info *s, const rvpf_planes *deblocked,
+              rvpf_planes *out, void *stream) {
+  if (!ctx) return fail(nullptr, RVPF_E_INVALID, "ctx is NULL");
+  int rc;
+  if ((rc = validate_params(p, ctx->err)) || (rc = validate_side(p, s, ctx->err)) ||
+      (rc = validate_planes(p, deblocked, "deblocked", 0, 0, ctx->err)) ||
+      (rc = validate_planes(p, out, "out", 0, 0, ctx->err)))
+    return rc;
+  if (deblocked->bytes_per_sample != out->bytes_per_sample)
+    return fail(ctx, RVPF_E_INVALID, "deblocked and out sample sizes differ");
+  const Geom g = make_geom(p);
+  PlanePtr in[3], o[3];
+  to_ptrs(deblocked, in);
+  to_ptrs(out, o);
+  if (!cdef_on(p)) {
+    int w[3], h[3];
+    for (int i = 0; i < g.nplanes; i++) area(p, i, &w[i], &h[i]);
+    return check_launch(ctx, launch_copy_planes(g, in, o, w, h, out->bytes_per_sample, (hipStream_t)stream),
+                        "cdef copy");
+  }
+  return check_launch(ctx,
+                      launch_cdef(g, make_cdef(p), s->mi, s->mi_stride, s->cdef_idx, s->cdef_stride, 1, in, o,
+                                  out->bytes_per_sample, (hipStream_t)stream),
+                      "cdef");
+}
+
+int rvpf_lr(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_side_info *s, const rvpf_planes *deblocked,
+            const rvpf_planes *cdef, rvpf_planes *out, void *stream) {
+  if (!ctx) return fail(nullptr, RVPF_E_INVALID, "ctx is NULL");
+  int rc;
+  if ((rc = validate_params(p, ctx->err)) || (rc = validate_side(p, s, ctx->err)) ||
+      (rc = validate_planes(p, deblocked, "deblocked", 0, 0, ctx->err)) ||
+      (rc = validate_planes(p, cdef, "cdef", 0, 0, ctx->err)) || (rc = validate_planes(p, out, "out", 0, 0, ctx->err)))
+    return rc;
+  if (deblocked->bytes_per_sample != out->bytes_per_sample || cdef->bytes_per_sample != out->bytes_per_sample)
+    return fail(ctx, RVPF_E_INVALID, "plane sample sizes differ");
+  const Geom g = make_geom(p);
+  PlanePtr d[3], c[3], o[3];
+  to_ptrs(deblocked, d);
+  to_ptrs(cdef, c);
+  to_ptrs(out, o);
+  int w[3] = {0, 0, 0}, h[3] = {0, 0, 0};
+  for (int i = 0; i < g.nplanes; i++)
+    if (!lr_plane_on(p, i)) area(p, i, &w[i], &h[i]);
+  hipStream_t st = (hipStream_t)stream;
+  if ((rc = check_launch(ctx, launch_copy_planes(g, c, o, w, h, out->bytes_per_sample, st), "lr copy"))) return rc;
+  return check_launch(ctx, launch_lr(g, make_lr(p, s), d, c, o, out->bytes_per_sample, st), "lr");
+}
+
+int rvpf_grain(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_planes *ref, rvpf_planes *display,
+               void *stream) {
+  if (!ctx) return fail(nullptr, RVPF_E_INVALID, "ctx is NULL");
+  int rc;
+  if ((rc = validate_params(p, ctx->err)) || (rc = validate_planes(p, ref, "ref", 1, 0, ctx->err)) ||
+      (rc = validate_planes(p, display, "display", 1, 0, ctx->err)))
+    return rc;
+  if (ref->bytes_per_sample != display->bytes_per_sample)
+    return fail(ctx, RVPF_E_INVALID, "ref and display sample sizes differ");
+  return run_output(ctx, p, ref, display, (hipStream_t)stream, false);
+}
+
+int rvpf_postfilter_frame(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_side_info *s, rvpf_planes *recon,
+                          rvpf_planes *ref_out, rvpf_planes *display, void *stream) {
+  if (!ctx) return fail(nullptr, RVPF_E_INVALID, "ctx is NULL");
+  int rc;
+  if ((rc = validate_params(p, ctx->err)) || (rc = validate_side(p, s, ctx->err)) ||
+      (rc = validate_planes(p, recon, "recon", 0, lf_on(p) ? 8 : 0, ctx->err)) ||
+      (rc = validate_planes(p, ref_out, "ref_out", 0, 0, ctx->err)))
+    return rc;
+  if (display && (rc = validate_planes(p, display, "display", 1, 0, ctx->err))) return rc;
+  const int bps = recon->bytes_per_sample;
+  if (ref_out->bytes_per_sample != bps || (display && display->bytes_per_sample != bps))
+    return fail(ctx, RVPF_E_INVALID, "plane sample sizes differ");
+  hipStream_t st = (hipStream_t)stream;
+  const Geom g = make_geom(p);
+  const bool show_grain = display && grain_on(p);
+  hipError_t e;
+  /* film grain templates on the side stream, overlapping the filters */
+  if (show_grain) {
+    if ((rc = ensure_grain_scratch(ctx, p->upscaled_width, p->frame_height))) return rc;
+    if ((e = hipEventRecord(ctx->ev_fork, st)) != hipSuccess) return hip_fail(ctx, e, "event record");
+    if ((e = hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0)) != hipSuccess) return hip_fail(ctx, e, "stream wait");
+    if ((rc = check_launch(ctx, launch_grain_prepare(make_grain(p), ctx->grain, ctx->side), "grain prepare")))
+      return rc;
+    if ((e = hipEventRecord(ctx->ev_join, ctx->side)) != hipSuccess) return hip_fail(ctx, e, "event record");
+  }
+  PlanePtr rec[3], ref[3];
+  to_ptrs(recon, rec);
+  to_ptrs(ref_out, ref);
+  /* 1. deblock in place */
+  if (lf_on(p)) {
+    LfSide side;
+    side.mi = s->mi;
+    side.mi_stride = s->mi_stride;
+    for (int i = 0; i < 3; i++) side.tx[i] = s->tx_size[i];
+    if ((rc = check_launch(ctx, launch_deblock(g, make_lf(p), side, rec, bps, st), "deblock"))) return rc;
+  }
+  /* 2. CDEF: planes restored next go to scratch, the others straight to ref_out */
+  const bool any_lr = lr_on(p);
+  PlanePtr cdef[3];
+  if (cdef_on(p)) {
+    if (any_lr) {
+      size_t off[3], bytes = 0;
+      int64_t stride[3];
+      for (int i = 0; i < g.nplanes; i++) {
+        int w, h;
+        area(p, i, &w, &h);
+        stride[i] = (w + 63) & ~63;
+        off[i] = bytes;
+        bytes += (size_t)stride[i] * h * bps;
+        bytes = (bytes + 255) & ~(size_t)255;
+      }
+      if ((rc = ensure_cdef_buf(ctx, bytes))) return rc;
+      for (int i = 0; i < 3; i++) {
+        if (i < g.nplanes && lr_plane_on(p, i)) {
+          cdef[i].data = (char *)ctx->cdef_buf + off[i];
+          cdef[i].stride = stride[i];
+        } else {
+          cdef[i] = ref[i];
+        }
+      }
+    } else {
+      for (int i = 0; i < 3; i++) cdef[i] = ref[i];
+    }
+    if ((rc = check_launch(ctx,
+                           launch_cdef(g, make_cdef(p), s->mi, s->mi_stride, s->cdef_idx, s->cdef_stride, 1, rec,
+                                       cdef, bps, st),
+                           "cdef")))
+      return rc;
+  } else {
+    /* CDEF is the identity: planes not restored are copied to ref_out */
+    int w[3] = {0, 0, 0}, h[3] = {0, 0, 0};
+    for (int i = 0; i < g.nplanes; i++) {
+      cdef[i] = rec[i];
+      if (!lr_plane_on(p, i)) area(p, i, &w[i], &h[i]);
+    }
+    if ((rc = check_launch(ctx, launch_copy_planes(g, rec, ref, w, h, bps, st), "copy"))) return rc;
+  }
+  /* 3. loop restoration of the restored planes into ref_out */
+  if (any_lr && (rc = check_launch(ctx, launch_lr(g, make_lr(p, s), rec, cdef, ref, bps, st), "lr"))) return rc;
+  /* 4. display */
+  if (!display) return RVPF_OK;
+  if (show_grain && (e = hipStreamWaitEvent(st, ctx->ev_join, 0)) != hipSuccess) return hip_fail(ctx, e, "stream wait");
+  return run_output(ctx, p, ref_out, display, st, true);
+}
+
+}  // extern "C"

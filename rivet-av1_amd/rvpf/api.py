@@ -1,0 +1,209 @@
+"""Device-side binding of librvpf.so (the HIP post-filter chain).
+
+Device memory and streams come from PyTorch (ROCm); every compute call
+goes through the C ABI of include/rvpf.h. There is no CPU fallback: if the
+library is missing or no GPU is present, the calls raise.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+_lib = None
+
+
+class RvpfError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("rvpf error %d: %s" % (code, msg))
+        self.code = code
+
+
+def lib():
+    """Loads librvpf.so (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        path = abi.lib_path("librvpf.so")
+        if not os.path.exists(path):
+            raise RuntimeError("librvpf.so is not built (run __graft_entry__.build()): %s" % path)
+        L = C.CDLL(path)
+        P = C.POINTER
+        L.rvpf_ctx_create.argtypes = [P(C.c_void_p)]
+        L.rvpf_ctx_destroy.argtypes = [C.c_void_p]
+        L.rvpf_ctx_destroy.restype = None
+        L.rvpf_ctx_sync.argtypes = [C.c_void_p, C.c_void_p]
+        L.rvpf_ctx_last_error.argtypes = [C.c_void_p]
+        L.rvpf_ctx_last_error.restype = C.c_char_p
+        L.rvpf_validate.argtypes = [P(abi.RvpfFrameParams), P(abi.RvpfSideInfo)]
+        L.rvpf_deblock.argtypes = [C.c_void_p, P(abi.RvpfFrameParams), P(abi.RvpfSideInfo), P(abi.RvpfPlanes), C.c_void_p]
+        L.rvpf_cdef.argtypes = [C.c_void_p, P(abi.RvpfFrameParams), P(abi.RvpfSideInfo), P(abi.RvpfPlanes),
+                                P(abi.RvpfPlanes), C.c_void_p]
+        L.rvpf_lr.argtypes = [C.c_void_p, P(abi.RvpfFrameParams), P(abi.RvpfSideInfo), P(abi.RvpfPlanes),
+                              P(abi.RvpfPlanes), P(abi.RvpfPlanes), C.c_void_p]
+        L.rvpf_grain.argtypes = [C.c_void_p, P(abi.RvpfFrameParams), P(abi.RvpfPlanes), P(abi.RvpfPlanes), C.c_void_p]
+        L.rvpf_postfilter_frame.argtypes = [C.c_void_p, P(abi.RvpfFrameParams), P(abi.RvpfSideInfo),
+                                            P(abi.RvpfPlanes), P(abi.RvpfPlanes), P(abi.RvpfPlanes), C.c_void_p]
+        _lib = L
+    return _lib
+
+
+def validate(params, side=None):
+    """Host-side validation (no GPU needed). Returns (code, message)."""
+    rc = lib().rvpf_validate(C.byref(params), C.byref(side) if side is not None else None)
+    return rc, lib().rvpf_ctx_last_error(None).decode()
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("rvpf needs a ROCm GPU (torch.cuda.is_available() is False)")
+    return torch
+
+
+def _stream_handle(stream):
+    torch = _torch()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)
+
+
+class Context:
+    """rvpf_ctx: owns device scratch; one per host thread."""
+
+    def __init__(self):
+        _torch()
+        h = C.c_void_p()
+        rc = lib().rvpf_ctx_create(C.byref(h))
+        if rc != 0:
+            raise RvpfError(rc, lib().rvpf_ctx_last_error(None).decode())
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().rvpf_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, rc):
+        if rc != 0:
+            raise RvpfError(rc, lib().rvpf_ctx_last_error(self.h).decode())
+
+    def sync(self, stream=None):
+        self.check(lib().rvpf_ctx_sync(self.h, _stream_handle(stream)))
+
+
+def planes_struct(tensors, bytes_per_sample=2):
+    """rvpf_planes over up to three 2-D device tensors (None = absent)."""
+    s = abi.RvpfPlanes()
+    s.bytes_per_sample = bytes_per_sample
+    for p, t in enumerate(tensors):
+        if t is None:
+            continue
+        assert t.dim() == 2 and t.stride(1) == 1
+        s.data[p] = t.data_ptr()
+        s.stride[p] = t.stride(0)
+        s.width[p] = t.shape[1]
+        s.height[p] = t.shape[0]
+    return s
+
+
+class DeviceFrame:
+    """A workload.Frame's side info and reconstruction in device memory."""
+
+    def __init__(self, frame, device="cuda", bytes_per_sample=2):
+        torch = _torch()
+        self.frame = frame
+        self.bps = bytes_per_sample
+        dt = torch.uint8 if bytes_per_sample == 1 else torch.int16
+
+        def dev(a):
+            return torch.from_numpy(np.ascontiguousarray(a)).to(device)
+
+        self.mi = dev(frame.mi)
+        self.tx = [dev(t) for t in frame.tx]
+        self.cdef_idx = dev(frame.cdef_idx)
+        self.lr_units = [dev(u) for u in frame.lr_units]
+        self.side = abi.RvpfSideInfo()
+        self.side.mi = self.mi.data_ptr()
+        self.side.mi_stride = frame.geom.mi_stride
+        for p in range(3):
+            self.side.tx_size[p] = self.tx[p].data_ptr()
+            self.side.lr_units[p] = self.lr_units[p].data_ptr()
+        self.side.cdef_idx = self.cdef_idx.data_ptr()
+        self.side.cdef_stride = frame.geom.cdef_stride
+        self.dtype = dt
+        self.device = device
+        self.rec_host = frame.rec
+
+    def upload_planes(self, arrs):
+        """Device copies of host uint16 planes in this frame's sample type."""
+        torch = _torch()
+        out = []
+        for a in arrs:
+            if a is None:
+                out.append(None)
+                continue
+            if self.bps == 1:
+                out.append(torch.from_numpy(a.astype(np.uint8)).to(self.device))
+            else:
+                out.append(torch.from_numpy(a.view(np.int16)).to(self.device))
+        return out
+
+    def alloc_like_recon(self):
+        torch = _torch()
+        return [None if a is None else torch.zeros(a.shape, dtype=self.dtype, device=self.device)
+                for a in self.rec_host]
+
+    def alloc_display(self):
+        torch = _torch()
+        return [torch.zeros(self.frame.crop_shape(p), dtype=self.dtype, device=self.device)
+                if p < self.frame.num_planes else None for p in range(3)]
+
+
+def to_host(tensors):
+    """Device planes -> host uint16 numpy arrays."""
+    out = []
+    for t in tensors:
+        if t is None:
+            out.append(None)
+            continue
+        a = t.cpu().numpy()
+        out.append(a.astype(np.uint16) if a.dtype == np.uint8 else a.view(np.uint16))
+    return out
+
+
+def deblock(ctx, df, planes, stream=None):
+    ps = planes_struct(planes, df.bps)
+    ctx.check(lib().rvpf_deblock(ctx.h, C.byref(df.frame.params), C.byref(df.side), C.byref(ps),
+                                 _stream_handle(stream)))
+
+
+def cdef(ctx, df, deblocked, out, stream=None):
+    pi, po = planes_struct(deblocked, df.bps), planes_struct(out, df.bps)
+    ctx.check(lib().rvpf_cdef(ctx.h, C.byref(df.frame.params), C.byref(df.side), C.byref(pi), C.byref(po),
+                              _stream_handle(stream)))
+
+
+def lr(ctx, df, deblocked, cdef_planes, out, stream=None):
+    pd, pc, po = planes_struct(deblocked, df.bps), planes_struct(cdef_planes, df.bps), planes_struct(out, df.bps)
+    ctx.check(lib().rvpf_lr(ctx.h, C.byref(df.frame.params), C.byref(df.side), C.byref(pd), C.byref(pc),
+                            C.byref(po), _stream_handle(stream)))
+
+
+def grain(ctx, df, ref, display, stream=None):
+    pr, pdsp = planes_struct(ref, df.bps), planes_struct(display, df.bps)
+    ctx.check(lib().rvpf_grain(ctx.h, C.byref(df.frame.params), C.byref(pr), C.byref(pdsp),
+                               _stream_handle(stream)))
+
+
+def postfilter_frame(ctx, df, recon, ref_out, display, stream=None):
+    pr, po = planes_struct(recon, df.bps), planes_struct(ref_out, df.bps)
+    pdsp = planes_struct(display, df.bps) if display is not None else None
+    ctx.check(lib().rvpf_postfilter_frame(ctx.h, C.byref(df.frame.params), C.byref(df.side), C.byref(pr),
+                                          C.byref(po), C.byref(pdsp) if pdsp is not None else None,
+                                          _stream_handle(stream)))

@@ -1,0 +1,55 @@
+"""Shared helpers of the test suite: synthetic frames, the oracle chain,
+the GPU chain, and parity comparisons."""
+import numpy as np
+
+from rvpf import workload
+
+# (name, Frame kwargs) -- small cases the oracle runs in well under a second
+SMALL_CASES = [
+    ("c1_352x288_8b_420_wiener", dict(width=352, height=288, bit_depth=8, qindex=128, lr=1, seed=11)),
+    ("66x66_8b_420_all", dict(width=66, height=66, bit_depth=8, qindex=100, lr=2, grain=True, seed=3)),
+    ("226x226_10b_420_all", dict(width=226, height=226, bit_depth=10, qindex=160, lr=2, grain=True,
+                                 lf_extras=True, seed=4)),
+    ("200x120_12b_420_all", dict(width=200, height=120, bit_depth=12, qindex=200, lr=2, grain=True,
+                                 lf_extras=True, seed=5, sb_size=128)),
+    ("170x90_10b_444_all", dict(width=170, height=90, bit_depth=10, ssx=0, ssy=0, qindex=64, lr=2,
+                                grain=True, seed=6, lr_unit_size=32)),
+    ("150x100_8b_422_all", dict(width=150, height=100, bit_depth=8, ssx=1, ssy=0, qindex=180, lr=2,
+                                grain=True, seed=7, lf_extras=True)),
+    ("130x70_12b_mono_all", dict(width=130, height=70, bit_depth=12, mono=True, qindex=120, lr=2,
+                                 grain=True, seed=8)),
+    ("257x129_10b_420_inter", dict(width=257, height=129, bit_depth=10, qindex=90, lr=2, grain=True,
+                                   frame_index=3, pan_px=1.5, seed=9, lr_unit_size=128, sb_size=128)),
+    ("96x64_8b_420_sharp", dict(width=96, height=64, bit_depth=8, qindex=250, lr=2, lf_extras=True,
+                                grain=True, grain_lag=0, seed=10, lr_unit_size=256)),
+    ("64x40_10b_444_lag1", dict(width=64, height=40, bit_depth=10, ssx=0, ssy=0, qindex=30, lr=2,
+                                grain=True, grain_lag=1, seed=12, mc_identity=True)),
+]
+
+
+def make_frame(**kw):
+    return workload.Frame(**kw)
+
+
+def region_equal(a, b, rows, cols):
+    return np.array_equal(a[:rows, :cols], b[:rows, :cols])
+
+
+def first_diff(a, b, rows, cols):
+    d = np.argwhere(a[:rows, :cols] != b[:rows, :cols])
+    if len(d) == 0:
+        return None
+    y, x = d[0]
+    return "n=%d first at (y=%d, x=%d): got %d want %d" % (len(d), y, x, a[y, x], b[y, x])
+
+
+def assert_planes(got, want, frame, region, what):
+    for p in range(frame.num_planes):
+        if region == "full":
+            rows, cols = want[p].shape
+        elif region == "mi":
+            rows, cols = frame.mi_area(p)
+        else:
+            rows, cols = frame.crop_shape(p)
+        msg = first_diff(got[p], want[p], rows, cols)
+        assert msg is None, "%s plane %d differs: %s" % (what, p, msg)
