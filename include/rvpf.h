@@ -184,6 +184,27 @@ int rvpf_ctx_sync(rvpf_ctx *ctx, void *stream);
 /* Message of the last error on this context (static storage, never NULL). */
 const char *rvpf_ctx_last_error(const rvpf_ctx *ctx);
 
+/*
+ * Per-stage profiling (no counterpart in the reference, which has no
+ * tracing; for callers that want stage timings without a profiler). While
+ * enabled, every stage's kernels are bracketed by HIP timing events on the
+ * stream they run on; rvpf_ctx_read_profile waits for them, adds the
+ * elapsed times per stage and resets the counters.
+ */
+#define RVPF_STAGE_DEBLOCK 0
+#define RVPF_STAGE_CDEF 1
+#define RVPF_STAGE_LR 2
+#define RVPF_STAGE_GRAIN_PREPARE 3
+#define RVPF_STAGE_GRAIN_APPLY 4
+#define RVPF_STAGE_COPY 5
+#define RVPF_STAGE_COUNT 6
+typedef struct rvpf_profile {
+  float ms[RVPF_STAGE_COUNT];       /* summed kernel time per stage */
+  int32_t launches[RVPF_STAGE_COUNT]; /* stage executions measured */
+} rvpf_profile;
+int rvpf_ctx_set_profiling(rvpf_ctx *ctx, int enable);
+int rvpf_ctx_read_profile(rvpf_ctx *ctx, rvpf_profile *out);
+
 /* Validates params + geometry exactly as the compute calls do (host only). */
 int rvpf_validate(const rvpf_frame_params *p, const rvpf_side_info *s);
 

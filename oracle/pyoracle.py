@@ -35,6 +35,7 @@ def lib():
         L.orc_loop_filter.argtypes = [P(abi.RvpfFrameParams), P(abi.RvpfSideInfo), P(OrcPlanes), C.c_int]
         L.orc_cdef_direction.argtypes = [C.c_void_p, C.c_int64, C.c_int, C.c_int, C.c_int, C.c_void_p]
         L.orc_cdef_direction.restype = C.c_int
+        L.orc_cdef_filter_window.argtypes = [C.c_void_p] + [C.c_int] * 7 + [C.c_void_p]
         L.orc_cdef.argtypes = [P(abi.RvpfFrameParams), P(abi.RvpfSideInfo), P(OrcPlanes), P(OrcPlanes), C.c_int]
         L.orc_lr.argtypes = [P(abi.RvpfFrameParams), P(abi.RvpfSideInfo), P(OrcPlanes), P(OrcPlanes),
                              P(OrcPlanes), C.c_int]
@@ -71,6 +72,14 @@ def cdef_direction(luma, x0, y0, bd):
     var = C.c_int32()
     d = lib().orc_cdef_direction(luma.ctypes.data, luma.shape[1], x0, y0, bd, C.byref(var))
     return d, var.value
+
+
+def cdef_filter_window(win, w, h, pri, sec, damping, direction, coeff_shift):
+    """win: int32 [144] (INT32_MIN = unavailable); returns int32 [h, w]."""
+    assert win.dtype == np.int32 and win.size == 144 and win.flags.c_contiguous
+    out = np.zeros(64, dtype=np.int32)
+    lib().orc_cdef_filter_window(win.ctypes.data, w, h, pri, sec, damping, direction, coeff_shift, out.ctypes.data)
+    return out[:w * h].reshape(h, w)
 
 
 def loop_filter(frame, planes, threads=1):

@@ -387,21 +387,11 @@ static inline int cdef_constrain(int diff, int threshold, int adj) {
   return diff < 0 ? -mag : mag;
 }
 
-/* src/decoder/postfilter.rs:449-513 with src/dsp/cdef.rs:54-90: the
- * w x h block at (x0, y0) of `src` into `dst`. */
-static void cdef_filter_block(const uint16_t *src, int64_t sstride, uint16_t *dst, int64_t dstride,
-                              int x0, int y0, int w, int h, int xlim, int ylim, int pri_str,
-                              int sec_str, int damping, int dir, int coeff_shift) {
-  int32_t win[CDEF_WS * CDEF_WS];
-  for (int i = 0; i < CDEF_WS * CDEF_WS; i++) win[i] = CDEF_NA;
-  for (int i = 0; i < h + 4; i++) {
-    const int y = y0 + i - 2;
-    if (y < 0 || y >= ylim) continue;
-    for (int j = 0; j < w + 4; j++) {
-      const int x = x0 + j - 2;
-      if (x >= 0 && x < xlim) win[i * CDEF_WS + j] = src[(int64_t)y * sstride + x];
-    }
-  }
+/* src/dsp/cdef.rs:54-90 (filter_scalar): the w x h block at window
+ * position (2, 2) of a 12x12 window (CDEF_NA = unavailable) into out (row
+ * stride w). Samples whose own window value is unavailable keep it. */
+void orc_cdef_filter_window(const int32_t *win, int w, int h, int pri_str, int sec_str, int damping,
+                            int dir, int coeff_shift, int32_t *out) {
   const int pri_adj = pri_str ? imax(damping - floor_log2((unsigned)pri_str), 0) : 0;
   const int sec_adj = sec_str ? imax(damping - floor_log2((unsigned)sec_str), 0) : 0;
   const int tsel = (pri_str >> coeff_shift) & 1;
@@ -416,7 +406,10 @@ static void cdef_filter_block(const uint16_t *src, int64_t sstride, uint16_t *ds
     for (int j = 0; j < w; j++) {
       const int ci = (i + 2) * CDEF_WS + j + 2;
       const int x = win[ci];
-      if (x == CDEF_NA) continue; /* beyond the MI edge: kept (see header) */
+      if (x == CDEF_NA) { /* beyond the MI edge: kept (see header) */
+        out[i * w + j] = x;
+        continue;
+      }
       int sum = 0, mx = x, mn = x;
       for (int k = 0; k < 2; k++) {
         for (int sgn = -1; sgn <= 1; sgn += 2) {
@@ -436,10 +429,31 @@ static void cdef_filter_block(const uint16_t *src, int64_t sstride, uint16_t *ds
           }
         }
       }
-      const int vv = x + ((8 + sum - (sum < 0)) >> 4);
-      dst[(int64_t)(y0 + i) * dstride + x0 + j] = (uint16_t)clip3(mn, mx, vv);
+      out[i * w + j] = clip3(mn, mx, x + ((8 + sum - (sum < 0)) >> 4));
     }
   }
+}
+
+/* src/decoder/postfilter.rs:449-513: the window of the w x h block at
+ * (x0, y0) of `src` (unavailable at or beyond xlim / ylim), filtered into
+ * `dst`. */
+static void cdef_filter_block(const uint16_t *src, int64_t sstride, uint16_t *dst, int64_t dstride,
+                              int x0, int y0, int w, int h, int xlim, int ylim, int pri_str,
+                              int sec_str, int damping, int dir, int coeff_shift) {
+  int32_t win[CDEF_WS * CDEF_WS], res[64];
+  for (int i = 0; i < CDEF_WS * CDEF_WS; i++) win[i] = CDEF_NA;
+  for (int i = 0; i < h + 4; i++) {
+    const int y = y0 + i - 2;
+    if (y < 0 || y >= ylim) continue;
+    for (int j = 0; j < w + 4; j++) {
+      const int x = x0 + j - 2;
+      if (x >= 0 && x < xlim) win[i * CDEF_WS + j] = src[(int64_t)y * sstride + x];
+    }
+  }
+  orc_cdef_filter_window(win, w, h, pri_str, sec_str, damping, dir, coeff_shift, res);
+  for (int i = 0; i < h; i++)
+    for (int j = 0; j < w; j++)
+      if (res[i * w + j] != CDEF_NA) dst[(int64_t)(y0 + i) * dstride + x0 + j] = (uint16_t)res[i * w + j];
 }
 
 typedef struct {

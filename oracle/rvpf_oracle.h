@@ -52,6 +52,11 @@ int orc_loop_filter(const rvpf_frame_params *p, const rvpf_side_info *s, orc_pla
 int orc_cdef_direction(const uint16_t *luma, int64_t stride, int x0, int y0, int bit_depth,
                        int32_t *var);
 
+/* 7.15.3 filter of the w x h block at position (2, 2) of a 12x12 window
+ * (INT32_MIN = unavailable) into out (row stride w); src/dsp/cdef.rs:54-90. */
+void orc_cdef_filter_window(const int32_t *win, int w, int h, int pri_str, int sec_str, int damping,
+                            int dir, int coeff_shift, int32_t *out);
+
 /* CDEF of a frame: out = in, then every filtered 8x8 block
  * (src/decoder/postfilter.rs:273-513, src/dsp/cdef.rs:48-90). */
 int orc_cdef(const rvpf_frame_params *p, const rvpf_side_info *s, const orc_planes *in,

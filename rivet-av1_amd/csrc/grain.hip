@@ -76,80 +76,143 @@ __device__ __forceinline__ int clip3(int lo, int hi, int x) { return min(max(x, 
 constexpr int TW = 82, TH = 73;
 
 /* Gaussian values of the LFSR sequence from `seed` into t (rows x cols, stride TW). */
-__device__ void gaussian_fill(int16_t *t, uint32_t seed, int rows, int cols, int shift, bool on) {
+__device__ void gaussian_fill(int16_t *t, const int16_t *gauss, uint32_t seed, int rows, int cols, int shift,
+                              bool on) {
   const int n = rows * cols;
   const int per = (n + 63) / 64;
-  const int i0 = threadIdx.x * per;
-  if (!on) {
-    for (int i = i0; i < min(i0 + per, n); i++) t[(i / cols) * TW + i % cols] = 0;
-    return;
-  }
-  uint32_t s = jump(seed, (uint32_t)i0);
-  for (int i = i0; i < min(i0 + per, n); i++) {
-    s = step(s);
-    const int gv = AV1_GAUSSIAN_SEQUENCE[(s >> 5) & 2047];
-    t[(i / cols) * TW + i % cols] = (int16_t)round2(gv, shift);
+  const int i0 = threadIdx.x * per, i1 = min(i0 + per, n);
+  int r = i0 / cols, c = i0 - r * cols;
+  uint32_t s = on ? jump(seed, (uint32_t)i0) : 0;
+  for (int i = i0; i < i1; i++) {
+    int v = 0;
+    if (on) {
+      s = step(s);
+      v = round2(gauss[(s >> 5) & 2047], shift);
+    }
+    t[r * TW + c] = (int16_t)v;
+    if (++c == cols) {
+      c = 0;
+      r++;
+    }
   }
 }
 
-__global__ __launch_bounds__(64) void grain_prepare_kernel(GrainHdr h, GrainScratch sc) {
-  __shared__ int16_t tl[TH * TW], tcb[TH * TW], tcr[TH * TW];
+/* Luma AR of 7.18.3.3 (grain.rs:57-73) for a fixed lag, along the
+ * diagonals T = (LAG+1)*y + x: at step T the active rows are those with
+ * 3 <= x <= 78, about 76 / (LAG+1) of them, one lane each. */
+template <int LAG>
+__device__ void luma_ar(int16_t *tl, const GrainHdr &h, int gmin, int gmax) {
+  constexpr int K = LAG + 1, NC = 2 * LAG * (LAG + 1);
+  int c[NC > 0 ? NC : 1];
+#pragma unroll
+  for (int i = 0; i < NC; i++) c[i] = h.ar_y[i];
+  const int ars = h.ar_shift;
   const int lane = threadIdx.x;
-  const int bd = h.bd;
-  const int gmin = -(128 << (bd - 8)), gmax = (256 << (bd - 8)) - 1 - (128 << (bd - 8));
-  const int shift = 12 - bd + h.grain_scale_shift;
-  const bool chroma = h.nplanes > 1;
-  const int cw = h.ssx ? 44 : 82, ch = h.ssy ? 38 : 73;
-  gaussian_fill(tl, h.seed, TH, TW, shift, h.num_y > 0);
-  if (chroma) {
-    gaussian_fill(tcb, h.seed ^ 0xb524, ch, cw, shift, h.num_cb > 0 || h.cfl);
-    gaussian_fill(tcr, h.seed ^ 0x49d8, ch, cw, shift, h.num_cr > 0 || h.cfl);
+  if (LAG == 0) {
+    /* no taps: every sample is clipped on its own */
+    for (int i = lane; i < 70 * 76; i += 64) {
+      const int y = 3 + i / 76, x = 3 + i % 76;
+      tl[y * TW + x] = (int16_t)clip3(gmin, gmax, tl[y * TW + x]);
+    }
+    __syncthreads();
+    return;
   }
-  __syncthreads();
-  const int lag = h.ar_lag, k = lag + 1, ars = h.ar_shift;
-  /* luma AR (grain.rs:57-73) */
-  for (int T = 3 * k + 3; T <= 72 * k + 78; T++) {
-    for (int y = 3 + lane; y < 73; y += 64) {
-      const int x = T - k * y;
-      if (x < 3 || x > 78) continue;
+  for (int T = 3 * K + 3; T <= 72 * K + 78; T++) {
+    const int ymin = max(3, (T - 78 + K - 1) / K);
+    const int y = ymin + lane;
+    const int x = T - K * y;
+    if (y <= 72 && x >= 3) {
       int s = 0, pos = 0;
-      for (int dr = -lag; dr <= 0; dr++)
-        for (int dc = -lag; dc <= lag; dc++) {
-          if (dr == 0 && dc == 0) break;
-          s += tl[(y + dr) * TW + x + dc] * h.ar_y[pos];
+#pragma unroll
+      for (int dr = -LAG; dr <= 0; dr++)
+#pragma unroll
+        for (int dc = -LAG; dc <= LAG; dc++) {
+          if (dr == 0 && dc >= 0) continue;
+          s += __mul24((int)tl[(y + dr) * TW + x + dc], c[pos]);
           pos++;
         }
       tl[y * TW + x] = (int16_t)clip3(gmin, gmax, tl[y * TW + x] + round2(s, ars));
     }
     __syncthreads();
   }
-  /* chroma AR (grain.rs:101-134) */
-  if (chroma) {
-    for (int T = 3 * k + 3; T <= (ch - 1) * k + (cw - 4); T++) {
-      for (int y = 3 + lane; y < ch; y += 64) {
-        const int x = T - k * y;
-        if (x < 3 || x > cw - 4) continue;
-        int s0 = 0, s1 = 0, pos = 0;
-        for (int dr = -lag; dr <= 0; dr++)
-          for (int dc = -lag; dc <= lag; dc++) {
-            if (dr == 0 && dc == 0) break;
-            s0 += tcb[(y + dr) * TW + x + dc] * h.ar_cb[pos];
-            s1 += tcr[(y + dr) * TW + x + dc] * h.ar_cr[pos];
-            pos++;
-          }
-        if (h.num_y > 0) {
-          const int lx = ((x - 3) << h.ssx) + 3, ly = ((y - 3) << h.ssy) + 3;
-          int l = 0;
-          for (int i = 0; i <= h.ssy; i++)
-            for (int j = 0; j <= h.ssx; j++) l += tl[(ly + i) * TW + lx + j];
-          l = round2(l, h.ssx + h.ssy);
-          s0 += l * h.ar_cb[pos];
-          s1 += l * h.ar_cr[pos];
+}
+
+/* Chroma AR (grain.rs:101-134): cb and cr together, plus the luma tap,
+ * along the same diagonals as luma. */
+template <int LAG>
+__device__ void chroma_ar(int16_t *tcb, int16_t *tcr, const int16_t *tl, const GrainHdr &h, int cw, int ch,
+                          int gmin, int gmax) {
+  constexpr int K = LAG + 1, NC = 2 * LAG * (LAG + 1) + 1;
+  int c0[NC], c1[NC];
+#pragma unroll
+  for (int i = 0; i < NC; i++) {
+    c0[i] = h.ar_cb[i];
+    c1[i] = h.ar_cr[i];
+  }
+  const int ars = h.ar_shift;
+  const int lane = threadIdx.x;
+  const bool luma_tap = h.num_y > 0;
+  const int xmax = cw - 4;
+  for (int T = 3 * K + 3; T <= (ch - 1) * K + xmax; T++) {
+    const int ymin = max(3, (T - xmax + K - 1) / K);
+    for (int y = ymin + lane; y < ch; y += 64) {
+      const int x = T - K * y;
+      if (x < 3) break;
+      int s0 = 0, s1 = 0, pos = 0;
+#pragma unroll
+      for (int dr = -LAG; dr <= 0; dr++)
+#pragma unroll
+        for (int dc = -LAG; dc <= LAG; dc++) {
+          if (dr == 0 && dc >= 0) continue;
+          s0 += __mul24((int)tcb[(y + dr) * TW + x + dc], c0[pos]);
+          s1 += __mul24((int)tcr[(y + dr) * TW + x + dc], c1[pos]);
+          pos++;
         }
-        tcb[y * TW + x] = (int16_t)clip3(gmin, gmax, tcb[y * TW + x] + round2(s0, ars));
-        tcr[y * TW + x] = (int16_t)clip3(gmin, gmax, tcr[y * TW + x] + round2(s1, ars));
+      if (luma_tap) {
+        const int lx = ((x - 3) << h.ssx) + 3, ly = ((y - 3) << h.ssy) + 3;
+        int l = 0;
+        for (int i = 0; i <= h.ssy; i++)
+          for (int j = 0; j <= h.ssx; j++) l += tl[(ly + i) * TW + lx + j];
+        l = round2(l, h.ssx + h.ssy);
+        s0 += __mul24(l, c0[NC - 1]);
+        s1 += __mul24(l, c1[NC - 1]);
       }
-      __syncthreads();
+      tcb[y * TW + x] = (int16_t)clip3(gmin, gmax, tcb[y * TW + x] + round2(s0, ars));
+      tcr[y * TW + x] = (int16_t)clip3(gmin, gmax, tcr[y * TW + x] + round2(s1, ars));
+    }
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(64) void grain_prepare_kernel(GrainHdr h, GrainScratch sc) {
+  __shared__ int16_t tl[TH * TW], tcb[TH * TW], tcr[TH * TW];
+  __shared__ int16_t gauss[2048];
+  const int lane = threadIdx.x;
+  for (int i = lane; i < 2048; i += 64) gauss[i] = AV1_GAUSSIAN_SEQUENCE[i];
+  __syncthreads();
+  const int bd = h.bd;
+  const int gmin = -(128 << (bd - 8)), gmax = (256 << (bd - 8)) - 1 - (128 << (bd - 8));
+  const int shift = 12 - bd + h.grain_scale_shift;
+  const bool chroma = h.nplanes > 1;
+  const int cw = h.ssx ? 44 : 82, ch = h.ssy ? 38 : 73;
+  gaussian_fill(tl, gauss, h.seed, TH, TW, shift, h.num_y > 0);
+  if (chroma) {
+    gaussian_fill(tcb, gauss, h.seed ^ 0xb524, ch, cw, shift, h.num_cb > 0 || h.cfl);
+    gaussian_fill(tcr, gauss, h.seed ^ 0x49d8, ch, cw, shift, h.num_cr > 0 || h.cfl);
+  }
+  __syncthreads();
+  switch (h.ar_lag) {
+    case 0: luma_ar<0>(tl, h, gmin, gmax); break;
+    case 1: luma_ar<1>(tl, h, gmin, gmax); break;
+    case 2: luma_ar<2>(tl, h, gmin, gmax); break;
+    default: luma_ar<3>(tl, h, gmin, gmax); break;
+  }
+  if (chroma) {
+    switch (h.ar_lag) {
+      case 0: chroma_ar<0>(tcb, tcr, tl, h, cw, ch, gmin, gmax); break;
+      case 1: chroma_ar<1>(tcb, tcr, tl, h, cw, ch, gmin, gmax); break;
+      case 2: chroma_ar<2>(tcb, tcr, tl, h, cw, ch, gmin, gmax); break;
+      default: chroma_ar<3>(tcb, tcr, tl, h, cw, ch, gmin, gmax); break;
     }
   }
   /* templates out */

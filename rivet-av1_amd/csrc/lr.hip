@@ -51,18 +51,23 @@ __device__ __forceinline__ const T *cpp(const PlanePtr &p) {
 
 __device__ __forceinline__ int round2(int x, int n) { return (x + ((1 << n) >> 1)) >> n; }
 
-/* SGR_A2 (postfilter.rs:884-898). */
-__device__ __forceinline__ int sgr_a2(int z) {
-  if (z >= 255) return 256;
-  if (z == 0) return 1;
-  return ((z << 8) + (z >> 1)) / (z + 1);
+/* SGR_A2 (postfilter.rs:884-898): 256 at 255 and above, 1 at 0, else
+ * ((z << 8) + z / 2) / (z + 1). */
+struct A2Table {
+  int16_t v[256];
+};
+constexpr A2Table make_a2() {
+  A2Table t{};
+  for (int z = 0; z < 256; z++) t.v[z] = (int16_t)(z >= 255 ? 256 : (z == 0 ? 1 : ((z << 8) + z / 2) / (z + 1)));
+  return t;
 }
+constexpr A2Table SGR_A2 = make_a2();
 
 template <typename T>
 __global__ __launch_bounds__(256) void lr_kernel(LrLaunch a) {
   __shared__ uint16_t win[WH * WW];
-  __shared__ int16_t inter[WH * SW]; /* Wiener horizontal output; SGR: A */
-  __shared__ int32_t bbuf[(64 + 2) * AW];  /* SGR: B */
+  __shared__ int16_t inter[WH * SW];      /* Wiener horizontal output; SGR: A */
+  __shared__ int32_t bbuf[(64 + 2) * AW]; /* SGR: B */
 
   const Geom &g = a.g;
   const int b = blockIdx.x;
@@ -77,7 +82,9 @@ __global__ __launch_bounds__(256) void lr_kernel(LrLaunch a) {
   const int plane_end_y = ((g.height + ssy) >> ssy) - 1;
   const int area_w = (g.mi_cols * 4) >> ssx, area_h = (g.mi_rows * 4) >> ssy;
   const int unit_size = a.h.unit_size[plane];
-  const int sw = min(SW, unit_size);
+  const int lg = unit_size >= SW ? 6 : 5; /* slice width 64 or 32 */
+  const int sw = 1 << lg;
+  const int tx = threadIdx.x & (sw - 1), ty = threadIdx.x >> lg, ny = 256 >> lg;
   /* lr_stripe, postfilter.rs:640-661 */
   const int stripe_start = (-8 + st * 64) >> ssy;
   const int stripe_end = stripe_start + (64 >> ssy) - 1;
@@ -98,7 +105,7 @@ __global__ __launch_bounds__(256) void lr_kernel(LrLaunch a) {
   if (y0 <= y1 && x0 <= xr) {
     const int luma_row = max(st * 64 - 8, 0);
     const int unit_row = min(a.h.unit_rows[plane] - 1, ((luma_row + 8) >> ssy) / unit_size);
-    const int unit_col = min(a.h.unit_cols[plane] - 1, x0 / unit_size);
+    const int unit_col = min(a.h.unit_cols[plane] - 1, x0 >> (31 - __clz(unit_size)));
     u = &a.h.units[plane][unit_row * a.h.unit_cols[plane] + unit_col];
     type = u->type;
     if (type != RVPF_RESTORE_WIENER && type != RVPF_RESTORE_SGRPROJ) type = 0;
@@ -106,12 +113,17 @@ __global__ __launch_bounds__(256) void lr_kernel(LrLaunch a) {
 
   /* copy what is not restored */
   {
-    const int w = x1c - x0 + 1, h = y1c - y0 + 1;
-    for (int i = threadIdx.x; i < w * h; i += blockDim.x) {
-      const int r = i / w, c = i - r * w;
-      const int x = x0 + c, y = y0 + r;
-      if (type != 0 && x <= xr && y <= y1) continue;
-      out[(int64_t)y * os + x] = cdef[(int64_t)y * cs_ + x];
+    const int wc = x1c - x0 + 1, hc = y1c - y0 + 1;
+    const int x = x0 + tx;
+    if (tx < wc) {
+      T *o = out + (int64_t)(y0 + ty) * os + x;
+      const T *c = cdef + (int64_t)(y0 + ty) * cs_ + x;
+      const int64_t ostep = (int64_t)ny * os, cstep = (int64_t)ny * cs_;
+      for (int r = ty; r < hc; r += ny, o += ostep, c += cstep) {
+        const int y = y0 + r;
+        if (type != 0 && x <= xr && y <= y1) continue;
+        *o = *c;
+      }
     }
   }
   if (type == 0) return;
@@ -122,21 +134,19 @@ __global__ __launch_bounds__(256) void lr_kernel(LrLaunch a) {
   {
     const T *deb = cpp<T>(a.deb[plane]);
     const int64_t ds = a.deb[plane].stride;
-    for (int i = threadIdx.x; i < ww * wh; i += blockDim.x) {
-      const int r = i / ww, c = i - r * ww;
-      const int x = min(max(x0 + c - 3, 0), plane_end_x);
+    for (int r = ty; r < wh; r += ny) {
       int y = min(max(y0 + r - 3, 0), plane_end_y);
-      int v;
+      const T *src;
       if (y < stripe_start) {
         y = max(stripe_start - 2, y);
-        v = deb[(int64_t)y * ds + x];
+        src = deb + (int64_t)y * ds;
       } else if (y > stripe_end) {
         y = min(stripe_end + 2, y);
-        v = deb[(int64_t)y * ds + x];
+        src = deb + (int64_t)y * ds;
       } else {
-        v = cdef[(int64_t)y * cs_ + x];
+        src = cdef + (int64_t)y * cs_;
       }
-      win[r * WW + c] = (uint16_t)v;
+      for (int c = tx; c < ww; c += sw) win[r * WW + c] = src[min(max(x0 + c - 3, 0), plane_end_x)];
     }
   }
   __syncthreads();
@@ -145,37 +155,45 @@ __global__ __launch_bounds__(256) void lr_kernel(LrLaunch a) {
 
   if (type == RVPF_RESTORE_WIENER) {
     /* wiener_coefficient (postfilter.rs:994-1004) + rounding_variables
-     * (predict.rs:1153-1166) */
+     * (predict.rs:1153-1166); symmetric taps */
     const int round0 = bd == 12 ? 5 : 3, round1 = bd == 12 ? 9 : 11;
     const int v0 = u->wiener[0][0], v1 = u->wiener[0][1], v2 = u->wiener[0][2];
     const int h0 = u->wiener[1][0], h1 = u->wiener[1][1], h2 = u->wiener[1][2];
     const int v3 = 128 - 2 * (v0 + v1 + v2), h3 = 128 - 2 * (h0 + h1 + h2);
     const int offset = 1 << (bd + 7 - round0 - 1);
     const int limit = (1 << (bd + 1 + 7 - round0)) - 1;
-    for (int i = threadIdx.x; i < wh * w; i += blockDim.x) {
-      const int r = i / w, c = i - r * w;
-      const uint16_t *s = &win[r * WW + c];
-      const int sum = h0 * (s[0] + s[6]) + h1 * (s[1] + s[5]) + h2 * (s[2] + s[4]) + h3 * s[3];
-      inter[r * SW + c] = (int16_t)min(max(round2(sum, round0), -offset), limit - offset);
-    }
+    if (tx < w)
+      for (int r = ty; r < wh; r += ny) {
+        const uint16_t *s = &win[r * WW + tx];
+        const int sum = __mul24(h0, s[0] + s[6]) + __mul24(h1, s[1] + s[5]) + __mul24(h2, s[2] + s[4]) +
+                        __mul24(h3, (int)s[3]);
+        inter[r * SW + tx] = (int16_t)min(max(round2(sum, round0), -offset), limit - offset);
+      }
     __syncthreads();
-    for (int i = threadIdx.x; i < h * w; i += blockDim.x) {
-      const int r = i / w, c = i - r * w;
-      const int16_t *s = &inter[r * SW + c];
-      const int sum = v0 * (s[0] + s[6 * SW]) + v1 * (s[SW] + s[5 * SW]) + v2 * (s[2 * SW] + s[4 * SW]) +
-                      v3 * s[3 * SW];
-      out[(int64_t)(y0 + r) * os + x0 + c] = (T)min(max(round2(sum, round1), 0), maxv);
+    if (tx < w) {
+      T *o = out + (int64_t)(y0 + ty) * os + x0 + tx;
+      const int64_t ostep = (int64_t)ny * os;
+      for (int r = ty; r < h; r += ny, o += ostep) {
+        const int16_t *s = &inter[r * SW + tx];
+        const int sum = __mul24(v0, s[0] + s[6 * SW]) + __mul24(v1, s[SW] + s[5 * SW]) +
+                        __mul24(v2, s[2 * SW] + s[4 * SW]) + __mul24(v3, (int)s[3 * SW]);
+        *o = (T)min(max(round2(sum, round1), 0), maxv);
+      }
     }
     return;
   }
 
-  /* self-guided: box_filter_rect (postfilter.rs:901-991) for both passes,
-   * the filtered values of pass 0 kept in registers */
+  /* self-guided: box_filter_rect (postfilter.rs:901-991) for both passes;
+   * pixel (r, tx) of row iteration k keeps pass 0's value in flt0[k] */
   const int set = u->sgr_set & 15;
+  const int r0 = AV1_SGR_PARAMS[set][0], r1 = AV1_SGR_PARAMS[set][2];
+  const int w0 = u->sgr_xqd[0], w1 = u->sgr_xqd[1];
+  const int w2 = 128 - w0 - w1;
   int flt0[PX_PER_THREAD];
   int16_t *A = inter;
+#pragma unroll
   for (int pass = 0; pass < 2; pass++) {
-    const int r = AV1_SGR_PARAMS[set][pass * 2];
+    const int r = pass == 0 ? r0 : r1;
     if (r == 0) continue;
     const int eps = AV1_SGR_PARAMS[set][pass * 2 + 1];
     const int n = (2 * r + 1) * (2 * r + 1);
@@ -185,86 +203,87 @@ __global__ __launch_bounds__(256) void lr_kernel(LrLaunch a) {
     /* p * s fits in 32 bits up to this bound; beyond it z saturates */
     const uint32_t p_lim = (0xFFFFFFFFu - (1u << 19)) / s;
     __syncthreads();
-    for (int i = threadIdx.x; i < (h + 2) * (w + 2); i += blockDim.x) {
-      const int ii = i / (w + 2), jj = i - ii * (w + 2);
+    /* A and B at rectangle positions (ii - 1, jj - 1) = window (ii + 2, jj + 2) */
+    for (int ii = ty; ii < h + 2; ii += ny) {
       if (pass == 0 && ((y0 + ii - 1) & 1) == 0) continue;
-      /* position (ii - 1, jj - 1) of the rectangle = window (ii + 2, jj + 2) */
-      uint32_t sum = 0, sq = 0;
-      for (int dy = -r; dy <= r; dy++) {
-        const uint16_t *row = &win[(ii + 2 + dy) * WW + jj + 2];
-        for (int dx = -r; dx <= r; dx++) {
-          const uint32_t v = row[dx];
-          sum += v;
-          sq += v * v;
+      for (int jj = tx; jj < w + 2; jj += sw) {
+        uint32_t sum = 0, sq = 0;
+        if (r == 2) {
+#pragma unroll
+          for (int dy = -2; dy <= 2; dy++) {
+            const uint16_t *row = &win[(ii + 2 + dy) * WW + jj + 2];
+#pragma unroll
+            for (int dx = -2; dx <= 2; dx++) {
+              const uint32_t v = row[dx];
+              sum += v;
+              sq += __umul24(v, v);
+            }
+          }
+        } else {
+#pragma unroll
+          for (int dy = -1; dy <= 1; dy++) {
+            const uint16_t *row = &win[(ii + 2 + dy) * WW + jj + 2];
+#pragma unroll
+            for (int dx = -1; dx <= 1; dx++) {
+              const uint32_t v = row[dx];
+              sum += v;
+              sq += __umul24(v, v);
+            }
+          }
         }
+        const uint32_t av = (uint32_t)round2((int)sq, 2 * (bd - 8));
+        const uint32_t d = (uint32_t)round2((int)sum, bd - 8);
+        const uint32_t an = __umul24(av, (uint32_t)n), dd = __umul24(d, d);
+        const uint32_t p = an > dd ? an - dd : 0;
+        const uint32_t z = p > p_lim ? 255u : min((p * s + (1u << 19)) >> 20, 255u);
+        const int a2 = SGR_A2.v[z];
+        const uint64_t b2 = (uint64_t)(uint32_t)(256 - a2) * (uint64_t)__umul24(sum, one_over_n);
+        A[ii * AW + jj] = (int16_t)a2;
+        bbuf[ii * AW + jj] = (int32_t)((b2 + 2048) >> 12);
       }
-      const uint32_t av = (uint32_t)round2((int)sq, 2 * (bd - 8));
-      const uint32_t d = (uint32_t)round2((int)sum, bd - 8);
-      const uint32_t an = av * (uint32_t)n, dd = d * d;
-      const uint32_t p = an > dd ? an - dd : 0;
-      const uint32_t z = p > p_lim ? 255u : min((p * s + (1u << 19)) >> 20, 255u);
-      const int a2 = sgr_a2((int)z);
-      const uint64_t b2 = (uint64_t)(uint32_t)(256 - a2) * (uint64_t)(sum * one_over_n);
-      A[ii * AW + jj] = (int16_t)a2;
-      bbuf[ii * AW + jj] = (int32_t)((b2 + 2048) >> 12);
     }
     __syncthreads();
+    if (tx < w) {
 #pragma unroll
-    for (int k = 0; k < PX_PER_THREAD; k++) {
-      const int i = threadIdx.x + k * 256;
-      if (i >= h * w) break;
-      const int ii = i / w, jj = i - ii * w;
-      const int yabs = y0 + ii;
-      int av = 0, bv = 0;
-      const int c = (ii + 1) * AW + jj + 1;
-      int shift;
-      if (pass == 0) {
-        if (yabs & 1) {
-          /* odd row: only this row's A/B, weights 6 (centre) / 5 */
-          av = 6 * A[c] + 5 * (A[c - 1] + A[c + 1]);
-          bv = 6 * bbuf[c] + 5 * (bbuf[c - 1] + bbuf[c + 1]);
-          shift = 4;
+      for (int k = 0; k < PX_PER_THREAD; k++) {
+        const int ii = ty + k * ny;
+        if (ii >= h) break;
+        const int yabs = y0 + ii;
+        const int c = (ii + 1) * AW + tx + 1;
+        int av, bv, shift;
+        if (pass == 0) {
+          if (yabs & 1) {
+            /* odd row: this row's A/B only, weights 6 (centre) / 5 */
+            av = 6 * A[c] + 5 * (A[c - 1] + A[c + 1]);
+            bv = 6 * bbuf[c] + 5 * (bbuf[c - 1] + bbuf[c + 1]);
+            shift = 4;
+          } else {
+            /* even row: the rows above and below */
+            av = 6 * (A[c - AW] + A[c + AW]) + 5 * (A[c - AW - 1] + A[c - AW + 1] + A[c + AW - 1] + A[c + AW + 1]);
+            bv = 6 * (bbuf[c - AW] + bbuf[c + AW]) +
+                 5 * (bbuf[c - AW - 1] + bbuf[c - AW + 1] + bbuf[c + AW - 1] + bbuf[c + AW + 1]);
+            shift = 5;
+          }
         } else {
-          /* even row: the rows above and below */
-          av = 6 * (A[c - AW] + A[c + AW]) + 5 * (A[c - AW - 1] + A[c - AW + 1] + A[c + AW - 1] + A[c + AW + 1]);
-          bv = 6 * (bbuf[c - AW] + bbuf[c + AW]) +
-               5 * (bbuf[c - AW - 1] + bbuf[c - AW + 1] + bbuf[c + AW - 1] + bbuf[c + AW + 1]);
+          av = 4 * (A[c] + A[c - 1] + A[c + 1] + A[c - AW] + A[c + AW]) +
+               3 * (A[c - AW - 1] + A[c - AW + 1] + A[c + AW - 1] + A[c + AW + 1]);
+          bv = 4 * (bbuf[c] + bbuf[c - 1] + bbuf[c + 1] + bbuf[c - AW] + bbuf[c + AW]) +
+               3 * (bbuf[c - AW - 1] + bbuf[c - AW + 1] + bbuf[c + AW - 1] + bbuf[c + AW + 1]);
           shift = 5;
         }
-      } else {
-        av = 4 * (A[c] + A[c - 1] + A[c + 1] + A[c - AW] + A[c + AW]) +
-             3 * (A[c - AW - 1] + A[c - AW + 1] + A[c + AW - 1] + A[c + AW + 1]);
-        bv = 4 * (bbuf[c] + bbuf[c - 1] + bbuf[c + 1] + bbuf[c - AW] + bbuf[c + AW]) +
-             3 * (bbuf[c - AW - 1] + bbuf[c - AW + 1] + bbuf[c + AW - 1] + bbuf[c + AW + 1]);
-        shift = 5;
+        const int uu = win[(ii + 3) * WW + tx + 3];
+        const int v = round2(__mul24(av, uu) + bv, 4 + shift);
+        if (pass == 0 && r1 != 0) {
+          flt0[k] = v;
+        } else {
+          /* projection (postfilter.rs:823-851); here the last pass ran */
+          const int ue = uu << 4;
+          const int f0 = pass == 0 ? v : (r0 ? flt0[k] : ue);
+          const int f1 = pass == 1 ? v : ue;
+          const int acc = __mul24(w1, ue) + __mul24(w0, f0) + __mul24(w2, f1);
+          out[(int64_t)(y0 + ii) * os + x0 + tx] = (T)min(max(round2(acc, 11), 0), maxv);
+        }
       }
-      const int uu = win[(ii + 3) * WW + jj + 3];
-      const int v = round2(av * uu + bv, 4 + shift);
-      if (pass == 0) {
-        flt0[k] = v;
-      } else {
-        /* projection (postfilter.rs:823-851) */
-        const int w0 = u->sgr_xqd[0], w1 = u->sgr_xqd[1];
-        const int w2 = 128 - w0 - w1;
-        const int r0 = AV1_SGR_PARAMS[set][0];
-        const int ue = uu << 4;
-        int acc = w1 * ue + (r0 ? w0 * flt0[k] : w0 * ue) + w2 * v;
-        out[(int64_t)(y0 + ii) * os + x0 + jj] = (T)min(max(round2(acc, 11), 0), maxv);
-      }
-    }
-  }
-  if (AV1_SGR_PARAMS[set][2] == 0) {
-    /* r1 == 0: the projection with flt1 = u */
-    const int w0 = u->sgr_xqd[0], w1 = u->sgr_xqd[1];
-    const int w2 = 128 - w0 - w1;
-#pragma unroll
-    for (int k = 0; k < PX_PER_THREAD; k++) {
-      const int i = threadIdx.x + k * 256;
-      if (i >= h * w) break;
-      const int ii = i / w, jj = i - ii * w;
-      const int ue = win[(ii + 3) * WW + jj + 3] << 4;
-      const int acc = w1 * ue + w0 * flt0[k] + w2 * ue;
-      out[(int64_t)(y0 + ii) * os + x0 + jj] = (T)min(max(round2(acc, 11), 0), maxv);
     }
   }
 }

@@ -13,19 +13,33 @@
 #include <string.h>
 
 #include <new>
+#include <vector>
 
 #include "av1_tables.h"
 #include "rvpf_internal.h"
 
 using namespace rvpf;
 
+/* Film grain scratch is double-buffered: the templates of frame n+1 are
+ * built on the side stream while frame n still filters. Slot s is rebuilt
+ * only after the apply that last read it (ev_done[s]); the apply waits for
+ * its templates (ev_ready[s]). */
+struct ProfMark {
+  int stage;
+  hipEvent_t a, b;
+};
+
 struct rvpf_ctx {
   int device;
+  int profiling;
+  std::vector<ProfMark> marks;      /* recorded, not yet read */
+  std::vector<hipEvent_t> free_evs; /* timing events for reuse */
   hipStream_t side;
-  hipEvent_t ev_fork, ev_join;
+  hipEvent_t ev_ready[2], ev_done[2];
+  int grain_slot;
   void *cdef_buf;
   size_t cdef_bytes;
-  GrainScratch grain;
+  GrainScratch grain[2];
   char err[256];
 };
 
@@ -138,8 +152,11 @@ int validate_params(const rvpf_frame_params *p, char *msg) {
   for (int pl = 0; pl < nplanes_of(p); pl++) {
     if (p->frame_restoration_type[pl] < 0 || p->frame_restoration_type[pl] > 3) BAD("frame_restoration_type");
     if (!lr_plane_on(p, pl)) continue;
+    /* lr_params (src/header.rs:1026-1043): luma 64, 128 or 256; chroma the
+     * same or, with 4:2:0, half of it. A 64-row stripe then lies in one unit row. */
     const int us = p->loop_restoration_size[pl];
-    if (us != 32 && us != 64 && us != 128 && us != 256) BAD("loop_restoration_size %d", us);
+    const bool half_ok = pl > 0 && p->ssx && p->ssy && us == 32;
+    if (us != 64 && us != 128 && us != 256 && !half_ok) BAD("loop_restoration_size[%d] = %d", pl, us);
     int cw, ch;
     crop(p, pl, &cw, &ch);
     if (p->lr_unit_rows[pl] != count_units(us, ch) || p->lr_unit_cols[pl] != count_units(us, cw))
@@ -391,12 +408,17 @@ int ensure_cdef_buf(rvpf_ctx *ctx, size_t bytes) {
   return RVPF_OK;
 }
 
-int ensure_grain_scratch(rvpf_ctx *ctx, int w, int h) {
+int ensure_grain_scratch(rvpf_ctx *ctx, int slot, int w, int h) {
   const int stripes = ((h + 1) / 2 + 15) / 16 + 1;
   const int blocks = ((w + 1) / 2 + 15) / 16 + 1;
-  GrainScratch &g = ctx->grain;
+  GrainScratch &g = ctx->grain[slot];
   if (g.tmpl && stripes <= g.max_stripes && blocks <= g.max_blocks) return RVPF_OK;
-  if (g.offsets) (void)hipFree(g.offsets);
+  if (g.offsets) {
+    /* pending work may still read the old offsets */
+    hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) return hip_fail(ctx, e, "synchronize");
+    (void)hipFree(g.offsets);
+  }
   g.offsets = nullptr;
   const int ms = stripes > g.max_stripes ? stripes : g.max_stripes;
   const int mb = blocks > g.max_blocks ? blocks : g.max_blocks;
@@ -413,29 +435,97 @@ int ensure_grain_scratch(rvpf_ctx *ctx, int w, int h) {
   return RVPF_OK;
 }
 
+/* Profiling: a StageMark records a timing event on `st` when built and
+ * another when it goes out of scope, bracketing the stage's kernels; it
+ * does nothing unless profiling is enabled. */
+hipEvent_t take_event(rvpf_ctx *ctx) {
+  if (!ctx->free_evs.empty()) {
+    hipEvent_t e = ctx->free_evs.back();
+    ctx->free_evs.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  return e;
+}
+
+struct StageMark {
+  rvpf_ctx *ctx;
+  int stage;
+  hipStream_t st;
+  hipEvent_t a;
+  StageMark(rvpf_ctx *c, int s, hipStream_t stream) : ctx(c), stage(s), st(stream), a(nullptr) {
+    if (!ctx->profiling) return;
+    a = take_event(ctx);
+    if (a && hipEventRecord(a, st) != hipSuccess) {
+      (void)hipGetLastError();
+      ctx->free_evs.push_back(a);
+      a = nullptr;
+    }
+  }
+  ~StageMark() {
+    if (!a) return;
+    hipEvent_t b = take_event(ctx);
+    if (b && hipEventRecord(b, st) == hipSuccess) {
+      ctx->marks.push_back({stage, a, b});
+    } else {
+      (void)hipGetLastError();
+      ctx->free_evs.push_back(a);
+      if (b) ctx->free_evs.push_back(b);
+    }
+  }
+};
+
 int check_launch(rvpf_ctx *ctx, int rc, const char *what) {
   if (rc != 0) return hip_fail(ctx, (hipError_t)rc, what);
   return RVPF_OK;
 }
 
-/* display output: crop (+ grain) of `ref` */
+/* Builds the film grain templates of `p` into a scratch slot on stream
+ * `prep` and returns the slot; the apply must wait for ev_ready[slot]. */
+int grain_prepare(rvpf_ctx *ctx, const rvpf_frame_params *p, hipStream_t prep, int *slot_out) {
+  const int slot = ctx->grain_slot;
+  ctx->grain_slot ^= 1;
+  int rc;
+  hipError_t e;
+  if ((rc = ensure_grain_scratch(ctx, slot, p->upscaled_width, p->frame_height))) return rc;
+  if ((e = hipStreamWaitEvent(prep, ctx->ev_done[slot], 0)) != hipSuccess) return hip_fail(ctx, e, "stream wait");
+  {
+    StageMark m(ctx, RVPF_STAGE_GRAIN_PREPARE, prep);
+    if ((rc = check_launch(ctx, launch_grain_prepare(make_grain(p), ctx->grain[slot], prep), "grain prepare")))
+      return rc;
+  }
+  if ((e = hipEventRecord(ctx->ev_ready[slot], prep)) != hipSuccess) return hip_fail(ctx, e, "event record");
+  *slot_out = slot;
+  return RVPF_OK;
+}
+
+/* display output: crop (+ grain with the templates of `slot`) of `ref` */
 int run_output(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_planes *ref, rvpf_planes *disp,
-               hipStream_t st, bool prepared) {
+               hipStream_t st, int slot) {
   const Geom g = make_geom(p);
   PlanePtr in[3], out[3];
   to_ptrs(ref, in);
   to_ptrs(disp, out);
   if (grain_on(p)) {
-    const GrainHdr gh = make_grain(p);
     int rc;
-    if (!prepared) {
-      if ((rc = ensure_grain_scratch(ctx, p->upscaled_width, p->frame_height))) return rc;
-      if ((rc = check_launch(ctx, launch_grain_prepare(gh, ctx->grain, st), "grain prepare"))) return rc;
+    hipError_t e;
+    if ((e = hipStreamWaitEvent(st, ctx->ev_ready[slot], 0)) != hipSuccess) return hip_fail(ctx, e, "stream wait");
+    {
+      StageMark m(ctx, RVPF_STAGE_GRAIN_APPLY, st);
+      if ((rc = check_launch(ctx, launch_grain_apply(make_grain(p), ctx->grain[slot], in, out, ref->bytes_per_sample, st),
+                             "grain apply")))
+        return rc;
     }
-    return check_launch(ctx, launch_grain_apply(gh, ctx->grain, in, out, ref->bytes_per_sample, st), "grain apply");
+    if ((e = hipEventRecord(ctx->ev_done[slot], st)) != hipSuccess) return hip_fail(ctx, e, "event record");
+    return RVPF_OK;
   }
   int w[3] = {0, 0, 0}, h[3] = {0, 0, 0};
   for (int i = 0; i < g.nplanes; i++) crop(p, i, &w[i], &h[i]);
+  StageMark m(ctx, RVPF_STAGE_COPY, st);
   return check_launch(ctx, launch_copy_planes(g, in, out, w, h, ref->bytes_per_sample, st), "display copy");
 }
 
@@ -446,13 +536,17 @@ extern "C" {
 int rvpf_ctx_create(rvpf_ctx **out) {
   if (!out) return fail(nullptr, RVPF_E_INVALID, "out is NULL");
   *out = nullptr;
+  /* value-initialised: every handle and counter starts at zero */
   rvpf_ctx *ctx = new (std::nothrow) rvpf_ctx();
   if (!ctx) return fail(nullptr, RVPF_E_NOMEM, "cannot allocate the context");
-  memset(ctx, 0, sizeof *ctx);
   hipError_t e = hipGetDevice(&ctx->device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking);
-  if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming);
-  if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming);
+  for (int i = 0; i < 2; i++) {
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_ready[i], hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_done[i], hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev_done[i], ctx->side);
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev_ready[i], ctx->side);
+  }
   if (e != hipSuccess) {
     int rc = hip_fail(nullptr, e, "rvpf_ctx_create");
     (void)hipGetLastError();
@@ -467,11 +561,19 @@ void rvpf_ctx_destroy(rvpf_ctx *ctx) {
   if (!ctx) return;
   if (ctx->side) (void)hipStreamSynchronize(ctx->side);
   if (ctx->cdef_buf) (void)hipFree(ctx->cdef_buf);
-  if (ctx->grain.tmpl) (void)hipFree(ctx->grain.tmpl);
-  if (ctx->grain.lut) (void)hipFree(ctx->grain.lut);
-  if (ctx->grain.offsets) (void)hipFree(ctx->grain.offsets);
-  if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
-  if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
+  (void)hipDeviceSynchronize();
+  for (auto &m : ctx->marks) {
+    (void)hipEventDestroy(m.a);
+    (void)hipEventDestroy(m.b);
+  }
+  for (auto e : ctx->free_evs) (void)hipEventDestroy(e);
+  for (int i = 0; i < 2; i++) {
+    if (ctx->grain[i].tmpl) (void)hipFree(ctx->grain[i].tmpl);
+    if (ctx->grain[i].lut) (void)hipFree(ctx->grain[i].lut);
+    if (ctx->grain[i].offsets) (void)hipFree(ctx->grain[i].offsets);
+    if (ctx->ev_ready[i]) (void)hipEventDestroy(ctx->ev_ready[i]);
+    if (ctx->ev_done[i]) (void)hipEventDestroy(ctx->ev_done[i]);
+  }
   if (ctx->side) (void)hipStreamDestroy(ctx->side);
   delete ctx;
 }
@@ -484,6 +586,33 @@ int rvpf_ctx_sync(rvpf_ctx *ctx, void *stream) {
 }
 
 const char *rvpf_ctx_last_error(const rvpf_ctx *ctx) { return ctx ? ctx->err : g_err; }
+
+int rvpf_ctx_set_profiling(rvpf_ctx *ctx, int enable) {
+  if (!ctx) return fail(nullptr, RVPF_E_INVALID, "ctx is NULL");
+  ctx->profiling = enable != 0;
+  return RVPF_OK;
+}
+
+int rvpf_ctx_read_profile(rvpf_ctx *ctx, rvpf_profile *out) {
+  if (!ctx || !out) return fail(ctx, RVPF_E_INVALID, "ctx or out is NULL");
+  memset(out, 0, sizeof *out);
+  int rc = RVPF_OK;
+  for (auto &m : ctx->marks) {
+    float ms = 0;
+    hipError_t e = hipEventSynchronize(m.b);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, m.a, m.b);
+    if (e != hipSuccess) {
+      rc = hip_fail(ctx, e, "profile read");
+    } else if (m.stage >= 0 && m.stage < RVPF_STAGE_COUNT) {
+      out->ms[m.stage] += ms;
+      out->launches[m.stage] += 1;
+    }
+    ctx->free_evs.push_back(m.a);
+    ctx->free_evs.push_back(m.b);
+  }
+  ctx->marks.clear();
+  return rc;
+}
 
 int rvpf_validate(const rvpf_frame_params *p, const rvpf_side_info *s) {
   int rc = validate_params(p, g_err);
@@ -506,6 +635,7 @@ int rvpf_deblock(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_side_info
   side.mi = s->mi;
   side.mi_stride = s->mi_stride;
   for (int i = 0; i < 3; i++) side.tx[i] = s->tx_size[i];
+  StageMark m(ctx, RVPF_STAGE_DEBLOCK, (hipStream_t)stream);
   return check_launch(ctx, launch_deblock(g, make_lf(p), side, pl, frame->bytes_per_sample, (hipStream_t)stream),
                       "deblock");
 }
@@ -527,9 +657,11 @@ int rvpf_cdef(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_side_info *s
   if (!cdef_on(p)) {
     int w[3], h[3];
     for (int i = 0; i < g.nplanes; i++) area(p, i, &w[i], &h[i]);
+    StageMark m(ctx, RVPF_STAGE_COPY, (hipStream_t)stream);
     return check_launch(ctx, launch_copy_planes(g, in, o, w, h, out->bytes_per_sample, (hipStream_t)stream),
                         "cdef copy");
   }
+  StageMark m(ctx, RVPF_STAGE_CDEF, (hipStream_t)stream);
   return check_launch(ctx,
                       launch_cdef(g, make_cdef(p), s->mi, s->mi_stride, s->cdef_idx, s->cdef_stride, 1, in, o,
                                   out->bytes_per_sample, (hipStream_t)stream),
@@ -555,7 +687,11 @@ int rvpf_lr(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_side_info *s, 
   for (int i = 0; i < g.nplanes; i++)
     if (!lr_plane_on(p, i)) area(p, i, &w[i], &h[i]);
   hipStream_t st = (hipStream_t)stream;
-  if ((rc = check_launch(ctx, launch_copy_planes(g, c, o, w, h, out->bytes_per_sample, st), "lr copy"))) return rc;
+  {
+    StageMark m(ctx, RVPF_STAGE_COPY, st);
+    if ((rc = check_launch(ctx, launch_copy_planes(g, c, o, w, h, out->bytes_per_sample, st), "lr copy"))) return rc;
+  }
+  StageMark m(ctx, RVPF_STAGE_LR, st);
   return check_launch(ctx, launch_lr(g, make_lr(p, s), d, c, o, out->bytes_per_sample, st), "lr");
 }
 
@@ -568,7 +704,10 @@ int rvpf_grain(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_planes *ref
     return rc;
   if (ref->bytes_per_sample != display->bytes_per_sample)
     return fail(ctx, RVPF_E_INVALID, "ref and display sample sizes differ");
-  return run_output(ctx, p, ref, display, (hipStream_t)stream, false);
+  hipStream_t st = (hipStream_t)stream;
+  int slot = 0;
+  if (grain_on(p) && (rc = grain_prepare(ctx, p, st, &slot))) return rc;
+  return run_output(ctx, p, ref, display, st, slot);
 }
 
 int rvpf_postfilter_frame(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_side_info *s, rvpf_planes *recon,
@@ -586,16 +725,9 @@ int rvpf_postfilter_frame(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_
   hipStream_t st = (hipStream_t)stream;
   const Geom g = make_geom(p);
   const bool show_grain = display && grain_on(p);
-  hipError_t e;
   /* film grain templates on the side stream, overlapping the filters */
-  if (show_grain) {
-    if ((rc = ensure_grain_scratch(ctx, p->upscaled_width, p->frame_height))) return rc;
-    if ((e = hipEventRecord(ctx->ev_fork, st)) != hipSuccess) return hip_fail(ctx, e, "event record");
-    if ((e = hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0)) != hipSuccess) return hip_fail(ctx, e, "stream wait");
-    if ((rc = check_launch(ctx, launch_grain_prepare(make_grain(p), ctx->grain, ctx->side), "grain prepare")))
-      return rc;
-    if ((e = hipEventRecord(ctx->ev_join, ctx->side)) != hipSuccess) return hip_fail(ctx, e, "event record");
-  }
+  int slot = 0;
+  if (show_grain && (rc = grain_prepare(ctx, p, ctx->side, &slot))) return rc;
   PlanePtr rec[3], ref[3];
   to_ptrs(recon, rec);
   to_ptrs(ref_out, ref);
@@ -605,6 +737,7 @@ int rvpf_postfilter_frame(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_
     side.mi = s->mi;
     side.mi_stride = s->mi_stride;
     for (int i = 0; i < 3; i++) side.tx[i] = s->tx_size[i];
+    StageMark m(ctx, RVPF_STAGE_DEBLOCK, st);
     if ((rc = check_launch(ctx, launch_deblock(g, make_lf(p), side, rec, bps, st), "deblock"))) return rc;
   }
   /* 2. CDEF: planes restored next go to scratch, the others straight to ref_out */
@@ -634,6 +767,7 @@ int rvpf_postfilter_frame(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_
     } else {
       for (int i = 0; i < 3; i++) cdef[i] = ref[i];
     }
+    StageMark m(ctx, RVPF_STAGE_CDEF, st);
     if ((rc = check_launch(ctx,
                            launch_cdef(g, make_cdef(p), s->mi, s->mi_stride, s->cdef_idx, s->cdef_stride, 1, rec,
                                        cdef, bps, st),
@@ -646,14 +780,17 @@ int rvpf_postfilter_frame(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_
       cdef[i] = rec[i];
       if (!lr_plane_on(p, i)) area(p, i, &w[i], &h[i]);
     }
+    StageMark m(ctx, RVPF_STAGE_COPY, st);
     if ((rc = check_launch(ctx, launch_copy_planes(g, rec, ref, w, h, bps, st), "copy"))) return rc;
   }
   /* 3. loop restoration of the restored planes into ref_out */
-  if (any_lr && (rc = check_launch(ctx, launch_lr(g, make_lr(p, s), rec, cdef, ref, bps, st), "lr"))) return rc;
+  if (any_lr) {
+    StageMark m(ctx, RVPF_STAGE_LR, st);
+    if ((rc = check_launch(ctx, launch_lr(g, make_lr(p, s), rec, cdef, ref, bps, st), "lr"))) return rc;
+  }
   /* 4. display */
   if (!display) return RVPF_OK;
-  if (show_grain && (e = hipStreamWaitEvent(st, ctx->ev_join, 0)) != hipSuccess) return hip_fail(ctx, e, "stream wait");
-  return run_output(ctx, p, ref_out, display, st, true);
+  return run_output(ctx, p, ref_out, display, st, slot);
 }
 
 }  // extern "C"

@@ -65,7 +65,7 @@ int synth_geometry(const synth_cfg *c, synth_geom *g) {
   g->cdef_rows = (g->mi_rows >> 4) + 2;
   g->num_planes = c->mono ? 1 : 3;
   const int aw = (g->mi_cols * 4 + 127) & ~127, ah = (g->mi_rows * 4 + 127) & ~127;
-  const int unit = c->lr_unit_size ? c->lr_unit_size : 64;
+  const int unit = c->lr_unit_size >= 64 ? c->lr_unit_size : 64; /* luma units: 64, 128, 256 */
   for (int p = 0; p < g->num_planes; p++) {
     const int sx = p ? c->ssx : 0, sy = p ? c->ssy : 0;
     g->plane_w[p] = (aw >> sx) + 32;
@@ -369,7 +369,7 @@ int synth_frame(const synth_cfg *c, rvpf_frame_params *p, rvpf_mi *mi, uint8_t *
     P.cdef_uv_pri_strength[i] = rng_int(&hr, 16);
     P.cdef_uv_sec_strength[i] = sec[rng_int(&hr, 4)];
   }
-  const int unit = c->lr_unit_size ? c->lr_unit_size : 64;
+  const int unit = c->lr_unit_size >= 64 ? c->lr_unit_size : 64; /* luma units: 64, 128, 256 */
   for (int pl = 0; pl < np; pl++) {
     P.frame_restoration_type[pl] = 0;
     if (c->lr == 1 && pl == 0) P.frame_restoration_type[pl] = RVPF_RESTORE_WIENER;

@@ -36,7 +36,7 @@ typedef struct synth_cfg {
   int32_t cdef;          /* CDEF on */
   int32_t cdef_bits;     /* 0..3 */
   int32_t lr;            /* 0 off, 1 Wiener on luma, 2 Wiener/SGR/none mixed on all planes */
-  int32_t lr_unit_size;  /* luma unit size 32..256 (0 = 64) */
+  int32_t lr_unit_size;  /* luma unit size 64, 128 or 256 (else 64); 4:2:0 chroma halves it */
   int32_t grain;         /* film grain on */
   int32_t grain_lag;     /* ar_coeff_lag */
   int32_t grain_y_points;

@@ -176,6 +176,13 @@ class SynthGeom(C.Structure):
     ]
 
 
+STAGES = ["deblock", "cdef", "lr", "grain_prepare", "grain_apply", "copy"]
+
+
+class RvpfProfile(C.Structure):
+    _fields_ = [("ms", C.c_float * 6), ("launches", C.c_int32 * 6)]
+
+
 STRUCT_SIZES = [
     C.sizeof(RvpfFrameParams),
     C.sizeof(RvpfSideInfo),
@@ -196,6 +203,8 @@ RVPF_EXPORTS = [
     "rvpf_ctx_destroy",
     "rvpf_ctx_sync",
     "rvpf_ctx_last_error",
+    "rvpf_ctx_set_profiling",
+    "rvpf_ctx_read_profile",
     "rvpf_validate",
     "rvpf_deblock",
     "rvpf_cdef",

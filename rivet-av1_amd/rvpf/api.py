@@ -35,6 +35,8 @@ def lib():
         L.rvpf_ctx_sync.argtypes = [C.c_void_p, C.c_void_p]
         L.rvpf_ctx_last_error.argtypes = [C.c_void_p]
         L.rvpf_ctx_last_error.restype = C.c_char_p
+        L.rvpf_ctx_set_profiling.argtypes = [C.c_void_p, C.c_int]
+        L.rvpf_ctx_read_profile.argtypes = [C.c_void_p, P(abi.RvpfProfile)]
         L.rvpf_validate.argtypes = [P(abi.RvpfFrameParams), P(abi.RvpfSideInfo)]
         L.rvpf_deblock.argtypes = [C.c_void_p, P(abi.RvpfFrameParams), P(abi.RvpfSideInfo), P(abi.RvpfPlanes), C.c_void_p]
         L.rvpf_cdef.argtypes = [C.c_void_p, P(abi.RvpfFrameParams), P(abi.RvpfSideInfo), P(abi.RvpfPlanes),
@@ -92,6 +94,15 @@ class Context:
     def check(self, rc):
         if rc != 0:
             raise RvpfError(rc, lib().rvpf_ctx_last_error(self.h).decode())
+
+    def set_profiling(self, enable):
+        self.check(lib().rvpf_ctx_set_profiling(self.h, int(bool(enable))))
+
+    def read_profile(self):
+        """{stage: (total ms, launches)} since the last read."""
+        pr = abi.RvpfProfile()
+        self.check(lib().rvpf_ctx_read_profile(self.h, C.byref(pr)))
+        return {name: (pr.ms[i], pr.launches[i]) for i, name in enumerate(abi.STAGES)}
 
     def sync(self, stream=None):
         self.check(lib().rvpf_ctx_sync(self.h, _stream_handle(stream)))

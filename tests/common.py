@@ -13,7 +13,7 @@ SMALL_CASES = [
     ("200x120_12b_420_all", dict(width=200, height=120, bit_depth=12, qindex=200, lr=2, grain=True,
                                  lf_extras=True, seed=5, sb_size=128)),
     ("170x90_10b_444_all", dict(width=170, height=90, bit_depth=10, ssx=0, ssy=0, qindex=64, lr=2,
-                                grain=True, seed=6, lr_unit_size=32)),
+                                grain=True, seed=6, lr_unit_size=64)),
     ("150x100_8b_422_all", dict(width=150, height=100, bit_depth=8, ssx=1, ssy=0, qindex=180, lr=2,
                                 grain=True, seed=7, lf_extras=True)),
     ("130x70_12b_mono_all", dict(width=130, height=70, bit_depth=12, mono=True, qindex=120, lr=2,

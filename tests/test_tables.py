@@ -1,0 +1,37 @@
+"""The spec tables in av1_tables.h equal the reference's src/tables.rs
+(hashes recorded by tests/golden/make_table_hashes.py)."""
+import json
+import os
+import re
+
+from golden.make_table_hashes import NAMES, values_hash
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+HDR = os.path.join(os.path.dirname(HERE), "rivet-av1_amd", "csrc", "av1_tables.h")
+
+
+def header_tables():
+    src = open(HDR).read()
+    out = {}
+    for m in re.finditer(r"AV1_TABLE \w+ AV1_(\w+)((?:\[\d+\])+) = (\{.*?\});\n", src, re.S):
+        vals = [int(v) for v in re.findall(r"-?\d+", m.group(3))]
+        out[m.group(1)] = vals
+    return out
+
+
+def test_tables_match_reference():
+    want = json.load(open(os.path.join(HERE, "golden", "table_hashes.json")))
+    got = header_tables()
+    for n in NAMES:
+        assert n in got, n
+        assert len(got[n]) == want[n]["count"], n
+        assert values_hash(got[n]) == want[n]["sha256"], n
+
+
+def test_derived_tables():
+    t = header_tables()
+    # Div_Table is 840 / k (7.15.2)
+    assert t["DIV_TABLE"] == [0] + [840 // k for k in range(1, 9)]
+    # Gaussian sequence: 2048 entries, multiples of 4 within 12 bits
+    g = t["GAUSSIAN_SEQUENCE"]
+    assert len(g) == 2048 and all(v % 4 == 0 and -2048 <= v < 2048 for v in g)
