@@ -80,6 +80,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-stages", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="first run a plain device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the post-filter chain shards by stream/frame with no exchange; "
@@ -92,6 +94,15 @@ def main():
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if args.calib_copy_mib:
+        # a plain copy of known size: reads and writes exactly n bytes each
+        n = args.calib_copy_mib << 20
+        a_ = torch.ones(n // 2, dtype=torch.int16, device=dev)
+        b_ = torch.empty_like(a_)
+        torch.cuda.synchronize()
+        b_.copy_(a_)
+        torch.cuda.synchronize()
+        del a_, b_
     log("generating %d frames %dx%d %d-bit" % (args.frames, args.width, args.height, args.bit_depth))
     frames = make_frames(args.frames, args.width, args.height, args.bit_depth, args.seed)
     ctx = api.Context()

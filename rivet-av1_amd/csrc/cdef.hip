@@ -9,12 +9,13 @@
  * cdef_idx):
  *   1. the luma tile plus a 2-sample border is staged in LDS; samples at or
  *      beyond the mode-info edge are the sentinel NA_S (unavailable);
- *   2. four lanes per 8x8 block run the direction search, each through one
- *      symmetry of the block (no divergence), and one shuffle gathers the
- *      eight costs;
- *   3. lanes filter one column each, every fourth row, from LDS (coalesced
- *      row stores); then each chroma plane is staged and filtered the same
- *      way.
+ *   2. the four lanes of an 8x8 block run its direction search, each
+ *      through one symmetry of the block (no divergence), and one shuffle
+ *      gathers the eight costs;
+ *   3. the same four lanes filter that block (two luma columns each), so
+ *      taps and strengths are set up once per lane; a wave's 16 blocks
+ *      span two block rows, and its stores cover whole 64-sample rows.
+ *      Each chroma plane is then staged and filtered the same way.
  * Unfiltered blocks (cdef_idx -1, all four 4x4 skipped, CDEF off) are
  * copied, so `out` receives the whole mode-info area.
  *
@@ -34,7 +35,8 @@ namespace rvpf {
 namespace {
 
 constexpr int NA_S = 0x7000;
-constexpr int TS = 68; /* luma tile stride: 64 + 2 * 2 */
+constexpr int TS = 68;                /* tile stride in samples: 64 + 2 * 2 */
+constexpr int SOFF = 68 * (TS / 2);   /* word offset of the shifted tile S */
 
 struct CdefLaunch {
   Geom g;
@@ -53,18 +55,15 @@ __device__ __forceinline__ T *pp(const PlanePtr &p) {
   return reinterpret_cast<T *>(p.data);
 }
 
-/* constrain() of 7.15.3: sign(d) * min(|d|, max(0, t - (|d| >> adj))),
- * i.e. d clamped to [-m, m] with m = max(0, t - (|d| >> adj)). */
-__device__ __forceinline__ int constrain(int diff, int threshold, int adj) {
-  const int m = max(threshold - (abs(diff) >> adj), 0);
-  return min(max(diff, -m), m);
-}
 
-/* Stages rows [y0-2, y0+h+2) x cols [x0-2, x0+w+2) of a plane into t
- * (stride TS); outside [0, xlim) x [0, ylim) -> NA_S. w <= 64. */
+/* Stages rows [y0-2, y0+h+2) x cols [x0-2, x0+w+2) of a plane into the
+ * u16 tile P (stride TS); outside [0, xlim) x [0, ylim) -> NA_S. The same
+ * samples shifted left by one go to S (S[r][c] = P[r][c + 1]), so that any
+ * horizontal pair (c + dx, c + dx + 1) with c even is one aligned 32-bit
+ * word of P (dx even) or of S (dx odd). w <= 64. */
 template <typename T>
-__device__ void stage_tile(uint16_t *t, const T *src, int64_t stride, int x0, int y0, int w, int h,
-                           int xlim, int ylim) {
+__device__ void stage_tile(uint16_t *t, uint16_t *sh, const T *src, int64_t stride, int x0, int y0, int w,
+                           int h, int xlim, int ylim) {
   const int tw = w + 4, th = h + 4;
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
   for (int r = ty; r < th; r += 4) {
@@ -73,79 +72,120 @@ __device__ void stage_tile(uint16_t *t, const T *src, int64_t stride, int x0, in
     const T *row = src + (int64_t)y * stride;
     for (int c = tx; c < tw; c += 64) {
       const int x = x0 + c - 2;
-      t[r * TS + c] = (uint16_t)((yok && x >= 0 && x < xlim) ? (int)row[x] : NA_S);
+      const uint16_t v = (uint16_t)((yok && x >= 0 && x < xlim) ? (int)row[x] : NA_S);
+      t[r * TS + c] = v;
+      if (c > 0) sh[r * TS + c - 1] = v;
     }
+    if (tx == 0) sh[r * TS + tw - 1] = (uint16_t)NA_S;
   }
 }
 
 /* Product of two values known to fit in 16 bits (one 24-bit multiply). */
 __device__ __forceinline__ int mul16(int a, int b) { return (int)(int16_t)a * (int)(int16_t)b; }
 
-/* One filtered sample (dsp::cdef::filter_scalar, src/dsp/cdef.rs:61-88). */
-struct Taps {
-  int pri, sec, pri_adj, sec_adj, pt0, pt1;
-  int po0, po1, so00, so01, so10, so11; /* tile offsets */
-  bool clamp;
-};
+/* Two horizontally adjacent samples filtered at once in packed 16-bit
+ * arithmetic (dsp::cdef::filter_scalar, src/dsp/cdef.rs:61-88). Every
+ * quantity fits in int16: samples <= 4095, the sentinel 0x7000, their
+ * differences, constrain() results <= 240 and |sum| <= 24 * 240. */
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 
-__device__ __forceinline__ int filter_px(const uint16_t *t, int ci, const Taps &k) {
-  const int x = t[ci];
-  int sum = 0, mx = x, mn = x;
-  if (k.pri) {
-    const int a = t[ci + k.po0], b = t[ci - k.po0], c = t[ci + k.po1], d = t[ci - k.po1];
-    sum += mul16(k.pt0, constrain(a - x, k.pri, k.pri_adj) + constrain(b - x, k.pri, k.pri_adj));
-    sum += mul16(k.pt1, constrain(c - x, k.pri, k.pri_adj) + constrain(d - x, k.pri, k.pri_adj));
-    if (k.clamp) {
-      mn = min(min(mn, a), min(b, min(c, d)));
-      mx = max(max(mx, a & 0xfff), max(b & 0xfff, max(c & 0xfff, d & 0xfff)));
-    }
-  }
-  if (k.sec) {
-    const int a = t[ci + k.so00], b = t[ci - k.so00], c = t[ci + k.so01], d = t[ci - k.so01];
-    const int e = t[ci + k.so10], f = t[ci - k.so10], gg = t[ci + k.so11], hh = t[ci - k.so11];
-    /* Cdef_Sec_Taps is {2, 1} in both rows */
-    sum += 2 * (constrain(a - x, k.sec, k.sec_adj) + constrain(b - x, k.sec, k.sec_adj) +
-                constrain(c - x, k.sec, k.sec_adj) + constrain(d - x, k.sec, k.sec_adj));
-    sum += constrain(e - x, k.sec, k.sec_adj) + constrain(f - x, k.sec, k.sec_adj) +
-           constrain(gg - x, k.sec, k.sec_adj) + constrain(hh - x, k.sec, k.sec_adj);
-    if (k.clamp) {
-      mn = min(min(min(mn, a), min(b, c)), min(min(d, e), min(f, min(gg, hh))));
-      mx = max(max(max(mx, a & 0xfff), max(b & 0xfff, c & 0xfff)),
-               max(max(d & 0xfff, e & 0xfff), max(f & 0xfff, max(gg & 0xfff, hh & 0xfff))));
-    }
-  }
-  int v = x + ((8 + sum - (sum < 0)) >> 4);
-  if (k.clamp) v = min(max(v, mn), mx);
-  return v;
+__device__ __forceinline__ s16x2 as_s2(uint32_t w) { return __builtin_bit_cast(s16x2, w); }
+__device__ __forceinline__ s16x2 splat(int v) { return (s16x2){(short)v, (short)v}; }
+
+/* constrain() of 7.15.3: sign(d) * min(|d|, max(0, t - (|d| >> adj))),
+ * i.e. d clamped to [-m, m] with m = max(0, t - (|d| >> adj)). */
+__device__ __forceinline__ s16x2 constrain2(s16x2 d, s16x2 t, u16x2 adj) {
+  const u16x2 ad = __builtin_bit_cast(u16x2, __builtin_elementwise_abs(d));
+  const s16x2 m = __builtin_elementwise_max(t - __builtin_bit_cast(s16x2, ad >> adj), splat(0));
+  return __builtin_elementwise_min(__builtin_elementwise_max(d, -m), m);
 }
+
+/* Per-lane filter set-up: strengths, damping shifts, primary tap weights
+ * and the 32-bit word offsets of the 12 taps relative to the pair's word. */
+struct Taps {
+  s16x2 pri, sec;
+  u16x2 pri_adj, sec_adj;
+  s16x2 pt0, pt1;
+  int po[4];  /* +d0, -d0, +d1, -d1 */
+  int so[8];  /* (lo, hi) x (k0, k1) x (+, -) */
+  bool has_pri, has_sec, clamp;
+};
 
 __device__ __forceinline__ int floor_log2(int x) { return 31 - __clz(x); }
 
-__device__ __forceinline__ int tap_off(int d, int k) {
-  return AV1_CDEF_DIRECTIONS[d][k][0] * TS + AV1_CDEF_DIRECTIONS[d][k][1];
+/* word offset of tap (dy, dx) for a pair at an even column */
+__device__ __forceinline__ int word_off(int dy, int dx) {
+  return (dx & 1) ? SOFF + dy * (TS / 2) + ((dx - 1) >> 1) : dy * (TS / 2) + (dx >> 1);
 }
 
 __device__ __forceinline__ Taps make_taps(int pri, int sec, int damping, int dir, int coeff_shift) {
   Taps k;
-  k.pri = pri;
-  k.sec = sec;
-  k.pri_adj = pri ? max(damping - floor_log2(pri), 0) : 0;
-  k.sec_adj = sec ? max(damping - floor_log2(sec), 0) : 0;
+  k.has_pri = pri != 0;
+  k.has_sec = sec != 0;
+  k.clamp = k.has_pri && k.has_sec;
+  k.pri = splat(pri);
+  k.sec = splat(sec);
+  const int pa = pri ? max(damping - floor_log2(pri), 0) : 0;
+  const int sa = sec ? max(damping - floor_log2(sec), 0) : 0;
+  k.pri_adj = (u16x2){(unsigned short)pa, (unsigned short)pa};
+  k.sec_adj = (u16x2){(unsigned short)sa, (unsigned short)sa};
   const int sel = (pri >> coeff_shift) & 1;
-  k.pt0 = AV1_CDEF_PRI_TAPS[sel][0];
-  k.pt1 = AV1_CDEF_PRI_TAPS[sel][1];
-  static_assert(AV1_CDEF_SEC_TAPS[0][0] == 2 && AV1_CDEF_SEC_TAPS[0][1] == 1 && AV1_CDEF_SEC_TAPS[1][0] == 2 &&
-                    AV1_CDEF_SEC_TAPS[1][1] == 1,
-                "secondary taps are 2, 1");
-  k.po0 = tap_off(dir, 0);
-  k.po1 = tap_off(dir, 1);
+  k.pt0 = splat(AV1_CDEF_PRI_TAPS[sel][0]);
+  k.pt1 = splat(AV1_CDEF_PRI_TAPS[sel][1]);
   const int dlo = (dir + 6) & 7, dhi = (dir + 2) & 7;
-  k.so00 = tap_off(dlo, 0);
-  k.so01 = tap_off(dhi, 0);
-  k.so10 = tap_off(dlo, 1);
-  k.so11 = tap_off(dhi, 1);
-  k.clamp = pri != 0 && sec != 0;
+#pragma unroll
+  for (int t = 0; t < 2; t++) {
+    const int dy = AV1_CDEF_DIRECTIONS[dir][t][0], dx = AV1_CDEF_DIRECTIONS[dir][t][1];
+    k.po[2 * t] = word_off(dy, dx);
+    k.po[2 * t + 1] = word_off(-dy, -dx);
+    const int ly = AV1_CDEF_DIRECTIONS[dlo][t][0], lx = AV1_CDEF_DIRECTIONS[dlo][t][1];
+    const int hy = AV1_CDEF_DIRECTIONS[dhi][t][0], hx = AV1_CDEF_DIRECTIONS[dhi][t][1];
+    k.so[4 * t + 0] = word_off(ly, lx);
+    k.so[4 * t + 1] = word_off(-ly, -lx);
+    k.so[4 * t + 2] = word_off(hy, hx);
+    k.so[4 * t + 3] = word_off(-hy, -hx);
+  }
   return k;
+}
+
+/* The filtered pair whose own word is w[base]. */
+__device__ __forceinline__ uint32_t filter_pair(const uint32_t *w, int base, const Taps &k) {
+  const s16x2 x = as_s2(w[base]);
+  s16x2 sum = splat(0), mn = x, mx = x;
+  if (k.has_pri) {
+    s16x2 s0 = splat(0), s1 = splat(0);
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      const uint32_t pw = w[base + k.po[t]];
+      const s16x2 c = constrain2(as_s2(pw) - x, k.pri, k.pri_adj);
+      if (t < 2) s0 += c; else s1 += c;
+      if (k.clamp) {
+        mn = __builtin_elementwise_min(mn, as_s2(pw));
+        mx = __builtin_elementwise_max(mx, as_s2(pw & 0x0fff0fffu));
+      }
+    }
+    sum = k.pt0 * s0 + k.pt1 * s1;
+  }
+  if (k.has_sec) {
+    /* Cdef_Sec_Taps is {2, 1} in both rows */
+    s16x2 s0 = splat(0), s1 = splat(0);
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+      const uint32_t pw = w[base + k.so[t]];
+      const s16x2 c = constrain2(as_s2(pw) - x, k.sec, k.sec_adj);
+      if (t < 4) s0 += c; else s1 += c;
+      if (k.clamp) {
+        mn = __builtin_elementwise_min(mn, as_s2(pw));
+        mx = __builtin_elementwise_max(mx, as_s2(pw & 0x0fff0fffu));
+      }
+    }
+    sum += s0 + s0 + s1;
+  }
+  /* x + ((8 + sum - (sum < 0)) >> 4) */
+  s16x2 v = x + ((sum + splat(8) + (sum >> splat(15))) >> splat(4));
+  if (k.clamp) v = __builtin_elementwise_min(__builtin_elementwise_max(v, mn), mx);
+  return __builtin_bit_cast(uint32_t, v);
 }
 
 /* Three of the direction costs of 7.15.2 (postfilter.rs:345-390) for an
@@ -196,15 +236,34 @@ __device__ void dir_costs3(const uint16_t *t, int b0, int rs, int cs, int sh, in
   *c2 = __mul24(c, AV1_DIV_TABLE[8]);
 }
 
+/* Filters this lane's pairs of one block: rows r0 .. r0 + nrows - 1, the
+ * column pair starting at block column c (even); t0 = the block's top-left
+ * tile sample (even), o = the block's top-left output sample. */
+template <typename T>
+__device__ __forceinline__ void filter_block_pairs(const uint32_t *w, int t0, T *o, int64_t os, int c, int r0,
+                                                   int nrows, const Taps &k, bool on) {
+  o += (int64_t)r0 * os + c;
+  for (int r = r0; r < r0 + nrows; r++, o += os) {
+    const int base = ((t0 + r * TS + c) >> 1);
+    const uint32_t v = on ? filter_pair(w, base, k) : w[base];
+    if (sizeof(T) == 2) {
+      *reinterpret_cast<uint32_t *>(o) = v;
+    } else {
+      *reinterpret_cast<uint16_t *>(o) = (uint16_t)((v & 0xff) | ((v >> 8) & 0xff00));
+    }
+  }
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void cdef_kernel(CdefLaunch a) {
-  __shared__ uint16_t tile[TS * TS];
-  __shared__ int8_t blk_dir[64];  /* luma direction (yDir); -1: block not filtered */
-  __shared__ int16_t blk_pri[64]; /* variance-adjusted luma primary strength */
+  __shared__ uint32_t words[2 * SOFF]; /* tile P, then the shifted tile S */
+  uint16_t *tile = reinterpret_cast<uint16_t *>(words);
+  uint16_t *shifted = reinterpret_cast<uint16_t *>(words + SOFF);
   const Geom &g = a.g;
   const int tid = threadIdx.x;
   const int tx = tid & 63, ty = tid >> 6;
-  const int fbx = blockIdx.x % a.fb_cols, fby = blockIdx.x / a.fb_cols;
+  const int tile_id = xcd_remap(blockIdx.x, gridDim.x);
+  const int fbx = tile_id % a.fb_cols, fby = tile_id / a.fb_cols;
   const int cs = g.bd - 8;
   int idx = -1;
   if (a.enabled) idx = a.cdef_idx[(int64_t)fby * a.cdef_stride + fbx];
@@ -218,127 +277,92 @@ __global__ __launch_bounds__(256) void cdef_kernel(CdefLaunch a) {
     for (int p = 0; p < g.nplanes; p++) {
       const int ssx = p ? g.ssx : 0, ssy = p ? g.ssy : 0;
       const int w = bw >> ssx, h = bh >> ssy, px = x0 >> ssx, py = y0 >> ssy;
-      const T *src = pp<T>(a.in[p]);
-      T *dst = pp<T>(a.out[p]);
-      if (tx < w)
-        for (int r = ty; r < h; r += 4)
-          dst[(int64_t)(py + r) * a.out[p].stride + px + tx] = src[(int64_t)(py + r) * a.in[p].stride + px + tx];
+      if (tx < w) {
+        const T *src = pp<T>(a.in[p]) + (int64_t)(py + ty) * a.in[p].stride + px + tx;
+        T *dst = pp<T>(a.out[p]) + (int64_t)(py + ty) * a.out[p].stride + px + tx;
+        for (int r = ty; r < h; r += 4, src += 4 * a.in[p].stride, dst += 4 * a.out[p].stride) *dst = *src;
+      }
     }
     return;
   }
 
-  stage_tile<T>(tile, pp<T>(a.in[0]), a.in[0].stride, x0, y0, 64, 64, xlim, ylim);
+  stage_tile<T>(tile, shifted, pp<T>(a.in[0]), a.in[0].stride, x0, y0, 64, 64, xlim, ylim);
   __syncthreads();
 
-  /* cdef_block (postfilter.rs:304-328): 4 lanes per 8x8 block, each reads
-   * the block through one symmetry and yields three direction costs */
-  {
-    const int b = tid >> 2, q = tid & 3;
-    const int by = b >> 3, bx = b & 7;
-    const bool inside = by * 8 < bh && bx * 8 < bw;
-    bool skip = true;
-    int32_t c0 = 0, c1 = 0, c2 = 0;
-    if (inside) {
-      const int r = (y0 >> 2) + by * 2, c = (x0 >> 2) + bx * 2;
-      const int ms = a.mi_stride;
-      skip = a.mi[(int64_t)r * ms + c].seg_skip & a.mi[(int64_t)(r + 1) * ms + c].seg_skip &
-             a.mi[(int64_t)r * ms + c + 1].seg_skip & a.mi[(int64_t)(r + 1) * ms + c + 1].seg_skip & 1;
-      if (!skip) {
-        const int base = (by * 8 + 2) * TS + bx * 8 + 2;
-        /* q: 0 identity, 1 horizontal flip, 2 transpose, 3 transpose + flip */
-        const int rs = q == 0 || q == 1 ? TS : (q == 2 ? 1 : -1);
-        const int csx = q == 0 ? 1 : (q == 1 ? -1 : TS);
-        const int b0 = base + (q == 1 ? 7 : 0) + (q == 3 ? 7 : 0);
-        dir_costs3(tile, b0, rs, csx, cs, &c0, &c1, &c2);
-      }
-    }
-    /* lane q of the block: q0 -> costs 0,1,2; q1 -> 4,3,2; q2 -> 0,7,6; q3 -> 4,5,6 */
-    const int lb = (tid & 63) & ~3;
-    const int32_t k0 = __shfl(c0, lb + 0, 64), k1 = __shfl(c1, lb + 0, 64), k2 = __shfl(c2, lb + 0, 64);
-    const int32_t k4 = __shfl(c0, lb + 1, 64), k3 = __shfl(c1, lb + 1, 64);
-    const int32_t k7 = __shfl(c1, lb + 2, 64), k6 = __shfl(c2, lb + 2, 64);
-    const int32_t k5 = __shfl(c1, lb + 3, 64);
-    if (q == 0) {
-      int dir = -1, pri = 0;
-      if (inside && !skip) {
-        const int32_t cost[8] = {k0, k1, k2, k3, k4, k5, k6, k7};
-        int32_t best = 0;
-        dir = 0;
-#pragma unroll
-        for (int k = 0; k < 8; k++)
-          if (cost[k] > best) {
-            best = cost[k];
-            dir = k;
-          }
-        const int32_t var = (best - cost[(dir + 4) & 7]) >> 10;
-        pri = a.h.y_pri[idx] << cs;
-        const int var_str = (var >> 6) != 0 ? min(floor_log2(var >> 6), 12) : 0;
-        pri = var != 0 ? (pri * (4 + var_str) + 8) >> 4 : 0;
-      }
-      blk_dir[b] = (int8_t)dir;
-      blk_pri[b] = (int16_t)pri;
-    }
+  /* cdef_block (postfilter.rs:304-341): the 4 lanes of block b = tid / 4
+   * run its direction search, each through one symmetry of the block, and
+   * then filter its samples (lane q: columns 2q, 2q + 1 of luma). */
+  const int b = tid >> 2, q = tid & 3;
+  const int by = b >> 3, bx = b & 7;
+  const bool inside = by * 8 < bh && bx * 8 < bw;
+  bool skip = true;
+  if (inside) {
+    const int r = (y0 >> 2) + by * 2, c = (x0 >> 2) + bx * 2;
+    const int ms = a.mi_stride;
+    skip = a.mi[(int64_t)r * ms + c].seg_skip & a.mi[(int64_t)(r + 1) * ms + c].seg_skip &
+           a.mi[(int64_t)r * ms + c + 1].seg_skip & a.mi[(int64_t)(r + 1) * ms + c + 1].seg_skip & 1;
   }
-  __syncthreads();
-
-  /* luma: thread = column tx, rows ty, ty + 4, ... */
-  {
-    const int sec = a.h.y_sec[idx] << cs;
-    const int damping = a.h.damping + cs;
-    const bool use_dir = a.h.y_pri[idx] != 0;
-    T *dst = pp<T>(a.out[0]) + (int64_t)(y0 + ty) * a.out[0].stride + x0 + tx;
-    const int64_t dstep = 4 * a.out[0].stride;
-    if (tx < bw) {
-#pragma unroll 2
-      for (int r = ty; r < bh; r += 4, dst += dstep) {
-        const int b = (r >> 3) * 8 + (tx >> 3);
-        const int d = blk_dir[b];
-        const int ci = (r + 2) * TS + tx + 2;
-        int v = tile[ci];
-        if (d >= 0) {
-          const int pri = blk_pri[b];
-          if (pri | sec) v = filter_px(tile, ci, make_taps(pri, sec, damping, use_dir ? d : 0, cs));
-        }
-        *dst = (T)v;
+  const bool filt = inside && !skip;
+  int32_t c0 = 0, c1 = 0, c2 = 0;
+  if (filt) {
+    const int base = (by * 8 + 2) * TS + bx * 8 + 2;
+    /* q: 0 identity, 1 horizontal flip, 2 transpose, 3 transpose + flip */
+    const int rs = q == 0 || q == 1 ? TS : (q == 2 ? 1 : -1);
+    const int csx = q == 0 ? 1 : (q == 1 ? -1 : TS);
+    const int b0 = base + (q == 1 || q == 3 ? 7 : 0);
+    dir_costs3(tile, b0, rs, csx, cs, &c0, &c1, &c2);
+  }
+  /* lane q of the block: q0 -> costs 0,1,2; q1 -> 4,3,2; q2 -> 0,7,6; q3 -> 4,5,6 */
+  const int lb = (tid & 63) & ~3;
+  const int32_t k0 = __shfl(c0, lb + 0, 64), k1 = __shfl(c1, lb + 0, 64), k2 = __shfl(c2, lb + 0, 64);
+  const int32_t k4 = __shfl(c0, lb + 1, 64), k3 = __shfl(c1, lb + 1, 64);
+  const int32_t k7 = __shfl(c1, lb + 2, 64), k6 = __shfl(c2, lb + 2, 64);
+  const int32_t k5 = __shfl(c1, lb + 3, 64);
+  int ydir = 0, pri = 0;
+  if (filt) {
+    const int32_t cost[8] = {k0, k1, k2, k3, k4, k5, k6, k7};
+    int32_t best = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+      if (cost[k] > best) {
+        best = cost[k];
+        ydir = k;
       }
-    }
+    const int32_t var = (best - cost[(ydir + 4) & 7]) >> 10;
+    pri = a.h.y_pri[idx] << cs;
+    const int var_str = (var >> 6) != 0 ? min(floor_log2(var >> 6), 12) : 0;
+    pri = var != 0 ? (pri * (4 + var_str) + 8) >> 4 : 0;
+  }
+
+  /* luma: lane q filters the column pair 2q, 2q + 1 of its block */
+  if (inside) {
+    const int sec = a.h.y_sec[idx] << cs;
+    const Taps k = make_taps(pri, sec, a.h.damping + cs, a.h.y_pri[idx] ? ydir : 0, cs);
+    T *o = pp<T>(a.out[0]) + (int64_t)(y0 + by * 8) * a.out[0].stride + x0 + bx * 8;
+    filter_block_pairs<T>(words, (by * 8 + 2) * TS + bx * 8 + 2, o, a.out[0].stride, 2 * q, 0, 8, k,
+                          filt && (pri | sec));
   }
   if (g.nplanes == 1) return;
 
-  /* chroma planes (postfilter.rs:332-341) */
+  /* chroma planes: block of (8 >> ssx) x (8 >> ssy) = ncp column pairs;
+   * lane q takes pair q % ncp and its share of the rows */
   const int ssx = g.ssx, ssy = g.ssy;
   const int cw = 64 >> ssx, ch = 64 >> ssy;
-  const int pri = a.h.uv_pri[idx] << cs, sec = a.h.uv_sec[idx] << cs;
-  const int damping = a.h.damping + cs - 1;
-  const int w = bw >> ssx, h = bh >> ssy;
-  /* thread -> (row, column) of the chroma block: cw columns per row */
-  const int lgw = 6 - ssx;
-  const int cx = tid & (cw - 1), cy0 = tid >> lgw, cny = 256 >> lgw;
+  const int upri = a.h.uv_pri[idx] << cs, usec = a.h.uv_sec[idx] << cs;
+  const int bws = 8 >> ssx, bhs = 8 >> ssy;
+  const int ncp = bws >> 1;               /* 4 (4:4:4) or 2 */
+  const int nrows = (bhs * ncp) >> 2;     /* rows per lane */
+  const int cpair = q % ncp, r0 = (q / ncp) * nrows;
+  const Taps k = make_taps(upri, usec, a.h.damping + cs - 1, upri ? AV1_CDEF_UV_DIR[ssx][ssy][ydir] : 0, cs);
   for (int p = 1; p < 3; p++) {
     __syncthreads();
-    stage_tile<T>(tile, pp<T>(a.in[p]), a.in[p].stride, x0 >> ssx, y0 >> ssy, cw, ch, xlim >> ssx,
+    stage_tile<T>(tile, shifted, pp<T>(a.in[p]), a.in[p].stride, x0 >> ssx, y0 >> ssy, cw, ch, xlim >> ssx,
                   ylim >> ssy);
     __syncthreads();
-    if (!(pri | sec) || cx >= w) {
-      if (cx < w) {
-        T *dst = pp<T>(a.out[p]);
-        for (int r = cy0; r < h; r += cny)
-          dst[(int64_t)((y0 >> ssy) + r) * a.out[p].stride + (x0 >> ssx) + cx] = (T)tile[(r + 2) * TS + cx + 2];
-      }
-      continue;
-    }
-    T *dst = pp<T>(a.out[p]) + (int64_t)((y0 >> ssy) + cy0) * a.out[p].stride + (x0 >> ssx) + cx;
-    const int64_t dstep = (int64_t)cny * a.out[p].stride;
-    for (int r = cy0; r < h; r += cny, dst += dstep) {
-      const int b = ((r << ssy) >> 3) * 8 + ((cx << ssx) >> 3);
-      const int d = blk_dir[b];
-      const int ci = (r + 2) * TS + cx + 2;
-      int v = tile[ci];
-      if (d >= 0) {
-        const int dir = pri == 0 ? 0 : AV1_CDEF_UV_DIR[ssx][ssy][d];
-        v = filter_px(tile, ci, make_taps(pri, sec, damping, dir, cs));
-      }
-      *dst = (T)v;
+    if (inside) {
+      T *o = pp<T>(a.out[p]) + (int64_t)((y0 >> ssy) + by * bhs) * a.out[p].stride + (x0 >> ssx) + bx * bws;
+      filter_block_pairs<T>(words, (by * bhs + 2) * TS + bx * bws + 2, o, a.out[p].stride, 2 * cpair, r0, nrows,
+                            k, filt && (upri | usec));
     }
   }
 }

@@ -3,7 +3,7 @@
  *
  * Follows rivet-av1's grain::apply (src/decoder/grain.rs:8-354).
  *
- * grain_prepare_kernel: one wave per frame.
+ * grain_prepare_kernel: one workgroup of two waves per frame.
  *   - Gaussian templates (7.18.3.3): luma 73x82, cb/cr 38x44 (4:2:0) or
  *     larger. The 16-bit LFSR is linear over GF(2), so lane t jumps to the
  *     state after t*C steps with precomputed powers A^(2^k) of the step
@@ -11,7 +11,8 @@
  *   - The auto-regressive filter is a recursion: sample (y, x) reads rows
  *     y-lag..y and columns x-lag..x+lag. Processing it along the diagonals
  *     T = (lag+1)*y + x visits every dependency at an earlier T, so all
- *     samples of one T run in parallel (one lane per template row).
+ *     samples of one T run in parallel (one lane per template row); the
+ *     chroma recursion runs on the second wave, interleaved with luma.
  *   - Scaling LUTs (7.18.3.4) and the per-stripe random block offsets.
  * grain_apply_kernel: one thread per chroma sample (and the 1, 2 or 4 luma
  *   samples it covers). The noise of a sample is rebuilt directly from the
@@ -77,10 +78,10 @@ constexpr int TW = 82, TH = 73;
 
 /* Gaussian values of the LFSR sequence from `seed` into t (rows x cols, stride TW). */
 __device__ void gaussian_fill(int16_t *t, const int16_t *gauss, uint32_t seed, int rows, int cols, int shift,
-                              bool on) {
+                              bool on, int lane) {
   const int n = rows * cols;
   const int per = (n + 63) / 64;
-  const int i0 = threadIdx.x * per, i1 = min(i0 + per, n);
+  const int i0 = lane * per, i1 = min(i0 + per, n);
   int r = i0 / cols, c = i0 - r * cols;
   uint32_t s = on ? jump(seed, (uint32_t)i0) : 0;
   for (int i = i0; i < i1; i++) {
@@ -97,126 +98,153 @@ __device__ void gaussian_fill(int16_t *t, const int16_t *gauss, uint32_t seed, i
   }
 }
 
-/* Luma AR of 7.18.3.3 (grain.rs:57-73) for a fixed lag, along the
- * diagonals T = (LAG+1)*y + x: at step T the active rows are those with
- * 3 <= x <= 78, about 76 / (LAG+1) of them, one lane each. */
+/* One diagonal step of the luma AR of 7.18.3.3 (grain.rs:57-73): the
+ * samples (y, x) with (LAG+1)*y + x = T, one lane each. Four partial sums
+ * keep the multiply-adds out of one dependency chain. */
 template <int LAG>
-__device__ void luma_ar(int16_t *tl, const GrainHdr &h, int gmin, int gmax) {
-  constexpr int K = LAG + 1, NC = 2 * LAG * (LAG + 1);
-  int c[NC > 0 ? NC : 1];
+__device__ __forceinline__ void luma_step(int16_t *tl, const int *c, int T, int lane, int ars, int gmin, int gmax) {
+  constexpr int K = LAG + 1;
+  const int ymin = max(3, (T - 78 + K - 1) / K);
+  const int y = ymin + lane;
+  const int x = T - K * y;
+  if (y > 72 || x < 3) return;
+  int s[4] = {0, 0, 0, 0}, pos = 0;
 #pragma unroll
-  for (int i = 0; i < NC; i++) c[i] = h.ar_y[i];
-  const int ars = h.ar_shift;
-  const int lane = threadIdx.x;
-  if (LAG == 0) {
-    /* no taps: every sample is clipped on its own */
-    for (int i = lane; i < 70 * 76; i += 64) {
-      const int y = 3 + i / 76, x = 3 + i % 76;
-      tl[y * TW + x] = (int16_t)clip3(gmin, gmax, tl[y * TW + x]);
+  for (int dr = -LAG; dr <= 0; dr++)
+#pragma unroll
+    for (int dc = -LAG; dc <= LAG; dc++) {
+      if (dr == 0 && dc >= 0) continue;
+      s[pos & 3] += __mul24((int)tl[(y + dr) * TW + x + dc], c[pos]);
+      pos++;
     }
-    __syncthreads();
-    return;
+  tl[y * TW + x] = (int16_t)clip3(gmin, gmax, tl[y * TW + x] + round2((s[0] + s[1]) + (s[2] + s[3]), ars));
+}
+
+/* One chroma AR sample (grain.rs:101-134): cb and cr together, plus the
+ * tap on the co-located (averaged) luma. */
+template <int LAG>
+__device__ __forceinline__ void chroma_one(int16_t *tcb, int16_t *tcr, const int16_t *tl, const int *c0,
+                                           const int *c1, const GrainHdr &h, int y, int x, int gmin, int gmax) {
+  constexpr int NC = 2 * LAG * (LAG + 1) + 1;
+  int s0[2] = {0, 0}, s1[2] = {0, 0}, pos = 0;
+#pragma unroll
+  for (int dr = -LAG; dr <= 0; dr++)
+#pragma unroll
+    for (int dc = -LAG; dc <= LAG; dc++) {
+      if (dr == 0 && dc >= 0) continue;
+      s0[pos & 1] += __mul24((int)tcb[(y + dr) * TW + x + dc], c0[pos]);
+      s1[pos & 1] += __mul24((int)tcr[(y + dr) * TW + x + dc], c1[pos]);
+      pos++;
+    }
+  int a0 = s0[0] + s0[1], a1 = s1[0] + s1[1];
+  if (h.num_y > 0) {
+    const int lx = ((x - 3) << h.ssx) + 3, ly = ((y - 3) << h.ssy) + 3;
+    int l = 0;
+    for (int i = 0; i <= h.ssy; i++)
+      for (int j = 0; j <= h.ssx; j++) l += tl[(ly + i) * TW + lx + j];
+    l = round2(l, h.ssx + h.ssy);
+    a0 += __mul24(l, c0[NC - 1]);
+    a1 += __mul24(l, c1[NC - 1]);
   }
-  for (int T = 3 * K + 3; T <= 72 * K + 78; T++) {
-    const int ymin = max(3, (T - 78 + K - 1) / K);
-    const int y = ymin + lane;
+  tcb[y * TW + x] = (int16_t)clip3(gmin, gmax, tcb[y * TW + x] + round2(a0, h.ar_shift));
+  tcr[y * TW + x] = (int16_t)clip3(gmin, gmax, tcr[y * TW + x] + round2(a1, h.ar_shift));
+}
+
+/* One diagonal step of the chroma AR: the samples with
+ * (LAG+1)*y + x = T, 3 <= x <= cw-4, 3 <= y < ch. */
+template <int LAG>
+__device__ __forceinline__ void chroma_step(int16_t *tcb, int16_t *tcr, const int16_t *tl, const int *c0,
+                                            const int *c1, const GrainHdr &h, int cw, int ch, int T, int lane,
+                                            int gmin, int gmax) {
+  constexpr int K = LAG + 1;
+  const int xmax = cw - 4;
+  const int ymin = max(3, (T - xmax + K - 1) / K);
+  for (int y = ymin + lane; y < ch; y += 64) {
     const int x = T - K * y;
-    if (y <= 72 && x >= 3) {
-      int s = 0, pos = 0;
-#pragma unroll
-      for (int dr = -LAG; dr <= 0; dr++)
-#pragma unroll
-        for (int dc = -LAG; dc <= LAG; dc++) {
-          if (dr == 0 && dc >= 0) continue;
-          s += __mul24((int)tl[(y + dr) * TW + x + dc], c[pos]);
-          pos++;
-        }
-      tl[y * TW + x] = (int16_t)clip3(gmin, gmax, tl[y * TW + x] + round2(s, ars));
-    }
-    __syncthreads();
+    if (x < 3) break;
+    chroma_one<LAG>(tcb, tcr, tl, c0, c1, h, y, x, gmin, gmax);
   }
 }
 
-/* Chroma AR (grain.rs:101-134): cb and cr together, plus the luma tap,
- * along the same diagonals as luma. */
+/* Both AR recursions in lock step: wave 0 walks the luma diagonals
+ * T = K*y + x (K = LAG+1) at global step G = T; wave 1 walks the chroma
+ * ones at G = 2T - 2K - 1 (subsampled chroma) or G = T + 1 (4:4:4). Every
+ * luma sample a chroma sample (y, x) reads was finished at a smaller G:
+ * for 4:2:0 the luma samples (2y-3+i, 2x-3+j), i, j in {0, 1}, finish by
+ * G = 2T - 2K - 2; for 4:2:2, (y, 2x-3+j) finish by T + x - 2 <= 2T - 3K - 2;
+ * for 4:4:4, (y, x) finishes at G = T. */
 template <int LAG>
-__device__ void chroma_ar(int16_t *tcb, int16_t *tcr, const int16_t *tl, const GrainHdr &h, int cw, int ch,
-                          int gmin, int gmax) {
-  constexpr int K = LAG + 1, NC = 2 * LAG * (LAG + 1) + 1;
-  int c0[NC], c1[NC];
+__device__ void ar_both(int16_t *tl, int16_t *tcb, int16_t *tcr, const GrainHdr &h, bool chroma, int cw, int ch,
+                        int gmin, int gmax) {
+  constexpr int K = LAG + 1, NL = 2 * LAG * (LAG + 1), NC = NL + 1;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  int cl[NL > 0 ? NL : 1], c0[NC], c1[NC];
+#pragma unroll
+  for (int i = 0; i < NL; i++) cl[i] = h.ar_y[i];
 #pragma unroll
   for (int i = 0; i < NC; i++) {
     c0[i] = h.ar_cb[i];
     c1[i] = h.ar_cr[i];
   }
-  const int ars = h.ar_shift;
-  const int lane = threadIdx.x;
-  const bool luma_tap = h.num_y > 0;
-  const int xmax = cw - 4;
-  for (int T = 3 * K + 3; T <= (ch - 1) * K + xmax; T++) {
-    const int ymin = max(3, (T - xmax + K - 1) / K);
-    for (int y = ymin + lane; y < ch; y += 64) {
-      const int x = T - K * y;
-      if (x < 3) break;
-      int s0 = 0, s1 = 0, pos = 0;
-#pragma unroll
-      for (int dr = -LAG; dr <= 0; dr++)
-#pragma unroll
-        for (int dc = -LAG; dc <= LAG; dc++) {
-          if (dr == 0 && dc >= 0) continue;
-          s0 += __mul24((int)tcb[(y + dr) * TW + x + dc], c0[pos]);
-          s1 += __mul24((int)tcr[(y + dr) * TW + x + dc], c1[pos]);
-          pos++;
-        }
-      if (luma_tap) {
-        const int lx = ((x - 3) << h.ssx) + 3, ly = ((y - 3) << h.ssy) + 3;
-        int l = 0;
-        for (int i = 0; i <= h.ssy; i++)
-          for (int j = 0; j <= h.ssx; j++) l += tl[(ly + i) * TW + lx + j];
-        l = round2(l, h.ssx + h.ssy);
-        s0 += __mul24(l, c0[NC - 1]);
-        s1 += __mul24(l, c1[NC - 1]);
+  if (LAG == 0) {
+    /* no neighbour taps: luma samples are only clipped, and each chroma
+     * sample depends on the (final) luma alone */
+    for (int i = threadIdx.x; i < 70 * 76; i += 128) {
+      const int y = 3 + i / 76, x = 3 + i % 76;
+      tl[y * TW + x] = (int16_t)clip3(gmin, gmax, tl[y * TW + x]);
+    }
+    __syncthreads();
+    if (chroma) {
+      const int rw = cw - 6, nr = ch - 3;
+      for (int i = threadIdx.x; i < rw * nr; i += 128) {
+        const int y = 3 + i / rw, x = 3 + i % rw;
+        chroma_one<0>(tcb, tcr, tl, c0, c1, h, y, x, gmin, gmax);
       }
-      tcb[y * TW + x] = (int16_t)clip3(gmin, gmax, tcb[y * TW + x] + round2(s0, ars));
-      tcr[y * TW + x] = (int16_t)clip3(gmin, gmax, tcr[y * TW + x] + round2(s1, ars));
+    }
+    __syncthreads();
+    return;
+  }
+  const int sub = h.ssx | h.ssy;
+  const int tl_lo = 3 * K + 3, tl_hi = 72 * K + 78;
+  const int tc_lo = 3 * K + 3, tc_hi = (ch - 1) * K + (cw - 4);
+  const int g_hi = max(tl_hi, chroma ? (sub ? 2 * tc_hi - 2 * K - 1 : tc_hi + 1) : 0);
+  for (int G = tl_lo; G <= g_hi; G++) {
+    if (wave == 0) {
+      if (G <= tl_hi) luma_step<LAG>(tl, cl, G, lane, h.ar_shift, gmin, gmax);
+    } else if (chroma) {
+      const int T = sub ? ((G & 1) ? (G + 2 * K + 1) >> 1 : -1) : G - 1;
+      if (T >= tc_lo && T <= tc_hi) chroma_step<LAG>(tcb, tcr, tl, c0, c1, h, cw, ch, T, lane, gmin, gmax);
     }
     __syncthreads();
   }
 }
 
-__global__ __launch_bounds__(64) void grain_prepare_kernel(GrainHdr h, GrainScratch sc) {
+__global__ __launch_bounds__(128) void grain_prepare_kernel(GrainHdr h, GrainScratch sc) {
   __shared__ int16_t tl[TH * TW], tcb[TH * TW], tcr[TH * TW];
   __shared__ int16_t gauss[2048];
-  const int lane = threadIdx.x;
-  for (int i = lane; i < 2048; i += 64) gauss[i] = AV1_GAUSSIAN_SEQUENCE[i];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int i = threadIdx.x; i < 2048; i += 128) gauss[i] = AV1_GAUSSIAN_SEQUENCE[i];
   __syncthreads();
   const int bd = h.bd;
   const int gmin = -(128 << (bd - 8)), gmax = (256 << (bd - 8)) - 1 - (128 << (bd - 8));
   const int shift = 12 - bd + h.grain_scale_shift;
   const bool chroma = h.nplanes > 1;
   const int cw = h.ssx ? 44 : 82, ch = h.ssy ? 38 : 73;
-  gaussian_fill(tl, gauss, h.seed, TH, TW, shift, h.num_y > 0);
-  if (chroma) {
-    gaussian_fill(tcb, gauss, h.seed ^ 0xb524, ch, cw, shift, h.num_cb > 0 || h.cfl);
-    gaussian_fill(tcr, gauss, h.seed ^ 0x49d8, ch, cw, shift, h.num_cr > 0 || h.cfl);
+  if (wave == 0) {
+    gaussian_fill(tl, gauss, h.seed, TH, TW, shift, h.num_y > 0, lane);
+  } else if (chroma) {
+    gaussian_fill(tcb, gauss, h.seed ^ 0xb524, ch, cw, shift, h.num_cb > 0 || h.cfl, lane);
+    gaussian_fill(tcr, gauss, h.seed ^ 0x49d8, ch, cw, shift, h.num_cr > 0 || h.cfl, lane);
   }
   __syncthreads();
   switch (h.ar_lag) {
-    case 0: luma_ar<0>(tl, h, gmin, gmax); break;
-    case 1: luma_ar<1>(tl, h, gmin, gmax); break;
-    case 2: luma_ar<2>(tl, h, gmin, gmax); break;
-    default: luma_ar<3>(tl, h, gmin, gmax); break;
-  }
-  if (chroma) {
-    switch (h.ar_lag) {
-      case 0: chroma_ar<0>(tcb, tcr, tl, h, cw, ch, gmin, gmax); break;
-      case 1: chroma_ar<1>(tcb, tcr, tl, h, cw, ch, gmin, gmax); break;
-      case 2: chroma_ar<2>(tcb, tcr, tl, h, cw, ch, gmin, gmax); break;
-      default: chroma_ar<3>(tcb, tcr, tl, h, cw, ch, gmin, gmax); break;
-    }
+    case 0: ar_both<0>(tl, tcb, tcr, h, chroma, cw, ch, gmin, gmax); break;
+    case 1: ar_both<1>(tl, tcb, tcr, h, chroma, cw, ch, gmin, gmax); break;
+    case 2: ar_both<2>(tl, tcb, tcr, h, chroma, cw, ch, gmin, gmax); break;
+    default: ar_both<3>(tl, tcb, tcr, h, chroma, cw, ch, gmin, gmax); break;
   }
   /* templates out */
-  for (int i = lane; i < TH * TW; i += 64) {
+  for (int i = threadIdx.x; i < TH * TW; i += 128) {
     sc.tmpl[i] = tl[i];
     sc.tmpl[TH * TW + i] = chroma ? tcb[i] : 0;
     sc.tmpl[2 * TH * TW + i] = chroma ? tcr[i] : 0;
@@ -225,7 +253,7 @@ __global__ __launch_bounds__(64) void grain_prepare_kernel(GrainHdr h, GrainScra
   for (int p = 0; p < h.nplanes; p++) {
     const int np = h.npoints[p];
     const int *xs = h.points[p][0], *ys = h.points[p][1];
-    for (int x = lane; x < 256; x += 64) {
+    for (int x = threadIdx.x; x < 256; x += 128) {
       int v = 0;
       if (np > 0) {
         if (x < xs[0]) {
@@ -246,7 +274,7 @@ __global__ __launch_bounds__(64) void grain_prepare_kernel(GrainHdr h, GrainScra
   /* block offsets per noise stripe (grain.rs:186-199) */
   const int nstripes = ((h.h + 1) / 2 + 15) / 16;
   const int nblocks = ((h.w + 1) / 2 + 15) / 16;
-  for (int ln = lane; ln < nstripes; ln += 64) {
+  for (int ln = threadIdx.x; ln < nstripes; ln += 128) {
     uint32_t r = h.seed;
     r ^= (uint32_t)(((ln * 37 + 178) & 255) << 8);
     r ^= (uint32_t)((ln * 173 + 105) & 255);
@@ -316,8 +344,10 @@ __global__ __launch_bounds__(256) void grain_apply_kernel(GrainHdr h, GrainScrat
   __syncthreads();
   const int ssx = h.nplanes > 1 ? h.ssx : 0, ssy = h.nplanes > 1 ? h.ssy : 0;
   const int cw = (h.w + ssx) >> ssx, ch = (h.h + ssy) >> ssy;
-  const int cx = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int cy = blockIdx.y * 4 + (threadIdx.x >> 6);
+  const int tile_id = xcd_remap(blockIdx.x, gridDim.x);
+  const int nbx = (cw + 63) / 64;
+  const int cx = (tile_id % nbx) * 64 + (threadIdx.x & 63);
+  const int cy = (tile_id / nbx) * 4 + (threadIdx.x >> 6);
   if (cx >= cw || cy >= ch) return;
   const int bd = h.bd;
   const int gmin = -(128 << (bd - 8)), gmax = (256 << (bd - 8)) - 1 - (128 << (bd - 8));
@@ -377,7 +407,7 @@ __global__ __launch_bounds__(256) void grain_apply_kernel(GrainHdr h, GrainScrat
 }  // namespace
 
 int launch_grain_prepare(const GrainHdr &h, const GrainScratch &s, hipStream_t st) {
-  grain_prepare_kernel<<<1, 64, 0, st>>>(h, s);
+  grain_prepare_kernel<<<1, 128, 0, st>>>(h, s);
   return (int)hipGetLastError();
 }
 
@@ -385,7 +415,7 @@ int launch_grain_apply(const GrainHdr &h, const GrainScratch &s, const PlanePtr 
                        PlanePtr out[3], int bps, hipStream_t st) {
   const int ssx = h.nplanes > 1 ? h.ssx : 0, ssy = h.nplanes > 1 ? h.ssy : 0;
   const int cw = (h.w + ssx) >> ssx, ch = (h.h + ssy) >> ssy;
-  dim3 grid((cw + 63) / 64, (ch + 3) / 4);
+  const int grid = ((cw + 63) / 64) * ((ch + 3) / 4);
   if (bps == 1)
     grain_apply_kernel<uint8_t, uint8_t><<<grid, 256, 0, st>>>(h, s, in[0], in[1], in[2], out[0], out[1], out[2]);
   else

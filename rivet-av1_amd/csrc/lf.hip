@@ -186,8 +186,10 @@ struct LfLaunch {
   LfSide s;
   PlanePtr planes[3];
   int plane_id[3];   /* planes filtered, in grid order */
-  int blk_start[4];  /* first block of each grid plane, and the total */
+  int blk_start[4];  /* first block of each grid plane (padded to 8), and the total */
   int blk_x[3];      /* blocks across each grid plane */
+  int blk_n[3];      /* blocks of each grid plane */
+  int nseg;
 };
 
 /* Edge decision of edge_loop_filter (postfilter.rs:115-172). Returns the
@@ -244,12 +246,10 @@ __device__ __forceinline__ int edge_decision(const LfLaunch &a, int plane, int p
 
 template <typename T, int PASS>
 __global__ __launch_bounds__(256) void lf_pass_kernel(LfLaunch a) {
-  const int b = blockIdx.x;
-  int gp = 0;
-  if (b >= a.blk_start[1]) gp = 1;
-  if (b >= a.blk_start[2]) gp = 2;
+  int lb;
+  const int gp = xcd_segment(blockIdx.x, a.blk_start, a.nseg, &lb);
+  if (gp >= a.nseg || lb >= a.blk_n[gp]) return; /* padding */
   const int plane = a.plane_id[gp];
-  const int lb = b - a.blk_start[gp];
   const int ex = (lb % a.blk_x[gp]) * 64 + (threadIdx.x & 63);
   const int ey = (lb / a.blk_x[gp]) * 4 + (threadIdx.x >> 6);
   const Geom &g = a.g;
@@ -349,13 +349,16 @@ int launch_deblock(const Geom &g, const LfHdr &h, const LfSide &s, PlanePtr plan
     const int rows4 = (g.mi_rows + (1 << ssy) - 1) >> ssy;
     a.plane_id[n] = p;
     a.blk_x[n] = (cols4 + 63) / 64;
+    a.blk_n[n] = a.blk_x[n] * ((rows4 + 3) / 4);
     a.blk_start[n] = total;
-    total += a.blk_x[n] * ((rows4 + 3) / 4);
+    total += (a.blk_n[n] + 7) & ~7;
     n++;
   }
+  a.nseg = n;
   for (int i = n; i < 3; i++) {
     a.plane_id[i] = 0;
     a.blk_x[i] = 1;
+    a.blk_n[i] = 0;
     a.blk_start[i] = total;
   }
   a.blk_start[3] = total;

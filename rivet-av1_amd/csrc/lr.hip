@@ -5,7 +5,7 @@
  * restore_stripe / LrCtx::sample / restore_rect_from_window /
  * box_filter_rect / wiener_coefficient (src/decoder/postfilter.rs:572-1004).
  *
- * One workgroup (256 threads) per (plane, stripe, slice): a stripe is 64
+ * One workgroup (512 threads) per (plane, stripe, slice): a stripe is 64
  * luma rows offset by -8, a slice is min(64, unit size) columns, so a slice
  * lies in one restoration unit (the last unit of a row absorbs the
  * remainder: its slices map to it by clamping the unit column). The
@@ -32,14 +32,16 @@ constexpr int SW = 64;            /* max slice width */
 constexpr int WW = SW + 6;        /* window stride */
 constexpr int WH = 64 + 6;        /* window rows */
 constexpr int AW = SW + 2;        /* A/B stride */
-constexpr int PX_PER_THREAD = 16; /* 64 x 64 / 256 */
+constexpr int NT = 512;           /* threads per workgroup */
+constexpr int PX_PER_THREAD = 64 * 64 / NT;
 
 struct LrLaunch {
   Geom g;
   LrHdr h;
   PlanePtr deb[3], cdef[3], out[3];
   int plane_id[3];
-  int blk_start[4];
+  int blk_start[4]; /* padded to multiples of 8 per plane */
+  int nseg;
   int nslices[3];
   int nstripes;
 };
@@ -64,18 +66,16 @@ constexpr A2Table make_a2() {
 constexpr A2Table SGR_A2 = make_a2();
 
 template <typename T>
-__global__ __launch_bounds__(256) void lr_kernel(LrLaunch a) {
+__global__ __launch_bounds__(NT) void lr_kernel(LrLaunch a) {
   __shared__ uint16_t win[WH * WW];
   __shared__ int16_t inter[WH * SW];      /* Wiener horizontal output; SGR: A */
   __shared__ int32_t bbuf[(64 + 2) * AW]; /* SGR: B */
 
   const Geom &g = a.g;
-  const int b = blockIdx.x;
-  int gp = 0;
-  if (b >= a.blk_start[1]) gp = 1;
-  if (b >= a.blk_start[2]) gp = 2;
+  int lb;
+  const int gp = xcd_segment(blockIdx.x, a.blk_start, a.nseg, &lb);
+  if (gp >= a.nseg || lb >= a.nslices[gp] * a.nstripes) return; /* padding */
   const int plane = a.plane_id[gp];
-  const int lb = b - a.blk_start[gp];
   const int st = lb / a.nslices[gp], sl = lb - st * a.nslices[gp];
   const int ssx = plane ? g.ssx : 0, ssy = plane ? g.ssy : 0;
   const int plane_end_x = ((g.width + ssx) >> ssx) - 1;
@@ -84,7 +84,7 @@ __global__ __launch_bounds__(256) void lr_kernel(LrLaunch a) {
   const int unit_size = a.h.unit_size[plane];
   const int lg = unit_size >= SW ? 6 : 5; /* slice width 64 or 32 */
   const int sw = 1 << lg;
-  const int tx = threadIdx.x & (sw - 1), ty = threadIdx.x >> lg, ny = 256 >> lg;
+  const int tx = threadIdx.x & (sw - 1), ty = threadIdx.x >> lg, ny = NT >> lg;
   /* lr_stripe, postfilter.rs:640-661 */
   const int stripe_start = (-8 + st * 64) >> ssy;
   const int stripe_end = stripe_start + (64 >> ssy) - 1;
@@ -310,9 +310,10 @@ int launch_lr(const Geom &g, const LrHdr &h, const PlanePtr deblocked[3], const 
     a.plane_id[n] = p;
     a.nslices[n] = (area_w + sw - 1) / sw;
     a.blk_start[n] = total;
-    total += a.nslices[n] * a.nstripes;
+    total += (a.nslices[n] * a.nstripes + 7) & ~7;
     n++;
   }
+  a.nseg = n;
   for (int i = n; i < 3; i++) {
     a.plane_id[i] = 0;
     a.nslices[i] = 1;
@@ -321,9 +322,9 @@ int launch_lr(const Geom &g, const LrHdr &h, const PlanePtr deblocked[3], const 
   a.blk_start[3] = total;
   if (total == 0) return 0;
   if (bps == 1)
-    lr_kernel<uint8_t><<<total, 256, 0, st>>>(a);
+    lr_kernel<uint8_t><<<total, NT, 0, st>>>(a);
   else
-    lr_kernel<uint16_t><<<total, 256, 0, st>>>(a);
+    lr_kernel<uint16_t><<<total, NT, 0, st>>>(a);
   return (int)hipGetLastError();
 }
 

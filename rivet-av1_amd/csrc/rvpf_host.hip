@@ -223,9 +223,10 @@ int validate_side(const rvpf_frame_params *p, const rvpf_side_info *s, char *msg
   return RVPF_OK;
 }
 
-/* margin: extra rows/cols beyond `need` the stage reads */
+/* margin: extra rows/cols beyond `need` the stage reads; pairs: the
+ * stage stores two adjacent samples at once at even columns */
 int validate_planes(const rvpf_frame_params *p, const rvpf_planes *pl, const char *name, int use_crop, int margin,
-                    char *msg) {
+                    char *msg, bool pairs = false) {
   if (!pl) {
     snprintf(msg, 256, "%s planes are NULL", name);
     return RVPF_E_INVALID;
@@ -254,6 +255,10 @@ int validate_planes(const rvpf_frame_params *p, const rvpf_planes *pl, const cha
     /* the loop filter moves 4 samples at a time */
     if (margin && (pl->stride[i] % 4 != 0 || ((uintptr_t)pl->data[i] & 7) != 0)) {
       snprintf(msg, 256, "%s plane %d: stride must be a multiple of 4 samples and data 8-byte aligned", name, i);
+      return RVPF_E_INVALID;
+    }
+    if (pairs && (pl->stride[i] % 2 != 0 || ((uintptr_t)pl->data[i] & (2 * pl->bytes_per_sample - 1)) != 0)) {
+      snprintf(msg, 256, "%s plane %d: stride must be even and data aligned to two samples", name, i);
       return RVPF_E_INVALID;
     }
     if (((uintptr_t)pl->data[i] & (pl->bytes_per_sample - 1)) != 0) {
@@ -646,7 +651,7 @@ int rvpf_cdef(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_side_info *s
   int rc;
   if ((rc = validate_params(p, ctx->err)) || (rc = validate_side(p, s, ctx->err)) ||
       (rc = validate_planes(p, deblocked, "deblocked", 0, 0, ctx->err)) ||
-      (rc = validate_planes(p, out, "out", 0, 0, ctx->err)))
+      (rc = validate_planes(p, out, "out", 0, 0, ctx->err, true)))
     return rc;
   if (deblocked->bytes_per_sample != out->bytes_per_sample)
     return fail(ctx, RVPF_E_INVALID, "deblocked and out sample sizes differ");
@@ -716,7 +721,7 @@ int rvpf_postfilter_frame(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_
   int rc;
   if ((rc = validate_params(p, ctx->err)) || (rc = validate_side(p, s, ctx->err)) ||
       (rc = validate_planes(p, recon, "recon", 0, lf_on(p) ? 8 : 0, ctx->err)) ||
-      (rc = validate_planes(p, ref_out, "ref_out", 0, 0, ctx->err)))
+      (rc = validate_planes(p, ref_out, "ref_out", 0, 0, ctx->err, true)))
     return rc;
   if (display && (rc = validate_planes(p, display, "display", 1, 0, ctx->err))) return rc;
   const int bps = recon->bytes_per_sample;

@@ -84,6 +84,37 @@ struct GrainScratch {
   int max_stripes, max_blocks;
 };
 
+/* Workgroups are dispatched round-robin over the 8 XCDs (each with its
+ * own L2): workgroup b runs on XCD b % 8. Renumbering so that XCD k
+ * receives one contiguous range of the n tiles keeps neighbouring tiles,
+ * and the halo lines they share, in one L2. A bijection of [0, n). */
+__device__ __forceinline__ int xcd_remap(int b, int n) {
+  constexpr int NX = 8;
+  const int q = n / NX, r = n % NX;
+  const int k = b % NX, j = b / NX;
+  return k * q + min(k, r) + j;
+}
+
+/* xcd_remap for a grid made of segments (one per plane) of unequal cost:
+ * each segment's block count is padded to a multiple of 8 (pad[s] =
+ * first block of segment s, pad[nseg] = grid size), and XCD k gets the
+ * k-th contiguous eighth of every segment, so every XCD sees the same mix.
+ * Returns the segment of block b and sets *local to its index there. */
+__device__ __forceinline__ int xcd_segment(int b, const int *pad, int nseg, int *local) {
+  const int k = b & 7, j = b >> 3;
+  int acc = 0;
+  for (int s = 0; s < nseg; s++) {
+    const int n8 = (pad[s + 1] - pad[s]) >> 3;
+    if (j < acc + n8) {
+      *local = k * n8 + (j - acc);
+      return s;
+    }
+    acc += n8;
+  }
+  *local = 0;
+  return nseg; /* not reached for b < pad[nseg] */
+}
+
 /* Launch entry points (each returns a hipError_t cast to int). */
 int launch_deblock(const Geom &g, const LfHdr &h, const LfSide &s, PlanePtr planes[3], int bps,
                    hipStream_t st);
