@@ -43,6 +43,32 @@ def make_frames(n, w, h, bd, seed):
     return frames
 
 
+# HBM bytes per launch of each stage's kernels on the default workload,
+# from the calibrated FETCH_SIZE / WRITE_SIZE counter passes summarised here
+PMC_SUMMARY = "profiles/chain_4k10b_pmc.md"
+STAGE_KERNELS = {"deblock": ["lf_pass_kernel<u16,V>", "lf_pass_kernel<u16,H>"], "cdef": ["cdef_kernel<u16>"],
+                 "lr": ["lr_kernel<u16>"], "grain_apply": ["grain_apply_kernel<u16,u16>"]}
+
+
+def measured_traffic(stage):
+    """Calibrated HBM bytes per launch of `stage` (None if not recorded)."""
+    path = os.path.join(ROOT, PMC_SUMMARY)
+    if not os.path.exists(path):
+        return None
+    rows = {}
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 5 and cells[0].startswith("`"):
+            try:
+                rows[cells[0].strip("`")] = int(cells[4])
+            except ValueError:
+                pass
+    names = STAGE_KERNELS.get(stage, [])
+    if not names or any(n not in rows for n in names):
+        return None
+    return sum(rows[n] for n in names)
+
+
 def samples_per_frame(f):
     return sum(f.crop_shape(p)[0] * f.crop_shape(p)[1] for p in range(f.num_planes))
 
@@ -194,9 +220,11 @@ def main():
     if main:
         dom = max(main, key=lambda k: stages[k]["ms"])
         s = stages[dom]
+        default_workload = (args.width, args.height, args.bit_depth, args.frames) == (3840, 2160, 10, 4)
         roofline = {"bound": "hbm", "achieved": s["GB_s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": s["frac"], "traffic": None, "kernel": dom,
-                    "alg_bytes_per_launch": s["alg_bytes"], "ms_per_launch": s["ms"]}
+                    "frac": s["frac"], "traffic": measured_traffic(dom) if default_workload else None,
+                    "kernel": dom, "alg_bytes_per_launch": s["alg_bytes"], "ms_per_launch": s["ms"],
+                    "traffic_source": PMC_SUMMARY if default_workload else None}
     chain_alg = samples_per_frame(frames[0]) * (8 * b + b * 4 / 64) * len(frames)
     chain_gbs = chain_alg / (ms_per_step / 1e3) / 1e9
 
