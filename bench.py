@@ -92,9 +92,162 @@ def cpu_baseline(frames, seconds):
             "sample": "%d frames of the same 4K 10-bit 4:2:0 chain (%.1f s), C oracle, %d threads" % (n, el, threads)}
 
 
+BATCH_SIZES = [(640, 360), (1280, 720), (1920, 1080), (2560, 1440), (3840, 2160)]
+
+
+def batch_streams(n_streams, frames_per_stream, seed):
+    """The 32-stream mixed workload: sizes uniform over BATCH_SIZES, half
+    8-bit (in 8-bit sample planes) and half 10-bit, 4:2:0 except four 4:4:4
+    streams, a random qindex in [48, 200] per stream, every filter on with
+    film grain, as the 1080p GOP configuration."""
+    import random
+    from rvpf import workload
+    rng = random.Random(seed)
+    streams = []
+    for s in range(n_streams):
+        w, h = BATCH_SIZES[rng.randrange(len(BATCH_SIZES))]
+        bd = 8 if s % 2 == 0 else 10
+        ss = 0 if s % 8 == 3 else 1
+        q = rng.randint(48, 200)
+        frames = [workload.Frame(w, h, bit_depth=bd, ssx=ss, ssy=ss, sb_size=128, qindex=q,
+                                 seed=seed * 1000 + s, frame_index=i, pan_px=1.5, lr=2, lr_unit_size=64,
+                                 grain=True, grain_lag=3, grain_y_points=8, with_source=False)
+                  for i in range(frames_per_stream)]
+        streams.append((bd, frames))
+    return streams
+
+
+def bench_batch(args):
+    """Batched path vs the plain per-frame drop-in on the same frames."""
+    import torch
+    from rvpf import api
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    log("generating %d streams x %d frames" % (args.streams, args.stream_frames))
+    streams = batch_streams(args.streams, args.stream_frames, args.seed)
+    ctx = api.Context()
+    items = []
+    for bd, frames in streams:
+        for f in frames:
+            df = api.DeviceFrame(f, bytes_per_sample=1 if bd == 8 else 2)
+            pristine = df.upload_planes(f.rec)
+            f.rec = None  # host copy no longer needed
+            items.append({"f": f, "df": df, "pristine": pristine,
+                          "rec": [None if t is None else t.clone() for t in pristine],
+                          "ref": df.alloc_like_recon(), "disp": df.alloc_display()})
+    luma_px = sum(it["f"].luma_pixels() for it in items)
+    log("%d frames, %.1f Mpix" % (len(items), luma_px / 1e6))
+    stream = torch.cuda.current_stream()
+
+    def refill():
+        for it in items:
+            for t, s_ in zip(it["rec"], it["pristine"]):
+                if t is not None:
+                    t.copy_(s_)
+
+    def timed(fn, reps):
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        total = 0.0
+        for _ in range(reps):
+            refill()
+            ev0.record(stream)
+            fn()
+            ev1.record(stream)
+            torch.cuda.synchronize()
+            total += ev0.elapsed_time(ev1)
+        return total / reps
+
+    def plain():
+        for it in items:
+            api.postfilter_frame(ctx, it["df"], it["rec"], it["ref"], it["disp"], stream)
+
+    for _ in range(args.warmup):
+        refill()
+        plain()
+    ms_plain = timed(plain, args.steps)
+    plain_out = [[t.clone() if t is not None else None for t in it["ref"] + it["disp"]] for it in items]
+    log("plain per-frame drop-in: %.3f ms per batch, %.1f Mpix/s" % (ms_plain, luma_px / ms_plain / 1e3))
+
+    batch = api.Batch(ctx, [(it["df"], it["rec"], it["ref"], it["disp"]) for it in items])
+    t0 = time.perf_counter()
+    batch.autotune(force=True)
+    tune_s = time.perf_counter() - t0
+    tiles = batch.get_tiles()
+    log("autotuned in %.1f s:" % tune_s, tiles)
+    for it in items:
+        for t in it["ref"] + it["disp"]:
+            if t is not None:
+                t.zero_()
+    for _ in range(args.warmup):
+        refill()
+        batch.run(stream)
+    ms_batch = timed(lambda: batch.run(stream), args.steps)
+    log("batched graph launch: %.3f ms per batch, %.1f Mpix/s" % (ms_batch, luma_px / ms_batch / 1e3))
+    same = True
+    for it, want in zip(items, plain_out):
+        for t, w in zip(it["ref"] + it["disp"], want):
+            if t is not None:
+                same &= bool(torch.equal(t, w))
+    log("batched outputs equal the per-frame outputs:", same)
+    bitexact = None
+    if not args.no_check:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+        import numpy as np
+        small = min(range(len(items)), key=lambda i: items[i]["f"].luma_pixels())
+        it = items[small]
+        f = it["f"]
+        f.rec = api.to_host(it["pristine"])
+        deb, ref, disp = pyoracle.postfilter_frame(f, threads=min(16, os.cpu_count() or 1))
+        got_ref, got_disp = api.to_host(it["ref"]), api.to_host(it["disp"])
+        bitexact = True
+        for p in range(f.num_planes):
+            r, c = f.mi_area(p)
+            bitexact &= bool(np.array_equal(got_ref[p][:r, :c], ref[p][:r, :c]))
+            bitexact &= bool(np.array_equal(got_disp[p], disp[p]))
+        log("bit-exact vs oracle (smallest frame):", bitexact)
+    out = {
+        "metric": "batched post-filter Mpix/s, 32-stream mixed workload (one graph launch vs per-frame drop-in)",
+        "value": round(luma_px / ms_batch / 1e3, 1),
+        "unit": "Mpix/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_batch, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8/u16",
+        "data": "synthetic",
+        "config": {
+            "workload": "%d streams x %d frames, sizes %s, half 8-bit (u8 planes) / half 10-bit, 4:2:0 + four "
+                        "4:4:4 streams, qindex U[48,200], all filters + film grain" %
+                        (args.streams, args.stream_frames, "/".join("%dx%d" % s for s in BATCH_SIZES)),
+            "frames": len(items),
+            "mpix_per_batch": round(luma_px / 1e6, 2),
+            "plain_ms": round(ms_plain, 4),
+            "plain_mpix_s": round(luma_px / ms_plain / 1e3, 1),
+            "batched_over_plain": round(ms_plain / ms_batch, 3),
+            "batched_equals_plain": same,
+            "bitexact_smallest_frame": bitexact,
+            "autotune_s": round(tune_s, 2),
+            "tiles": {k: {"shape": v[0], "workgroups_per_cu": v[1], "chunk": v[2], "us": round(v[3], 1)}
+                      for k, v in tiles.items()},
+        },
+    }
+    print(json.dumps(out), flush=True)
+    batch.close()
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", choices=["gop4k", "batch32"], default="gop4k",
+                    help="gop4k: the per-frame chain on the 4K 10-bit GOP (default); "
+                         "batch32: the batched path against the per-frame drop-in on 32 mixed streams")
+    ap.add_argument("--streams", type=int, default=32)
+    ap.add_argument("--stream-frames", type=int, default=4)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--frames", type=int, default=4)
@@ -113,6 +266,10 @@ def main():
         print("bench.py measures one GPU: the post-filter chain shards by stream/frame with no exchange; "
               "multi-GPU runs are not part of this build (use --gpus 1)", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "batch32":
+        bench_batch(args)
+        return
 
     import numpy as np
     import torch

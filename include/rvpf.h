@@ -258,6 +258,65 @@ int rvpf_postfilter_frame(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_
                           rvpf_planes *recon, rvpf_planes *ref_out, rvpf_planes *display,
                           void *stream);
 
+/*
+ * Batched multi-stream path (no counterpart in the reference, whose
+ * decoder filters one frame at a time; for servers and transcoders that
+ * decode many streams at once).
+ *
+ * A batch is built once over a fixed set of frames — any mix of sizes,
+ * bit depths, subsamplings and sample sizes — and their caller-owned
+ * buffers. Every stage then runs as one launch over all frames (a fixed
+ * grid of workgroups taking tiles from a device queue), the launches are
+ * captured in one HIP graph, and rvpf_batch_run replays it. To filter new
+ * content, write it into the same buffers and run again. The result of
+ * every frame equals rvpf_postfilter_frame's on the same inputs.
+ */
+typedef struct rvpf_batch_frame {
+  const rvpf_frame_params *params; /* host memory, read by rvpf_batch_create */
+  rvpf_side_info side;             /* device pointers, as for the per-frame calls */
+  rvpf_planes recon;               /* deblocked in place */
+  rvpf_planes ref_out;             /* the reference frame */
+  rvpf_planes display;             /* data[0] == NULL: the frame is not shown */
+} rvpf_batch_frame;
+
+typedef struct rvpf_batch rvpf_batch;
+
+/* Launch shape of one batched stage. */
+#define RVPF_TUNE_DEBLOCK 0
+#define RVPF_TUNE_CDEF 1
+#define RVPF_TUNE_LR 2
+#define RVPF_TUNE_GRAIN_PREPARE 3
+#define RVPF_TUNE_GRAIN_APPLY 4
+#define RVPF_TUNE_COPY 5
+#define RVPF_TUNE_COUNT 6
+typedef struct rvpf_tile_config {
+  int32_t shape;            /* tile shape: deblock 0..2 = 64x4, 32x8, 16x16 edges;
+                               grain apply 0..2 = 64x4, 32x8, 16x16 chroma samples; others 0 */
+  int32_t workgroups_per_cu;/* persistent grid = CUs x this; 0 = one workgroup per grab */
+  int32_t chunk;            /* tiles taken per queue grab, >= 1 */
+  float us;                 /* measured stage time of this choice (output; 0 = not measured) */
+} rvpf_tile_config;
+
+/* Validates every frame as rvpf_postfilter_frame would (the message of a
+ * failure names the frame) and allocates the batch's device scratch. */
+int rvpf_batch_create(rvpf_ctx *ctx, const rvpf_batch_frame *frames, int32_t n, rvpf_batch **out);
+void rvpf_batch_destroy(rvpf_batch *b);
+/*
+ * Autotuner: times each stage over the batch's own frames for every
+ * candidate tile shape and grid, keeps the fastest, and remembers the
+ * choice in the context for batches of the same shape (frame sizes, bit
+ * depths, sample sizes and enabled filters), which then skip the timing.
+ * force != 0 times again. The timing runs the stages on the batch's
+ * buffers: deblocking works in place, so refill `recon` before the next
+ * rvpf_batch_run. Synchronous.
+ */
+int rvpf_batch_autotune(rvpf_ctx *ctx, rvpf_batch *b, int force, void *stream);
+int rvpf_batch_get_tiles(const rvpf_batch *b, rvpf_tile_config out[RVPF_TUNE_COUNT]);
+int rvpf_batch_set_tiles(rvpf_batch *b, const rvpf_tile_config in[RVPF_TUNE_COUNT]);
+/* Enqueues the whole batch (one graph launch; the graph is captured on
+ * the first run after a tile change). */
+int rvpf_batch_run(rvpf_ctx *ctx, rvpf_batch *b, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

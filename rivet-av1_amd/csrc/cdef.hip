@@ -254,15 +254,15 @@ __device__ __forceinline__ void filter_block_pairs(const uint32_t *w, int t0, T 
   }
 }
 
+/* One 64x64 filter block (tile_id in raster order); words holds the
+ * tile P, then the shifted tile S. */
 template <typename T>
-__global__ __launch_bounds__(256) void cdef_kernel(CdefLaunch a) {
-  __shared__ uint32_t words[2 * SOFF]; /* tile P, then the shifted tile S */
+__device__ __forceinline__ void cdef_tile(const CdefLaunch &a, int tile_id, uint32_t *words) {
   uint16_t *tile = reinterpret_cast<uint16_t *>(words);
   uint16_t *shifted = reinterpret_cast<uint16_t *>(words + SOFF);
   const Geom &g = a.g;
   const int tid = threadIdx.x;
   const int tx = tid & 63, ty = tid >> 6;
-  const int tile_id = xcd_remap(blockIdx.x, gridDim.x);
   const int fbx = tile_id % a.fb_cols, fby = tile_id / a.fb_cols;
   const int cs = g.bd - 8;
   int idx = -1;
@@ -367,12 +367,20 @@ __global__ __launch_bounds__(256) void cdef_kernel(CdefLaunch a) {
   }
 }
 
-}  // namespace
+template <typename T>
+__global__ __launch_bounds__(256) void cdef_kernel(CdefLaunch a) {
+  __shared__ uint32_t words[2 * SOFF];
+  cdef_tile<T>(a, xcd_remap(blockIdx.x, gridDim.x), words);
+}
 
-int launch_cdef(const Geom &g, const CdefHdr &h, const rvpf_mi *mi, int mi_stride,
-                const int8_t *cdef_idx, int cdef_stride, int enabled, const PlanePtr in[3],
-                PlanePtr out[3], int bps, hipStream_t st) {
-  CdefLaunch a;
+template <typename T>
+__global__ __launch_bounds__(256) void cdef_batch(BatchWork w) {
+  __shared__ uint32_t words[2 * SOFF];
+  batch_loop(w, [&](int f, int j) { cdef_tile<T>(static_cast<const CdefLaunch *>(w.rec)[f], j, words); });
+}
+
+int plan(const Geom &g, const CdefHdr &h, const rvpf_mi *mi, int mi_stride, const int8_t *cdef_idx,
+         int cdef_stride, int enabled, const PlanePtr in[3], const PlanePtr out[3], CdefLaunch &a) {
   a.g = g;
   a.h = h;
   a.mi = mi;
@@ -385,8 +393,32 @@ int launch_cdef(const Geom &g, const CdefHdr &h, const rvpf_mi *mi, int mi_strid
   }
   a.enabled = enabled;
   a.fb_cols = (g.mi_cols + 15) / 16;
-  const int fb_rows = (g.mi_rows + 15) / 16;
-  const int n = a.fb_cols * fb_rows;
+  return a.fb_cols * ((g.mi_rows + 15) / 16);
+}
+
+}  // namespace
+
+size_t cdef_record_size() { return sizeof(CdefLaunch); }
+
+int cdef_plan(const Geom &g, const CdefHdr &h, const rvpf_mi *mi, int mi_stride, const int8_t *cdef_idx,
+              int cdef_stride, int enabled, const PlanePtr in[3], const PlanePtr out[3], void *rec) {
+  return plan(g, h, mi, mi_stride, cdef_idx, cdef_stride, enabled, in, out, *static_cast<CdefLaunch *>(rec));
+}
+
+int launch_cdef_batch(const BatchWork &w, const StagePlan &sp, int bps, hipStream_t st) {
+  if (w.total == 0) return 0;
+  if (bps == 1)
+    cdef_batch<uint8_t><<<sp.grid, 256, 0, st>>>(w);
+  else
+    cdef_batch<uint16_t><<<sp.grid, 256, 0, st>>>(w);
+  return (int)hipGetLastError();
+}
+
+int launch_cdef(const Geom &g, const CdefHdr &h, const rvpf_mi *mi, int mi_stride,
+                const int8_t *cdef_idx, int cdef_stride, int enabled, const PlanePtr in[3],
+                PlanePtr out[3], int bps, hipStream_t st) {
+  CdefLaunch a;
+  const int n = plan(g, h, mi, mi_stride, cdef_idx, cdef_stride, enabled, in, out, a);
   if (bps == 1)
     cdef_kernel<uint8_t><<<n, 256, 0, st>>>(a);
   else

@@ -16,10 +16,9 @@ struct CopyLaunch {
   int blk_start[4];
 };
 
-/* 64 x 4 samples per block; rows of 64 lanes -> coalesced */
+/* Block b: 64 x 4 samples; rows of 64 lanes -> coalesced. */
 template <typename T>
-__global__ __launch_bounds__(256) void copy_kernel(CopyLaunch a) {
-  const int b = blockIdx.x;
+__device__ __forceinline__ void copy_block(const CopyLaunch &a, int b) {
   int p = 0;
   if (b >= a.blk_start[1]) p = 1;
   if (b >= a.blk_start[2]) p = 2;
@@ -31,11 +30,18 @@ __global__ __launch_bounds__(256) void copy_kernel(CopyLaunch a) {
       reinterpret_cast<const T *>(a.in[p].data)[(int64_t)y * a.in[p].stride + x];
 }
 
-}  // namespace
+template <typename T>
+__global__ __launch_bounds__(256) void copy_kernel(CopyLaunch a) {
+  copy_block<T>(a, blockIdx.x);
+}
 
-int launch_copy_planes(const Geom &g, const PlanePtr in[3], PlanePtr out[3], const int w[3],
-                       const int h[3], int bps, hipStream_t st) {
-  CopyLaunch a;
+template <typename T>
+__global__ __launch_bounds__(256) void copy_batch(BatchWork w) {
+  batch_loop(w, [&](int f, int j) { copy_block<T>(static_cast<const CopyLaunch *>(w.rec)[f], j); });
+}
+
+int plan(const Geom &g, const PlanePtr in[3], const PlanePtr out[3], const int w[3], const int h[3],
+         CopyLaunch &a) {
   int total = 0;
   for (int p = 0; p < 3; p++) {
     const bool on = p < g.nplanes && w[p] > 0 && h[p] > 0;
@@ -48,6 +54,31 @@ int launch_copy_planes(const Geom &g, const PlanePtr in[3], PlanePtr out[3], con
     if (on) total += a.blk_x[p] * ((h[p] + 3) / 4);
   }
   a.blk_start[3] = total;
+  return total;
+}
+
+}  // namespace
+
+size_t copy_record_size() { return sizeof(CopyLaunch); }
+
+int copy_plan(const Geom &g, const PlanePtr in[3], const PlanePtr out[3], const int w[3], const int h[3],
+              int, void *rec) {
+  return plan(g, in, out, w, h, *static_cast<CopyLaunch *>(rec));
+}
+
+int launch_copy_batch(const BatchWork &w, const StagePlan &sp, int bps, hipStream_t st) {
+  if (w.total == 0) return 0;
+  if (bps == 1)
+    copy_batch<uint8_t><<<sp.grid, 256, 0, st>>>(w);
+  else
+    copy_batch<uint16_t><<<sp.grid, 256, 0, st>>>(w);
+  return (int)hipGetLastError();
+}
+
+int launch_copy_planes(const Geom &g, const PlanePtr in[3], PlanePtr out[3], const int w[3],
+                       const int h[3], int bps, hipStream_t st) {
+  CopyLaunch a;
+  const int total = plan(g, in, out, w, h, a);
   if (total == 0) return 0;
   if (bps == 1)
     copy_kernel<uint8_t><<<total, 256, 0, st>>>(a);

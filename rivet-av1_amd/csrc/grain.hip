@@ -219,9 +219,14 @@ __device__ void ar_both(int16_t *tl, int16_t *tcb, int16_t *tcr, const GrainHdr 
   }
 }
 
-__global__ __launch_bounds__(128) void grain_prepare_kernel(GrainHdr h, GrainScratch sc) {
-  __shared__ int16_t tl[TH * TW], tcb[TH * TW], tcr[TH * TW];
-  __shared__ int16_t gauss[2048];
+struct PrepShared {
+  int16_t tl[TH * TW], tcb[TH * TW], tcr[TH * TW];
+  int16_t gauss[2048];
+};
+
+/* Templates, LUTs and block offsets of one frame (128 threads). */
+__device__ __forceinline__ void grain_prepare_body(const GrainHdr &h, const GrainScratch &sc, PrepShared &sm) {
+  int16_t *tl = sm.tl, *tcb = sm.tcb, *tcr = sm.tcr, *gauss = sm.gauss;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int i = threadIdx.x; i < 2048; i += 128) gauss[i] = AV1_GAUSSIAN_SEQUENCE[i];
   __syncthreads();
@@ -335,19 +340,18 @@ __device__ __forceinline__ int scale_lut(const int16_t *lut, int bd, int index) 
   return start + round2((end - start) * rem, sh);
 }
 
-template <typename TI, typename TO>
-__global__ __launch_bounds__(256) void grain_apply_kernel(GrainHdr h, GrainScratch sc, PlanePtr i0,
-                                                          PlanePtr i1, PlanePtr i2, PlanePtr o0,
-                                                          PlanePtr o1, PlanePtr o2) {
-  __shared__ int16_t lut[3 * 256];
-  for (int i = threadIdx.x; i < h.nplanes * 256; i += blockDim.x) lut[i] = sc.lut[i];
+/* Tile tile_id of EX x (256 / EX) chroma positions (raster order). */
+template <typename TI, typename TO, int EX>
+__device__ __forceinline__ void grain_apply_tile(const GrainHdr &h, const GrainScratch &sc, const PlanePtr &i0,
+                                                 const PlanePtr &i1, const PlanePtr &i2, const PlanePtr &o0,
+                                                 const PlanePtr &o1, const PlanePtr &o2, int tile_id, int16_t *lut) {
+  for (int i = threadIdx.x; i < h.nplanes * 256; i += 256) lut[i] = sc.lut[i];
   __syncthreads();
   const int ssx = h.nplanes > 1 ? h.ssx : 0, ssy = h.nplanes > 1 ? h.ssy : 0;
   const int cw = (h.w + ssx) >> ssx, ch = (h.h + ssy) >> ssy;
-  const int tile_id = xcd_remap(blockIdx.x, gridDim.x);
-  const int nbx = (cw + 63) / 64;
-  const int cx = (tile_id % nbx) * 64 + (threadIdx.x & 63);
-  const int cy = (tile_id / nbx) * 4 + (threadIdx.x >> 6);
+  const int nbx = (cw + EX - 1) / EX;
+  const int cx = (tile_id % nbx) * EX + (threadIdx.x % EX);
+  const int cy = (tile_id / nbx) * (256 / EX) + (threadIdx.x / EX);
   if (cx >= cw || cy >= ch) return;
   const int bd = h.bd;
   const int gmin = -(128 << (bd - 8)), gmax = (256 << (bd - 8)) - 1 - (128 << (bd - 8));
@@ -404,6 +408,64 @@ __global__ __launch_bounds__(256) void grain_apply_kernel(GrainHdr h, GrainScrat
   }
 }
 
+__global__ __launch_bounds__(128) void grain_prepare_kernel(GrainHdr h, GrainScratch sc) {
+  __shared__ PrepShared sm;
+  grain_prepare_body(h, sc, sm);
+}
+
+template <typename TI, typename TO>
+__global__ __launch_bounds__(256) void grain_apply_kernel(GrainHdr h, GrainScratch sc, PlanePtr i0,
+                                                          PlanePtr i1, PlanePtr i2, PlanePtr o0,
+                                                          PlanePtr o1, PlanePtr o2) {
+  __shared__ int16_t lut[3 * 256];
+  grain_apply_tile<TI, TO, 64>(h, sc, i0, i1, i2, o0, o1, o2, xcd_remap(blockIdx.x, gridDim.x), lut);
+}
+
+struct PrepRecord {
+  GrainHdr h;
+  GrainScratch s;
+};
+
+struct ApplyRecord {
+  GrainHdr h;
+  GrainScratch s;
+  PlanePtr in[3], out[3];
+};
+
+__global__ __launch_bounds__(128) void grain_prepare_batch(BatchWork w) {
+  __shared__ PrepShared sm;
+  batch_loop(w, [&](int f, int) {
+    const PrepRecord &r = static_cast<const PrepRecord *>(w.rec)[f];
+    grain_prepare_body(r.h, r.s, sm);
+  });
+}
+
+template <typename T, int EX>
+__global__ __launch_bounds__(256) void grain_apply_batch(BatchWork w) {
+  __shared__ int16_t lut[3 * 256];
+  batch_loop(w, [&](int f, int j) {
+    const ApplyRecord &r = static_cast<const ApplyRecord *>(w.rec)[f];
+    grain_apply_tile<T, T, EX>(r.h, r.s, r.in[0], r.in[1], r.in[2], r.out[0], r.out[1], r.out[2], j, lut);
+  });
+}
+
+constexpr int SHAPE_EX[GRAIN_SHAPES] = {64, 32, 16};
+
+int apply_tiles(const GrainHdr &h, int ex) {
+  const int ssx = h.nplanes > 1 ? h.ssx : 0, ssy = h.nplanes > 1 ? h.ssy : 0;
+  const int cw = (h.w + ssx) >> ssx, ch = (h.h + ssy) >> ssy;
+  return ((cw + ex - 1) / ex) * ((ch + 256 / ex - 1) / (256 / ex));
+}
+
+template <typename T>
+void launch_apply_shape(const BatchWork &w, const StagePlan &sp, hipStream_t st) {
+  switch (sp.shape) {
+    case 1: grain_apply_batch<T, 32><<<sp.grid, 256, 0, st>>>(w); break;
+    case 2: grain_apply_batch<T, 16><<<sp.grid, 256, 0, st>>>(w); break;
+    default: grain_apply_batch<T, 64><<<sp.grid, 256, 0, st>>>(w); break;
+  }
+}
+
 }  // namespace
 
 int launch_grain_prepare(const GrainHdr &h, const GrainScratch &s, hipStream_t st) {
@@ -411,11 +473,47 @@ int launch_grain_prepare(const GrainHdr &h, const GrainScratch &s, hipStream_t s
   return (int)hipGetLastError();
 }
 
+size_t grain_prep_record_size() { return sizeof(PrepRecord); }
+
+int grain_prep_plan(const GrainHdr &h, const GrainScratch &s, void *rec) {
+  PrepRecord &r = *static_cast<PrepRecord *>(rec);
+  r.h = h;
+  r.s = s;
+  return 1;
+}
+
+int launch_grain_prepare_batch(const BatchWork &w, const StagePlan &sp, hipStream_t st) {
+  if (w.total == 0) return 0;
+  grain_prepare_batch<<<sp.grid, 128, 0, st>>>(w);
+  return (int)hipGetLastError();
+}
+
+size_t grain_apply_record_size() { return sizeof(ApplyRecord); }
+
+int grain_apply_plan(const GrainHdr &h, const GrainScratch &s, const PlanePtr in[3], const PlanePtr out[3],
+                     int shape, void *rec) {
+  ApplyRecord &r = *static_cast<ApplyRecord *>(rec);
+  r.h = h;
+  r.s = s;
+  for (int p = 0; p < 3; p++) {
+    r.in[p] = in[p];
+    r.out[p] = out[p];
+  }
+  return apply_tiles(h, SHAPE_EX[shape < 0 || shape >= GRAIN_SHAPES ? 0 : shape]);
+}
+
+int launch_grain_apply_batch(const BatchWork &w, const StagePlan &sp, int bps, hipStream_t st) {
+  if (w.total == 0) return 0;
+  if (bps == 1)
+    launch_apply_shape<uint8_t>(w, sp, st);
+  else
+    launch_apply_shape<uint16_t>(w, sp, st);
+  return (int)hipGetLastError();
+}
+
 int launch_grain_apply(const GrainHdr &h, const GrainScratch &s, const PlanePtr in[3],
                        PlanePtr out[3], int bps, hipStream_t st) {
-  const int ssx = h.nplanes > 1 ? h.ssx : 0, ssy = h.nplanes > 1 ? h.ssy : 0;
-  const int cw = (h.w + ssx) >> ssx, ch = (h.h + ssy) >> ssy;
-  const int grid = ((cw + 63) / 64) * ((ch + 3) / 4);
+  const int grid = apply_tiles(h, 64);
   if (bps == 1)
     grain_apply_kernel<uint8_t, uint8_t><<<grid, 256, 0, st>>>(h, s, in[0], in[1], in[2], out[0], out[1], out[2]);
   else

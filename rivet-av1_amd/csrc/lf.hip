@@ -244,14 +244,12 @@ __device__ __forceinline__ int edge_decision(const LfLaunch &a, int plane, int p
   return filter_size;
 }
 
-template <typename T, int PASS>
-__global__ __launch_bounds__(256) void lf_pass_kernel(LfLaunch a) {
-  int lb;
-  const int gp = xcd_segment(blockIdx.x, a.blk_start, a.nseg, &lb);
-  if (gp >= a.nseg || lb >= a.blk_n[gp]) return; /* padding */
+/* Block lb of grid plane gp: EX x (256 / EX) edges, one per thread. */
+template <typename T, int PASS, int EX>
+__device__ __forceinline__ void lf_block(const LfLaunch &a, int gp, int lb) {
   const int plane = a.plane_id[gp];
-  const int ex = (lb % a.blk_x[gp]) * 64 + (threadIdx.x & 63);
-  const int ey = (lb / a.blk_x[gp]) * 4 + (threadIdx.x >> 6);
+  const int ex = (lb % a.blk_x[gp]) * EX + (threadIdx.x % EX);
+  const int ey = (lb / a.blk_x[gp]) * (256 / EX) + (threadIdx.x / EX);
   const Geom &g = a.g;
   const int ssx = plane ? g.ssx : 0, ssy = plane ? g.ssy : 0;
   if ((ey << ssy) >= g.mi_rows || (ex << ssx) >= g.mi_cols) return;
@@ -332,26 +330,47 @@ __global__ __launch_bounds__(256) void lf_pass_kernel(LfLaunch a) {
   }
 }
 
-}  // namespace
+template <typename T, int PASS>
+__global__ __launch_bounds__(256) void lf_pass_kernel(LfLaunch a) {
+  int lb;
+  const int gp = xcd_segment(blockIdx.x, a.blk_start, a.nseg, &lb);
+  if (gp >= a.nseg || lb >= a.blk_n[gp]) return; /* padding */
+  lf_block<T, PASS, 64>(a, gp, lb);
+}
 
-int launch_deblock(const Geom &g, const LfHdr &h, const LfSide &s, PlanePtr planes[3], int bps,
-                   hipStream_t st) {
-  LfLaunch a;
+/* Batched: item j of a frame is block j of its (unpadded) grid. */
+template <typename T, int PASS, int EX>
+__global__ __launch_bounds__(256) void lf_pass_batch(BatchWork w) {
+  batch_loop(w, [&](int f, int j) {
+    const LfLaunch &a = static_cast<const LfLaunch *>(w.rec)[f];
+    int gp = 0;
+    while (gp + 1 < a.nseg && j >= a.blk_start[gp + 1]) gp++;
+    const int lb = j - a.blk_start[gp];
+    if (lb < a.blk_n[gp]) lf_block<T, PASS, EX>(a, gp, lb);
+  });
+}
+
+/* Fills the launch record; blocks of EX x (256 / EX) edges, each plane's
+ * first block padded to a multiple of 8 when pad8 (per-frame launch with
+ * the XCD segment mapping). Returns the number of blocks. */
+int plan(const Geom &g, const LfHdr &h, const LfSide &s, const PlanePtr planes[3], int ex, bool pad8,
+         LfLaunch &a) {
   a.g = g;
   a.h = h;
   a.s = s;
   int n = 0, total = 0;
+  const int ey = 256 / ex;
+  for (int p = 0; p < 3; p++) a.planes[p] = planes[p];
   for (int p = 0; p < g.nplanes; p++) {
-    a.planes[p] = planes[p];
     if (!(p == 0 || h.level[1 + p] != 0)) continue;
     const int ssx = p ? g.ssx : 0, ssy = p ? g.ssy : 0;
     const int cols4 = (g.mi_cols + (1 << ssx) - 1) >> ssx;
     const int rows4 = (g.mi_rows + (1 << ssy) - 1) >> ssy;
     a.plane_id[n] = p;
-    a.blk_x[n] = (cols4 + 63) / 64;
-    a.blk_n[n] = a.blk_x[n] * ((rows4 + 3) / 4);
+    a.blk_x[n] = (cols4 + ex - 1) / ex;
+    a.blk_n[n] = a.blk_x[n] * ((rows4 + ey - 1) / ey);
     a.blk_start[n] = total;
-    total += (a.blk_n[n] + 7) & ~7;
+    total += pad8 ? (a.blk_n[n] + 7) & ~7 : a.blk_n[n];
     n++;
   }
   a.nseg = n;
@@ -362,6 +381,47 @@ int launch_deblock(const Geom &g, const LfHdr &h, const LfSide &s, PlanePtr plan
     a.blk_start[i] = total;
   }
   a.blk_start[3] = total;
+  return total;
+}
+
+constexpr int SHAPE_EX[LF_SHAPES] = {64, 32, 16};
+
+template <typename T, int EX>
+void launch_batch_t(const BatchWork &wv, const BatchWork &wh, const StagePlan &sp, hipStream_t st) {
+  if (wv.total > 0) lf_pass_batch<T, 0, EX><<<sp.grid, 256, 0, st>>>(wv);
+  if (wh.total > 0) lf_pass_batch<T, 1, EX><<<sp.grid, 256, 0, st>>>(wh);
+}
+
+template <typename T>
+void launch_batch_shape(const BatchWork &wv, const BatchWork &wh, const StagePlan &sp, hipStream_t st) {
+  switch (sp.shape) {
+    case 1: launch_batch_t<T, 32>(wv, wh, sp, st); break;
+    case 2: launch_batch_t<T, 16>(wv, wh, sp, st); break;
+    default: launch_batch_t<T, 64>(wv, wh, sp, st); break;
+  }
+}
+
+}  // namespace
+
+size_t lf_record_size() { return sizeof(LfLaunch); }
+
+int lf_plan(const Geom &g, const LfHdr &h, const LfSide &s, const PlanePtr planes[3], int shape, void *rec) {
+  return plan(g, h, s, planes, SHAPE_EX[shape < 0 || shape >= LF_SHAPES ? 0 : shape], false,
+              *static_cast<LfLaunch *>(rec));
+}
+
+int launch_deblock_batch(const BatchWork &wv, const BatchWork &wh, const StagePlan &sp, int bps, hipStream_t st) {
+  if (bps == 1)
+    launch_batch_shape<uint8_t>(wv, wh, sp, st);
+  else
+    launch_batch_shape<uint16_t>(wv, wh, sp, st);
+  return (int)hipGetLastError();
+}
+
+int launch_deblock(const Geom &g, const LfHdr &h, const LfSide &s, PlanePtr planes[3], int bps,
+                   hipStream_t st) {
+  LfLaunch a;
+  const int total = plan(g, h, s, planes, 64, true, a);
   if (total == 0) return 0;
   if (bps == 1) {
     lf_pass_kernel<uint8_t, 0><<<total, 256, 0, st>>>(a);

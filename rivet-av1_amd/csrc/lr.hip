@@ -65,16 +65,20 @@ constexpr A2Table make_a2() {
 }
 constexpr A2Table SGR_A2 = make_a2();
 
-template <typename T>
-__global__ __launch_bounds__(NT) void lr_kernel(LrLaunch a) {
-  __shared__ uint16_t win[WH * WW];
-  __shared__ int16_t inter[WH * SW];      /* Wiener horizontal output; SGR: A */
-  __shared__ int32_t bbuf[(64 + 2) * AW]; /* SGR: B */
+/* LDS of one workgroup. */
+struct LrShared {
+  uint16_t win[WH * WW];
+  int16_t inter[WH * SW];      /* Wiener horizontal output; SGR: A */
+  int32_t bbuf[(64 + 2) * AW]; /* SGR: B */
+};
 
+/* Slice lb of grid plane gp (stripe-major). */
+template <typename T>
+__device__ __forceinline__ void lr_slice(const LrLaunch &a, int gp, int lb, LrShared &sm) {
+  uint16_t *win = sm.win;
+  int16_t *inter = sm.inter;
+  int32_t *bbuf = sm.bbuf;
   const Geom &g = a.g;
-  int lb;
-  const int gp = xcd_segment(blockIdx.x, a.blk_start, a.nseg, &lb);
-  if (gp >= a.nseg || lb >= a.nslices[gp] * a.nstripes) return; /* padding */
   const int plane = a.plane_id[gp];
   const int st = lb / a.nslices[gp], sl = lb - st * a.nslices[gp];
   const int ssx = plane ? g.ssx : 0, ssy = plane ? g.ssy : 0;
@@ -288,11 +292,29 @@ __global__ __launch_bounds__(NT) void lr_kernel(LrLaunch a) {
   }
 }
 
-}  // namespace
+template <typename T>
+__global__ __launch_bounds__(NT) void lr_kernel(LrLaunch a) {
+  __shared__ LrShared sm;
+  int lb;
+  const int gp = xcd_segment(blockIdx.x, a.blk_start, a.nseg, &lb);
+  if (gp >= a.nseg || lb >= a.nslices[gp] * a.nstripes) return; /* padding */
+  lr_slice<T>(a, gp, lb, sm);
+}
 
-int launch_lr(const Geom &g, const LrHdr &h, const PlanePtr deblocked[3], const PlanePtr cdef[3],
-              PlanePtr out[3], int bps, hipStream_t st) {
-  LrLaunch a;
+/* Batched: item j of a frame is slice j of its (unpadded) grid. */
+template <typename T>
+__global__ __launch_bounds__(NT) void lr_batch(BatchWork w) {
+  __shared__ LrShared sm;
+  batch_loop(w, [&](int f, int j) {
+    const LrLaunch &a = static_cast<const LrLaunch *>(w.rec)[f];
+    int gp = 0;
+    while (gp + 1 < a.nseg && j >= a.blk_start[gp + 1]) gp++;
+    lr_slice<T>(a, gp, j - a.blk_start[gp], sm);
+  });
+}
+
+int plan(const Geom &g, const LrHdr &h, const PlanePtr deblocked[3], const PlanePtr cdef[3],
+         const PlanePtr out[3], bool pad8, LrLaunch &a) {
   a.g = g;
   a.h = h;
   for (int p = 0; p < 3; p++) {
@@ -310,7 +332,7 @@ int launch_lr(const Geom &g, const LrHdr &h, const PlanePtr deblocked[3], const 
     a.plane_id[n] = p;
     a.nslices[n] = (area_w + sw - 1) / sw;
     a.blk_start[n] = total;
-    total += (a.nslices[n] * a.nstripes + 7) & ~7;
+    total += pad8 ? (a.nslices[n] * a.nstripes + 7) & ~7 : a.nslices[n] * a.nstripes;
     n++;
   }
   a.nseg = n;
@@ -320,6 +342,31 @@ int launch_lr(const Geom &g, const LrHdr &h, const PlanePtr deblocked[3], const 
     a.blk_start[i] = total;
   }
   a.blk_start[3] = total;
+  return total;
+}
+
+}  // namespace
+
+size_t lr_record_size() { return sizeof(LrLaunch); }
+
+int lr_plan(const Geom &g, const LrHdr &h, const PlanePtr deblocked[3], const PlanePtr cdef[3],
+            const PlanePtr out[3], void *rec) {
+  return plan(g, h, deblocked, cdef, out, false, *static_cast<LrLaunch *>(rec));
+}
+
+int launch_lr_batch(const BatchWork &w, const StagePlan &sp, int bps, hipStream_t st) {
+  if (w.total == 0) return 0;
+  if (bps == 1)
+    lr_batch<uint8_t><<<sp.grid, NT, 0, st>>>(w);
+  else
+    lr_batch<uint16_t><<<sp.grid, NT, 0, st>>>(w);
+  return (int)hipGetLastError();
+}
+
+int launch_lr(const Geom &g, const LrHdr &h, const PlanePtr deblocked[3], const PlanePtr cdef[3],
+              PlanePtr out[3], int bps, hipStream_t st) {
+  LrLaunch a;
+  const int total = plan(g, h, deblocked, cdef, out, true, a);
   if (total == 0) return 0;
   if (bps == 1)
     lr_kernel<uint8_t><<<total, NT, 0, st>>>(a);

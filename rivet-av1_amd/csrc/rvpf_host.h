@@ -1,0 +1,75 @@
+/*
+ * rvpf_host.h — host-side state and helpers shared by the C ABI files
+ * (rvpf_host.hip: per-frame calls; batch.hip: the batched path).
+ */
+#pragma once
+
+#include <stdint.h>
+
+#include <vector>
+
+#include "rvpf_internal.h"
+
+struct ProfMark {
+  int stage;
+  hipEvent_t a, b;
+};
+
+/* Tile choices of one batch shape, remembered by the context. */
+struct TuneEntry {
+  uint64_t key;
+  rvpf_tile_config cfg[RVPF_TUNE_COUNT];
+};
+
+/* Film grain scratch is double-buffered: the templates of frame n+1 are
+ * built on the side stream while frame n still filters. Slot s is rebuilt
+ * only after the apply that last read it (ev_done[s]); the apply waits for
+ * its templates (ev_ready[s]). */
+struct rvpf_ctx {
+  int device;
+  int cus;                          /* compute units of the device */
+  int profiling;
+  std::vector<ProfMark> marks;      /* recorded, not yet read */
+  std::vector<hipEvent_t> free_evs; /* timing events for reuse */
+  hipStream_t side;
+  hipEvent_t ev_ready[2], ev_done[2];
+  int grain_slot;
+  void *cdef_buf;
+  size_t cdef_bytes;
+  rvpf::GrainScratch grain[2];
+  std::vector<TuneEntry> tune_cache;
+  char err[256];
+};
+
+namespace rvpf {
+namespace host {
+
+int fail(rvpf_ctx *ctx, int code, const char *msg);
+int hip_fail(rvpf_ctx *ctx, hipError_t e, const char *where);
+int nplanes_of(const rvpf_frame_params *p);
+bool lf_on(const rvpf_frame_params *p);
+bool cdef_on(const rvpf_frame_params *p);
+bool lr_plane_on(const rvpf_frame_params *p, int pl);
+bool lr_on(const rvpf_frame_params *p);
+bool grain_on(const rvpf_frame_params *p);
+/* the mode-info area / the display crop of a plane */
+void area(const rvpf_frame_params *p, int pl, int *w, int *h);
+void crop(const rvpf_frame_params *p, int pl, int *w, int *h);
+int validate_params(const rvpf_frame_params *p, char *msg);
+int validate_side(const rvpf_frame_params *p, const rvpf_side_info *s, char *msg);
+/* margin: extra rows/cols beyond the area the stage reads; pairs: the
+ * stage stores two adjacent samples at once at even columns */
+int validate_planes(const rvpf_frame_params *p, const rvpf_planes *pl, const char *name, int use_crop, int margin,
+                    char *msg, bool pairs = false);
+Geom make_geom(const rvpf_frame_params *p);
+void to_ptrs(const rvpf_planes *pl, PlanePtr out[3]);
+LfHdr make_lf(const rvpf_frame_params *p);
+CdefHdr make_cdef(const rvpf_frame_params *p);
+LrHdr make_lr(const rvpf_frame_params *p, const rvpf_side_info *s);
+GrainHdr make_grain(const rvpf_frame_params *p);
+int check_launch(rvpf_ctx *ctx, int rc, const char *what);
+/* bytes of the offsets table of a grain scratch for a w x h frame */
+void grain_offsets_dims(int w, int h, int *stripes, int *blocks);
+
+}  // namespace host
+}  // namespace rvpf

@@ -16,36 +16,16 @@
 #include <vector>
 
 #include "av1_tables.h"
-#include "rvpf_internal.h"
+#include "rvpf_host.h"
 
 using namespace rvpf;
 
-/* Film grain scratch is double-buffered: the templates of frame n+1 are
- * built on the side stream while frame n still filters. Slot s is rebuilt
- * only after the apply that last read it (ev_done[s]); the apply waits for
- * its templates (ev_ready[s]). */
-struct ProfMark {
-  int stage;
-  hipEvent_t a, b;
-};
-
-struct rvpf_ctx {
-  int device;
-  int profiling;
-  std::vector<ProfMark> marks;      /* recorded, not yet read */
-  std::vector<hipEvent_t> free_evs; /* timing events for reuse */
-  hipStream_t side;
-  hipEvent_t ev_ready[2], ev_done[2];
-  int grain_slot;
-  void *cdef_buf;
-  size_t cdef_bytes;
-  GrainScratch grain[2];
-  char err[256];
-};
-
 namespace {
-
 thread_local char g_err[256] = "";
+}  // namespace
+
+namespace rvpf {
+namespace host {
 
 int fail(rvpf_ctx *ctx, int code, const char *msg) {
   char *dst = ctx ? ctx->err : g_err;
@@ -223,10 +203,8 @@ int validate_side(const rvpf_frame_params *p, const rvpf_side_info *s, char *msg
   return RVPF_OK;
 }
 
-/* margin: extra rows/cols beyond `need` the stage reads; pairs: the
- * stage stores two adjacent samples at once at even columns */
 int validate_planes(const rvpf_frame_params *p, const rvpf_planes *pl, const char *name, int use_crop, int margin,
-                    char *msg, bool pairs = false) {
+                    char *msg, bool pairs) {
   if (!pl) {
     snprintf(msg, 256, "%s planes are NULL", name);
     return RVPF_E_INVALID;
@@ -413,9 +391,14 @@ int ensure_cdef_buf(rvpf_ctx *ctx, size_t bytes) {
   return RVPF_OK;
 }
 
+void grain_offsets_dims(int w, int h, int *stripes, int *blocks) {
+  *stripes = ((h + 1) / 2 + 15) / 16 + 1;
+  *blocks = ((w + 1) / 2 + 15) / 16 + 1;
+}
+
 int ensure_grain_scratch(rvpf_ctx *ctx, int slot, int w, int h) {
-  const int stripes = ((h + 1) / 2 + 15) / 16 + 1;
-  const int blocks = ((w + 1) / 2 + 15) / 16 + 1;
+  int stripes, blocks;
+  grain_offsets_dims(w, h, &stripes, &blocks);
   GrainScratch &g = ctx->grain[slot];
   if (g.tmpl && stripes <= g.max_stripes && blocks <= g.max_blocks) return RVPF_OK;
   if (g.offsets) {
@@ -534,7 +517,10 @@ int run_output(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_planes *ref
   return check_launch(ctx, launch_copy_planes(g, in, out, w, h, ref->bytes_per_sample, st), "display copy");
 }
 
-}  // namespace
+}  // namespace host
+}  // namespace rvpf
+
+using namespace rvpf::host;
 
 extern "C" {
 
@@ -545,6 +531,7 @@ int rvpf_ctx_create(rvpf_ctx **out) {
   rvpf_ctx *ctx = new (std::nothrow) rvpf_ctx();
   if (!ctx) return fail(nullptr, RVPF_E_NOMEM, "cannot allocate the context");
   hipError_t e = hipGetDevice(&ctx->device);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&ctx->cus, hipDeviceAttributeMultiprocessorCount, ctx->device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking);
   for (int i = 0; i < 2; i++) {
     if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_ready[i], hipEventDisableTiming);

@@ -115,6 +115,106 @@ __device__ __forceinline__ int xcd_segment(int b, const int *pad, int nseg, int 
   return nseg; /* not reached for b < pad[nseg] */
 }
 
+/* --- batched launches ------------------------------------------------
+ * One launch of a stage over many frames. Every frame contributes `items`
+ * work items (tiles) described by its own launch record; the records sit
+ * in a device array and start[f] is the first item of frame f (prefix
+ * sums, start[n] = total). A fixed grid of workgroups takes `chunk` items
+ * at a time from a device counter until all are taken. */
+struct BatchWork {
+  const void *rec;  /* device: n launch records of the stage's type */
+  const int *start; /* device: n + 1 prefix sums of work items */
+  int n, total;
+  int chunk;        /* items per grab */
+  int *queue;       /* device, 2 ints: next item, workgroups finished */
+};
+
+/* Frame of item i: the last f with start[f] <= i. */
+__device__ __forceinline__ int batch_frame(const int *start, int n, int i) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= i)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+/* The grab loop of a batched kernel: body(f, j) runs item j of frame f
+ * with the whole workgroup (f and j are wave-uniform). The queue is
+ * re-armed by the last workgroup to finish, so the next launch of the same
+ * work (a graph replay) starts from zero without a memset. Every workgroup
+ * leaves once the counter passes the total. */
+template <class Body>
+__device__ __forceinline__ void batch_loop(const BatchWork &w, Body &&body) {
+  __shared__ int s_item;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = atomicAdd(&w.queue[0], w.chunk);
+    __syncthreads();
+    const int i0 = s_item;
+    __syncthreads();
+    if (i0 >= w.total) break;
+    const int i1 = min(i0 + w.chunk, w.total);
+    for (int i = i0; i < i1; i++) {
+      const int f = __builtin_amdgcn_readfirstlane(batch_frame(w.start, w.n, i));
+      const int j = __builtin_amdgcn_readfirstlane(i - w.start[f]);
+      body(f, j);
+      __syncthreads(); /* the next item reuses the LDS */
+    }
+  }
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(&w.queue[1], 1) == (int)gridDim.x - 1) {
+      atomicExch(&w.queue[0], 0);
+      atomicExch(&w.queue[1], 0);
+    }
+  }
+}
+
+/* Tile shapes a batched stage can run with (the autotuner's choices). */
+struct StagePlan {
+  int shape;       /* stage-specific tile shape index */
+  int grid;        /* workgroups of the launch */
+  int chunk;       /* items per grab */
+};
+
+/* Per-stage batch interface: record size, planning one frame into a
+ * record (returns its work items), and the batched launch. */
+size_t lf_record_size();
+int lf_plan(const Geom &g, const LfHdr &h, const LfSide &s, const PlanePtr planes[3], int shape, void *rec);
+int launch_deblock_batch(const BatchWork &wv, const BatchWork &wh, const StagePlan &sp, int bps, hipStream_t st);
+
+size_t cdef_record_size();
+int cdef_plan(const Geom &g, const CdefHdr &h, const rvpf_mi *mi, int mi_stride, const int8_t *cdef_idx,
+              int cdef_stride, int enabled, const PlanePtr in[3], const PlanePtr out[3], void *rec);
+int launch_cdef_batch(const BatchWork &w, const StagePlan &sp, int bps, hipStream_t st);
+
+size_t lr_record_size();
+int lr_plan(const Geom &g, const LrHdr &h, const PlanePtr deblocked[3], const PlanePtr cdef[3],
+            const PlanePtr out[3], void *rec);
+int launch_lr_batch(const BatchWork &w, const StagePlan &sp, int bps, hipStream_t st);
+
+size_t copy_record_size();
+int copy_plan(const Geom &g, const PlanePtr in[3], const PlanePtr out[3], const int w[3], const int h[3],
+              int shape, void *rec);
+int launch_copy_batch(const BatchWork &w, const StagePlan &sp, int bps, hipStream_t st);
+
+size_t grain_prep_record_size();
+int grain_prep_plan(const GrainHdr &h, const GrainScratch &s, void *rec);
+int launch_grain_prepare_batch(const BatchWork &w, const StagePlan &sp, hipStream_t st);
+
+size_t grain_apply_record_size();
+int grain_apply_plan(const GrainHdr &h, const GrainScratch &s, const PlanePtr in[3], const PlanePtr out[3],
+                     int shape, void *rec);
+int launch_grain_apply_batch(const BatchWork &w, const StagePlan &sp, int bps, hipStream_t st);
+
+/* Number of tile shapes of each shape-tunable stage. */
+constexpr int LF_SHAPES = 3;    /* 64x4, 32x8, 16x16 edges per workgroup */
+constexpr int GRAIN_SHAPES = 3; /* 64x4, 32x8, 16x16 chroma positions per workgroup */
+constexpr int COPY_SHAPES = 1;
+
 /* Launch entry points (each returns a hipError_t cast to int). */
 int launch_deblock(const Geom &g, const LfHdr &h, const LfSide &s, PlanePtr planes[3], int bps,
                    hipStream_t st);

@@ -176,6 +176,28 @@ class SynthGeom(C.Structure):
     ]
 
 
+class RvpfBatchFrame(C.Structure):
+    _fields_ = [
+        ("params", C.c_void_p),
+        ("side", RvpfSideInfo),
+        ("recon", RvpfPlanes),
+        ("ref_out", RvpfPlanes),
+        ("display", RvpfPlanes),
+    ]
+
+
+class RvpfTileConfig(C.Structure):
+    _fields_ = [
+        ("shape", C.c_int32),
+        ("workgroups_per_cu", C.c_int32),
+        ("chunk", C.c_int32),
+        ("us", C.c_float),
+    ]
+
+
+TUNE_STAGES = ["deblock", "cdef", "lr", "grain_prepare", "grain_apply", "copy"]
+TUNE_SHAPES = [3, 1, 1, 1, 3, 1]
+
 STAGES = ["deblock", "cdef", "lr", "grain_prepare", "grain_apply", "copy"]
 
 
@@ -190,6 +212,8 @@ STRUCT_SIZES = [
     C.sizeof(RvpfLrUnit),
     C.sizeof(RvpfGrainParams),
     C.sizeof(RvpfPlanes),
+    C.sizeof(RvpfBatchFrame),
+    C.sizeof(RvpfTileConfig),
 ]
 
 
@@ -211,4 +235,10 @@ RVPF_EXPORTS = [
     "rvpf_lr",
     "rvpf_grain",
     "rvpf_postfilter_frame",
+    "rvpf_batch_create",
+    "rvpf_batch_destroy",
+    "rvpf_batch_autotune",
+    "rvpf_batch_get_tiles",
+    "rvpf_batch_set_tiles",
+    "rvpf_batch_run",
 ]

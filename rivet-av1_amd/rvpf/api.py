@@ -46,6 +46,13 @@ def lib():
         L.rvpf_grain.argtypes = [C.c_void_p, P(abi.RvpfFrameParams), P(abi.RvpfPlanes), P(abi.RvpfPlanes), C.c_void_p]
         L.rvpf_postfilter_frame.argtypes = [C.c_void_p, P(abi.RvpfFrameParams), P(abi.RvpfSideInfo),
                                             P(abi.RvpfPlanes), P(abi.RvpfPlanes), P(abi.RvpfPlanes), C.c_void_p]
+        L.rvpf_batch_create.argtypes = [C.c_void_p, P(abi.RvpfBatchFrame), C.c_int32, P(C.c_void_p)]
+        L.rvpf_batch_destroy.argtypes = [C.c_void_p]
+        L.rvpf_batch_destroy.restype = None
+        L.rvpf_batch_autotune.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
+        L.rvpf_batch_get_tiles.argtypes = [C.c_void_p, P(abi.RvpfTileConfig)]
+        L.rvpf_batch_set_tiles.argtypes = [C.c_void_p, P(abi.RvpfTileConfig)]
+        L.rvpf_batch_run.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         _lib = L
     return _lib
 
@@ -218,3 +225,58 @@ def postfilter_frame(ctx, df, recon, ref_out, display, stream=None):
     ctx.check(lib().rvpf_postfilter_frame(ctx.h, C.byref(df.frame.params), C.byref(df.side), C.byref(pr),
                                           C.byref(po), C.byref(pdsp) if pdsp is not None else None,
                                           _stream_handle(stream)))
+
+
+class Batch:
+    """rvpf_batch: many frames' buffers, filtered by one graph launch.
+
+    entries: list of (DeviceFrame, recon planes, ref_out planes, display
+    planes or None). The tensors must stay alive as long as the batch."""
+
+    def __init__(self, ctx, entries):
+        arr = (abi.RvpfBatchFrame * len(entries))()
+        self._keep = []
+        for i, (df, recon, ref, disp) in enumerate(entries):
+            arr[i].params = C.addressof(df.frame.params)
+            arr[i].side = df.side
+            arr[i].recon = planes_struct(recon, df.bps)
+            arr[i].ref_out = planes_struct(ref, df.bps)
+            if disp is not None:
+                arr[i].display = planes_struct(disp, df.bps)
+            self._keep.append((df, recon, ref, disp))
+        self.ctx = ctx
+        h = C.c_void_p()
+        ctx.check(lib().rvpf_batch_create(ctx.h, arr, len(entries), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().rvpf_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def autotune(self, force=False, stream=None):
+        self.ctx.check(lib().rvpf_batch_autotune(self.ctx.h, self.h, int(bool(force)), _stream_handle(stream)))
+
+    def get_tiles(self):
+        arr = (abi.RvpfTileConfig * len(abi.TUNE_STAGES))()
+        self.ctx.check(lib().rvpf_batch_get_tiles(self.h, arr))
+        return {n: (arr[i].shape, arr[i].workgroups_per_cu, arr[i].chunk, arr[i].us)
+                for i, n in enumerate(abi.TUNE_STAGES)}
+
+    def set_tiles(self, tiles):
+        """tiles: {stage: (shape, workgroups_per_cu, chunk)} (others unchanged)."""
+        arr = (abi.RvpfTileConfig * len(abi.TUNE_STAGES))()
+        self.ctx.check(lib().rvpf_batch_get_tiles(self.h, arr))
+        for i, n in enumerate(abi.TUNE_STAGES):
+            if n in tiles:
+                arr[i].shape, arr[i].workgroups_per_cu, arr[i].chunk = tiles[n][:3]
+        self.ctx.check(lib().rvpf_batch_set_tiles(self.h, arr))
+
+    def run(self, stream=None):
+        self.ctx.check(lib().rvpf_batch_run(self.ctx.h, self.h, _stream_handle(stream)))

@@ -77,3 +77,23 @@ def test_chain(ctx, name, kw):
     common.assert_planes(api.to_host(rec), deb, f, "full", name + " deblocked")
     common.assert_planes(api.to_host(ref_dev), ref, f, "mi", name + " reference")
     common.assert_planes(api.to_host(disp_dev), disp, f, "crop", name + " display")
+
+
+U8_CASES = [c for c in common.SMALL_CASES if c[1]["bit_depth"] == 8]
+
+
+@pytest.mark.parametrize("name,kw", U8_CASES, ids=[c[0] for c in U8_CASES])
+def test_chain_u8_planes(ctx, name, kw):
+    """8-bit frames in 8-bit sample planes (bytes_per_sample 1)."""
+    _say("case u8", name)
+    f = common.make_frame(**kw)
+    deb, ref, disp = pyoracle.postfilter_frame(f, threads=4)
+    df = api.DeviceFrame(f, bytes_per_sample=1)
+    rec = df.upload_planes(f.rec)
+    ref_dev = df.alloc_like_recon()
+    disp_dev = df.alloc_display()
+    api.postfilter_frame(ctx, df, rec, ref_dev, disp_dev)
+    ctx.sync()
+    common.assert_planes(api.to_host(rec), deb, f, "full", name + " deblocked")
+    common.assert_planes(api.to_host(ref_dev), ref, f, "mi", name + " reference")
+    common.assert_planes(api.to_host(disp_dev), disp, f, "crop", name + " display")
