@@ -10,6 +10,7 @@
  */
 #include "rvpf_oracle.h"
 
+#include <math.h>
 #include <pthread.h>
 #include <stdatomic.h>
 #include <stdlib.h>
@@ -642,7 +643,7 @@ static void box_filter_rect(const lr_ctx *c, const int32_t *win, int bd, int x0,
 
 /* restore_rect + restore_rect_from_window, src/decoder/postfilter.rs:749-853. */
 static void restore_rect(const rvpf_frame_params *f, const lr_ctx *c, int x0, int y0, int w, int h,
-                         const rvpf_lr_unit *u, uint16_t *out, int64_t ostride) {
+                         const rvpf_lr_unit *u, uint16_t *out, int64_t ostride, int ox, int oy) {
   const int ww = w + 2 * LRB, wh = h + 2 * LRB;
   int32_t *win = (int32_t *)malloc(sizeof(int32_t) * (size_t)ww * wh);
   for (int r = 0; r < wh; r++)
@@ -673,7 +674,7 @@ static void restore_rect(const rvpf_frame_params *f, const lr_ctx *c, int x0, in
       for (int cc = 0; cc < w; cc++) {
         int sum = 0;
         for (int t = 0; t < 7; t++) sum += vf[t] * inter[(r + t) * w + cc];
-        out[(int64_t)(y0 + r) * ostride + x0 + cc] = (uint16_t)clip3(0, maxv, round2(sum, round1));
+        out[(int64_t)(oy + r) * ostride + ox + cc] = (uint16_t)clip3(0, maxv, round2(sum, round1));
       }
     free(inter);
   } else {
@@ -691,7 +692,7 @@ static void restore_rect(const rvpf_frame_params *f, const lr_ctx *c, int x0, in
         int v = w1 * uu;
         v += r0 ? w0 * flt0[i * w + jj] : w0 * uu;
         v += r1 ? w2 * flt1[i * w + jj] : w2 * uu;
-        out[(int64_t)(y0 + i) * ostride + x0 + jj] = (uint16_t)clip3(0, maxv, round2(v, 11));
+        out[(int64_t)(oy + i) * ostride + ox + jj] = (uint16_t)clip3(0, maxv, round2(v, 11));
       }
     free(flt0);
     free(flt1);
@@ -735,7 +736,7 @@ static void lr_stripe_job(void *arg, int i) {
     const rvpf_lr_unit *u = &j->s->lr_units[plane][unit_row * unit_cols + uc];
     if (u->type != RVPF_RESTORE_WIENER && u->type != RVPF_RESTORE_SGRPROJ) continue;
     restore_rect(f, &c, x0, y0, x1 - x0 + 1, y1 - y0 + 1, u, j->out->data[plane],
-                 j->out->stride[plane]);
+                 j->out->stride[plane], x0, y0);
   }
 }
 
@@ -767,6 +768,328 @@ int orc_lr(const rvpf_frame_params *p, const rvpf_side_info *s, const orc_planes
       }
   parallel_map(n, threads, lr_stripe_job, j);
   free(j);
+  return 0;
+}
+
+/* ---------------------------------------------------------------------
+ * Encoder loop restoration search, src/encoder/lr.rs:64-430
+ * ------------------------------------------------------------------- */
+
+/* One stripe rectangle of a unit (lr.rs:82-127 with lr_stripe,
+ * postfilter.rs:640-661): its LrCtx and rows y0..y0+h-1. */
+typedef struct {
+  lr_ctx c;
+  int y0, h;
+} lr_rect;
+
+/* The rectangles of unit (plane, ur, uc) and its columns x0..x1; returns
+ * the count (0: the unit has no samples). */
+static int unit_rects(const rvpf_frame_params *f, const orc_planes *deb, const orc_planes *cdef, int plane,
+                      int ur, int uc, lr_rect *rects, int max_rects, int *x0_out, int *x1_out) {
+  int sub_x, sub_y;
+  plane_ss(f, plane, &sub_x, &sub_y);
+  const int plane_end_y = round2(f->frame_height, sub_y) - 1;
+  const int plane_end_x = round2(f->upscaled_width, sub_x) - 1;
+  const int unit_size = f->loop_restoration_size[plane];
+  const int unit_rows = f->lr_unit_rows[plane], unit_cols = f->lr_unit_cols[plane];
+  const int x0 = uc * unit_size;
+  const int x1 = uc + 1 == unit_cols ? plane_end_x : imin((uc + 1) * unit_size - 1, plane_end_x);
+  *x0_out = x0;
+  *x1_out = x1;
+  if (x0 > plane_end_x) return 0;
+  const int stripes = (f->frame_height + 8 + 63) / 64;
+  int n = 0;
+  for (int st = 0; st < stripes && n < max_rects; st++) {
+    const int ss = (-8 + st * 64) >> sub_y;
+    const int se = ss + (64 >> sub_y) - 1;
+    const int y0 = imax(ss, 0), y1 = imin(se, plane_end_y);
+    if (y0 > y1) continue;
+    const int luma_row = imax(st * 64 - 8, 0);
+    const int row = imin(unit_rows - 1, ((luma_row + 8) >> sub_y) / unit_size);
+    if (row != ur) continue;
+    lr_ctx c = {deb->data[plane], cdef->data[plane], deb->stride[plane], cdef->stride[plane],
+                ss, se, plane_end_x, plane_end_y};
+    rects[n].c = c;
+    rects[n].y0 = y0;
+    rects[n].h = y1 - y0 + 1;
+    n++;
+  }
+  return n;
+}
+
+/* lr_window, postfilter.rs:765-777: the rectangle plus LRB on every side. */
+static void rect_window(const lr_rect *r, int x0, int w, int32_t *win) {
+  const int ww = w + 2 * LRB, wh = r->h + 2 * LRB;
+  for (int y = 0; y < wh; y++)
+    for (int x = 0; x < ww; x++) win[y * ww + x] = lr_sample(&r->c, x0 + x - LRB, r->y0 + y - LRB);
+}
+
+/* solve, lr.rs:209-234: Gaussian elimination with partial pivoting
+ * (first largest pivot), zeros when singular. */
+static void solve_n(int n, double *m, double *v, double *out) {
+  for (int i = 0; i < n; i++) {
+    int piv = i;
+    for (int r = i + 1; r < n; r++)
+      if (fabs(m[r * n + i]) > fabs(m[piv * n + i])) piv = r;
+    if (piv != i) {
+      for (int c = 0; c < n; c++) {
+        const double t = m[i * n + c];
+        m[i * n + c] = m[piv * n + c];
+        m[piv * n + c] = t;
+      }
+      const double t = v[i];
+      v[i] = v[piv];
+      v[piv] = t;
+    }
+    if (fabs(m[i * n + i]) < 1e-9) {
+      for (int k = 0; k < n; k++) out[k] = 0.0;
+      return;
+    }
+    for (int r = 0; r < n; r++)
+      if (r != i) {
+        const double fct = m[r * n + i] / m[i * n + i];
+        for (int c = i; c < n; c++) m[r * n + c] -= fct * m[i * n + c];
+        v[r] -= fct * v[i];
+      }
+  }
+  for (int i = 0; i < n; i++) out[i] = v[i] / m[i * n + i];
+}
+
+/* fit_wiener, lr.rs:240-325: three alternating least-squares solves
+ * (vertical, horizontal, vertical) from zero taps, then rounding (half
+ * away from zero) and clamping to the coded ranges. */
+int orc_fit_wiener(const rvpf_frame_params *f, const orc_planes *deb, const orc_planes *cdef,
+                   const orc_planes *src, int plane, int ur, int uc, int step, int32_t out[2][3],
+                   double fit[2][3]) {
+  lr_rect rects[8];
+  int x0, x1;
+  const int nr = unit_rects(f, deb, cdef, plane, ur, uc, rects, 8, &x0, &x1);
+  memset(out, 0, sizeof(int32_t) * 6);
+  if (fit) memset(fit, 0, sizeof(double) * 6);
+  if (nr == 0) return 0;
+  const int w = x1 - x0 + 1, ww = w + 2 * LRB;
+  const int first = plane == 0 ? 0 : 1;
+  double hc[3] = {0, 0, 0}, vc[3] = {0, 0, 0};
+  int32_t *win = (int32_t *)malloc(sizeof(int32_t) * (size_t)ww * (64 + 2 * LRB));
+  for (int iter = 0; iter < 3; iter++) {
+    const int vertical = iter % 2 == 0;
+    double m[9] = {0}, v[3] = {0};
+    for (int q = 0; q < nr; q++) {
+      rect_window(&rects[q], x0, w, win);
+#define AT(r_, c_) ((double)win[(r_) * ww + (c_)])
+      for (int i = 0; i < rects[q].h; i += step) {
+        const uint16_t *srow = src->data[plane] + (int64_t)(rects[q].y0 + i) * src->stride[plane] + x0;
+        for (int jj = 0; jj < w; jj += step) {
+          double line[7];
+          for (int t = 0; t < 7; t++) {
+            const int r = vertical ? i + t : i + LRB, c = vertical ? jj + LRB : jj + t;
+            const double *other = vertical ? hc : vc;
+            const double center = AT(r, c);
+            double sacc = center;
+            for (int k = 0; k < 3; k++) {
+              const int d = 3 - k;
+              const double a = vertical ? AT(r, c - d) : AT(r - d, c);
+              const double b = vertical ? AT(r, c + d) : AT(r + d, c);
+              sacc += other[k] * (a + b - 2.0 * center);
+            }
+            line[t] = sacc;
+          }
+          const double target = (double)srow[jj] - line[3];
+          double d[3];
+          for (int k = 0; k < 3; k++) d[k] = line[k] + line[6 - k] - 2.0 * line[3];
+          for (int a = first; a < 3; a++) {
+            for (int b = first; b < 3; b++) m[a * 3 + b] += d[a] * d[b];
+            v[a] += d[a] * target;
+          }
+        }
+      }
+#undef AT
+    }
+    double sol[3] = {0, 0, 0};
+    if (first == 0) {
+      solve_n(3, m, v, sol);
+    } else {
+      double m2[4] = {m[4], m[5], m[7], m[8]}, v2[2] = {v[1], v[2]}, s2[2];
+      solve_n(2, m2, v2, s2);
+      sol[1] = s2[0];
+      sol[2] = s2[1];
+    }
+    for (int k = 0; k < 3; k++) {
+      if (vertical)
+        vc[k] = sol[k];
+      else
+        hc[k] = sol[k];
+    }
+  }
+  free(win);
+  for (int k = 0; k < 3; k++) {
+    if (fit) {
+      fit[0][k] = vc[k];
+      fit[1][k] = hc[k];
+    }
+    if (k < first) continue;
+    out[0][k] = clip3(AV1_WIENER_TAPS_MIN[k], AV1_WIENER_TAPS_MAX[k], (int)round(vc[k] * 128.0));
+    out[1][k] = clip3(AV1_WIENER_TAPS_MIN[k], AV1_WIENER_TAPS_MAX[k], (int)round(hc[k] * 128.0));
+  }
+  return 0;
+}
+
+/* The exact (integer) restoration error of one unit with `u` (or of the
+ * CDEF output when u->type is none) against the source: sse_of, lr.rs:128-153. */
+uint64_t orc_unit_sse(const rvpf_frame_params *f, const orc_planes *deb, const orc_planes *cdef,
+                      const orc_planes *src, int plane, int ur, int uc, const rvpf_lr_unit *u) {
+  lr_rect rects[8];
+  int x0, x1;
+  const int nr = unit_rects(f, deb, cdef, plane, ur, uc, rects, 8, &x0, &x1);
+  const int w = x1 - x0 + 1;
+  uint64_t total = 0;
+  uint16_t *buf = (uint16_t *)malloc(sizeof(uint16_t) * (size_t)w * 64);
+  for (int q = 0; q < nr; q++) {
+    const lr_rect *r = &rects[q];
+    const int restore = u && (u->type == RVPF_RESTORE_WIENER || u->type == RVPF_RESTORE_SGRPROJ);
+    if (restore) restore_rect(f, &r->c, x0, r->y0, w, r->h, u, buf, w, 0, 0);
+    for (int i = 0; i < r->h; i++) {
+      const uint16_t *srow = src->data[plane] + (int64_t)(r->y0 + i) * src->stride[plane] + x0;
+      const uint16_t *orow = restore ? buf + (int64_t)i * w
+                                     : cdef->data[plane] + (int64_t)(r->y0 + i) * cdef->stride[plane] + x0;
+      for (int jj = 0; jj < w; jj++) {
+        const int64_t d = (int64_t)orow[jj] - srow[jj];
+        total += (uint64_t)(d * d);
+      }
+    }
+  }
+  free(buf);
+  return total;
+}
+
+/* fit_sgr, lr.rs:332-430: least-squares projection weights for `set`
+ * over the unit, clamped to the coded ranges; *sse = the exact error. */
+int orc_fit_sgr(const rvpf_frame_params *f, const orc_planes *deb, const orc_planes *cdef, const orc_planes *src,
+                int plane, int ur, int uc, int set, int step, int32_t xqd[2], double wfit[2], uint64_t *sse) {
+  lr_rect rects[8];
+  int x0, x1;
+  const int nr = unit_rects(f, deb, cdef, plane, ur, uc, rects, 8, &x0, &x1);
+  xqd[0] = xqd[1] = 0;
+  if (wfit) wfit[0] = wfit[1] = 0.0;
+  if (sse) *sse = 0;
+  if (nr == 0) return 0;
+  const int w = x1 - x0 + 1, ww = w + 2 * LRB;
+  const int r0 = AV1_SGR_PARAMS[set][0], r1 = AV1_SGR_PARAMS[set][2];
+  int32_t *win = (int32_t *)malloc(sizeof(int32_t) * (size_t)ww * (64 + 2 * LRB));
+  int32_t *flt = (int32_t *)calloc((size_t)nr * 2 * w * 64, sizeof(int32_t));
+  double m00 = 0, m01 = 0, m11 = 0, v0 = 0, v1 = 0;
+  for (int q = 0; q < nr; q++) {
+    const lr_rect *r = &rects[q];
+    rect_window(r, x0, w, win);
+    int32_t *f0 = flt + (size_t)q * 2 * w * 64, *f1 = f0 + (size_t)w * 64;
+    box_filter_rect(&r->c, win, f->bit_depth, x0, r->y0, w, r->h, set, 0, f0);
+    box_filter_rect(&r->c, win, f->bit_depth, x0, r->y0, w, r->h, set, 1, f1);
+    for (int i = 0; i < r->h; i += step) {
+      const uint16_t *srow = src->data[plane] + (int64_t)(r->y0 + i) * src->stride[plane] + x0;
+      const uint16_t *crow = cdef->data[plane] + (int64_t)(r->y0 + i) * cdef->stride[plane] + x0;
+      for (int jj = 0; jj < w; jj += step) {
+        const double u = (double)((int32_t)crow[jj] << 4);
+        const double a = r0 ? f0[i * w + jj] - u : 0.0;
+        const double b = r1 ? f1[i * w + jj] - u : 0.0;
+        const double t = 128.0 * ((double)((int32_t)srow[jj] << 4) - u);
+        m00 += a * a;
+        m01 += a * b;
+        m11 += b * b;
+        v0 += a * t;
+        v1 += b * t;
+      }
+    }
+  }
+  double w0, w2;
+  if (r0 && r1) {
+    double m[4] = {m00, m01, m01, m11}, v[2] = {v0, v1}, s2[2];
+    solve_n(2, m, v, s2);
+    w0 = s2[0];
+    w2 = s2[1];
+  } else if (r0) {
+    w0 = m00 > 0.0 ? v0 / m00 : 0.0;
+    w2 = 0.0;
+  } else {
+    w0 = 0.0;
+    w2 = m11 > 0.0 ? v1 / m11 : 0.0;
+  }
+  if (wfit) {
+    wfit[0] = w0;
+    wfit[1] = w2;
+  }
+  const int x0q = r0 ? clip3(AV1_SGRPROJ_XQD_MIN[0], AV1_SGRPROJ_XQD_MAX[0], (int)round(w0)) : 0;
+  const int x1q = r1 ? clip3(AV1_SGRPROJ_XQD_MIN[1], AV1_SGRPROJ_XQD_MAX[1], 128 - x0q - (int)round(w2))
+                     : clip3(AV1_SGRPROJ_XQD_MIN[1], AV1_SGRPROJ_XQD_MAX[1], 128 - x0q);
+  xqd[0] = x0q;
+  xqd[1] = x1q;
+  if (sse) {
+    const int ww0 = x0q, ww1 = x1q, ww2 = 128 - ww0 - ww1;
+    const int maxv = (1 << f->bit_depth) - 1;
+    uint64_t total = 0;
+    for (int q = 0; q < nr; q++) {
+      const lr_rect *r = &rects[q];
+      const int32_t *f0 = flt + (size_t)q * 2 * w * 64, *f1 = f0 + (size_t)w * 64;
+      for (int i = 0; i < r->h; i++) {
+        const uint16_t *srow = src->data[plane] + (int64_t)(r->y0 + i) * src->stride[plane] + x0;
+        const uint16_t *crow = cdef->data[plane] + (int64_t)(r->y0 + i) * cdef->stride[plane] + x0;
+        for (int jj = 0; jj < w; jj++) {
+          const int32_t u = (int32_t)crow[jj] << 4;
+          int32_t val = ww1 * u;
+          val += r0 ? ww0 * f0[i * w + jj] : ww0 * u;
+          val += r1 ? ww2 * f1[i * w + jj] : ww2 * u;
+          const int64_t d = (int64_t)clip3(0, maxv, round2(val, 11)) - srow[jj];
+          total += (uint64_t)(d * d);
+        }
+      }
+    }
+    *sse = total;
+  }
+  free(win);
+  free(flt);
+  return 0;
+}
+
+/* The per-unit choice of search(), lr.rs:153-186: none (cost SSE + lambda),
+ * the fitted Wiener filter (lambda x 40 luma / 28 chroma), each self-guided
+ * set of `sets` (lambda x 20); ties keep the earlier candidate. */
+int orc_lr_search_unit(const rvpf_frame_params *f, const orc_planes *deb, const orc_planes *cdef,
+                       const orc_planes *src, int plane, int ur, int uc, double lambda, const int *sets,
+                       int nsets, int step, rvpf_lr_unit *out, double *cost_out) {
+  memset(out, 0, sizeof *out);
+  lr_rect rects[8];
+  int x0, x1;
+  if (unit_rects(f, deb, cdef, plane, ur, uc, rects, 8, &x0, &x1) == 0) {
+    if (cost_out) *cost_out = 0.0;
+    return 0;
+  }
+  double best = (double)orc_unit_sse(f, deb, cdef, src, plane, ur, uc, NULL) + lambda;
+  rvpf_lr_unit cand;
+  memset(&cand, 0, sizeof cand);
+  int32_t coef[2][3];
+  orc_fit_wiener(f, deb, cdef, src, plane, ur, uc, step, coef, NULL);
+  cand.type = RVPF_RESTORE_WIENER;
+  for (int d = 0; d < 2; d++)
+    for (int k = 0; k < 3; k++) cand.wiener[d][k] = (int8_t)coef[d][k];
+  double cost = (double)orc_unit_sse(f, deb, cdef, src, plane, ur, uc, &cand) + lambda * (plane == 0 ? 40.0 : 28.0);
+  if (cost < best) {
+    best = cost;
+    *out = cand;
+  }
+  for (int i = 0; i < nsets; i++) {
+    int32_t xqd[2];
+    uint64_t e;
+    orc_fit_sgr(f, deb, cdef, src, plane, ur, uc, sets[i], step, xqd, NULL, &e);
+    cost = (double)e + lambda * 20.0;
+    if (cost < best) {
+      best = cost;
+      memset(out, 0, sizeof *out);
+      out->type = RVPF_RESTORE_SGRPROJ;
+      out->sgr_set = (uint8_t)sets[i];
+      out->sgr_xqd[0] = (int8_t)xqd[0];
+      out->sgr_xqd[1] = (int8_t)xqd[1];
+    }
+  }
+  if (cost_out) *cost_out = best;
   return 0;
 }
 

@@ -67,6 +67,28 @@ int orc_cdef(const rvpf_frame_params *p, const rvpf_side_info *s, const orc_plan
 int orc_lr(const rvpf_frame_params *p, const rvpf_side_info *s, const orc_planes *deblocked,
            const orc_planes *cdef, orc_planes *out, int threads);
 
+/* Encoder loop restoration search (src/encoder/lr.rs:64-430), for one
+ * unit (plane, unit row ur, unit column uc) of a frame whose deblocked,
+ * CDEF and source planes are given.
+ * fit_wiener (lr.rs:240-325): quantised taps out[0] vertical, out[1]
+ * horizontal, and (fit != NULL) the unquantised fractions. */
+int orc_fit_wiener(const rvpf_frame_params *f, const orc_planes *deb, const orc_planes *cdef,
+                   const orc_planes *src, int plane, int ur, int uc, int step, int32_t out[2][3],
+                   double fit[2][3]);
+/* fit_sgr (lr.rs:332-430): quantised xqd, the unquantised (w0, w2) and the
+ * exact error with xqd. */
+int orc_fit_sgr(const rvpf_frame_params *f, const orc_planes *deb, const orc_planes *cdef, const orc_planes *src,
+                int plane, int ur, int uc, int set, int step, int32_t xqd[2], double wfit[2], uint64_t *sse);
+/* Exact error of the unit restored with u (u NULL or type none: the CDEF
+ * output) against the source (lr.rs:128-153). */
+uint64_t orc_unit_sse(const rvpf_frame_params *f, const orc_planes *deb, const orc_planes *cdef,
+                      const orc_planes *src, int plane, int ur, int uc, const rvpf_lr_unit *u);
+/* The unit's choice (lr.rs:153-186) among none, the fitted Wiener filter
+ * and the fitted self-guided filter of each set in sets[]. */
+int orc_lr_search_unit(const rvpf_frame_params *f, const orc_planes *deb, const orc_planes *cdef,
+                       const orc_planes *src, int plane, int ur, int uc, double lambda, const int *sets,
+                       int nsets, int step, rvpf_lr_unit *out, double *cost_out);
+
 /* Film grain templates (7.18.3.3), grain.rs:40-135: luma 73x82 and
  * cb/cr (rows x 82 used up to the chroma template size), row stride 82. */
 void orc_grain_templates(const rvpf_frame_params *p, int32_t luma[73 * 82], int32_t cb[73 * 82],

@@ -52,7 +52,7 @@ struct CdefLaunch {
 
 template <typename T>
 __device__ __forceinline__ T *pp(const PlanePtr &p) {
-  return reinterpret_cast<T *>(p.data);
+  return gptr(reinterpret_cast<T *>(p.data));
 }
 
 
@@ -266,7 +266,7 @@ __device__ __forceinline__ void cdef_tile(const CdefLaunch &a, int tile_id, uint
   const int fbx = tile_id % a.fb_cols, fby = tile_id / a.fb_cols;
   const int cs = g.bd - 8;
   int idx = -1;
-  if (a.enabled) idx = a.cdef_idx[(int64_t)fby * a.cdef_stride + fbx];
+  if (a.enabled) idx = gptr(a.cdef_idx)[(int64_t)fby * a.cdef_stride + fbx];
   if (idx >= 0) idx &= 7; /* cdef_bits <= 3 */
   const int xlim = g.mi_cols * 4, ylim = g.mi_rows * 4;
   const int x0 = fbx * 64, y0 = fby * 64;
@@ -299,8 +299,9 @@ __device__ __forceinline__ void cdef_tile(const CdefLaunch &a, int tile_id, uint
   if (inside) {
     const int r = (y0 >> 2) + by * 2, c = (x0 >> 2) + bx * 2;
     const int ms = a.mi_stride;
-    skip = a.mi[(int64_t)r * ms + c].seg_skip & a.mi[(int64_t)(r + 1) * ms + c].seg_skip &
-           a.mi[(int64_t)r * ms + c + 1].seg_skip & a.mi[(int64_t)(r + 1) * ms + c + 1].seg_skip & 1;
+    const rvpf_mi *mi = gptr(a.mi);
+    skip = mi[(int64_t)r * ms + c].seg_skip & mi[(int64_t)(r + 1) * ms + c].seg_skip &
+           mi[(int64_t)r * ms + c + 1].seg_skip & mi[(int64_t)(r + 1) * ms + c + 1].seg_skip & 1;
   }
   const bool filt = inside && !skip;
   int32_t c0 = 0, c1 = 0, c2 = 0;
@@ -376,7 +377,7 @@ __global__ __launch_bounds__(256) void cdef_kernel(CdefLaunch a) {
 template <typename T>
 __global__ __launch_bounds__(256) void cdef_batch(BatchWork w) {
   __shared__ uint32_t words[2 * SOFF];
-  batch_loop(w, [&](int f, int j) { cdef_tile<T>(static_cast<const CdefLaunch *>(w.rec)[f], j, words); });
+  batch_loop(w, [&](int f, int j) { cdef_tile<T>(batch_table<CdefLaunch>(w.rec, f), j, words); });
 }
 
 int plan(const Geom &g, const CdefHdr &h, const rvpf_mi *mi, int mi_stride, const int8_t *cdef_idx,

@@ -26,8 +26,8 @@ __device__ __forceinline__ void copy_block(const CopyLaunch &a, int b) {
   const int x = (lb % a.blk_x[p]) * 64 + (threadIdx.x & 63);
   const int y = (lb / a.blk_x[p]) * 4 + (threadIdx.x >> 6);
   if (x >= a.w[p] || y >= a.h[p]) return;
-  reinterpret_cast<T *>(a.out[p].data)[(int64_t)y * a.out[p].stride + x] =
-      reinterpret_cast<const T *>(a.in[p].data)[(int64_t)y * a.in[p].stride + x];
+  gptr(reinterpret_cast<T *>(a.out[p].data))[(int64_t)y * a.out[p].stride + x] =
+      gptr(reinterpret_cast<const T *>(a.in[p].data))[(int64_t)y * a.in[p].stride + x];
 }
 
 template <typename T>
@@ -37,7 +37,7 @@ __global__ __launch_bounds__(256) void copy_kernel(CopyLaunch a) {
 
 template <typename T>
 __global__ __launch_bounds__(256) void copy_batch(BatchWork w) {
-  batch_loop(w, [&](int f, int j) { copy_block<T>(static_cast<const CopyLaunch *>(w.rec)[f], j); });
+  batch_loop(w, [&](int f, int j) { copy_block<T>(batch_table<CopyLaunch>(w.rec, f), j); });
 }
 
 int plan(const Geom &g, const PlanePtr in[3], const PlanePtr out[3], const int w[3], const int h[3],

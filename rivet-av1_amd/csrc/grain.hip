@@ -357,8 +357,8 @@ __device__ __forceinline__ void grain_apply_tile(const GrainHdr &h, const GrainS
   const int gmin = -(128 << (bd - 8)), gmax = (256 << (bd - 8)) - 1 - (128 << (bd - 8));
   const bool ov = h.overlap;
   const int mb = sc.max_blocks;
-  const TI *lin = reinterpret_cast<const TI *>(i0.data);
-  TO *lout = reinterpret_cast<TO *>(o0.data);
+  const TI *lin = gptr(reinterpret_cast<const TI *>(i0.data));
+  TO *lout = gptr(reinterpret_cast<TO *>(o0.data));
   if (h.nplanes > 1) {
     const int lx = cx << ssx, ly = cy << ssy;
     const int lnx = min(lx + 1, h.w - 1);
@@ -368,7 +368,7 @@ __device__ __forceinline__ void grain_apply_tile(const GrainHdr &h, const GrainS
     for (int p = 1; p < 3; p++) {
       const PlanePtr &ip = p == 1 ? i1 : i2;
       const PlanePtr &op = p == 1 ? o1 : o2;
-      const int orig = reinterpret_cast<const TI *>(ip.data)[(int64_t)cy * ip.stride + cx];
+      const int orig = gptr(reinterpret_cast<const TI *>(ip.data))[(int64_t)cy * ip.stride + cx];
       int v = orig;
       const int npts = p == 1 ? h.num_cb : h.num_cr;
       if (npts > 0 || h.cfl) {
@@ -386,7 +386,7 @@ __device__ __forceinline__ void grain_apply_tile(const GrainHdr &h, const GrainS
         const int sn = round2(scale_lut(lut + p * 256, bd, merged) * noise, h.scaling_shift);
         v = clip3(h.min_value, h.max_chroma, orig + sn);
       }
-      reinterpret_cast<TO *>(op.data)[(int64_t)cy * op.stride + cx] = (TO)v;
+      gptr(reinterpret_cast<TO *>(op.data))[(int64_t)cy * op.stride + cx] = (TO)v;
     }
   }
   /* luma samples of this chroma position */
@@ -435,7 +435,7 @@ struct ApplyRecord {
 __global__ __launch_bounds__(128) void grain_prepare_batch(BatchWork w) {
   __shared__ PrepShared sm;
   batch_loop(w, [&](int f, int) {
-    const PrepRecord &r = static_cast<const PrepRecord *>(w.rec)[f];
+    const PrepRecord &r = batch_table<PrepRecord>(w.rec, f);
     grain_prepare_body(r.h, r.s, sm);
   });
 }
@@ -444,7 +444,7 @@ template <typename T, int EX>
 __global__ __launch_bounds__(256) void grain_apply_batch(BatchWork w) {
   __shared__ int16_t lut[3 * 256];
   batch_loop(w, [&](int f, int j) {
-    const ApplyRecord &r = static_cast<const ApplyRecord *>(w.rec)[f];
+    const ApplyRecord &r = batch_table<ApplyRecord>(w.rec, f);
     grain_apply_tile<T, T, EX>(r.h, r.s, r.in[0], r.in[1], r.in[2], r.out[0], r.out[1], r.out[2], j, lut);
   });
 }

@@ -205,8 +205,9 @@ __device__ __forceinline__ int edge_decision(const LfLaunch &a, int plane, int p
   const int xp = x >> ssx, yp = y >> ssy;
   const int rowo = row | ssy, colo = col | ssx;
   const int ms = a.s.mi_stride;
-  const rvpf_mi m = a.s.mi[(int64_t)rowo * ms + colo];
-  const uint8_t *tx = a.s.tx[plane];
+  const rvpf_mi *mi = gptr(a.s.mi);
+  const rvpf_mi m = mi[(int64_t)rowo * ms + colo];
+  const uint8_t *tx = gptr(a.s.tx[plane]);
   const int tx_sz = min((int)tx[(int64_t)ey * ms + ex], 18);
   const int prev_tx_sz = min((int)(pass == 0 ? tx[(int64_t)ey * ms + ex - 1] : tx[(int64_t)(ey - 1) * ms + ex]), 18);
   const int plane_size = AV1_SUBSAMPLED_SIZE[min((int)m.mi_size, 21)][ssx][ssy];
@@ -230,7 +231,7 @@ __device__ __forceinline__ int edge_decision(const LfLaunch &a, int plane, int p
   if (lvl == 0) {
     const int prow = pass == 0 ? rowo : rowo - (1 << ssy);
     const int pcol = pass == 0 ? colo - (1 << ssx) : colo;
-    const rvpf_mi pm = a.s.mi[(int64_t)prow * ms + pcol];
+    const rvpf_mi pm = mi[(int64_t)prow * ms + pcol];
     lvl = lf_level(a.h, pm, li);
   }
   if (lvl == 0) return 0;
@@ -257,7 +258,7 @@ __device__ __forceinline__ void lf_block(const LfLaunch &a, int gp, int lb) {
   const int fs = edge_decision(a, plane, PASS, ex, ey, &limit_bd, &blimit_bd, &thresh_bd);
   if (fs == 0) return;
   const bool chroma = plane != 0;
-  T *base = reinterpret_cast<T *>(a.planes[plane].data);
+  T *base = gptr(reinterpret_cast<T *>(a.planes[plane].data));
   const int64_t stride = a.planes[plane].stride;
   const int xp = ex * 4, yp = ey * 4;
   if (PASS == 0) {
@@ -342,7 +343,7 @@ __global__ __launch_bounds__(256) void lf_pass_kernel(LfLaunch a) {
 template <typename T, int PASS, int EX>
 __global__ __launch_bounds__(256) void lf_pass_batch(BatchWork w) {
   batch_loop(w, [&](int f, int j) {
-    const LfLaunch &a = static_cast<const LfLaunch *>(w.rec)[f];
+    const LfLaunch &a = batch_table<LfLaunch>(w.rec, f);
     int gp = 0;
     while (gp + 1 < a.nseg && j >= a.blk_start[gp + 1]) gp++;
     const int lb = j - a.blk_start[gp];

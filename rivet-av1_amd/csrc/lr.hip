@@ -48,7 +48,7 @@ struct LrLaunch {
 
 template <typename T>
 __device__ __forceinline__ const T *cpp(const PlanePtr &p) {
-  return reinterpret_cast<const T *>(p.data);
+  return gptr(reinterpret_cast<const T *>(p.data));
 }
 
 __device__ __forceinline__ int round2(int x, int n) { return (x + ((1 << n) >> 1)) >> n; }
@@ -101,7 +101,7 @@ __device__ __forceinline__ void lr_slice(const LrLaunch &a, int gp, int lb, LrSh
   if (y0 > y1c || x0 > x1c) return;
   const T *cdef = cpp<T>(a.cdef[plane]);
   const int64_t cs_ = a.cdef[plane].stride;
-  T *out = reinterpret_cast<T *>(a.out[plane].data);
+  T *out = gptr(reinterpret_cast<T *>(a.out[plane].data));
   const int64_t os = a.out[plane].stride;
 
   int type = 0;
@@ -110,7 +110,7 @@ __device__ __forceinline__ void lr_slice(const LrLaunch &a, int gp, int lb, LrSh
     const int luma_row = max(st * 64 - 8, 0);
     const int unit_row = min(a.h.unit_rows[plane] - 1, ((luma_row + 8) >> ssy) / unit_size);
     const int unit_col = min(a.h.unit_cols[plane] - 1, x0 >> (31 - __clz(unit_size)));
-    u = &a.h.units[plane][unit_row * a.h.unit_cols[plane] + unit_col];
+    u = &gptr(a.h.units[plane])[unit_row * a.h.unit_cols[plane] + unit_col];
     type = u->type;
     if (type != RVPF_RESTORE_WIENER && type != RVPF_RESTORE_SGRPROJ) type = 0;
   }
@@ -306,7 +306,7 @@ template <typename T>
 __global__ __launch_bounds__(NT) void lr_batch(BatchWork w) {
   __shared__ LrShared sm;
   batch_loop(w, [&](int f, int j) {
-    const LrLaunch &a = static_cast<const LrLaunch *>(w.rec)[f];
+    const LrLaunch &a = batch_table<LrLaunch>(w.rec, f);
     int gp = 0;
     while (gp + 1 < a.nseg && j >= a.blk_start[gp + 1]) gp++;
     lr_slice<T>(a, gp, j - a.blk_start[gp], sm);

@@ -129,12 +129,31 @@ struct BatchWork {
   int *queue;       /* device, 2 ints: next item, workgroups finished */
 };
 
+/* A batch table is read-only while a batched kernel runs; reading it
+ * through the constant address space lets the compiler use scalar loads
+ * for the (wave-uniform) record fields, and keeps the plane pointers
+ * stored in them known as global memory. */
+template <class R>
+__device__ __forceinline__ const R &batch_table(const void *p, int i) {
+  typedef const R __attribute__((address_space(4))) ConstR;
+  return *(const R *)((ConstR *)p + i);
+}
+
+/* A pointer known to address global memory (plane and side-info pointers
+ * read from a record): the round trip through the global address space
+ * lets the compiler emit global rather than flat memory instructions. */
+template <class T>
+__device__ __forceinline__ T *gptr(T *p) {
+  typedef T __attribute__((address_space(1))) GlobalT;
+  return (T *)((GlobalT *)p);
+}
+
 /* Frame of item i: the last f with start[f] <= i. */
 __device__ __forceinline__ int batch_frame(const int *start, int n, int i) {
   int lo = 0, hi = n - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
-    if (start[mid] <= i)
+    if (batch_table<int>(start, mid) <= i)
       lo = mid;
     else
       hi = mid - 1;
@@ -159,7 +178,7 @@ __device__ __forceinline__ void batch_loop(const BatchWork &w, Body &&body) {
     const int i1 = min(i0 + w.chunk, w.total);
     for (int i = i0; i < i1; i++) {
       const int f = __builtin_amdgcn_readfirstlane(batch_frame(w.start, w.n, i));
-      const int j = __builtin_amdgcn_readfirstlane(i - w.start[f]);
+      const int j = __builtin_amdgcn_readfirstlane(i - batch_table<int>(w.start, f));
       body(f, j);
       __syncthreads(); /* the next item reuses the LDS */
     }
