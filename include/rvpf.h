@@ -317,6 +317,65 @@ int rvpf_batch_set_tiles(rvpf_batch *b, const rvpf_tile_config in[RVPF_TUNE_COUN
  * the first run after a tile change). */
 int rvpf_batch_run(rvpf_ctx *ctx, rvpf_batch *b, void *stream);
 
+/*
+ * Differentiable loop restoration, for the encoder's restoration search
+ * (the reference fits Wiener taps and self-guided weights by least squares
+ * in src/encoder/lr.rs:226-430 and judges them with the exact filter,
+ * :64-205). These calls give the squared error of float-relaxed filters
+ * (no rounding, no clipping) and its gradient with respect to each unit's
+ * taps, for gradient-based fitting; rvpf_lrd_rect_sse and rvpf_lr judge
+ * the quantised result exactly.
+ *
+ * Work items are rectangles of at most 64 x 64 samples, one stripe of one
+ * restoration unit each (the rectangles of lr.rs:82-127, split every 64
+ * columns). Per item the caller-allocated float buffers hold a 70 x 70
+ * window (the rectangle with a 3-sample border, 7.17.6's stripe rule) and
+ * 64 x 64 tiles (row stride 64) of the CDEF output u, the source, the
+ * residual and the self-guided filter outputs. All pointers are device
+ * pointers; items with an out-of-range unit or plane count as empty.
+ */
+typedef struct rvpf_lrd_item {
+  int32_t plane;
+  int32_t x0, y0;                   /* first sample of the rectangle */
+  int32_t w, h;                     /* 1..64 */
+  int32_t stripe_start, stripe_end; /* StripeStartY / StripeEndY of its stripe */
+  int32_t unit;                     /* index of its unit in the per-unit arrays */
+} rvpf_lrd_item;
+
+/* win[n][70][70], u[n][64][64], src[n][64][64] from the deblocked, CDEF
+ * and source planes (LrCtx::sample, src/decoder/postfilter.rs:584-596). */
+int rvpf_lrd_gather(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_planes *deblocked,
+                    const rvpf_planes *cdef, const rvpf_planes *source, const rvpf_lrd_item *items,
+                    int32_t n, int32_t nunits, float *win, float *u, float *src, void *stream);
+/* Relaxed Wiener filter with taps[unit][2][3] (vertical, horizontal; c0..c2
+ * as fractions of 128, the model of fit_wiener, lr.rs:240-325).
+ * backward == 0: resid[n][64][64] = out - src, out[n] = squared error;
+ * backward != 0: out[n][6] = d(squared error)/d(v0, v1, v2, h0, h1, h2)
+ * from the forward residual. */
+int rvpf_lrd_wiener(rvpf_ctx *ctx, int backward, const rvpf_lrd_item *items, int32_t n, int32_t nunits,
+                    const float *win, const float *src, const float *taps, float *resid, double *out,
+                    void *stream);
+/* Relaxed self-guided filter outputs flt[n][2][64][64] (in 1/16 sample
+ * units; 16 u for a pass whose radius is 0) for the set of each unit
+ * (sets[unit], 0..15; box_filter_rect, postfilter.rs:901-991). */
+int rvpf_lrd_sgr_filter(rvpf_ctx *ctx, int32_t bit_depth, const rvpf_lrd_item *items, int32_t n,
+                        int32_t nunits, const float *win, const float *u, const int32_t *sets, float *flt,
+                        void *stream);
+/* Relaxed projection with w[unit][2] = (w0, w2) (7.17.2, fit_sgr's model,
+ * lr.rs:332-430): forward / backward as rvpf_lrd_wiener, out[n][2] = the
+ * gradient with respect to (w0, w2). */
+int rvpf_lrd_sgr(rvpf_ctx *ctx, int backward, const rvpf_lrd_item *items, int32_t n, int32_t nunits,
+                 const float *flt, const float *u, const float *src, const float *w, float *resid, double *out,
+                 void *stream);
+/* AC_QLOOKUP[(bit_depth - 8) >> 1][qindex] (src/tables.rs), from which
+ * the search's Lagrange multiplier is derived (src/encoder/mod.rs:1449-1455:
+ * lambda = 0.1 (AC_QLOOKUP / 8)^2). Returns -1 for arguments out of range. */
+int32_t rvpf_ac_q(int32_t bit_depth, int32_t qindex);
+/* Exact squared error between two frames' samples over each item:
+ * out[n] (the judge's error, lr.rs:128-153). */
+int rvpf_lrd_rect_sse(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_lrd_item *items, int32_t n,
+                      int32_t nunits, const rvpf_planes *a, const rvpf_planes *b, double *out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

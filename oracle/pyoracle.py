@@ -46,6 +46,15 @@ def lib():
                                            P(OrcPlanes), C.c_void_p]
         L.orc_grain_rng_step.argtypes = [C.c_uint32]
         L.orc_grain_rng_step.restype = C.c_uint32
+        OP = P(OrcPlanes)
+        L.orc_fit_wiener.argtypes = [P(abi.RvpfFrameParams), OP, OP, OP, C.c_int, C.c_int, C.c_int, C.c_int,
+                                     C.c_void_p, C.c_void_p]
+        L.orc_fit_sgr.argtypes = [P(abi.RvpfFrameParams), OP, OP, OP, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                  C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_unit_sse.argtypes = [P(abi.RvpfFrameParams), OP, OP, OP, C.c_int, C.c_int, C.c_int, C.c_void_p]
+        L.orc_unit_sse.restype = C.c_uint64
+        L.orc_lr_search_unit.argtypes = [P(abi.RvpfFrameParams), OP, OP, OP, C.c_int, C.c_int, C.c_int, C.c_double,
+                                         C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
         _lib = L
     return _lib
 
@@ -149,3 +158,49 @@ def postfilter_frame(frame, threads=1, recon=None):
     pr, po = planes_struct(rec), planes_struct(ref)
     lib().orc_postfilter_frame(C.byref(frame.params), C.byref(s), C.byref(pr), C.byref(po), arr, threads)
     return rec, ref, disp
+
+
+class LrSearchFrame:
+    """Host planes of one frame for the encoder restoration search:
+    deblocked, CDEF output and source (uint16 arrays)."""
+
+    def __init__(self, frame, deblocked, cdef_planes, source):
+        self.frame = frame
+        self.keep = (deblocked, cdef_planes, source)
+        self.pd = planes_struct(deblocked)
+        self.pc = planes_struct(cdef_planes)
+        self.ps = planes_struct(source)
+
+    def _args(self):
+        return (C.byref(self.frame.params), C.byref(self.pd), C.byref(self.pc), C.byref(self.ps))
+
+    def fit_wiener(self, plane, ur, uc, step=1):
+        """(quantised taps [2, 3] int, unquantised fractions [2, 3])."""
+        q = np.zeros((2, 3), dtype=np.int32)
+        f = np.zeros((2, 3), dtype=np.float64)
+        lib().orc_fit_wiener(*self._args(), plane, ur, uc, step, q.ctypes.data, f.ctypes.data)
+        return q, f
+
+    def fit_sgr(self, plane, ur, uc, set_, step=1):
+        """(xqd [2] int, unquantised (w0, w2), exact squared error)."""
+        q = np.zeros(2, dtype=np.int32)
+        w = np.zeros(2, dtype=np.float64)
+        e = C.c_uint64()
+        lib().orc_fit_sgr(*self._args(), plane, ur, uc, set_, step, q.ctypes.data, w.ctypes.data, C.byref(e))
+        return q, w, e.value
+
+    def unit_sse(self, plane, ur, uc, unit_bytes=None):
+        """Exact squared error of the unit with the rvpf_lr_unit bytes (None: CDEF output)."""
+        if unit_bytes is None:
+            return lib().orc_unit_sse(*self._args(), plane, ur, uc, None)
+        u = np.ascontiguousarray(unit_bytes, dtype=np.uint8)
+        return lib().orc_unit_sse(*self._args(), plane, ur, uc, u.ctypes.data)
+
+    def search_unit(self, plane, ur, uc, lam, sets, step=1):
+        """(chosen rvpf_lr_unit bytes [12], cost)."""
+        out = np.zeros(12, dtype=np.uint8)
+        cost = C.c_double()
+        st = np.ascontiguousarray(sets, dtype=np.int32)
+        lib().orc_lr_search_unit(*self._args(), plane, ur, uc, lam, st.ctypes.data, len(st), step, out.ctypes.data,
+                                 C.byref(cost))
+        return out, cost.value

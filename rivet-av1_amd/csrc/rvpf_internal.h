@@ -234,6 +234,19 @@ constexpr int LF_SHAPES = 3;    /* 64x4, 32x8, 16x16 edges per workgroup */
 constexpr int GRAIN_SHAPES = 3; /* 64x4, 32x8, 16x16 chroma positions per workgroup */
 constexpr int COPY_SHAPES = 1;
 
+/* Differentiable loop restoration (lrdiff.hip). */
+int launch_lrd_gather(const PlanePtr deb[3], const PlanePtr cdef[3], const PlanePtr src[3], int bps,
+                      const int pex[3], const int pey[3], const rvpf_lrd_item *items, int n, int nunits,
+                      float *win, float *u, float *s, hipStream_t st);
+int launch_lrd_wiener(int backward, const rvpf_lrd_item *items, int n, int nunits, const float *win,
+                      const float *src, const float *taps, float *resid, double *out, hipStream_t st);
+int launch_lrd_sgr_filter(const rvpf_lrd_item *items, int n, int nunits, const float *win, const float *u,
+                          const int32_t *sets, int bd, float *flt, hipStream_t st);
+int launch_lrd_sgr(int backward, const rvpf_lrd_item *items, int n, int nunits, const float *flt,
+                   const float *u, const float *src, const float *w, float *resid, double *out, hipStream_t st);
+int launch_lrd_rect_sse(const rvpf_lrd_item *items, int n, int nunits, const int pex[3], const int pey[3],
+                        const PlanePtr a[3], const PlanePtr b[3], int bps, double *out, hipStream_t st);
+
 /* Launch entry points (each returns a hipError_t cast to int). */
 int launch_deblock(const Geom &g, const LfHdr &h, const LfSide &s, PlanePtr planes[3], int bps,
                    hipStream_t st);

@@ -195,6 +195,19 @@ class RvpfTileConfig(C.Structure):
     ]
 
 
+class RvpfLrdItem(C.Structure):
+    _fields_ = [
+        ("plane", C.c_int32),
+        ("x0", C.c_int32),
+        ("y0", C.c_int32),
+        ("w", C.c_int32),
+        ("h", C.c_int32),
+        ("stripe_start", C.c_int32),
+        ("stripe_end", C.c_int32),
+        ("unit", C.c_int32),
+    ]
+
+
 TUNE_STAGES = ["deblock", "cdef", "lr", "grain_prepare", "grain_apply", "copy"]
 TUNE_SHAPES = [3, 1, 1, 1, 3, 1]
 
@@ -214,6 +227,7 @@ STRUCT_SIZES = [
     C.sizeof(RvpfPlanes),
     C.sizeof(RvpfBatchFrame),
     C.sizeof(RvpfTileConfig),
+    C.sizeof(RvpfLrdItem),
 ]
 
 
@@ -241,4 +255,10 @@ RVPF_EXPORTS = [
     "rvpf_batch_get_tiles",
     "rvpf_batch_set_tiles",
     "rvpf_batch_run",
+    "rvpf_lrd_gather",
+    "rvpf_lrd_wiener",
+    "rvpf_lrd_sgr_filter",
+    "rvpf_lrd_sgr",
+    "rvpf_lrd_rect_sse",
+    "rvpf_ac_q",
 ]

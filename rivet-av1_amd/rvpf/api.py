@@ -53,6 +53,16 @@ def lib():
         L.rvpf_batch_get_tiles.argtypes = [C.c_void_p, P(abi.RvpfTileConfig)]
         L.rvpf_batch_set_tiles.argtypes = [C.c_void_p, P(abi.RvpfTileConfig)]
         L.rvpf_batch_run.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        V = C.c_void_p
+        L.rvpf_lrd_gather.argtypes = [V, P(abi.RvpfFrameParams), P(abi.RvpfPlanes), P(abi.RvpfPlanes),
+                                      P(abi.RvpfPlanes), V, C.c_int32, C.c_int32, V, V, V, V]
+        L.rvpf_lrd_wiener.argtypes = [V, C.c_int, V, C.c_int32, C.c_int32, V, V, V, V, V, V]
+        L.rvpf_lrd_sgr_filter.argtypes = [V, C.c_int32, V, C.c_int32, C.c_int32, V, V, V, V, V]
+        L.rvpf_lrd_sgr.argtypes = [V, C.c_int, V, C.c_int32, C.c_int32, V, V, V, V, V, V, V]
+        L.rvpf_lrd_rect_sse.argtypes = [V, P(abi.RvpfFrameParams), V, C.c_int32, C.c_int32, P(abi.RvpfPlanes),
+                                        P(abi.RvpfPlanes), V, V]
+        L.rvpf_ac_q.argtypes = [C.c_int32, C.c_int32]
+        L.rvpf_ac_q.restype = C.c_int32
         _lib = L
     return _lib
 
