@@ -240,12 +240,141 @@ def bench_batch(args):
     ctx.close()
 
 
+def bench_lrdiff(args):
+    """The encoder's restoration search on the GPU: 1080p 10-bit frames at
+    qindex 160 (deblocked + CDEF'd reconstruction), 64-sample units; the
+    relaxed forward + adjoint kernels, the fits and the exact judge."""
+    import numpy as np
+    import torch
+    from rvpf import api, lrdiff, workload
+    torch.cuda.set_device(0)
+    nf = args.lr_frames
+    log("generating %d frames 1920x1080 10-bit" % nf)
+    frames = [workload.Frame(1920, 1080, bit_depth=10, qindex=160, sb_size=128, seed=args.seed + i, frame_index=i,
+                             pan_px=1.5, lr=2, lr_unit_size=64) for i in range(nf)]
+    ctx = api.Context()
+    stream = torch.cuda.current_stream()
+    entries = []
+    for f in frames:
+        df = api.DeviceFrame(f)
+        rec = df.upload_planes(f.rec)
+        api.deblock(ctx, df, rec, stream)
+        cd = df.alloc_like_recon()
+        api.cdef(ctx, df, rec, cd, stream)
+        entries.append((df, rec, cd, df.upload_planes(f.src)))
+    search = lrdiff.Search(ctx, entries, stream=stream)
+    torch.cuda.synchronize()
+    units = search.U
+    luma_units = int((search.units[:, 0] == 0).sum())
+    px = sum(f.luma_pixels() for f in frames)
+    lam = lrdiff.rd_lambda(10, 160)
+
+    def timed(fn, reps):
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        fn()
+        torch.cuda.synchronize()
+        ev0.record(stream)
+        for _ in range(reps):
+            fn()
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        return ev0.elapsed_time(ev1) / reps
+
+    taps = torch.full((search.U, 2, 3), 0.02, device="cuda") * search.tap_mask
+    ms_fb = timed(lambda: search.wiener_value_grad(taps), args.steps)
+    log("relaxed Wiener forward+adjoint: %.3f ms for %d frames (%.1f Mpix/s)" % (ms_fb, nf, px / ms_fb / 1e3))
+    ms_search = timed(lambda: search.search(lam), max(1, args.steps // 5))
+    log("full search: %.3f ms for %d frames (%.2f ms/frame)" % (ms_search, nf, ms_search / nf))
+    # checks: finite differences (all frames) and the reference's least squares (frame 0)
+    t0 = ((torch.rand((search.U, 2, 3), generator=torch.Generator().manual_seed(7)) - 0.5) * 0.3).double().cuda()
+    luma = search.plane_of_unit == 0
+    _, g = search.wiener_value_grad(t0)
+    worst = 0.0
+    for d in range(2):
+        for k in range(3):
+            tp, tm = t0.clone(), t0.clone()
+            tp[:, d, k] += 1e-3
+            tm[:, d, k] -= 1e-3
+            fd = (search.wiener_value_grad(tp)[0] - search.wiener_value_grad(tm)[0]) / 2e-3
+            scale = torch.maximum(fd.abs(), 1e-2 * g.view(-1, 6).abs().amax(dim=1))
+            ok = luma & (scale > 0)
+            worst = max(worst, ((g[:, d, k] - fd).abs()[ok] / scale[ok]).max().item())
+    log("adjoint vs central differences (h=1e-3): max rel err %.2e" % worst)
+    f0 = frames[0]
+    df0, rec0, cd0, _ = entries[0]
+    ls = None
+    if not args.no_check:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+        host = pyoracle.LrSearchFrame(f0, api.to_host(rec0), api.to_host(cd0), f0.src)
+        taps3 = search.fit_wiener(3).cpu()
+        q3 = search.quantise_wiener(taps3)
+        taps3 = taps3.numpy()
+        u0, u1 = search.frame_units[0]
+        sample = list(range(u0, u1, max(1, (u1 - u0) // 64)))
+        dmax, same = 0.0, 0
+        for u in sample:
+            p, ur, uc = (int(x) for x in search.units[u])
+            qr, fr = host.fit_wiener(p, ur, uc)
+            dmax = max(dmax, float(np.abs(taps3[u] - fr).max()))
+            same += int(np.array_equal(q3[u], qr))
+        ls = {"units": len(sample), "max_abs_diff_fraction": dmax, "quantised_equal": same}
+        log("least-squares taps vs the oracle's fit_wiener on %d units: max |diff| %.2e, %d quantised equal"
+            % (len(sample), dmax, same))
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+        host = pyoracle.LrSearchFrame(f0, api.to_host(rec0), api.to_host(cd0), f0.src)
+        t_start, n = time.perf_counter(), 0
+        u1 = search.frame_units[0][1]
+        while time.perf_counter() - t_start < args.cpu_seconds and n < u1:
+            p, ur, uc = (int(x) for x in search.units[n])
+            host.search_unit(p, ur, uc, lam, lrdiff.SGR_SETS_NORMAL)
+            n += 1
+        el = time.perf_counter() - t_start
+        cpu = {"value": round(n / el, 2), "unit": "units/s", "cores": 1, "kind": "port",
+               "sample": "%d units of a 1080p 10-bit frame (%.1f s), oracle search (fit_wiener + 6 SGR sets + "
+                         "exact judge), 1 thread" % (n, el)}
+        log("cpu:", cpu)
+    out = {
+        "metric": "encoder LR search units/s (relaxed forward+adjoint fits + exact judge), 1080p 10-bit",
+        "value": round(units / (ms_search / 1e3), 1),
+        "unit": "units/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": 1,
+        "ms_per_step": round(ms_search, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (relaxed fits), u16 (judge)",
+        "data": "synthetic",
+        "config": {
+            "workload": "%d frames 1920x1080 10-bit 4:2:0, qindex 160, deblocked+CDEF'd reconstruction, "
+                        "64-sample units (%d luma units/frame), Wiener + SGR sets %s" %
+                        (nf, luma_units // nf, list(lrdiff.SGR_SETS_NORMAL)),
+            "units_per_step": units,
+            "forward_adjoint_ms": round(ms_fb, 4),
+            "forward_adjoint_mpix_s": round(px / ms_fb / 1e3, 1),
+            "search_ms_per_frame": round(ms_search / nf, 3),
+            "fd_max_rel_err": worst,
+            "least_squares_check": ls,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=["gop4k", "batch32"], default="gop4k",
+    ap.add_argument("--workload", choices=["gop4k", "batch32", "lrdiff"], default="gop4k",
                     help="gop4k: the per-frame chain on the 4K 10-bit GOP (default); "
-                         "batch32: the batched path against the per-frame drop-in on 32 mixed streams")
+                         "batch32: the batched path against the per-frame drop-in on 32 mixed streams; "
+                         "lrdiff: the encoder's restoration search on 1080p 10-bit frames")
+    ap.add_argument("--lr-frames", type=int, default=16)
     ap.add_argument("--streams", type=int, default=32)
     ap.add_argument("--stream-frames", type=int, default=4)
     ap.add_argument("--steps", type=int, default=10)
@@ -269,6 +398,9 @@ def main():
 
     if args.workload == "batch32":
         bench_batch(args)
+        return
+    if args.workload == "lrdiff":
+        bench_lrdiff(args)
         return
 
     import numpy as np

@@ -154,30 +154,50 @@ __device__ __forceinline__ void load_window(WienerShared &sm, const float *win) 
   for (int k = threadIdx.x; k < WS * WS; k += NTH) sm.w[k] = win[k];
 }
 
+/* The forward pass runs in double: its error is differenced against
+ * itself by finite-difference checks at small tap steps, so its rounding
+ * must stay far below the change a step of 1e-3 makes. */
+struct ForwardShared {
+  float w[WS * WS];
+  double hbuf[WS * T];
+  double red[NTH / 64];
+};
+
 __global__ __launch_bounds__(NTH) void wiener_forward_kernel(WienerArgs a) {
-  __shared__ WienerShared sm;
+  __shared__ ForwardShared sm;
   const rvpf_lrd_item it = load_item(a.items, blockIdx.x, a.nunits);
   const float *tp = a.taps + (size_t)it.unit * 6;
-  const float v[3] = {tp[0], tp[1], tp[2]}, h[3] = {tp[3], tp[4], tp[5]};
-  load_window(sm, a.win + (size_t)blockIdx.x * WS * WS);
+  const double v[3] = {tp[0], tp[1], tp[2]}, h[3] = {tp[3], tp[4], tp[5]};
+  const float *win = a.win + (size_t)blockIdx.x * WS * WS;
+  for (int k = threadIdx.x; k < WS * WS; k += NTH) sm.w[k] = win[k];
   __syncthreads();
-  wiener_h(sm, sm.hbuf, h, it.h + 6, it.w);
+  for (int k = threadIdx.x; k < (it.h + 6) * T; k += NTH) {
+    const int r = k >> 6, c = k & 63;
+    double val = 0.0;
+    if (c < it.w) {
+      const float *row = sm.w + r * WS + c + 3;
+      const double ctr = row[0];
+      val = ctr + h[0] * ((double)row[-3] + row[3] - 2.0 * ctr) + h[1] * ((double)row[-2] + row[2] - 2.0 * ctr) +
+            h[2] * ((double)row[-1] + row[1] - 2.0 * ctr);
+    }
+    sm.hbuf[k] = val;
+  }
   __syncthreads();
   const float *src = a.src + (size_t)blockIdx.x * T * T;
   float *res = a.resid + (size_t)blockIdx.x * T * T;
   double acc = 0.0;
   for (int k = threadIdx.x; k < T * T; k += NTH) {
     const int i = k >> 6, j = k & 63;
-    float e = 0.f;
+    double e = 0.0;
     if (i < it.h && j < it.w) {
-      const float *col = sm.hbuf + (i + 3) * T + j;
-      const float ctr = col[0];
-      const float out = ctr + v[0] * (col[-3 * T] + col[3 * T] - 2.f * ctr) +
-                        v[1] * (col[-2 * T] + col[2 * T] - 2.f * ctr) + v[2] * (col[-T] + col[T] - 2.f * ctr);
-      e = out - src[k];
-      acc += (double)e * (double)e;
+      const double *col = sm.hbuf + (i + 3) * T + j;
+      const double ctr = col[0];
+      const double out = ctr + v[0] * (col[-3 * T] + col[3 * T] - 2.0 * ctr) +
+                         v[1] * (col[-2 * T] + col[2 * T] - 2.0 * ctr) + v[2] * (col[-T] + col[T] - 2.0 * ctr);
+      e = out - (double)src[k];
+      acc += e * e;
     }
-    res[k] = e;
+    res[k] = (float)e;
   }
   const double s = block_sum(acc, sm.red);
   if (threadIdx.x == 0) a.out[blockIdx.x] = s;

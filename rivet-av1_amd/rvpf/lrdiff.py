@@ -161,38 +161,66 @@ class SgrSSE:
 
 
 class Search:
-    """One frame's restoration search state on the GPU.
+    """The restoration search state of one or more frames on the GPU.
 
-    ctx: api.Context; df: api.DeviceFrame whose frame.params give the unit
-    sizes and counts; deblocked, cdef, source: device planes (the frame's
-    sample type)."""
+    ctx: api.Context; entries: list of (df, deblocked, cdef, source), one
+    per frame, where df is an api.DeviceFrame whose frame.params give the
+    unit sizes and counts, and the planes are device planes of the frame's
+    sample type. (Search(ctx, df, deblocked, cdef, source) is one frame.)
+    Units of all frames are numbered together; every kernel runs once over
+    all frames' items."""
 
-    def __init__(self, ctx, df, deblocked, cdef, source, stream=None):
+    def __init__(self, ctx, entries, deblocked=None, cdef=None, source=None, stream=None):
         torch = _torch()
         self.torch = torch
-        self.ctx, self.df, self.stream = ctx, df, stream
-        self.params = df.frame.params
-        self.deblocked, self.cdef, self.source = deblocked, cdef, source
-        units, items = unit_layout(self.params)
-        self.units = units
-        self.U, self.n = len(units), len(items)
-        dev = deblocked[0].device
+        if not isinstance(entries, (list, tuple)):
+            entries = [(entries, deblocked, cdef, source)]
+        self.ctx, self.stream, self.entries = ctx, stream, list(entries)
+        units, items, self.frame_units, self.frame_items = [], [], [], []
+        u0 = n0 = 0
+        for df, _, _, _ in self.entries:
+            fu, fi = unit_layout(df.frame.params)
+            fi = fi.copy()
+            fi[:, 7] += u0
+            units.append(fu)
+            items.append(fi)
+            self.frame_units.append((u0, u0 + len(fu)))
+            self.frame_items.append((n0, n0 + len(fi)))
+            u0 += len(fu)
+            n0 += len(fi)
+        self.units = np.concatenate(units).astype(np.int32)
+        self.frame_of_unit = np.concatenate([np.full(len(fu), k, np.int32) for k, fu in enumerate(units)])
+        items = np.concatenate(items).astype(np.int32)
+        self.U, self.n = len(self.units), len(items)
+        dev = self.entries[0][1][0].device
         self.items_host = items
         self.items = torch.from_numpy(items).to(dev)
         self.unit_of_item = self.items[:, 7].long()
-        self.plane_of_unit = torch.from_numpy(units[:, 0]).to(dev).long()
+        self.plane_of_unit = torch.from_numpy(self.units[:, 0]).to(dev).long()
         self.win = torch.empty((self.n, WS, WS), dtype=torch.float32, device=dev)
         self.u = torch.empty((self.n, T, T), dtype=torch.float32, device=dev)
         self.src = torch.empty((self.n, T, T), dtype=torch.float32, device=dev)
-        pd = api.planes_struct(deblocked, df.bps)
-        pc = api.planes_struct(cdef, df.bps)
-        ps = api.planes_struct(source, df.bps)
-        ctx.check(api.lib().rvpf_lrd_gather(ctx.h, C.byref(self.params), C.byref(pd), C.byref(pc), C.byref(ps),
-                                            _ptr(self.items), self.n, self.U, _ptr(self.win), _ptr(self.u),
-                                            _ptr(self.src), api._stream_handle(stream)))
+        for k, (df, deb, cd, so) in enumerate(self.entries):
+            i0, i1 = self.frame_items[k]
+            if i1 == i0:
+                continue
+            pd, pc, ps = (api.planes_struct(x, df.bps) for x in (deb, cd, so))
+            ctx.check(api.lib().rvpf_lrd_gather(ctx.h, C.byref(df.frame.params), C.byref(pd), C.byref(pc),
+                                                C.byref(ps), _ptr(self.items[i0:]), i1 - i0, self.U,
+                                                _ptr(self.win[i0:]), _ptr(self.u[i0:]), _ptr(self.src[i0:]),
+                                                api._stream_handle(stream)))
         # chroma Wiener filters have no outermost tap (lr.rs:247)
         self.tap_mask = torch.ones((self.U, 2, 3), dtype=torch.float32, device=dev)
         self.tap_mask[self.plane_of_unit > 0, :, 0] = 0.0
+
+    # single-frame conveniences
+    @property
+    def df(self):
+        return self.entries[0][0]
+
+    @property
+    def params(self):
+        return self.entries[0][0].frame.params
 
     # -- kernels ---------------------------------------------------------
     def per_unit(self, per_item):
@@ -227,13 +255,17 @@ class Search:
         return self.per_unit(sse), g
 
     def sgr_filter(self, sets):
-        """Relaxed filter outputs [n, 2, 64, 64] for sets[U] (int)."""
+        """Relaxed filter outputs [n, 2, 64, 64] for sets (one int, or [U])."""
         torch = self.torch
         sets = torch.as_tensor(sets, dtype=torch.int32, device=self.src.device).expand(self.U).contiguous()
         flt = torch.empty((self.n, 2, T, T), dtype=torch.float32, device=self.src.device)
-        self.ctx.check(api.lib().rvpf_lrd_sgr_filter(self.ctx.h, self.params.bit_depth, _ptr(self.items), self.n,
-                                                     self.U, _ptr(self.win), _ptr(self.u), _ptr(sets), _ptr(flt),
-                                                     api._stream_handle(self.stream)))
+        for k, (df, _, _, _) in enumerate(self.entries):
+            i0, i1 = self.frame_items[k]
+            if i1 == i0:
+                continue
+            self.ctx.check(api.lib().rvpf_lrd_sgr_filter(
+                self.ctx.h, df.frame.params.bit_depth, _ptr(self.items[i0:]), i1 - i0, self.U, _ptr(self.win[i0:]),
+                _ptr(self.u[i0:]), _ptr(sets), _ptr(flt[i0:]), api._stream_handle(self.stream)))
         return flt
 
     def sgr_forward(self, w, flt):
@@ -364,58 +396,75 @@ class Search:
 
     def exact_sse(self, recs):
         """Exact (integer) squared error per unit with the units' filters
-        recs[U, 12] (rvpf_lr over the frame, then per-rectangle errors);
+        recs[U, 12] (rvpf_lr over each frame, then per-rectangle errors);
         recs None: the CDEF output itself."""
         torch = self.torch
         dev = self.src.device
-        if recs is None:
-            planes = self.cdef
-        else:
-            params = abi.RvpfFrameParams.from_buffer_copy(self.params)
-            side = abi.RvpfSideInfo.from_buffer_copy(self.df.side)
-            keep = []
-            nplanes = 1 if params.mono_chrome else 3
-            for p in range(nplanes):
-                params.frame_restoration_type[p] = abi.RESTORE_SWITCHABLE
-                sel = self.units[:, 0] == p
-                t = torch.from_numpy(np.ascontiguousarray(recs[sel])).to(dev)
-                keep.append(t)
-                side.lr_units[p] = t.data_ptr()
-            planes = [None if a is None else torch.empty_like(a) for a in self.cdef]
-            pd, pc, po = (api.planes_struct(x, self.df.bps) for x in (self.deblocked, self.cdef, planes))
-            self.ctx.check(api.lib().rvpf_lr(self.ctx.h, C.byref(params), C.byref(side), C.byref(pd), C.byref(pc),
-                                             C.byref(po), api._stream_handle(self.stream)))
         out = torch.empty(self.n, dtype=torch.float64, device=dev)
-        pa, pb = api.planes_struct(planes, self.df.bps), api.planes_struct(self.source, self.df.bps)
-        self.ctx.check(api.lib().rvpf_lrd_rect_sse(self.ctx.h, C.byref(self.params), _ptr(self.items), self.n, self.U,
-                                                   C.byref(pa), C.byref(pb), _ptr(out),
-                                                   api._stream_handle(self.stream)))
+        for k, (df, deb, cd, so) in enumerate(self.entries):
+            i0, i1 = self.frame_items[k]
+            u0, u1 = self.frame_units[k]
+            if i1 == i0:
+                continue
+            if recs is None:
+                planes = cd
+            else:
+                params = abi.RvpfFrameParams.from_buffer_copy(df.frame.params)
+                side = abi.RvpfSideInfo.from_buffer_copy(df.side)
+                keep = []
+                nplanes = 1 if params.mono_chrome else 3
+                fu = self.units[u0:u1]
+                for p in range(nplanes):
+                    params.frame_restoration_type[p] = abi.RESTORE_SWITCHABLE
+                    t = torch.from_numpy(np.ascontiguousarray(recs[u0:u1][fu[:, 0] == p])).to(dev)
+                    keep.append(t)
+                    side.lr_units[p] = t.data_ptr()
+                planes = [None if x is None else torch.empty_like(x) for x in cd]
+                pd, pc, po = (api.planes_struct(x, df.bps) for x in (deb, cd, planes))
+                self.ctx.check(api.lib().rvpf_lr(self.ctx.h, C.byref(params), C.byref(side), C.byref(pd),
+                                                 C.byref(pc), C.byref(po), api._stream_handle(self.stream)))
+            pa, pb = api.planes_struct(planes, df.bps), api.planes_struct(so, df.bps)
+            self.ctx.check(api.lib().rvpf_lrd_rect_sse(self.ctx.h, C.byref(df.frame.params), _ptr(self.items[i0:]),
+                                                       i1 - i0, self.U, C.byref(pa), C.byref(pb), _ptr(out[i0:]),
+                                                       api._stream_handle(self.stream)))
         return self.per_unit(out)
 
     def search(self, lam, sets=SGR_SETS_NORMAL, wiener_steps=3):
-        """The per-unit choice of lr.rs:153-186: returns (recs [U, 12] of
-        the chosen rvpf_lr_unit bytes, cost [U], frame_type per plane)."""
+        """The per-unit choice of lr.rs:153-186, with lam one Lagrange
+        multiplier or one per frame: returns (recs [U, 12] of the chosen
+        rvpf_lr_unit bytes, cost [U], frame types [frame][plane])."""
         torch = self.torch
-        best = self.exact_sse(None) + lam * RATE_NONE
-        choice = np.zeros((self.U, 12), dtype=np.uint8)
+        dev = self.src.device
+        lam_u = torch.as_tensor(np.asarray(lam, dtype=np.float64), device=dev)
+        if lam_u.dim() == 1:
+            lam_u = lam_u[torch.from_numpy(self.frame_of_unit).to(dev).long()]
+        cands = [np.zeros((self.U, 12), dtype=np.uint8)]
+        costs = [self.exact_sse(None) + lam_u * RATE_NONE]
         taps = self.fit_wiener(wiener_steps)
         rw = self.unit_records("wiener", self.quantise_wiener(taps))
-        cost = self.exact_sse(rw) + lam * torch.where(self.plane_of_unit == 0, RATE_WIENER_LUMA, RATE_WIENER_CHROMA)
-        better = (cost < best).cpu().numpy()
-        choice[better] = rw[better]
-        best = torch.minimum(best, cost)
-        for s in sets:
-            w, _ = self.fit_sgr(s)
-            rs = self.unit_records("sgr", self.quantise_sgr(w, s), s)
-            cost = self.exact_sse(rs) + lam * RATE_SGR
-            better = (cost < best).cpu().numpy()
-            choice[better] = rs[better]
-            best = torch.minimum(best, cost)
-        nplanes = 1 if self.params.mono_chrome else 3
-        ftype = []
-        for p in range(nplanes):
-            t = choice[self.units[:, 0] == p, 0]
-            w, s = bool((t == abi.RESTORE_WIENER).any()), bool((t == abi.RESTORE_SGRPROJ).any())
-            ftype.append(abi.RESTORE_SWITCHABLE if w and s else abi.RESTORE_WIENER if w else
-                         abi.RESTORE_SGRPROJ if s else abi.RESTORE_NONE)
-        return choice, best, ftype
+        cands.append(rw)
+        costs.append(self.exact_sse(rw) + lam_u * torch.where(self.plane_of_unit == 0, RATE_WIENER_LUMA,
+                                                              RATE_WIENER_CHROMA))
+        for s_ in sets:
+            w, _ = self.fit_sgr(s_)
+            rs = self.unit_records("sgr", self.quantise_sgr(w, s_), s_)
+            cands.append(rs)
+            costs.append(self.exact_sse(rs) + lam_u * RATE_SGR)
+        # the first candidate of least cost (strict <, in the reference's order)
+        c = torch.stack(costs, dim=1).cpu().numpy()
+        pick = np.argmin(c, axis=1)
+        choice = np.stack(cands, axis=0)[pick, np.arange(self.U)]
+        best = torch.from_numpy(c[np.arange(self.U), pick]).to(dev)
+        ftypes = []
+        for k, (df, _, _, _) in enumerate(self.entries):
+            u0, u1 = self.frame_units[k]
+            fu, fc = self.units[u0:u1], choice[u0:u1]
+            nplanes = 1 if df.frame.params.mono_chrome else 3
+            ft = []
+            for p in range(nplanes):
+                t = fc[fu[:, 0] == p, 0]
+                w_, s2 = bool((t == abi.RESTORE_WIENER).any()), bool((t == abi.RESTORE_SGRPROJ).any())
+                ft.append(abi.RESTORE_SWITCHABLE if w_ and s2 else abi.RESTORE_WIENER if w_ else
+                          abi.RESTORE_SGRPROJ if s2 else abi.RESTORE_NONE)
+            ftypes.append(ft)
+        return choice, best, (ftypes[0] if len(ftypes) == 1 else ftypes)

@@ -229,3 +229,16 @@ def test_search_matches_reference(cases):
         _say("search", c.f.params.frame_width, "same", same, "of", c.s.U, "cost", cost.sum(), "ref", total_ref)
         assert same >= 0.9 * c.s.U
         assert cost.sum() <= total_ref * 1.001
+
+
+def test_multi_frame_search_equals_per_frame(ctx, cases):
+    """One Search over several frames (one launch per kernel) chooses as the
+    per-frame searches do."""
+    multi = lrdiff.Search(ctx, [(c.df, c.deb, c.cdef, c.src) for c in cases])
+    lams = [lrdiff.rd_lambda(c.f.bit_depth, c.f.cfg.qindex) for c in cases]
+    choice, cost, ftypes = multi.search(lams)
+    for k, c in enumerate(cases):
+        ch1, co1, ft1 = c.s.search(lams[k])
+        u0, u1 = multi.frame_units[k]
+        assert np.array_equal(choice[u0:u1], ch1)
+        assert ftypes[k] == ft1
