@@ -3,7 +3,7 @@
  *
  * Follows rivet-av1's grain::apply (src/decoder/grain.rs:8-354).
  *
- * grain_prepare_kernel: one workgroup of two waves per frame.
+ * grain_prepare_kernel: one workgroup of three waves per frame.
  *   - Gaussian templates (7.18.3.3): luma 73x82, cb/cr 38x44 (4:2:0) or
  *     larger. The 16-bit LFSR is linear over GF(2), so lane t jumps to the
  *     state after t*C steps with precomputed powers A^(2^k) of the step
@@ -11,8 +11,11 @@
  *   - The auto-regressive filter is a recursion: sample (y, x) reads rows
  *     y-lag..y and columns x-lag..x+lag. Processing it along the diagonals
  *     T = (lag+1)*y + x visits every dependency at an earlier T, so all
- *     samples of one T run in parallel (one lane per template row); the
- *     chroma recursion runs on the second wave, interleaved with luma.
+ *     samples of one T run in parallel (one lane per template row). The
+ *     rows above a lane's row belong to the lanes below it, so the taps
+ *     are passed between lanes by DPP rotations and stay in registers.
+ *     Luma runs on wave 0, cb and cr on waves 1 and 2, a fixed number of
+ *     diagonals behind the luma samples they read.
  *   - Scaling LUTs (7.18.3.4) and the per-stripe random block offsets.
  * grain_apply_kernel: one thread per chroma sample (and the 1, 2 or 4 luma
  *   samples it covers). The noise of a sample is rebuilt directly from the
@@ -75,6 +78,7 @@ __device__ __forceinline__ int round2(int x, int n) { return (x + ((1 << n) >> 1
 __device__ __forceinline__ int clip3(int lo, int hi, int x) { return min(max(x, lo), hi); }
 
 constexpr int TW = 82, TH = 73;
+constexpr int PREP_THREADS = 192; /* three waves: luma, cb, cr */
 
 /* Gaussian values of the LFSR sequence from `seed` into t (rows x cols, stride TW). */
 __device__ void gaussian_fill(int16_t *t, const int16_t *gauss, uint32_t seed, int rows, int cols, int shift,
@@ -98,124 +102,176 @@ __device__ void gaussian_fill(int16_t *t, const int16_t *gauss, uint32_t seed, i
   }
 }
 
-/* One diagonal step of the luma AR of 7.18.3.3 (grain.rs:57-73): the
- * samples (y, x) with (LAG+1)*y + x = T, one lane each. Four partial sums
- * keep the multiply-adds out of one dependency chain. */
-template <int LAG>
-__device__ __forceinline__ void luma_step(int16_t *tl, const int *c, int T, int lane, int ars, int gmin, int gmax) {
-  constexpr int K = LAG + 1;
-  const int ymin = max(3, (T - 78 + K - 1) / K);
-  const int y = ymin + lane;
-  const int x = T - K * y;
-  if (y > 72 || x < 3) return;
-  int s[4] = {0, 0, 0, 0}, pos = 0;
-#pragma unroll
-  for (int dr = -LAG; dr <= 0; dr++)
-#pragma unroll
-    for (int dc = -LAG; dc <= LAG; dc++) {
-      if (dr == 0 && dc >= 0) continue;
-      s[pos & 3] += __mul24((int)tl[(y + dr) * TW + x + dc], c[pos]);
-      pos++;
-    }
-  tl[y * TW + x] = (int16_t)clip3(gmin, gmax, tl[y * TW + x] + round2((s[0] + s[1]) + (s[2] + s[3]), ars));
-}
-
-/* One chroma AR sample (grain.rs:101-134): cb and cr together, plus the
- * tap on the co-located (averaged) luma. */
-template <int LAG>
-__device__ __forceinline__ void chroma_one(int16_t *tcb, int16_t *tcr, const int16_t *tl, const int *c0,
-                                           const int *c1, const GrainHdr &h, int y, int x, int gmin, int gmax) {
-  constexpr int NC = 2 * LAG * (LAG + 1) + 1;
-  int s0[2] = {0, 0}, s1[2] = {0, 0}, pos = 0;
-#pragma unroll
-  for (int dr = -LAG; dr <= 0; dr++)
-#pragma unroll
-    for (int dc = -LAG; dc <= LAG; dc++) {
-      if (dr == 0 && dc >= 0) continue;
-      s0[pos & 1] += __mul24((int)tcb[(y + dr) * TW + x + dc], c0[pos]);
-      s1[pos & 1] += __mul24((int)tcr[(y + dr) * TW + x + dc], c1[pos]);
-      pos++;
-    }
-  int a0 = s0[0] + s0[1], a1 = s1[0] + s1[1];
+/* One chroma AR sample (grain.rs:101-134) with no neighbour taps (lag 0):
+ * cb and cr from the co-located (averaged) luma alone. */
+__device__ __forceinline__ void chroma_lag0(int16_t *tcb, int16_t *tcr, const int16_t *tl, const GrainHdr &h, int y,
+                                            int x, int gmin, int gmax) {
+  int a0 = 0, a1 = 0;
   if (h.num_y > 0) {
     const int lx = ((x - 3) << h.ssx) + 3, ly = ((y - 3) << h.ssy) + 3;
     int l = 0;
     for (int i = 0; i <= h.ssy; i++)
       for (int j = 0; j <= h.ssx; j++) l += tl[(ly + i) * TW + lx + j];
     l = round2(l, h.ssx + h.ssy);
-    a0 += __mul24(l, c0[NC - 1]);
-    a1 += __mul24(l, c1[NC - 1]);
+    a0 = l * h.ar_cb[0];
+    a1 = l * h.ar_cr[0];
   }
   tcb[y * TW + x] = (int16_t)clip3(gmin, gmax, tcb[y * TW + x] + round2(a0, h.ar_shift));
   tcr[y * TW + x] = (int16_t)clip3(gmin, gmax, tcr[y * TW + x] + round2(a1, h.ar_shift));
 }
 
-/* One diagonal step of the chroma AR: the samples with
- * (LAG+1)*y + x = T, 3 <= x <= cw-4, 3 <= y < ch. */
-template <int LAG>
-__device__ __forceinline__ void chroma_step(int16_t *tcb, int16_t *tcr, const int16_t *tl, const int *c0,
-                                            const int *c1, const GrainHdr &h, int cw, int ch, int T, int lane,
-                                            int gmin, int gmax) {
-  constexpr int K = LAG + 1;
-  const int xmax = cw - 4;
-  const int ymin = max(3, (T - xmax + K - 1) / K);
-  for (int y = ymin + lane; y < ch; y += 64) {
-    const int x = T - K * y;
-    if (x < 3) break;
-    chroma_one<LAG>(tcb, tcr, tl, c0, c1, h, y, x, gmin, gmax);
+/* Lane l receives the value of lane (l - 1) mod 64 (DPP wave_ror:1). */
+__device__ __forceinline__ int from_lane_below(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, 0x13C, 0xF, 0xF, false);
+}
+
+/* Diagonals per round of the AR recursion (one workgroup barrier each). */
+constexpr int AR_ROUND = 16;
+
+/* Schedule of the three AR waves: rounds of AR_ROUND diagonals, a barrier
+ * after each. Wave 0 (luma) runs its round r in barrier round r. A chroma
+ * round j reads luma samples finished by diagonal 2*(16j+15) - 2K - 2
+ * (subsampled) or 16j+15 (4:4:4), so it runs in barrier round 2j+2 or j+1,
+ * when every luma round it depends on is behind a barrier. */
+struct ArPlan {
+  int first, every, rounds;
+};
+
+__device__ __forceinline__ int ar_rounds(int K, int rows, int cols) {
+  return (K * (rows - 1) + cols - 1) / AR_ROUND + 1;
+}
+
+/* The auto-regressive filter of 7.18.3.3 (grain.rs:57-73 luma, 101-134
+ * chroma) over one template, on one wave. Samples are visited along the
+ * diagonals T = K*y + x (K = LAG + 1): every sample (y - d, x + dc) that
+ * (y, x) reads lies on an earlier diagonal, T - (d*K - dc).
+ *
+ * Lane l owns the template rows y = (l + 3) mod 64 and y + 64 and handles,
+ * on diagonal T, the sample of its row at x = T - K*y (the two rows of a
+ * lane are never both inside the template on one diagonal). Each lane
+ * emits one value per diagonal: the filtered sample inside the filtered
+ * area, the unfiltered template value on the 3-sample border. It keeps the
+ * last 16 values emitted by itself (O) and by the lanes 1, 2 and 3 below
+ * it (N1, N2, N3, each filled by one lane rotation of the previous array
+ * per diagonal), which are exactly the rows above its row: sample
+ * (y - d, x + dc) is N_d[T - (d*K - dc)] (O for d = 0). The taps therefore
+ * come from registers, in registers indexed by T mod 16 (the rounds of 16
+ * diagonals are unrolled, so every index is a constant).
+ *
+ * CHROMA adds the tap on the co-located luma average (read from the finished
+ * luma template in LDS). coef holds the NL neighbour coefficients (then the
+ * luma one). */
+template <int LAG, bool CHROMA>
+__device__ void ar_wave(int16_t *t, int rows, int cols, const int *coef, int luma_coef, const int16_t *tl, int lsx,
+                        int lsy, bool luma_tap, int ars, int gmin, int gmax, ArPlan plan) {
+  constexpr int K = LAG + 1, NL = 2 * LAG * K;
+  const int lane = threadIdx.x & 63;
+  const int ya = (lane + 3) & 63;
+  int c[NL > 0 ? NL : 1];
+#pragma unroll
+  for (int i = 0; i < NL; i++) c[i] = coef[i];
+  int O[AR_ROUND], N1[AR_ROUND], N2[AR_ROUND], N3[AR_ROUND];
+#pragma unroll
+  for (int i = 0; i < AR_ROUND; i++) O[i] = N1[i] = N2[i] = N3[i] = 0;
+  const int nchunks = ar_rounds(K, rows, cols);
+  int chunk = 0;
+  for (int r = 0; r < plan.rounds; r++) {
+    if (r >= plan.first && (r - plan.first) % plan.every == 0 && chunk < nchunks) {
+      const int T0 = chunk * AR_ROUND;
+      int tv[AR_ROUND], lv[AR_ROUND], addr[AR_ROUND];
+      bool act[AR_ROUND];
+      /* every sample of this round is read before any is written: each
+       * template position is written only on its own diagonal */
+#pragma unroll
+      for (int s = 0; s < AR_ROUND; s++) {
+        int x = T0 + s - K * ya, y = ya;
+        if (x >= cols) {
+          x -= 64 * K;
+          y += 64;
+        }
+        const bool valid = y < rows && x >= 0 && x < cols;
+        act[s] = valid && y >= 3 && x >= 3 && x < cols - 3;
+        addr[s] = valid ? y * TW + x : 0;
+        tv[s] = t[addr[s]];
+        lv[s] = 0;
+        if (CHROMA && luma_tap) {
+          const int ly = act[s] ? ((y - 3) << lsy) + 3 : 0, lx = act[s] ? ((x - 3) << lsx) + 3 : 0;
+          const int16_t *q = tl + ly * TW + lx;
+          int l = q[0];
+          if (lsx) l += q[1];
+          if (lsy) l += q[TW];
+          if (lsx && lsy) l += q[TW + 1];
+          lv[s] = round2(l, lsx + lsy);
+        }
+      }
+#pragma unroll
+      for (int s = 0; s < AR_ROUND; s++) {
+        int acc[4] = {0, 0, 0, 0};
+        int pos = 0;
+#pragma unroll
+        for (int d = LAG; d >= 0; d--)
+#pragma unroll
+          for (int dc = -LAG; dc <= (d ? LAG : -1); dc++) {
+            const int idx = (s - (d * K - dc)) & (AR_ROUND - 1);
+            const int v = d == 0 ? O[idx] : d == 1 ? N1[idx] : d == 2 ? N2[idx] : N3[idx];
+            acc[pos & 3] += __mul24(v, c[pos]);
+            pos++;
+          }
+        int sum = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+        if (CHROMA && luma_tap) sum += __mul24(lv[s], luma_coef);
+        const int out = act[s] ? clip3(gmin, gmax, tv[s] + round2(sum, ars)) : tv[s];
+        if (act[s]) t[addr[s]] = (int16_t)out;
+        O[s] = out;
+        N1[s] = from_lane_below(O[s]);
+        N2[s] = from_lane_below(N1[s]);
+        N3[s] = from_lane_below(N2[s]);
+      }
+      chunk++;
+    }
+    __syncthreads();
   }
 }
 
-/* Both AR recursions in lock step: wave 0 walks the luma diagonals
- * T = K*y + x (K = LAG+1) at global step G = T; wave 1 walks the chroma
- * ones at G = 2T - 2K - 1 (subsampled chroma) or G = T + 1 (4:4:4). Every
- * luma sample a chroma sample (y, x) reads was finished at a smaller G:
- * for 4:2:0 the luma samples (2y-3+i, 2x-3+j), i, j in {0, 1}, finish by
- * G = 2T - 2K - 2; for 4:2:2, (y, 2x-3+j) finish by T + x - 2 <= 2T - 3K - 2;
- * for 4:4:4, (y, x) finishes at G = T. */
+/* The AR stage of the three templates: wave 0 luma, waves 1 and 2 cb and
+ * cr (on the schedule of ArPlan). Lag 0 has no recursion: luma samples are
+ * only clipped and each chroma sample depends on the finished luma alone. */
 template <int LAG>
-__device__ void ar_both(int16_t *tl, int16_t *tcb, int16_t *tcr, const GrainHdr &h, bool chroma, int cw, int ch,
-                        int gmin, int gmax) {
-  constexpr int K = LAG + 1, NL = 2 * LAG * (LAG + 1), NC = NL + 1;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  int cl[NL > 0 ? NL : 1], c0[NC], c1[NC];
-#pragma unroll
-  for (int i = 0; i < NL; i++) cl[i] = h.ar_y[i];
-#pragma unroll
-  for (int i = 0; i < NC; i++) {
-    c0[i] = h.ar_cb[i];
-    c1[i] = h.ar_cr[i];
-  }
-  if (LAG == 0) {
-    /* no neighbour taps: luma samples are only clipped, and each chroma
-     * sample depends on the (final) luma alone */
-    for (int i = threadIdx.x; i < 70 * 76; i += 128) {
+__device__ void ar_all(int16_t *tl, int16_t *tcb, int16_t *tcr, const GrainHdr &h, bool chroma, int cw, int ch,
+                       int gmin, int gmax) {
+  constexpr int K = LAG + 1;
+  const int wave = threadIdx.x >> 6;
+  if constexpr (LAG == 0) {
+    for (int i = threadIdx.x; i < 70 * 76; i += blockDim.x) {
       const int y = 3 + i / 76, x = 3 + i % 76;
       tl[y * TW + x] = (int16_t)clip3(gmin, gmax, tl[y * TW + x]);
     }
     __syncthreads();
     if (chroma) {
       const int rw = cw - 6, nr = ch - 3;
-      for (int i = threadIdx.x; i < rw * nr; i += 128) {
-        const int y = 3 + i / rw, x = 3 + i % rw;
-        chroma_one<0>(tcb, tcr, tl, c0, c1, h, y, x, gmin, gmax);
-      }
+      for (int i = threadIdx.x; i < rw * nr; i += blockDim.x) chroma_lag0(tcb, tcr, tl, h, 3 + i / rw, 3 + i % rw, gmin, gmax);
     }
     __syncthreads();
-    return;
-  }
+  } else {
+  constexpr int NL = 2 * LAG * K;
   const int sub = h.ssx | h.ssy;
-  const int tl_lo = 3 * K + 3, tl_hi = 72 * K + 78;
-  const int tc_lo = 3 * K + 3, tc_hi = (ch - 1) * K + (cw - 4);
-  const int g_hi = max(tl_hi, chroma ? (sub ? 2 * tc_hi - 2 * K - 1 : tc_hi + 1) : 0);
-  for (int G = tl_lo; G <= g_hi; G++) {
-    if (wave == 0) {
-      if (G <= tl_hi) luma_step<LAG>(tl, cl, G, lane, h.ar_shift, gmin, gmax);
-    } else if (chroma) {
-      const int T = sub ? ((G & 1) ? (G + 2 * K + 1) >> 1 : -1) : G - 1;
-      if (T >= tc_lo && T <= tc_hi) chroma_step<LAG>(tcb, tcr, tl, c0, c1, h, cw, ch, T, lane, gmin, gmax);
-    }
-    __syncthreads();
+  const int lr = ar_rounds(K, TH, TW);
+  const int cr = chroma ? ar_rounds(K, ch, cw) : 0;
+  const ArPlan pl_luma = {0, 1, 0}, pl_chroma = {sub ? 2 : 1, sub ? 2 : 1, 0};
+  const int rounds = max(lr, chroma ? pl_chroma.first + pl_chroma.every * (cr - 1) + 1 : 0);
+  if (wave == 0) {
+    ArPlan p = pl_luma;
+    p.rounds = rounds;
+    ar_wave<LAG, false>(tl, TH, TW, h.ar_y, 0, tl, 0, 0, false, h.ar_shift, gmin, gmax, p);
+  } else if (wave == 1 || wave == 2) {
+    ArPlan p = pl_chroma;
+    p.rounds = chroma ? rounds : 0;
+    const int *cf = wave == 1 ? h.ar_cb : h.ar_cr;
+    int16_t *t = wave == 1 ? tcb : tcr;
+    if (chroma)
+      ar_wave<LAG, true>(t, ch, cw, cf, cf[NL], tl, h.ssx, h.ssy, h.num_y > 0, h.ar_shift, gmin, gmax, p);
+    else
+      for (int r = 0; r < rounds; r++) __syncthreads();
+  }
   }
 }
 
@@ -224,32 +280,32 @@ struct PrepShared {
   int16_t gauss[2048];
 };
 
-/* Templates, LUTs and block offsets of one frame (128 threads). */
+/* Templates, LUTs and block offsets of one frame (PREP_THREADS threads). */
 __device__ __forceinline__ void grain_prepare_body(const GrainHdr &h, const GrainScratch &sc, PrepShared &sm) {
   int16_t *tl = sm.tl, *tcb = sm.tcb, *tcr = sm.tcr, *gauss = sm.gauss;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int i = threadIdx.x; i < 2048; i += 128) gauss[i] = AV1_GAUSSIAN_SEQUENCE[i];
+  for (int i = threadIdx.x; i < 2048; i += PREP_THREADS) gauss[i] = AV1_GAUSSIAN_SEQUENCE[i];
   __syncthreads();
   const int bd = h.bd;
   const int gmin = -(128 << (bd - 8)), gmax = (256 << (bd - 8)) - 1 - (128 << (bd - 8));
   const int shift = 12 - bd + h.grain_scale_shift;
   const bool chroma = h.nplanes > 1;
   const int cw = h.ssx ? 44 : 82, ch = h.ssy ? 38 : 73;
-  if (wave == 0) {
+  if (wave == 0)
     gaussian_fill(tl, gauss, h.seed, TH, TW, shift, h.num_y > 0, lane);
-  } else if (chroma) {
+  else if (wave == 1 && chroma)
     gaussian_fill(tcb, gauss, h.seed ^ 0xb524, ch, cw, shift, h.num_cb > 0 || h.cfl, lane);
+  else if (wave == 2 && chroma)
     gaussian_fill(tcr, gauss, h.seed ^ 0x49d8, ch, cw, shift, h.num_cr > 0 || h.cfl, lane);
-  }
   __syncthreads();
   switch (h.ar_lag) {
-    case 0: ar_both<0>(tl, tcb, tcr, h, chroma, cw, ch, gmin, gmax); break;
-    case 1: ar_both<1>(tl, tcb, tcr, h, chroma, cw, ch, gmin, gmax); break;
-    case 2: ar_both<2>(tl, tcb, tcr, h, chroma, cw, ch, gmin, gmax); break;
-    default: ar_both<3>(tl, tcb, tcr, h, chroma, cw, ch, gmin, gmax); break;
+    case 0: ar_all<0>(tl, tcb, tcr, h, chroma, cw, ch, gmin, gmax); break;
+    case 1: ar_all<1>(tl, tcb, tcr, h, chroma, cw, ch, gmin, gmax); break;
+    case 2: ar_all<2>(tl, tcb, tcr, h, chroma, cw, ch, gmin, gmax); break;
+    default: ar_all<3>(tl, tcb, tcr, h, chroma, cw, ch, gmin, gmax); break;
   }
   /* templates out */
-  for (int i = threadIdx.x; i < TH * TW; i += 128) {
+  for (int i = threadIdx.x; i < TH * TW; i += PREP_THREADS) {
     sc.tmpl[i] = tl[i];
     sc.tmpl[TH * TW + i] = chroma ? tcb[i] : 0;
     sc.tmpl[2 * TH * TW + i] = chroma ? tcr[i] : 0;
@@ -258,7 +314,7 @@ __device__ __forceinline__ void grain_prepare_body(const GrainHdr &h, const Grai
   for (int p = 0; p < h.nplanes; p++) {
     const int np = h.npoints[p];
     const int *xs = h.points[p][0], *ys = h.points[p][1];
-    for (int x = threadIdx.x; x < 256; x += 128) {
+    for (int x = threadIdx.x; x < 256; x += PREP_THREADS) {
       int v = 0;
       if (np > 0) {
         if (x < xs[0]) {
@@ -279,7 +335,7 @@ __device__ __forceinline__ void grain_prepare_body(const GrainHdr &h, const Grai
   /* block offsets per noise stripe (grain.rs:186-199) */
   const int nstripes = ((h.h + 1) / 2 + 15) / 16;
   const int nblocks = ((h.w + 1) / 2 + 15) / 16;
-  for (int ln = threadIdx.x; ln < nstripes; ln += 128) {
+  for (int ln = threadIdx.x; ln < nstripes; ln += PREP_THREADS) {
     uint32_t r = h.seed;
     r ^= (uint32_t)(((ln * 37 + 178) & 255) << 8);
     r ^= (uint32_t)((ln * 173 + 105) & 255);
@@ -408,7 +464,7 @@ __device__ __forceinline__ void grain_apply_tile(const GrainHdr &h, const GrainS
   }
 }
 
-__global__ __launch_bounds__(128) void grain_prepare_kernel(GrainHdr h, GrainScratch sc) {
+__global__ __launch_bounds__(PREP_THREADS) void grain_prepare_kernel(GrainHdr h, GrainScratch sc) {
   __shared__ PrepShared sm;
   grain_prepare_body(h, sc, sm);
 }
@@ -432,7 +488,7 @@ struct ApplyRecord {
   PlanePtr in[3], out[3];
 };
 
-__global__ __launch_bounds__(128) void grain_prepare_batch(BatchWork w) {
+__global__ __launch_bounds__(PREP_THREADS) void grain_prepare_batch(BatchWork w) {
   __shared__ PrepShared sm;
   batch_loop(w, [&](int f, int) {
     const PrepRecord &r = batch_table<PrepRecord>(w.rec, f);
@@ -469,7 +525,7 @@ void launch_apply_shape(const BatchWork &w, const StagePlan &sp, hipStream_t st)
 }  // namespace
 
 int launch_grain_prepare(const GrainHdr &h, const GrainScratch &s, hipStream_t st) {
-  grain_prepare_kernel<<<1, 128, 0, st>>>(h, s);
+  grain_prepare_kernel<<<1, PREP_THREADS, 0, st>>>(h, s);
   return (int)hipGetLastError();
 }
 
@@ -484,7 +540,7 @@ int grain_prep_plan(const GrainHdr &h, const GrainScratch &s, void *rec) {
 
 int launch_grain_prepare_batch(const BatchWork &w, const StagePlan &sp, hipStream_t st) {
   if (w.total == 0) return 0;
-  grain_prepare_batch<<<sp.grid, 128, 0, st>>>(w);
+  grain_prepare_batch<<<sp.grid, PREP_THREADS, 0, st>>>(w);
   return (int)hipGetLastError();
 }
 

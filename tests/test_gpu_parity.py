@@ -97,3 +97,39 @@ def test_chain_u8_planes(ctx, name, kw):
     common.assert_planes(api.to_host(rec), deb, f, "full", name + " deblocked")
     common.assert_planes(api.to_host(ref_dev), ref, f, "mi", name + " reference")
     common.assert_planes(api.to_host(disp_dev), disp, f, "crop", name + " display")
+
+
+GRAIN_LAYOUTS = {"420": dict(ssx=1, ssy=1), "422": dict(ssx=1, ssy=0), "444": dict(ssx=0, ssy=0),
+                 "mono": dict(mono=True)}
+
+
+@pytest.mark.parametrize("lag", [0, 1, 2, 3])
+@pytest.mark.parametrize("layout", sorted(GRAIN_LAYOUTS))
+@pytest.mark.parametrize("bd", [8, 10, 12])
+def test_grain_lags(ctx, lag, layout, bd):
+    """Film grain alone (templates, LUTs, offsets, blend) at every AR lag,
+    chroma layout and bit depth, on random planes."""
+    _say("grain lag", lag, layout, bd)
+    f = common.make_frame(width=98, height=70, bit_depth=bd, qindex=120, lr=0, grain=True, grain_lag=lag,
+                          seed=100 + 10 * lag + bd, **GRAIN_LAYOUTS[layout])
+    g = f.params.grain
+    if f.num_planes > 1 and not g.chroma_scaling_from_luma:
+        # every template in use: cr takes cb's scaling points (or four fixed ones)
+        if g.num_cb_points == 0:
+            g.num_cb_points = 4
+            for i, (x, y) in enumerate([(16, 40), (80, 200), (150, 90), (240, 160)]):
+                g.point_cb_value[i], g.point_cb_scaling[i] = x, y
+        g.num_cr_points = g.num_cb_points
+        for i in range(g.num_cb_points):
+            g.point_cr_value[i], g.point_cr_scaling[i] = g.point_cb_value[i], 255 - g.point_cb_scaling[i]
+    rng = np.random.default_rng(lag * 100 + bd)
+    planes = [None if p >= f.num_planes else
+              rng.integers(0, 1 << bd, size=f.crop_shape(p), dtype=np.uint16) for p in range(3)]
+    want = [None if a is None else a.copy() for a in planes]
+    pyoracle.grain_apply(f.params, f.params.upscaled_width, f.params.frame_height, want)
+    df = api.DeviceFrame(f)
+    src = df.upload_planes(planes)
+    disp = df.alloc_display()
+    api.grain(ctx, df, src, disp)
+    ctx.sync()
+    common.assert_planes(api.to_host(disp), want, f, "crop", "grain lag %d %s %d-bit" % (lag, layout, bd))
