@@ -66,17 +66,38 @@ __device__ void stage_tile(uint16_t *t, uint16_t *sh, const T *src, int64_t stri
                            int h, int xlim, int ylim) {
   const int tw = w + 4, th = h + 4;
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-  for (int r = ty; r < th; r += 4) {
-    const int y = y0 + r - 2;
-    const bool yok = y >= 0 && y < ylim;
-    const T *row = src + (int64_t)y * stride;
-    for (int c = tx; c < tw; c += 64) {
-      const int x = x0 + c - 2;
-      const uint16_t v = (uint16_t)((yok && x >= 0 && x < xlim) ? (int)row[x] : NA_S);
-      t[r * TS + c] = v;
-      if (c > 0) sh[r * TS + c - 1] = v;
+  const int xa = x0 + tx - 2, xb = xa + 64;
+  const bool xa_ok = xa >= 0 && xa < xlim, xb_ok = tx + 64 < tw && xb < xlim;
+  const int xac = min(max(xa, 0), xlim - 1), xbc = min(xb, xlim - 1);
+  /* ROWS rows per batch: every load (clamped into the plane) is issued
+   * before the LDS stores */
+  constexpr int ROWS = 6;
+  for (int r0 = ty; r0 < th; r0 += 4 * ROWS) {
+    uint16_t va[ROWS], vb[ROWS];
+#pragma unroll
+    for (int k = 0; k < ROWS; k++) {
+      const int y = y0 + r0 + 4 * k - 2;
+      const bool yok = y >= 0 && y < ylim;
+      const T *row = src + (int64_t)min(max(y, 0), ylim - 1) * stride;
+      const int a = row[xac], b = row[xbc];
+      va[k] = (uint16_t)(yok && xa_ok ? a : NA_S);
+      vb[k] = (uint16_t)(yok && xb_ok ? b : NA_S);
     }
-    if (tx == 0) sh[r * TS + tw - 1] = (uint16_t)NA_S;
+#pragma unroll
+    for (int k = 0; k < ROWS; k++) {
+      const int r = r0 + 4 * k;
+      if (r < th) {
+        if (tx < tw) {
+          t[r * TS + tx] = va[k];
+          if (tx > 0) sh[r * TS + tx - 1] = va[k];
+        }
+        if (tx + 64 < tw) {
+          t[r * TS + tx + 64] = vb[k];
+          sh[r * TS + tx + 63] = vb[k];
+        }
+        if (tx == 0) sh[r * TS + tw - 1] = (uint16_t)NA_S;
+      }
+    }
   }
 }
 
@@ -278,9 +299,18 @@ __device__ __forceinline__ void cdef_tile(const CdefLaunch &a, int tile_id, uint
       const int ssx = p ? g.ssx : 0, ssy = p ? g.ssy : 0;
       const int w = bw >> ssx, h = bh >> ssy, px = x0 >> ssx, py = y0 >> ssy;
       if (tx < w) {
-        const T *src = pp<T>(a.in[p]) + (int64_t)(py + ty) * a.in[p].stride + px + tx;
-        T *dst = pp<T>(a.out[p]) + (int64_t)(py + ty) * a.out[p].stride + px + tx;
-        for (int r = ty; r < h; r += 4, src += 4 * a.in[p].stride, dst += 4 * a.out[p].stride) *dst = *src;
+        const T *src = pp<T>(a.in[p]) + (int64_t)py * a.in[p].stride + px + tx;
+        T *dst = pp<T>(a.out[p]) + (int64_t)py * a.out[p].stride + px + tx;
+        /* all loads of a batch (rows clamped to the block) before the stores */
+        constexpr int ROWS = 8;
+        for (int r0 = ty; r0 < h; r0 += 4 * ROWS) {
+          T v[ROWS];
+#pragma unroll
+          for (int k = 0; k < ROWS; k++) v[k] = src[(int64_t)min(r0 + 4 * k, h - 1) * a.in[p].stride];
+#pragma unroll
+          for (int k = 0; k < ROWS; k++)
+            if (r0 + 4 * k < h) dst[(int64_t)(r0 + 4 * k) * a.out[p].stride] = v[k];
+        }
       }
     }
     return;

@@ -262,57 +262,57 @@ __device__ __forceinline__ void lf_block(const LfLaunch &a, int gp, int lb) {
   const int64_t stride = a.planes[plane].stride;
   const int xp = ex * 4, yp = ey * 4;
   if (PASS == 0) {
-    /* vertical edge at column xp: each lane is a row */
-#pragma unroll 1
+    /* vertical edge at column xp: each lane is a row. All 16 loads of
+     * the 4 rows are issued before the first filter; the outer quads
+     * (x - 8, x + 4) are only used by 16-wide edges, which have x >= 8 */
+    T *row0 = base + (int64_t)yp * stride + xp;
+    const int lo8 = xp >= 8 ? -8 : -4;
+    int s[4][16];
+#pragma unroll
     for (int l = 0; l < 4; l++) {
-      T *row = base + (int64_t)(yp + l) * stride + xp;
-      int s[16];
-      if (fs == 16) {
-        Io<T>::ld4(row - 8, s);
-        Io<T>::ld4(row - 4, s + 4);
-        Io<T>::ld4(row, s + 8);
-        Io<T>::ld4(row + 4, s + 12);
-      } else {
-        Io<T>::ld4(row - 4, s + 4);
-        Io<T>::ld4(row, s + 8);
-      }
+      T *row = row0 + (int64_t)l * stride;
+      Io<T>::ld4(row + lo8, s[l]);
+      Io<T>::ld4(row - 4, s[l] + 4);
+      Io<T>::ld4(row, s[l] + 8);
+      Io<T>::ld4(row + 4, s[l] + 12);
+    }
+#pragma unroll
+    for (int l = 0; l < 4; l++) {
+      T *row = row0 + (int64_t)l * stride;
       int v[14];
 #pragma unroll
-      for (int k = 0; k < 14; k++) v[k] = s[k + 1]; /* v[k] = sample at xp - 7 + k */
+      for (int k = 0; k < 14; k++) v[k] = s[l][k + 1]; /* v[k] = sample at xp - 7 + k */
       const int r = filter_lane(v, limit_bd, blimit_bd, thresh_bd, fs, chroma, g.bd);
       if (r == 0) continue;
 #pragma unroll
-      for (int k = 0; k < 14; k++) s[k + 1] = v[k];
+      for (int k = 0; k < 14; k++) s[l][k + 1] = v[k];
       if (fs == 4) {
-        Io<T>::st2(row - 2, s[6], s[7]);
-        Io<T>::st2(row, s[8], s[9]);
+        Io<T>::st2(row - 2, s[l][6], s[l][7]);
+        Io<T>::st2(row, s[l][8], s[l][9]);
       } else if (fs == 8) {
-        Io<T>::st4(row - 4, s + 4);
-        Io<T>::st4(row, s + 8);
+        Io<T>::st4(row - 4, s[l] + 4);
+        Io<T>::st4(row, s[l] + 8);
       } else {
-        Io<T>::st4(row - 8, s);
-        Io<T>::st4(row - 4, s + 4);
-        Io<T>::st4(row, s + 8);
-        Io<T>::st4(row + 4, s + 12);
+        Io<T>::st4(row - 8, s[l]);
+        Io<T>::st4(row - 4, s[l] + 4);
+        Io<T>::st4(row, s[l] + 8);
+        Io<T>::st4(row + 4, s[l] + 12);
       }
     }
   } else {
-    /* horizontal edge at row yp: each lane is a column */
+    /* horizontal edge at row yp: each lane is a column. The 14 row loads
+     * are issued together; rows beyond +-4 are only used by 16-wide edges,
+     * which have y >= 8 (rows above 0 repeat row yp - 4 otherwise) */
     T *col = base + (int64_t)yp * stride + xp;
-    const int reach = fs == 16 ? 7 : 4;
     int v[4][14];
 #pragma unroll
     for (int k = 0; k < 14; k++) {
       const int d = k - 7;
-      if (d >= -reach && d < reach) {
-        int q[4];
-        Io<T>::ld4(col + (int64_t)d * stride, q);
+      const int dd = (d < -4 && yp < 8) ? -4 : d;
+      int q[4];
+      Io<T>::ld4(col + (int64_t)dd * stride, q);
 #pragma unroll
-        for (int l = 0; l < 4; l++) v[l][k] = q[l];
-      } else {
-#pragma unroll
-        for (int l = 0; l < 4; l++) v[l][k] = 0;
-      }
+      for (int l = 0; l < 4; l++) v[l][k] = q[l];
     }
     int any = 0;
 #pragma unroll

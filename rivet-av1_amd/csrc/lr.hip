@@ -5,19 +5,25 @@
  * restore_stripe / LrCtx::sample / restore_rect_from_window /
  * box_filter_rect / wiener_coefficient (src/decoder/postfilter.rs:572-1004).
  *
- * One workgroup (512 threads) per (plane, stripe, slice): a stripe is 64
- * luma rows offset by -8, a slice is min(64, unit size) columns, so a slice
- * lies in one restoration unit (the last unit of a row absorbs the
- * remainder: its slices map to it by clamping the unit column). The
- * workgroup stages its window (slice + 3-sample border) in LDS with the
+ * One workgroup (256 threads, 4 waves) per (plane, stripe, slice): a
+ * stripe is 64 luma rows offset by -8, a slice is min(64, unit size)
+ * columns, so a slice lies in one restoration unit (the last unit of a row
+ * absorbs the remainder: its slices map to it by clamping the unit column).
+ * The workgroup stages its window (slice + 3-sample border) in LDS with the
  * stripe rule of 7.17.6 (rows above / below the stripe come from the
- * deblocked frame, at most 2 rows out), then runs the unit's filter:
+ * deblocked frame, at most 2 rows out). Threads form SW-wide lane groups,
+ * one per band of rows; a thread owns one column and walks down its band,
+ * so every vertical stencil slides through registers and reads one new row
+ * per output row:
  *   Wiener: horizontal 7-tap pass into an int16 LDS buffer (the clip3
- *           range fits 16 bits at 8/10/12 bits), then the vertical pass;
- *   SGR:    for each pass with r > 0, A and B per position from the
- *           (2r+1)^2 box sums (B's product in 64 bits, z saturated to 255
- *           through a 32-bit bound), the 3x3 weighting (6/5 on odd rows
- *           for r = 2, 4/3 for r = 1), then the projection.
+ *           range fits 16 bits at 8/10/12 bits), then the vertical 7-tap
+ *           pass over a 7-row register window.
+ *   SGR:    one walk per column computes the 5- and 3-row sums of the
+ *           5- and 3-sample horizontal sums (sum and sum of squares) and
+ *           from them A and B of both passes (pass 0, r = 2, on odd rows
+ *           only), each packed as (A << 20) | B; a second walk applies
+ *           the 3x3 weights (6/5 on odd rows for r = 2, 4/3 for r = 1)
+ *           from a 3-row register window and the projection.
  * Samples outside restored units, beyond the frame but inside the
  * mode-info area, are copied from the CDEF output.
  */
@@ -28,12 +34,12 @@ namespace rvpf {
 
 namespace {
 
-constexpr int SW = 64;            /* max slice width */
-constexpr int WW = SW + 6;        /* window stride */
-constexpr int WH = 64 + 6;        /* window rows */
-constexpr int AW = SW + 2;        /* A/B stride */
-constexpr int NT = 512;           /* threads per workgroup */
-constexpr int PX_PER_THREAD = 64 * 64 / NT;
+constexpr int NT = 256;        /* threads per workgroup */
+constexpr int MAXW = 64;       /* max slice width */
+constexpr int WS = MAXW + 8;   /* window row stride (samples) */
+constexpr int WR = 64 + 6;     /* window rows: stripe + 3 above and below */
+constexpr int ABW = MAXW + 2;  /* A/B row stride: positions -1..w */
+constexpr int ABR = 64 + 2;    /* A/B rows: -1..h */
 
 struct LrLaunch {
   Geom g;
@@ -56,48 +62,105 @@ __device__ __forceinline__ int round2(int x, int n) { return (x + ((1 << n) >> 1
 /* SGR_A2 (postfilter.rs:884-898): 256 at 255 and above, 1 at 0, else
  * ((z << 8) + z / 2) / (z + 1). */
 struct A2Table {
-  int16_t v[256];
+  uint16_t v[256];
 };
 constexpr A2Table make_a2() {
   A2Table t{};
-  for (int z = 0; z < 256; z++) t.v[z] = (int16_t)(z >= 255 ? 256 : (z == 0 ? 1 : ((z << 8) + z / 2) / (z + 1)));
+  for (int z = 0; z < 256; z++) t.v[z] = (uint16_t)(z >= 255 ? 256 : (z == 0 ? 1 : ((z << 8) + z / 2) / (z + 1)));
   return t;
 }
 constexpr A2Table SGR_A2 = make_a2();
 
 /* LDS of one workgroup. */
 struct LrShared {
-  uint16_t win[WH * WW];
-  int16_t inter[WH * SW];      /* Wiener horizontal output; SGR: A */
-  int32_t bbuf[(64 + 2) * AW]; /* SGR: B */
+  uint16_t win[WR * WS];
+  union {
+    int16_t hbuf[WR * MAXW]; /* Wiener: horizontal pass output */
+    struct {
+      uint32_t ab0[(ABR / 2 + 1) * ABW]; /* pass 0 (r = 2): odd rows i at (i + 1) >> 1 */
+      uint32_t ab1[ABR * ABW];           /* pass 1 (r = 1): row i at i + 1 */
+    } sgr;
+  };
+  uint16_t a2[256];
 };
 
-/* Slice lb of grid plane gp (stripe-major). */
-template <typename T>
-__device__ __forceinline__ void lr_slice(const LrLaunch &a, int gp, int lb, LrShared &sm) {
-  uint16_t *win = sm.win;
-  int16_t *inter = sm.inter;
-  int32_t *bbuf = sm.bbuf;
+/* Per-pass constants of the self-guided filter (postfilter.rs:915-935). */
+struct SgrPass {
+  uint32_t n, s, one_over_n, p_lim;
+};
+
+__device__ __forceinline__ SgrPass sgr_pass(int r, int eps) {
+  SgrPass q;
+  const int n = (2 * r + 1) * (2 * r + 1);
+  const int n2e = n * n * eps;
+  q.n = (uint32_t)n;
+  q.s = ((1u << 20) + (uint32_t)(n2e / 2)) / (uint32_t)n2e;
+  q.one_over_n = ((1u << 12) + (uint32_t)(n / 2)) / (uint32_t)n;
+  /* p * s fits in 32 bits up to this bound; beyond it z saturates */
+  q.p_lim = (0xFFFFFFFFu - (1u << 19)) / q.s;
+  return q;
+}
+
+/* A and B at one position from the box sum and sum of squares
+ * (postfilter.rs:940-965), packed as (A << 20) | B: A <= 256 and
+ * B <= (255 * 25 * 4095 * 164 + 2048) >> 12 < 2^20. Every product fits
+ * 32 bits unsigned at 8/10/12 bits. */
+__device__ __forceinline__ uint32_t sgr_ab(uint32_t sum, uint32_t sq, int bd, const SgrPass &q, const uint16_t *a2t) {
+  const int sh = bd - 8;
+  const uint32_t av = (sq + ((1u << (2 * sh)) >> 1)) >> (2 * sh);
+  const uint32_t d = (sum + ((1u << sh) >> 1)) >> sh;
+  const uint32_t an = __umul24(av, q.n), dd = __umul24(d, d);
+  const uint32_t p = an > dd ? an - dd : 0;
+  const uint32_t z = p > q.p_lim ? 255u : min((p * q.s + (1u << 19)) >> 20, 255u);
+  const uint32_t a2 = a2t[z];
+  const uint32_t b2 = ((256u - a2) * __umul24(sum, q.one_over_n) + 2048u) >> 12;
+  return (a2 << 20) | b2;
+}
+
+/* Rows [lo, hi) of n split evenly over nb bands: band b. */
+__device__ __forceinline__ void band_rows(int n, int nb, int b, int &lo, int &hi) {
+  lo = (b * n) / nb;
+  hi = ((b + 1) * n) / nb;
+}
+
+/* out[y][x] = in[y][x] for rows y = y, y + NB, ... <= y_last of one
+ * column, COPIES rows per batch with every load issued (rows past y_last
+ * repeat it) before the stores. */
+template <typename T, int NB>
+__device__ __forceinline__ void copy_column(T *out, int64_t os, const T *in, int64_t is, int x, int y, int y_last) {
+  constexpr int COPIES = 8;
+  for (; y <= y_last; y += COPIES * NB) {
+    T v[COPIES];
+#pragma unroll
+    for (int k = 0; k < COPIES; k++) v[k] = in[(int64_t)min(y + k * NB, y_last) * is + x];
+#pragma unroll
+    for (int k = 0; k < COPIES; k++)
+      if (y + k * NB <= y_last) out[(int64_t)(y + k * NB) * os + x] = v[k];
+  }
+}
+
+/* Tile (plane, stripe st, slice sl) with slice width SW (64, or 32 for
+ * 32-sample units). */
+template <typename T, int SW>
+__device__ __forceinline__ void lr_tile(const LrLaunch &a, int plane, int st, int sl, LrShared &sm) {
+  constexpr int LG = SW == 64 ? 6 : 5;
+  constexpr int NB = NT / SW; /* lane groups = row bands */
+  const int col = threadIdx.x & (SW - 1), band = threadIdx.x >> LG;
   const Geom &g = a.g;
-  const int plane = a.plane_id[gp];
-  const int st = lb / a.nslices[gp], sl = lb - st * a.nslices[gp];
   const int ssx = plane ? g.ssx : 0, ssy = plane ? g.ssy : 0;
-  const int plane_end_x = ((g.width + ssx) >> ssx) - 1;
-  const int plane_end_y = ((g.height + ssy) >> ssy) - 1;
+  const int pex = ((g.width + ssx) >> ssx) - 1;
+  const int pey = ((g.height + ssy) >> ssy) - 1;
   const int area_w = (g.mi_cols * 4) >> ssx, area_h = (g.mi_rows * 4) >> ssy;
   const int unit_size = a.h.unit_size[plane];
-  const int lg = unit_size >= SW ? 6 : 5; /* slice width 64 or 32 */
-  const int sw = 1 << lg;
-  const int tx = threadIdx.x & (sw - 1), ty = threadIdx.x >> lg, ny = NT >> lg;
   /* lr_stripe, postfilter.rs:640-661 */
   const int stripe_start = (-8 + st * 64) >> ssy;
   const int stripe_end = stripe_start + (64 >> ssy) - 1;
   const int y0 = max(stripe_start, 0);
-  const int y1 = min(stripe_end, plane_end_y);
+  const int y1 = min(stripe_end, pey);
   const int y1c = st == a.nstripes - 1 ? area_h - 1 : min(stripe_end, area_h - 1); /* rows written */
-  const int x0 = sl * sw;
-  const int x1c = min(x0 + sw, area_w) - 1; /* columns written */
-  const int xr = min(x1c, plane_end_x);     /* columns restored */
+  const int x0 = sl * SW;
+  const int x1c = min(x0 + SW, area_w) - 1; /* columns written */
+  const int xr = min(x1c, pex);             /* columns restored */
   if (y0 > y1c || x0 > x1c) return;
   const T *cdef = cpp<T>(a.cdef[plane]);
   const int64_t cs_ = a.cdef[plane].stride;
@@ -115,20 +178,13 @@ __device__ __forceinline__ void lr_slice(const LrLaunch &a, int gp, int lb, LrSh
     if (type != RVPF_RESTORE_WIENER && type != RVPF_RESTORE_SGRPROJ) type = 0;
   }
 
-  /* copy what is not restored */
+  /* copy what is not restored: everything (no filter), else the rows
+   * below the frame and the columns right of it inside the mode-info area */
   {
-    const int wc = x1c - x0 + 1, hc = y1c - y0 + 1;
-    const int x = x0 + tx;
-    if (tx < wc) {
-      T *o = out + (int64_t)(y0 + ty) * os + x;
-      const T *c = cdef + (int64_t)(y0 + ty) * cs_ + x;
-      const int64_t ostep = (int64_t)ny * os, cstep = (int64_t)ny * cs_;
-      for (int r = ty; r < hc; r += ny, o += ostep, c += cstep) {
-        const int y = y0 + r;
-        if (type != 0 && x <= xr && y <= y1) continue;
-        *o = *c;
-      }
-    }
+    const int x = x0 + col;
+    const int ya = type ? y1 + 1 : y0;
+    if (x <= x1c) copy_column<T, NB>(out, os, cdef, cs_, x, ya + band, y1c);
+    if (type && x > xr && x <= x1c) copy_column<T, NB>(out, os, cdef, cs_, x, y0 + band, y1);
   }
   if (type == 0) return;
 
@@ -138,26 +194,41 @@ __device__ __forceinline__ void lr_slice(const LrLaunch &a, int gp, int lb, LrSh
   {
     const T *deb = cpp<T>(a.deb[plane]);
     const int64_t ds = a.deb[plane].stride;
-    for (int r = ty; r < wh; r += ny) {
-      int y = min(max(y0 + r - 3, 0), plane_end_y);
-      const T *src;
-      if (y < stripe_start) {
-        y = max(stripe_start - 2, y);
-        src = deb + (int64_t)y * ds;
-      } else if (y > stripe_end) {
-        y = min(stripe_end + 2, y);
-        src = deb + (int64_t)y * ds;
-      } else {
-        src = cdef + (int64_t)y * cs_;
+    const int xa = min(max(x0 + col - 3, 0), pex);
+    const int xb = min(max(x0 + col + SW - 3, 0), pex);
+    const bool second = col + SW < ww;
+    /* LOADS rows per batch: every load is issued (rows past the window
+     * repeat its last row) before the LDS stores */
+    constexpr int LOADS = 6;
+    for (int r0 = band; r0 < wh; r0 += LOADS * NB) {
+      uint16_t va[LOADS], vb[LOADS];
+#pragma unroll
+      for (int k = 0; k < LOADS; k++) {
+        const int r = min(r0 + k * NB, wh - 1);
+        const int yc = min(max(y0 + r - 3, 0), pey);
+        /* rows outside the stripe: the deblocked frame, at most 2 out */
+        const bool above = yc < stripe_start, below = yc > stripe_end;
+        const int y = above ? max(stripe_start - 2, yc) : (below ? min(stripe_end + 2, yc) : yc);
+        const T *src = (above || below) ? deb + (int64_t)y * ds : cdef + (int64_t)y * cs_;
+        va[k] = src[xa];
+        vb[k] = src[xb];
       }
-      for (int c = tx; c < ww; c += sw) win[r * WW + c] = src[min(max(x0 + c - 3, 0), plane_end_x)];
+#pragma unroll
+      for (int k = 0; k < LOADS; k++) {
+        const int r = r0 + k * NB;
+        if (r < wh) {
+          uint16_t *wr = &sm.win[r * WS + col];
+          wr[0] = va[k];
+          if (second) wr[SW] = vb[k];
+        }
+      }
     }
   }
-  __syncthreads();
   const int bd = g.bd;
   const int maxv = (1 << bd) - 1;
 
   if (type == RVPF_RESTORE_WIENER) {
+    __syncthreads();
     /* wiener_coefficient (postfilter.rs:994-1004) + rounding_variables
      * (predict.rs:1153-1166); symmetric taps */
     const int round0 = bd == 12 ? 5 : 3, round1 = bd == 12 ? 9 : 11;
@@ -166,130 +237,185 @@ __device__ __forceinline__ void lr_slice(const LrLaunch &a, int gp, int lb, LrSh
     const int v3 = 128 - 2 * (v0 + v1 + v2), h3 = 128 - 2 * (h0 + h1 + h2);
     const int offset = 1 << (bd + 7 - round0 - 1);
     const int limit = (1 << (bd + 1 + 7 - round0)) - 1;
-    if (tx < w)
-      for (int r = ty; r < wh; r += ny) {
-        const uint16_t *s = &win[r * WW + tx];
+    if (col < w)
+      for (int r = band; r < wh; r += NB) {
+        const uint16_t *s = &sm.win[r * WS + col];
         const int sum = __mul24(h0, s[0] + s[6]) + __mul24(h1, s[1] + s[5]) + __mul24(h2, s[2] + s[4]) +
                         __mul24(h3, (int)s[3]);
-        inter[r * SW + tx] = (int16_t)min(max(round2(sum, round0), -offset), limit - offset);
+        sm.hbuf[r * MAXW + col] = (int16_t)min(max(round2(sum, round0), -offset), limit - offset);
       }
     __syncthreads();
-    if (tx < w) {
-      T *o = out + (int64_t)(y0 + ty) * os + x0 + tx;
-      const int64_t ostep = (int64_t)ny * os;
-      for (int r = ty; r < h; r += ny, o += ostep) {
-        const int16_t *s = &inter[r * SW + tx];
-        const int sum = __mul24(v0, s[0] + s[6 * SW]) + __mul24(v1, s[SW] + s[5 * SW]) +
-                        __mul24(v2, s[2 * SW] + s[4 * SW]) + __mul24(v3, (int)s[3 * SW]);
+    if (col < w) {
+      int lo, hi;
+      band_rows(h, NB, band, lo, hi);
+      const int16_t *hb = &sm.hbuf[lo * MAXW + col];
+      int r0 = hb[0], r1 = hb[MAXW], r2 = hb[2 * MAXW], r3 = hb[3 * MAXW], r4 = hb[4 * MAXW], r5 = hb[5 * MAXW];
+      T *o = out + (int64_t)(y0 + lo) * os + x0 + col;
+      for (int i = lo; i < hi; i++, o += os) {
+        const int r6 = sm.hbuf[(i + 6) * MAXW + col];
+        const int sum = __mul24(v0, r0 + r6) + __mul24(v1, r1 + r5) + __mul24(v2, r2 + r4) + __mul24(v3, r3);
         *o = (T)min(max(round2(sum, round1), 0), maxv);
+        r0 = r1;
+        r1 = r2;
+        r2 = r3;
+        r3 = r4;
+        r4 = r5;
+        r5 = r6;
       }
     }
     return;
   }
 
-  /* self-guided: box_filter_rect (postfilter.rs:901-991) for both passes;
-   * pixel (r, tx) of row iteration k keeps pass 0's value in flt0[k] */
+  /* self-guided: box_filter_rect (postfilter.rs:901-991) for both passes */
+  for (int i = threadIdx.x; i < 256; i += NT) sm.a2[i] = SGR_A2.v[i];
+  __syncthreads();
   const int set = u->sgr_set & 15;
-  const int r0 = AV1_SGR_PARAMS[set][0], r1 = AV1_SGR_PARAMS[set][2];
-  const int w0 = u->sgr_xqd[0], w1 = u->sgr_xqd[1];
-  const int w2 = 128 - w0 - w1;
-  int flt0[PX_PER_THREAD];
-  int16_t *A = inter;
-#pragma unroll
-  for (int pass = 0; pass < 2; pass++) {
-    const int r = pass == 0 ? r0 : r1;
-    if (r == 0) continue;
-    const int eps = AV1_SGR_PARAMS[set][pass * 2 + 1];
-    const int n = (2 * r + 1) * (2 * r + 1);
-    const int n2e = n * n * eps;
-    const uint32_t s = ((1u << 20) + (uint32_t)(n2e / 2)) / (uint32_t)n2e;
-    const uint32_t one_over_n = ((1u << 12) + (uint32_t)(n / 2)) / (uint32_t)n;
-    /* p * s fits in 32 bits up to this bound; beyond it z saturates */
-    const uint32_t p_lim = (0xFFFFFFFFu - (1u << 19)) / s;
-    __syncthreads();
-    /* A and B at rectangle positions (ii - 1, jj - 1) = window (ii + 2, jj + 2) */
-    for (int ii = ty; ii < h + 2; ii += ny) {
-      if (pass == 0 && ((y0 + ii - 1) & 1) == 0) continue;
-      for (int jj = tx; jj < w + 2; jj += sw) {
-        uint32_t sum = 0, sq = 0;
-        if (r == 2) {
-#pragma unroll
-          for (int dy = -2; dy <= 2; dy++) {
-            const uint16_t *row = &win[(ii + 2 + dy) * WW + jj + 2];
-#pragma unroll
-            for (int dx = -2; dx <= 2; dx++) {
-              const uint32_t v = row[dx];
-              sum += v;
-              sq += __umul24(v, v);
-            }
-          }
-        } else {
-#pragma unroll
-          for (int dy = -1; dy <= 1; dy++) {
-            const uint16_t *row = &win[(ii + 2 + dy) * WW + jj + 2];
-#pragma unroll
-            for (int dx = -1; dx <= 1; dx++) {
-              const uint32_t v = row[dx];
-              sum += v;
-              sq += __umul24(v, v);
-            }
-          }
-        }
-        const uint32_t av = (uint32_t)round2((int)sq, 2 * (bd - 8));
-        const uint32_t d = (uint32_t)round2((int)sum, bd - 8);
-        const uint32_t an = __umul24(av, (uint32_t)n), dd = __umul24(d, d);
-        const uint32_t p = an > dd ? an - dd : 0;
-        const uint32_t z = p > p_lim ? 255u : min((p * s + (1u << 19)) >> 20, 255u);
-        const int a2 = SGR_A2.v[z];
-        const uint64_t b2 = (uint64_t)(uint32_t)(256 - a2) * (uint64_t)__umul24(sum, one_over_n);
-        A[ii * AW + jj] = (int16_t)a2;
-        bbuf[ii * AW + jj] = (int32_t)((b2 + 2048) >> 12);
+  const int rad0 = AV1_SGR_PARAMS[set][0], rad1 = AV1_SGR_PARAMS[set][2];
+  const SgrPass q0 = sgr_pass(2, rad0 ? AV1_SGR_PARAMS[set][1] : 1);
+  const SgrPass q1 = sgr_pass(1, rad1 ? AV1_SGR_PARAMS[set][3] : 1);
+  const uint16_t *a2t = sm.a2;
+  /* A/B at positions (i, j), i in [-1, h], j in [-1, w]; the box of
+   * radius 2 covers window rows i+1..i+5 and columns j+1..j+5. The walk
+   * of thread (band, col) covers column j = col - 1 (the columns past
+   * SW - 2 are done directly below). */
+  {
+    const int j = col - 1;
+    int lo, hi;
+    band_rows(h + 2, NB, band, lo, hi);
+    lo -= 1;
+    hi -= 1;
+    if (j <= w) {
+      uint32_t h5a = 0, h5b = 0, h5c = 0, h5d = 0, q5a = 0, q5b = 0, q5c = 0, q5d = 0;
+      uint32_t h3a = 0, h3b = 0, q3a = 0, q3b = 0;
+      const uint16_t *wp = &sm.win[(lo + 1) * WS + j + 1];
+      for (int R = lo + 1; R <= hi + 4; R++, wp += WS) {
+        const uint32_t w1 = wp[0], w2 = wp[1], w3 = wp[2], w4 = wp[3], w5 = wp[4];
+        const uint32_t h3n = w2 + w3 + w4, h5n = h3n + w1 + w5;
+        const uint32_t q3n = __umul24(w2, w2) + __umul24(w3, w3) + __umul24(w4, w4);
+        const uint32_t q5n = q3n + __umul24(w1, w1) + __umul24(w5, w5);
+        const int i5 = R - 5, i3 = R - 4;
+        if (rad0 && i5 >= lo && ((y0 + i5) & 1))
+          sm.sgr.ab0[((i5 + 1) >> 1) * ABW + j + 1] =
+              sgr_ab(h5a + h5b + h5c + h5d + h5n, q5a + q5b + q5c + q5d + q5n, bd, q0, a2t);
+        if (rad1 && i3 >= lo && i3 < hi) sm.sgr.ab1[(i3 + 1) * ABW + j + 1] = sgr_ab(h3a + h3b + h3n, q3a + q3b + q3n, bd, q1, a2t);
+        h5a = h5b;
+        h5b = h5c;
+        h5c = h5d;
+        h5d = h5n;
+        q5a = q5b;
+        q5b = q5c;
+        q5c = q5d;
+        q5d = q5n;
+        h3a = h3b;
+        h3b = h3n;
+        q3a = q3b;
+        q3b = q3n;
       }
     }
-    __syncthreads();
-    if (tx < w) {
-#pragma unroll
-      for (int k = 0; k < PX_PER_THREAD; k++) {
-        const int ii = ty + k * ny;
-        if (ii >= h) break;
-        const int yabs = y0 + ii;
-        const int c = (ii + 1) * AW + tx + 1;
-        int av, bv, shift;
-        if (pass == 0) {
-          if (yabs & 1) {
-            /* odd row: this row's A/B only, weights 6 (centre) / 5 */
-            av = 6 * A[c] + 5 * (A[c - 1] + A[c + 1]);
-            bv = 6 * bbuf[c] + 5 * (bbuf[c - 1] + bbuf[c + 1]);
-            shift = 4;
-          } else {
-            /* even row: the rows above and below */
-            av = 6 * (A[c - AW] + A[c + AW]) + 5 * (A[c - AW - 1] + A[c - AW + 1] + A[c + AW - 1] + A[c + AW + 1]);
-            bv = 6 * (bbuf[c - AW] + bbuf[c + AW]) +
-                 5 * (bbuf[c - AW - 1] + bbuf[c - AW + 1] + bbuf[c + AW - 1] + bbuf[c + AW + 1]);
-            shift = 5;
+    /* columns SW - 1 .. w, every row, directly */
+    const int nl = w - SW + 2;
+    for (int t = threadIdx.x; t < nl * (h + 2); t += NT) {
+      const int jj = SW - 1 + t / (h + 2), i = t % (h + 2) - 1;
+      uint32_t s5 = 0, sq5 = 0, s3 = 0, sq3 = 0;
+      for (int dy = 0; dy < 5; dy++)
+        for (int dx = 0; dx < 5; dx++) {
+          const uint32_t v = sm.win[(i + 1 + dy) * WS + jj + 1 + dx];
+          s5 += v;
+          sq5 += __umul24(v, v);
+          if (dy >= 1 && dy <= 3 && dx >= 1 && dx <= 3) {
+            s3 += v;
+            sq3 += __umul24(v, v);
           }
-        } else {
-          av = 4 * (A[c] + A[c - 1] + A[c + 1] + A[c - AW] + A[c + AW]) +
-               3 * (A[c - AW - 1] + A[c - AW + 1] + A[c + AW - 1] + A[c + AW + 1]);
-          bv = 4 * (bbuf[c] + bbuf[c - 1] + bbuf[c + 1] + bbuf[c - AW] + bbuf[c + AW]) +
-               3 * (bbuf[c - AW - 1] + bbuf[c - AW + 1] + bbuf[c + AW - 1] + bbuf[c + AW + 1]);
-          shift = 5;
         }
-        const int uu = win[(ii + 3) * WW + tx + 3];
-        const int v = round2(__mul24(av, uu) + bv, 4 + shift);
-        if (pass == 0 && r1 != 0) {
-          flt0[k] = v;
-        } else {
-          /* projection (postfilter.rs:823-851); here the last pass ran */
-          const int ue = uu << 4;
-          const int f0 = pass == 0 ? v : (r0 ? flt0[k] : ue);
-          const int f1 = pass == 1 ? v : ue;
-          const int acc = __mul24(w1, ue) + __mul24(w0, f0) + __mul24(w2, f1);
-          out[(int64_t)(y0 + ii) * os + x0 + tx] = (T)min(max(round2(acc, 11), 0), maxv);
-        }
-      }
+      if (rad0 && ((y0 + i) & 1)) sm.sgr.ab0[((i + 1) >> 1) * ABW + jj + 1] = sgr_ab(s5, sq5, bd, q0, a2t);
+      if (rad1) sm.sgr.ab1[(i + 1) * ABW + jj + 1] = sgr_ab(s3, sq3, bd, q1, a2t);
     }
   }
+  __syncthreads();
+  /* 3x3 weighting of both passes and the projection (postfilter.rs:823-851) */
+  if (col < w) {
+    const int j = col;
+    const int w0 = u->sgr_xqd[0], w1 = u->sgr_xqd[1];
+    const int w2 = 128 - w0 - w1;
+    int lo, hi;
+    band_rows(h, NB, band, lo, hi);
+    /* per A/B row: E = A(j-1) + A(j+1), C = A(j) (and the same for B) */
+    const uint32_t *p1 = &sm.sgr.ab1[lo * ABW + j];
+    int ea_m = 0, ca_m = 0, eb_m = 0, cb_m = 0, ea_c = 0, ca_c = 0, eb_c = 0, cb_c = 0;
+    if (rad1) {
+      uint32_t l = p1[0], c = p1[1], r = p1[2];
+      ea_m = (int)((l >> 20) + (r >> 20));
+      ca_m = (int)(c >> 20);
+      eb_m = (int)((l & 0xFFFFF) + (r & 0xFFFFF));
+      cb_m = (int)(c & 0xFFFFF);
+      l = p1[ABW];
+      c = p1[ABW + 1];
+      r = p1[ABW + 2];
+      ea_c = (int)((l >> 20) + (r >> 20));
+      ca_c = (int)(c >> 20);
+      eb_c = (int)((l & 0xFFFFF) + (r & 0xFFFFF));
+      cb_c = (int)(c & 0xFFFFF);
+    }
+    /* pass 0: 6*C + 5*E of the last odd row at or above i */
+    int ra_p = 0, rb_p = 0;
+    if (rad0) {
+      const int io = ((y0 + lo) & 1) ? lo : lo - 1;
+      const uint32_t *p0 = &sm.sgr.ab0[((io + 1) >> 1) * ABW + j];
+      const uint32_t l = p0[0], c = p0[1], r = p0[2];
+      ra_p = 6 * (int)(c >> 20) + 5 * (int)((l >> 20) + (r >> 20));
+      rb_p = 6 * (int)(c & 0xFFFFF) + 5 * (int)((l & 0xFFFFF) + (r & 0xFFFFF));
+    }
+    T *o = out + (int64_t)(y0 + lo) * os + x0 + j;
+    for (int i = lo; i < hi; i++, o += os) {
+      const int uu = sm.win[(i + 3) * WS + j + 3];
+      const int ue = uu << 4;
+      int f0 = ue, f1 = ue;
+      if (rad0) {
+        if ((y0 + i) & 1) {
+          f0 = round2(__mul24(ra_p, uu) + rb_p, 8);
+        } else {
+          const uint32_t *p0 = &sm.sgr.ab0[((i + 2) >> 1) * ABW + j];
+          const uint32_t l = p0[0], c = p0[1], r = p0[2];
+          const int ra_n = 6 * (int)(c >> 20) + 5 * (int)((l >> 20) + (r >> 20));
+          const int rb_n = 6 * (int)(c & 0xFFFFF) + 5 * (int)((l & 0xFFFFF) + (r & 0xFFFFF));
+          f0 = round2(__mul24(ra_p + ra_n, uu) + rb_p + rb_n, 9);
+          ra_p = ra_n;
+          rb_p = rb_n;
+        }
+      }
+      if (rad1) {
+        const uint32_t *pn = &sm.sgr.ab1[(i + 2) * ABW + j];
+        const uint32_t l = pn[0], c = pn[1], r = pn[2];
+        const int ea_n = (int)((l >> 20) + (r >> 20)), ca_n = (int)(c >> 20);
+        const int eb_n = (int)((l & 0xFFFFF) + (r & 0xFFFFF)), cb_n = (int)(c & 0xFFFFF);
+        /* weights 3 4 3 / 4 4 4 / 3 4 3 */
+        const int av = 3 * (ea_m + ea_n) + 4 * (ca_m + ca_n + ea_c + ca_c);
+        const int bv = 3 * (eb_m + eb_n) + 4 * (cb_m + cb_n + eb_c + cb_c);
+        f1 = round2(__mul24(av, uu) + bv, 9);
+        ea_m = ea_c;
+        ca_m = ca_c;
+        eb_m = eb_c;
+        cb_m = cb_c;
+        ea_c = ea_n;
+        ca_c = ca_n;
+        eb_c = eb_n;
+        cb_c = cb_n;
+      }
+      const int acc = __mul24(w1, ue) + __mul24(w0, f0) + __mul24(w2, f1);
+      *o = (T)min(max(round2(acc, 11), 0), maxv);
+    }
+  }
+}
+
+/* Slice lb of grid plane gp (stripe-major). */
+template <typename T>
+__device__ __forceinline__ void lr_slice(const LrLaunch &a, int gp, int lb, LrShared &sm) {
+  const int plane = a.plane_id[gp];
+  const int st = lb / a.nslices[gp], sl = lb - st * a.nslices[gp];
+  if (a.h.unit_size[plane] >= MAXW)
+    lr_tile<T, 64>(a, plane, st, sl, sm);
+  else
+    lr_tile<T, 32>(a, plane, st, sl, sm);
 }
 
 template <typename T>
@@ -328,7 +454,7 @@ int plan(const Geom &g, const LrHdr &h, const PlanePtr deblocked[3], const Plane
     if (h.type[p] == 0) continue;
     const int ssx = p ? g.ssx : 0;
     const int area_w = (g.mi_cols * 4) >> ssx;
-    const int sw = h.unit_size[p] < SW ? h.unit_size[p] : SW;
+    const int sw = h.unit_size[p] < MAXW ? h.unit_size[p] : MAXW;
     a.plane_id[n] = p;
     a.nslices[n] = (area_w + sw - 1) / sw;
     a.blk_start[n] = total;
