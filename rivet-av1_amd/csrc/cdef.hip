@@ -170,42 +170,53 @@ __device__ __forceinline__ Taps make_taps(int pri, int sec, int damping, int dir
   return k;
 }
 
-/* The filtered pair whose own word is w[base]. */
+/* The filtered pair whose own word is w[base]. PRI / SEC: the tile has
+ * primary / secondary taps at all (a lane's strength of 0 makes its taps
+ * contribute nothing, since constrain() with t = 0 is 0). The clamp to
+ * [min, max] of the available taps is applied always: with one kind of
+ * tap the update already stays inside (total weight 12/16), and with no
+ * taps it is the identity. Every tap is loaded before the arithmetic, and
+ * the taps are independent, so the packed operations interleave. */
+template <bool PRI, bool SEC>
 __device__ __forceinline__ uint32_t filter_pair(const uint32_t *w, int base, const Taps &k) {
   const s16x2 x = as_s2(w[base]);
+  uint32_t pw[4], sw[8];
+  if (PRI) {
+#pragma unroll
+    for (int t = 0; t < 4; t++) pw[t] = w[base + k.po[t]];
+  }
+  if (SEC) {
+#pragma unroll
+    for (int t = 0; t < 8; t++) sw[t] = w[base + k.so[t]];
+  }
   s16x2 sum = splat(0), mn = x, mx = x;
-  if (k.has_pri) {
-    s16x2 s0 = splat(0), s1 = splat(0);
+  if (PRI) {
+    s16x2 c[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) c[t] = constrain2(as_s2(pw[t]) - x, k.pri, k.pri_adj);
 #pragma unroll
     for (int t = 0; t < 4; t++) {
-      const uint32_t pw = w[base + k.po[t]];
-      const s16x2 c = constrain2(as_s2(pw) - x, k.pri, k.pri_adj);
-      if (t < 2) s0 += c; else s1 += c;
-      if (k.clamp) {
-        mn = __builtin_elementwise_min(mn, as_s2(pw));
-        mx = __builtin_elementwise_max(mx, as_s2(pw & 0x0fff0fffu));
-      }
+      mn = __builtin_elementwise_min(mn, as_s2(pw[t]));
+      mx = __builtin_elementwise_max(mx, as_s2(pw[t] & 0x0fff0fffu));
     }
-    sum = k.pt0 * s0 + k.pt1 * s1;
+    sum = k.pt0 * (c[0] + c[1]) + k.pt1 * (c[2] + c[3]);
   }
-  if (k.has_sec) {
+  if (SEC) {
     /* Cdef_Sec_Taps is {2, 1} in both rows */
-    s16x2 s0 = splat(0), s1 = splat(0);
+    s16x2 c[8];
+#pragma unroll
+    for (int t = 0; t < 8; t++) c[t] = constrain2(as_s2(sw[t]) - x, k.sec, k.sec_adj);
 #pragma unroll
     for (int t = 0; t < 8; t++) {
-      const uint32_t pw = w[base + k.so[t]];
-      const s16x2 c = constrain2(as_s2(pw) - x, k.sec, k.sec_adj);
-      if (t < 4) s0 += c; else s1 += c;
-      if (k.clamp) {
-        mn = __builtin_elementwise_min(mn, as_s2(pw));
-        mx = __builtin_elementwise_max(mx, as_s2(pw & 0x0fff0fffu));
-      }
+      mn = __builtin_elementwise_min(mn, as_s2(sw[t]));
+      mx = __builtin_elementwise_max(mx, as_s2(sw[t] & 0x0fff0fffu));
     }
+    const s16x2 s0 = (c[0] + c[1]) + (c[2] + c[3]), s1 = (c[4] + c[5]) + (c[6] + c[7]);
     sum += s0 + s0 + s1;
   }
   /* x + ((8 + sum - (sum < 0)) >> 4) */
   s16x2 v = x + ((sum + splat(8) + (sum >> splat(15))) >> splat(4));
-  if (k.clamp) v = __builtin_elementwise_min(__builtin_elementwise_max(v, mn), mx);
+  v = __builtin_elementwise_min(__builtin_elementwise_max(v, mn), mx);
   return __builtin_bit_cast(uint32_t, v);
 }
 
@@ -259,20 +270,37 @@ __device__ void dir_costs3(const uint16_t *t, int b0, int rs, int cs, int sh, in
 
 /* Filters this lane's pairs of one block: rows r0 .. r0 + nrows - 1, the
  * column pair starting at block column c (even); t0 = the block's top-left
- * tile sample (even), o = the block's top-left output sample. */
-template <typename T>
+ * tile sample (even), o = the block's top-left output sample. Lanes whose
+ * block is not filtered (on == false) store the input pair. */
+template <typename T, bool PRI, bool SEC>
 __device__ __forceinline__ void filter_block_pairs(const uint32_t *w, int t0, T *o, int64_t os, int c, int r0,
                                                    int nrows, const Taps &k, bool on) {
   o += (int64_t)r0 * os + c;
+#pragma unroll 2
   for (int r = r0; r < r0 + nrows; r++, o += os) {
     const int base = ((t0 + r * TS + c) >> 1);
-    const uint32_t v = on ? filter_pair(w, base, k) : w[base];
+    const uint32_t f = filter_pair<PRI, SEC>(w, base, k);
+    const uint32_t v = on ? f : w[base];
     if (sizeof(T) == 2) {
       *reinterpret_cast<uint32_t *>(o) = v;
     } else {
       *reinterpret_cast<uint16_t *>(o) = (uint16_t)((v & 0xff) | ((v >> 8) & 0xff00));
     }
   }
+}
+
+/* filter_block_pairs with the tile's tap kinds as template arguments. */
+template <typename T>
+__device__ __forceinline__ void filter_block_any(bool pri, bool sec, const uint32_t *w, int t0, T *o, int64_t os,
+                                                 int c, int r0, int nrows, const Taps &k, bool on) {
+  if (pri && sec)
+    filter_block_pairs<T, true, true>(w, t0, o, os, c, r0, nrows, k, on);
+  else if (pri)
+    filter_block_pairs<T, true, false>(w, t0, o, os, c, r0, nrows, k, on);
+  else if (sec)
+    filter_block_pairs<T, false, true>(w, t0, o, os, c, r0, nrows, k, on);
+  else
+    filter_block_pairs<T, false, false>(w, t0, o, os, c, r0, nrows, k, false);
 }
 
 /* One 64x64 filter block (tile_id in raster order); words holds the
@@ -370,8 +398,8 @@ __device__ __forceinline__ void cdef_tile(const CdefLaunch &a, int tile_id, uint
     const int sec = a.h.y_sec[idx] << cs;
     const Taps k = make_taps(pri, sec, a.h.damping + cs, a.h.y_pri[idx] ? ydir : 0, cs);
     T *o = pp<T>(a.out[0]) + (int64_t)(y0 + by * 8) * a.out[0].stride + x0 + bx * 8;
-    filter_block_pairs<T>(words, (by * 8 + 2) * TS + bx * 8 + 2, o, a.out[0].stride, 2 * q, 0, 8, k,
-                          filt && (pri | sec));
+    filter_block_any<T>(a.h.y_pri[idx] != 0, sec != 0, words, (by * 8 + 2) * TS + bx * 8 + 2, o, a.out[0].stride,
+                        2 * q, 0, 8, k, filt && (pri | sec));
   }
   if (g.nplanes == 1) return;
 
@@ -392,8 +420,8 @@ __device__ __forceinline__ void cdef_tile(const CdefLaunch &a, int tile_id, uint
     __syncthreads();
     if (inside) {
       T *o = pp<T>(a.out[p]) + (int64_t)((y0 >> ssy) + by * bhs) * a.out[p].stride + (x0 >> ssx) + bx * bws;
-      filter_block_pairs<T>(words, (by * bhs + 2) * TS + bx * bws + 2, o, a.out[p].stride, 2 * cpair, r0, nrows,
-                            k, filt && (upri | usec));
+      filter_block_any<T>(upri != 0, usec != 0, words, (by * bhs + 2) * TS + bx * bws + 2, o, a.out[p].stride,
+                          2 * cpair, r0, nrows, k, filt && (upri | usec));
     }
   }
 }

@@ -254,20 +254,22 @@ __device__ __forceinline__ void lf_block(const LfLaunch &a, int gp, int lb) {
   const Geom &g = a.g;
   const int ssx = plane ? g.ssx : 0, ssy = plane ? g.ssy : 0;
   if ((ey << ssy) >= g.mi_rows || (ex << ssx) >= g.mi_cols) return;
-  int limit_bd, blimit_bd, thresh_bd;
-  const int fs = edge_decision(a, plane, PASS, ex, ey, &limit_bd, &blimit_bd, &thresh_bd);
-  if (fs == 0) return;
+  const int xp = ex * 4, yp = ey * 4;
+  /* the edge is not filtered at the frame's left / top border */
+  if (PASS == 0 ? xp == 0 : yp == 0) return;
   const bool chroma = plane != 0;
   T *base = gptr(reinterpret_cast<T *>(a.planes[plane].data));
   const int64_t stride = a.planes[plane].stride;
-  const int xp = ex * 4, yp = ey * 4;
+  /* The samples are loaded before the edge decision reads the mode info,
+   * so both memory round trips overlap; edges that turn out unfiltered
+   * only cost reads of a plane that the pass reads anyway. */
+  int s[4][16];
+  int v[4][14];
   if (PASS == 0) {
-    /* vertical edge at column xp: each lane is a row. All 16 loads of
-     * the 4 rows are issued before the first filter; the outer quads
+    /* vertical edge at column xp: each lane is a row. The outer quads
      * (x - 8, x + 4) are only used by 16-wide edges, which have x >= 8 */
     T *row0 = base + (int64_t)yp * stride + xp;
     const int lo8 = xp >= 8 ? -8 : -4;
-    int s[4][16];
 #pragma unroll
     for (int l = 0; l < 4; l++) {
       T *row = row0 + (int64_t)l * stride;
@@ -276,16 +278,36 @@ __device__ __forceinline__ void lf_block(const LfLaunch &a, int gp, int lb) {
       Io<T>::ld4(row, s[l] + 8);
       Io<T>::ld4(row + 4, s[l] + 12);
     }
+  } else {
+    /* horizontal edge at row yp: each lane is a column. Rows beyond +-4
+     * are only used by 16-wide edges, which have y >= 8 (rows above 0
+     * repeat row yp - 4 otherwise) */
+    const T *col = base + (int64_t)yp * stride + xp;
+#pragma unroll
+    for (int k = 0; k < 14; k++) {
+      const int d = k - 7;
+      const int dd = (d < -4 && yp < 8) ? -4 : d;
+      int q[4];
+      Io<T>::ld4(col + (int64_t)dd * stride, q);
+#pragma unroll
+      for (int l = 0; l < 4; l++) v[l][k] = q[l];
+    }
+  }
+  int limit_bd, blimit_bd, thresh_bd;
+  const int fs = edge_decision(a, plane, PASS, ex, ey, &limit_bd, &blimit_bd, &thresh_bd);
+  if (fs == 0) return;
+  if (PASS == 0) {
+    T *row0 = base + (int64_t)yp * stride + xp;
 #pragma unroll
     for (int l = 0; l < 4; l++) {
       T *row = row0 + (int64_t)l * stride;
-      int v[14];
+      int vr[14];
 #pragma unroll
-      for (int k = 0; k < 14; k++) v[k] = s[l][k + 1]; /* v[k] = sample at xp - 7 + k */
-      const int r = filter_lane(v, limit_bd, blimit_bd, thresh_bd, fs, chroma, g.bd);
+      for (int k = 0; k < 14; k++) vr[k] = s[l][k + 1]; /* vr[k] = sample at xp - 7 + k */
+      const int r = filter_lane(vr, limit_bd, blimit_bd, thresh_bd, fs, chroma, g.bd);
       if (r == 0) continue;
 #pragma unroll
-      for (int k = 0; k < 14; k++) s[l][k + 1] = v[k];
+      for (int k = 0; k < 14; k++) s[l][k + 1] = vr[k];
       if (fs == 4) {
         Io<T>::st2(row - 2, s[l][6], s[l][7]);
         Io<T>::st2(row, s[l][8], s[l][9]);
@@ -300,20 +322,7 @@ __device__ __forceinline__ void lf_block(const LfLaunch &a, int gp, int lb) {
       }
     }
   } else {
-    /* horizontal edge at row yp: each lane is a column. The 14 row loads
-     * are issued together; rows beyond +-4 are only used by 16-wide edges,
-     * which have y >= 8 (rows above 0 repeat row yp - 4 otherwise) */
     T *col = base + (int64_t)yp * stride + xp;
-    int v[4][14];
-#pragma unroll
-    for (int k = 0; k < 14; k++) {
-      const int d = k - 7;
-      const int dd = (d < -4 && yp < 8) ? -4 : d;
-      int q[4];
-      Io<T>::ld4(col + (int64_t)dd * stride, q);
-#pragma unroll
-      for (int l = 0; l < 4; l++) v[l][k] = q[l];
-    }
     int any = 0;
 #pragma unroll
     for (int l = 0; l < 4; l++) any |= filter_lane(v[l], limit_bd, blimit_bd, thresh_bd, fs, chroma, g.bd);
