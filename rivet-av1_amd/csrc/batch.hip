@@ -145,7 +145,7 @@ int plan_frame(const rvpf_batch *b, int k, const FrameIn &x, void *rec) {
       return lr_plan(g, make_lr(p, &x.s), rec_, x.cdef, ref, rec);
     case K_APPLY:
       if (!x.shown || !grain_on(p)) return 0;
-      return grain_apply_plan(make_grain(p), x.gs, ref, disp, b->cfg[RVPF_TUNE_GRAIN_APPLY].shape, rec);
+      return grain_apply_plan(make_grain(p), x.gs, ref, disp, x.recon.bytes_per_sample, rec);
     case K_COPY_OUT: {
       if (!x.shown || grain_on(p)) return 0;
       int w[3] = {0, 0, 0}, h[3] = {0, 0, 0};
@@ -489,7 +489,7 @@ int rvpf_batch_create(rvpf_ctx *ctx, const rvpf_batch_frame *frames, int32_t n, 
     if (x.shown && grain_on(p)) {
       int stripes, blocks;
       grain_offsets_dims(p->upscaled_width, p->frame_height, &stripes, &blocks);
-      grain_bytes += align_up(3 * 73 * 82 * sizeof(int16_t), 256) + align_up(3 * 256 * sizeof(int16_t), 256) +
+      grain_bytes += align_up(GRAIN_TMPL_BYTES, 256) + align_up(3 * 256 * sizeof(int16_t), 256) +
                      align_up((size_t)stripes * blocks, 256);
     }
   }
@@ -528,7 +528,7 @@ int rvpf_batch_create(rvpf_ctx *ctx, const rvpf_batch_frame *frames, int32_t n, 
         grain_offsets_dims(p->upscaled_width, p->frame_height, &stripes, &blocks);
         char *base = static_cast<char *>(b->grain_mem) + grain_off[i];
         x.gs.tmpl = reinterpret_cast<int16_t *>(base);
-        base += align_up(3 * 73 * 82 * sizeof(int16_t), 256);
+        base += align_up(GRAIN_TMPL_BYTES, 256);
         x.gs.lut = reinterpret_cast<int16_t *>(base);
         base += align_up(3 * 256 * sizeof(int16_t), 256);
         x.gs.offsets = reinterpret_cast<uint8_t *>(base);

@@ -17,12 +17,14 @@
  *     Luma runs on wave 0, cb and cr on waves 1 and 2, a fixed number of
  *     diagonals behind the luma samples they read.
  *   - Scaling LUTs (7.18.3.4) and the per-stripe random block offsets.
- * grain_apply_kernel: one thread per chroma sample (and the 1, 2 or 4 luma
- *   samples it covers). The noise of a sample is rebuilt directly from the
- *   template: its 32x32 block's offset, the horizontal blend with the block
- *   to the left, then the vertical blend with the stripe above (whose value
- *   is itself horizontally blended), exactly the order in which the
- *   reference fills its noise stripes and noise image. Chroma reads the
+ * grain_apply_kernel: one workgroup per (noise stripe, 64 luma columns).
+ *   The noise of a sample is rebuilt directly from the template: its 32x32
+ *   block's offset, the horizontal blend with the block to the left, then
+ *   the vertical blend with the stripe above (whose value is itself
+ *   horizontally blended), exactly the order in which the reference fills
+ *   its noise stripes and noise image. A lane owns a column of the stripe,
+ *   so the block offsets and template origins are set up once and every
+ *   row's template and plane accesses are coalesced. Chroma reads the
  *   un-noised luma of the input, as the reference blends chroma first.
  */
 #include "av1_tables.h"
@@ -304,11 +306,35 @@ __device__ __forceinline__ void grain_prepare_body(const GrainHdr &h, const Grai
     case 2: ar_all<2>(tl, tcb, tcr, h, chroma, cw, ch, gmin, gmax); break;
     default: ar_all<3>(tl, tcb, tcr, h, chroma, cw, ch, gmin, gmax); break;
   }
-  /* templates out */
-  for (int i = threadIdx.x; i < TH * TW; i += PREP_THREADS) {
-    sc.tmpl[i] = tl[i];
-    sc.tmpl[TH * TW + i] = chroma ? tcb[i] : 0;
-    sc.tmpl[2 * TH * TW + i] = chroma ? tcr[i] : 0;
+  /* templates out: per plane the 8 shifted copies (GRAIN_TPLANE). A thread
+   * takes 16 samples of a row (8 aligned words of the LDS template) and
+   * writes the 8 shifted 8-sample chunks starting at its first sample. */
+  {
+    constexpr int Q = GRAIN_TWS / 8;
+    for (int i = threadIdx.x; i < 3 * TH * Q; i += PREP_THREADS) {
+      const int q = i % Q, r = i / Q;
+      const int y = r % TH, p = r / TH;
+      const int16_t *t = p == 0 ? tl : (p == 1 ? tcb : tcr);
+      const bool on = p == 0 || chroma;
+      uint32_t w[9];
+#pragma unroll
+      for (int k = 0; k < 9; k++) {
+        const int x = q * 8 + 2 * k;
+        w[k] = (on && x < TW) ? *reinterpret_cast<const uint32_t *>(t + y * TW + x) : 0u;
+      }
+      /* samples past the template row read as 0 */
+      int16_t *dst = sc.tmpl + (size_t)p * GRAIN_TPLANE + y * GRAIN_TWS + q * 8;
+#pragma unroll
+      for (int sft = 0; sft < 8; sft++) {
+        uint32_t c[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          const int a = (sft >> 1) + k;
+          c[k] = (sft & 1) ? __builtin_amdgcn_alignbyte(w[a + 1], w[a], 2) : w[a];
+        }
+        *reinterpret_cast<uint4 *>(dst + sft * GRAIN_TCOPY) = make_uint4(c[0], c[1], c[2], c[3]);
+      }
+    }
   }
   /* scaling LUTs (grain.rs:137-165) */
   for (int p = 0; p < h.nplanes; p++) {
@@ -346,47 +372,6 @@ __device__ __forceinline__ void grain_prepare_body(const GrainHdr &h, const Grai
   }
 }
 
-/* Noise of plane sample (x, y): blocks of 32 (16 subsampled) with a 2 (1)
- * sample overlap, horizontally blended, in noise-stripe row i of stripe ln. */
-__device__ __forceinline__ int stripe_noise(const int16_t *t, const uint8_t *offs, int mb, int ln,
-                                            int i, int x, int psx, int psy, bool overlap, int gmin,
-                                            int gmax) {
-  const int bs = 32 >> psx;
-  const int bx = x / bs, j = x - bx * bs;
-  const int o = offs[ln * mb + bx];
-  const int ox = o >> 4, oy = o & 15;
-  const int pox = psx ? 6 + ox : 9 + ox * 2, poy = psy ? 6 + oy : 9 + oy * 2;
-  int gv = t[(poy + i) * TW + pox + j];
-  if (overlap && bx > 0 && j < (psx ? 1 : 2)) {
-    const int o2 = offs[ln * mb + bx - 1];
-    const int ox2 = o2 >> 4, oy2 = o2 & 15;
-    const int pox2 = psx ? 6 + ox2 : 9 + ox2 * 2, poy2 = psy ? 6 + oy2 : 9 + oy2 * 2;
-    const int old = t[(poy2 + i) * TW + pox2 + bs + j];
-    if (psx)
-      gv = old * 23 + gv * 22;
-    else
-      gv = j == 0 ? old * 27 + gv * 17 : old * 17 + gv * 27;
-    gv = clip3(gmin, gmax, round2(gv, 5));
-  }
-  return gv;
-}
-
-__device__ __forceinline__ int noise_at(const int16_t *t, const uint8_t *offs, int mb, int x, int y,
-                                        int psx, int psy, bool overlap, int gmin, int gmax) {
-  const int sh = 5 - psy;
-  const int ln = y >> sh, i = y - (ln << sh);
-  int gv = stripe_noise(t, offs, mb, ln, i, x, psx, psy, overlap, gmin, gmax);
-  if (overlap && ln > 0 && i < (psy ? 1 : 2)) {
-    const int old = stripe_noise(t, offs, mb, ln - 1, i + (32 >> psy), x, psx, psy, overlap, gmin, gmax);
-    if (psy)
-      gv = old * 23 + gv * 22;
-    else
-      gv = i == 0 ? old * 27 + gv * 17 : old * 17 + gv * 27;
-    gv = clip3(gmin, gmax, round2(gv, 5));
-  }
-  return gv;
-}
-
 __device__ __forceinline__ int scale_lut(const int16_t *lut, int bd, int index) {
   const int sh = bd - 8;
   const int x = index >> sh;
@@ -396,38 +381,272 @@ __device__ __forceinline__ int scale_lut(const int16_t *lut, int bd, int index) 
   return start + round2((end - start) * rem, sh);
 }
 
-/* Tile tile_id of EX x (256 / EX) chroma positions (raster order). */
-template <typename TI, typename TO, int EX>
+/* Template blend of two noise values (the horizontal and vertical
+ * overlaps of 7.18.3.5): weights 23/22 for a one-sample overlap, 27/17
+ * and 17/27 for the two samples of a two-sample overlap. */
+__device__ __forceinline__ int overlap_blend(int old, int gv, bool one, int k, int gmin, int gmax) {
+  const int v = one ? old * 23 + gv * 22 : (k == 0 ? old * 27 + gv * 17 : old * 17 + gv * 27);
+  return clip3(gmin, gmax, round2(v, 5));
+}
+
+/* Template sample (y, x) of a block whose offset byte is o: the block's
+ * origin is (9 + 2 oy, 9 + 2 ox), or 6 + o in a subsampled direction. */
+__device__ __forceinline__ int origin_x(int o, int psx) { return psx ? 6 + (o >> 4) : 9 + (o >> 4) * 2; }
+__device__ __forceinline__ int origin_y(int o, int psy) { return psy ? 6 + (o & 15) : 9 + (o & 15) * 2; }
+
+/* 8 template samples of plane template pt at row y, columns x .. x + 7:
+ * one aligned 16-byte load from the copy shifted by x & 7. */
+__device__ __forceinline__ void tmpl8(const int16_t *pt, int y, int x, int *o) {
+  const uint4 v = *reinterpret_cast<const uint4 *>(pt + (x & 7) * GRAIN_TCOPY + y * GRAIN_TWS + (x & ~7));
+  const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    o[2 * k] = (int)(int16_t)(w[k] & 0xffff);
+    o[2 * k + 1] = (int)(int16_t)(w[k] >> 16);
+  }
+}
+
+/* Noise of 8 consecutive samples (block bx, columns j0 .. j0 + 7 of the
+ * block, row i of noise stripe ln): the template samples, blended with the
+ * block to the left in the overlap columns, then with the stripe above
+ * (itself blended horizontally) in the overlap rows. Every offset and
+ * template origin is the same for the 8 samples. */
+__device__ __forceinline__ void seg_noise(const int16_t *pt, const uint8_t *offs, int mb, int ln, int bx, int j0,
+                                          int i, int psx, int psy, bool ov, int gmin, int gmax, int *g) {
+  const int bs = 32 >> psx, bsy = 32 >> psy;
+  const int o = offs[ln * mb + bx];
+  tmpl8(pt, origin_y(o, psy) + i, origin_x(o, psx) + j0, g);
+  const bool hov = ov && bx > 0 && j0 == 0;
+  const int nh = psx ? 1 : 2;
+  if (hov) {
+    const int o2 = offs[ln * mb + bx - 1];
+    const int16_t *lp = pt + (origin_y(o2, psy) + i) * GRAIN_TWS + origin_x(o2, psx) + bs;
+    for (int k = 0; k < nh; k++) g[k] = overlap_blend(lp[k], g[k], psx, k, gmin, gmax);
+  }
+  if (ov && ln > 0 && i < (psy ? 1 : 2)) {
+    const int ou = offs[(ln - 1) * mb + bx];
+    int up[8];
+    tmpl8(pt, origin_y(ou, psy) + i + bsy, origin_x(ou, psx) + j0, up);
+    if (hov) {
+      const int ou2 = offs[(ln - 1) * mb + bx - 1];
+      const int16_t *lp = pt + (origin_y(ou2, psy) + i + bsy) * GRAIN_TWS + origin_x(ou2, psx) + bs;
+      for (int k = 0; k < nh; k++) up[k] = overlap_blend(lp[k], up[k], psx, k, gmin, gmax);
+    }
+#pragma unroll
+    for (int k = 0; k < 8; k++) g[k] = overlap_blend(up[k], g[k], psy, i, gmin, gmax);
+  }
+}
+
+/* 8 consecutive samples of a plane row: one 16-byte (u16) or 8-byte (u8)
+ * access when the whole segment lies inside the row, sample by sample at
+ * the row's end. */
+template <typename T>
+struct Seg8;
+template <>
+struct Seg8<uint16_t> {
+  static __device__ __forceinline__ void load(const uint16_t *p, int n, int *o) {
+    if (n >= 8) {
+      const uint4 v = *reinterpret_cast<const uint4 *>(p);
+      const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        o[2 * k] = w[k] & 0xffff;
+        o[2 * k + 1] = w[k] >> 16;
+      }
+    } else {
+      for (int k = 0; k < 8; k++) o[k] = k < n ? p[k] : 0;
+    }
+  }
+  static __device__ __forceinline__ void store(uint16_t *p, int n, const int *o) {
+    if (n >= 8) {
+      *reinterpret_cast<uint4 *>(p) = make_uint4((uint32_t)o[0] | ((uint32_t)o[1] << 16), (uint32_t)o[2] | ((uint32_t)o[3] << 16),
+                                                 (uint32_t)o[4] | ((uint32_t)o[5] << 16), (uint32_t)o[6] | ((uint32_t)o[7] << 16));
+    } else {
+      for (int k = 0; k < n; k++) p[k] = (uint16_t)o[k];
+    }
+  }
+};
+template <>
+struct Seg8<uint8_t> {
+  static __device__ __forceinline__ void load(const uint8_t *p, int n, int *o) {
+    if (n >= 8) {
+      const uint2 v = *reinterpret_cast<const uint2 *>(p);
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        o[k] = (v.x >> (8 * k)) & 0xff;
+        o[4 + k] = (v.y >> (8 * k)) & 0xff;
+      }
+    } else {
+      for (int k = 0; k < 8; k++) o[k] = k < n ? p[k] : 0;
+    }
+  }
+  static __device__ __forceinline__ void store(uint8_t *p, int n, const int *o) {
+    if (n >= 8) {
+      uint32_t a = 0, b = 0;
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        a |= (uint32_t)o[k] << (8 * k);
+        b |= (uint32_t)o[4 + k] << (8 * k);
+      }
+      *reinterpret_cast<uint2 *>(p) = make_uint2(a, b);
+    } else {
+      for (int k = 0; k < n; k++) p[k] = (uint8_t)o[k];
+    }
+  }
+};
+
+/* One tile of the noise-applying stage: noise stripe ln (32 luma rows) and
+ * luma columns [x0, x0 + 64), with the chroma samples they cover. A thread
+ * takes one 8-sample row segment of luma (8 per row, 32 rows) and CI
+ * chroma segments per pass (one pass, or two for 4:4:4 with CI = 1);
+ * every segment lies in one 32-column noise block, so its noise is 8
+ * consecutive template samples. The luma plane load is issued before the
+ * scaling tables are staged. The planes' rows must be 16-byte (u16) or
+ * 8-byte (u8) aligned. Chroma reads the un-noised luma of the input, as
+ * the reference blends chroma first. */
+template <typename T, int CI>
 __device__ __forceinline__ void grain_apply_tile(const GrainHdr &h, const GrainScratch &sc, const PlanePtr &i0,
                                                  const PlanePtr &i1, const PlanePtr &i2, const PlanePtr &o0,
                                                  const PlanePtr &o1, const PlanePtr &o2, int tile_id, int16_t *lut) {
-  for (int i = threadIdx.x; i < h.nplanes * 256; i += 256) lut[i] = sc.lut[i];
-  __syncthreads();
-  const int ssx = h.nplanes > 1 ? h.ssx : 0, ssy = h.nplanes > 1 ? h.ssy : 0;
-  const int cw = (h.w + ssx) >> ssx, ch = (h.h + ssy) >> ssy;
-  const int nbx = (cw + EX - 1) / EX;
-  const int cx = (tile_id % nbx) * EX + (threadIdx.x % EX);
-  const int cy = (tile_id / nbx) * (256 / EX) + (threadIdx.x / EX);
-  if (cx >= cw || cy >= ch) return;
+  const int ntx = (h.w + 63) >> 6;
+  const int ln = tile_id / ntx, x0 = (tile_id - ln * ntx) * 64, y0 = ln * 32;
   const int bd = h.bd;
   const int gmin = -(128 << (bd - 8)), gmax = (256 << (bd - 8)) - 1 - (128 << (bd - 8));
   const bool ov = h.overlap;
   const int mb = sc.max_blocks;
-  const TI *lin = gptr(reinterpret_cast<const TI *>(i0.data));
-  TO *lout = gptr(reinterpret_cast<TO *>(o0.data));
-  if (h.nplanes > 1) {
-    const int lx = cx << ssx, ly = cy << ssy;
-    const int lnx = min(lx + 1, h.w - 1);
-    const int64_t lrow = (int64_t)ly * i0.stride;
-    const int avg = ssx ? round2((int)lin[lrow + lx] + (int)lin[lrow + lnx], 1) : (int)lin[lrow + lx];
-    const int maxv = (1 << bd) - 1;
-    for (int p = 1; p < 3; p++) {
-      const PlanePtr &ip = p == 1 ? i1 : i2;
+  const T *lin = gptr(reinterpret_cast<const T *>(i0.data));
+  const int64_t ls = i0.stride;
+  /* luma segment of this thread */
+  const int lx = x0 + (threadIdx.x & 7) * 8, li = threadIdx.x >> 3, ly = y0 + li;
+  const bool lon = lx < h.w && ly < h.h;
+  int lorig[8];
+  if (lon) Seg8<T>::load(lin + (int64_t)ly * ls + lx, h.w - lx, lorig);
+  for (int i = threadIdx.x; i < h.nplanes * 256; i += 256) lut[i] = sc.lut[i];
+  __syncthreads();
+  int lz[8];
+  if (lon) seg_noise(sc.tmpl, sc.offsets, mb, ln, lx >> 5, lx & 31, li, 0, 0, ov, gmin, gmax, lz);
+  /* chroma segments: items threadIdx.x + 256 (u + CI pass), u < CI */
+  const bool chroma = h.nplanes > 1;
+  const int ssx = chroma ? h.ssx : 0, ssy = chroma ? h.ssy : 0;
+  const int cw = (h.w + ssx) >> ssx, ch = (h.h + ssy) >> ssy;
+  const int segs = 8 >> ssx, per = segs * (32 >> ssy);
+  const int maxv = (1 << bd) - 1;
+  for (int pass = 0; chroma && pass * 256 * CI < 2 * per; pass++) {
+    int cp[CI], cx[CI], cy[CI], ci[CI], corig[CI][8], cavg[CI][8], cz[CI][8];
+    bool con[CI];
+#pragma unroll
+    for (int u = 0; u < CI; u++) {
+      const int it = threadIdx.x + 256 * (u + CI * pass);
+      cp[u] = 1 + (it >= per);
+      const int r = it - (cp[u] - 1) * per;
+      ci[u] = r / segs;
+      cx[u] = (x0 >> ssx) + (r - ci[u] * segs) * 8;
+      cy[u] = (y0 >> ssy) + ci[u];
+      con[u] = it < 2 * per && cx[u] < cw && cy[u] < ch;
+      if (con[u]) {
+        const int n = cw - cx[u];
+        const PlanePtr &ip = cp[u] == 1 ? i1 : i2;
+        Seg8<T>::load(gptr(reinterpret_cast<const T *>(ip.data)) + (int64_t)cy[u] * ip.stride + cx[u], n, corig[u]);
+        const T *lrow = lin + (int64_t)(cy[u] << ssy) * ls + (cx[u] << ssx);
+        if (ssx) {
+          int l[16];
+          const int nl = h.w - (cx[u] << 1);
+          Seg8<T>::load(lrow, nl, l);
+          Seg8<T>::load(lrow + 8, nl - 8, l + 8);
+#pragma unroll
+          for (int k = 0; k < 8; k++) {
+            /* the right neighbour of the last luma column is itself */
+            const int b = 2 * k + 1 < nl ? l[2 * k + 1] : l[2 * k];
+            cavg[u][k] = round2(l[2 * k] + b, 1);
+          }
+        } else {
+          Seg8<T>::load(lrow, n, cavg[u]);
+        }
+        seg_noise(sc.tmpl + (size_t)cp[u] * GRAIN_TPLANE, sc.offsets, mb, ln, cx[u] / (32 >> ssx),
+                  cx[u] & ((32 >> ssx) - 1), ci[u], ssx, ssy, ov, gmin, gmax, cz[u]);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < CI; u++) {
+      if (!con[u]) continue;
+      const int p = cp[u];
+      const int npts = p == 1 ? h.num_cb : h.num_cr;
+      const bool on = npts > 0 || h.cfl;
+      const int lm = p == 1 ? h.cb_luma_mult : h.cr_luma_mult;
+      const int mu = p == 1 ? h.cb_mult : h.cr_mult;
+      const int of = p == 1 ? h.cb_offset : h.cr_offset;
+      int v[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        v[k] = corig[u][k];
+        if (on) {
+          int merged;
+          if (h.cfl) {
+            merged = cavg[u][k];
+          } else {
+            const int combined = cavg[u][k] * (lm - 128) + corig[u][k] * (mu - 128);
+            merged = clip3(0, maxv, (combined >> 6) + (of - 256) * (1 << (bd - 8)));
+          }
+          const int sn = round2(scale_lut(lut + p * 256, bd, merged) * cz[u][k], h.scaling_shift);
+          v[k] = clip3(h.min_value, h.max_chroma, corig[u][k] + sn);
+        }
+      }
       const PlanePtr &op = p == 1 ? o1 : o2;
-      const int orig = gptr(reinterpret_cast<const TI *>(ip.data))[(int64_t)cy * ip.stride + cx];
+      Seg8<T>::store(gptr(reinterpret_cast<T *>(op.data)) + (int64_t)cy[u] * op.stride + cx[u], cw - cx[u], v);
+    }
+  }
+  if (lon) {
+    int v[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      v[k] = lorig[k];
+      if (h.num_y > 0) {
+        const int sn = round2(scale_lut(lut, bd, lorig[k]) * lz[k], h.scaling_shift);
+        v[k] = clip3(h.min_value, h.max_luma, lorig[k] + sn);
+      }
+    }
+    Seg8<T>::store(gptr(reinterpret_cast<T *>(o0.data)) + (int64_t)ly * o0.stride + lx, h.w - lx, v);
+  }
+}
+
+/* Sample-by-sample variant for planes whose rows are not aligned for the
+ * segment accesses: thread t of the tile takes samples t, t + 256, ... of
+ * the tile's luma (64 x 32) and then of each chroma plane. */
+template <typename T>
+__device__ __forceinline__ void grain_apply_tile_unaligned(const GrainHdr &h, const GrainScratch &sc,
+                                                           const PlanePtr &i0, const PlanePtr &i1, const PlanePtr &i2,
+                                                           const PlanePtr &o0, const PlanePtr &o1, const PlanePtr &o2,
+                                                           int tile_id, int16_t *lut) {
+  for (int i = threadIdx.x; i < h.nplanes * 256; i += 256) lut[i] = sc.lut[i];
+  __syncthreads();
+  const int ntx = (h.w + 63) >> 6;
+  const int ln = tile_id / ntx, x0 = (tile_id - ln * ntx) * 64, y0 = ln * 32;
+  const int bd = h.bd;
+  const int gmin = -(128 << (bd - 8)), gmax = (256 << (bd - 8)) - 1 - (128 << (bd - 8));
+  const bool ov = h.overlap;
+  const int mb = sc.max_blocks;
+  const T *lin = gptr(reinterpret_cast<const T *>(i0.data));
+  const int64_t ls = i0.stride;
+  const int maxv = (1 << bd) - 1;
+  const int np = h.nplanes > 1 ? 3 : 1;
+  for (int p = 1; p < np; p++) {
+    const int ssx = h.ssx, ssy = h.ssy;
+    const int cw = (h.w + ssx) >> ssx, ch = (h.h + ssy) >> ssy;
+    const int tw = 64 >> ssx, th = 32 >> ssy;
+    const PlanePtr &ip = p == 1 ? i1 : i2;
+    const PlanePtr &op = p == 1 ? o1 : o2;
+    const int16_t *pt = sc.tmpl + (size_t)p * GRAIN_TPLANE;
+    for (int t = threadIdx.x; t < tw * th; t += 256) {
+      const int ci = t / tw, c = t - ci * tw;
+      const int cx = (x0 >> ssx) + c, cy = (y0 >> ssy) + ci;
+      if (cx >= cw || cy >= ch) continue;
+      const int orig = gptr(reinterpret_cast<const T *>(ip.data))[(int64_t)cy * ip.stride + cx];
       int v = orig;
       const int npts = p == 1 ? h.num_cb : h.num_cr;
       if (npts > 0 || h.cfl) {
+        const T *lrow = lin + (int64_t)(cy << ssy) * ls;
+        const int lx = cx << ssx, lnx = min(lx + 1, h.w - 1);
+        const int avg = ssx ? round2((int)lrow[lx] + (int)lrow[lnx], 1) : (int)lrow[lx];
         int merged;
         if (h.cfl) {
           merged = avg;
@@ -438,29 +657,27 @@ __device__ __forceinline__ void grain_apply_tile(const GrainHdr &h, const GrainS
           const int combined = avg * (lm - 128) + orig * (mu - 128);
           merged = clip3(0, maxv, (combined >> 6) + (of - 256) * (1 << (bd - 8)));
         }
-        const int noise = noise_at(sc.tmpl + p * TH * TW, sc.offsets, mb, cx, cy, ssx, ssy, ov, gmin, gmax);
-        const int sn = round2(scale_lut(lut + p * 256, bd, merged) * noise, h.scaling_shift);
+        const int bs = 32 >> ssx;
+        int z[8];
+        seg_noise(pt, sc.offsets, mb, ln, cx / bs, (cx % bs) & ~7, ci, ssx, ssy, ov, gmin, gmax, z);
+        const int sn = round2(scale_lut(lut + p * 256, bd, merged) * z[cx & 7], h.scaling_shift);
         v = clip3(h.min_value, h.max_chroma, orig + sn);
       }
-      gptr(reinterpret_cast<TO *>(op.data))[(int64_t)cy * op.stride + cx] = (TO)v;
+      gptr(reinterpret_cast<T *>(op.data))[(int64_t)cy * op.stride + cx] = (T)v;
     }
   }
-  /* luma samples of this chroma position */
-  for (int dy = 0; dy <= ssy; dy++) {
-    const int ly = (cy << ssy) + dy;
-    if (ly >= h.h) break;
-    for (int dx = 0; dx <= ssx; dx++) {
-      const int lx = (cx << ssx) + dx;
-      if (lx >= h.w) break;
-      const int orig = lin[(int64_t)ly * i0.stride + lx];
-      int v = orig;
-      if (h.num_y > 0) {
-        const int noise = noise_at(sc.tmpl, sc.offsets, mb, lx, ly, 0, 0, ov, gmin, gmax);
-        const int sn = round2(scale_lut(lut, bd, orig) * noise, h.scaling_shift);
-        v = clip3(h.min_value, h.max_luma, orig + sn);
-      }
-      lout[(int64_t)ly * o0.stride + lx] = (TO)v;
+  for (int t = threadIdx.x; t < 64 * 32; t += 256) {
+    const int i = t >> 6, x = x0 + (t & 63), y = y0 + i;
+    if (x >= h.w || y >= h.h) continue;
+    const int orig = lin[(int64_t)y * ls + x];
+    int v = orig;
+    if (h.num_y > 0) {
+      int z[8];
+      seg_noise(sc.tmpl, sc.offsets, mb, ln, x >> 5, x & 24, i, 0, 0, ov, gmin, gmax, z);
+      const int sn = round2(scale_lut(lut, bd, orig) * z[x & 7], h.scaling_shift);
+      v = clip3(h.min_value, h.max_luma, orig + sn);
     }
+    gptr(reinterpret_cast<T *>(o0.data))[(int64_t)y * o0.stride + x] = (T)v;
   }
 }
 
@@ -469,12 +686,17 @@ __global__ __launch_bounds__(PREP_THREADS) void grain_prepare_kernel(GrainHdr h,
   grain_prepare_body(h, sc, sm);
 }
 
-template <typename TI, typename TO>
-__global__ __launch_bounds__(256) void grain_apply_kernel(GrainHdr h, GrainScratch sc, PlanePtr i0,
-                                                          PlanePtr i1, PlanePtr i2, PlanePtr o0,
-                                                          PlanePtr o1, PlanePtr o2) {
+/* ALIGNED: 1 (segments, one chroma segment per thread), 2 (segments,
+ * two: 4:4:4), 0 (sample by sample). */
+template <typename T, int ALIGNED>
+__global__ __launch_bounds__(256) void grain_apply_kernel(GrainHdr h, GrainScratch sc, PlanePtr i0, PlanePtr i1,
+                                                          PlanePtr i2, PlanePtr o0, PlanePtr o1, PlanePtr o2) {
   __shared__ int16_t lut[3 * 256];
-  grain_apply_tile<TI, TO, 64>(h, sc, i0, i1, i2, o0, o1, o2, xcd_remap(blockIdx.x, gridDim.x), lut);
+  const int tile = xcd_remap(blockIdx.x, gridDim.x);
+  if constexpr (ALIGNED != 0)
+    grain_apply_tile<T, ALIGNED>(h, sc, i0, i1, i2, o0, o1, o2, tile, lut);
+  else
+    grain_apply_tile_unaligned<T>(h, sc, i0, i1, i2, o0, o1, o2, tile, lut);
 }
 
 struct PrepRecord {
@@ -486,6 +708,7 @@ struct ApplyRecord {
   GrainHdr h;
   GrainScratch s;
   PlanePtr in[3], out[3];
+  int aligned;
 };
 
 __global__ __launch_bounds__(PREP_THREADS) void grain_prepare_batch(BatchWork w) {
@@ -496,30 +719,30 @@ __global__ __launch_bounds__(PREP_THREADS) void grain_prepare_batch(BatchWork w)
   });
 }
 
-template <typename T, int EX>
+template <typename T>
 __global__ __launch_bounds__(256) void grain_apply_batch(BatchWork w) {
   __shared__ int16_t lut[3 * 256];
   batch_loop(w, [&](int f, int j) {
     const ApplyRecord &r = batch_table<ApplyRecord>(w.rec, f);
-    grain_apply_tile<T, T, EX>(r.h, r.s, r.in[0], r.in[1], r.in[2], r.out[0], r.out[1], r.out[2], j, lut);
+    if (r.aligned)
+      grain_apply_tile<T, 1>(r.h, r.s, r.in[0], r.in[1], r.in[2], r.out[0], r.out[1], r.out[2], j, lut);
+    else
+      grain_apply_tile_unaligned<T>(r.h, r.s, r.in[0], r.in[1], r.in[2], r.out[0], r.out[1], r.out[2], j, lut);
   });
 }
 
-constexpr int SHAPE_EX[GRAIN_SHAPES] = {64, 32, 16};
+/* Tiles of the noise-applying stage: 64 luma columns of one noise stripe. */
+int apply_tiles(const GrainHdr &h) { return ((h.w + 63) / 64) * ((h.h + 31) / 32); }
 
-int apply_tiles(const GrainHdr &h, int ex) {
-  const int ssx = h.nplanes > 1 ? h.ssx : 0, ssy = h.nplanes > 1 ? h.ssy : 0;
-  const int cw = (h.w + ssx) >> ssx, ch = (h.h + ssy) >> ssy;
-  return ((cw + ex - 1) / ex) * ((ch + 256 / ex - 1) / (256 / ex));
-}
-
-template <typename T>
-void launch_apply_shape(const BatchWork &w, const StagePlan &sp, hipStream_t st) {
-  switch (sp.shape) {
-    case 1: grain_apply_batch<T, 32><<<sp.grid, 256, 0, st>>>(w); break;
-    case 2: grain_apply_batch<T, 16><<<sp.grid, 256, 0, st>>>(w); break;
-    default: grain_apply_batch<T, 64><<<sp.grid, 256, 0, st>>>(w); break;
+/* Every plane row in and out starts on a segment boundary (8 samples). */
+bool rows_aligned(const GrainHdr &h, const PlanePtr in[3], const PlanePtr out[3], int bps) {
+  const uintptr_t a = 8 * (uintptr_t)bps;
+  for (int p = 0; p < h.nplanes; p++) {
+    const PlanePtr *q[2] = {&in[p], &out[p]};
+    for (const PlanePtr *x : q)
+      if (((uintptr_t)x->data % a) != 0 || (x->stride % 8) != 0) return false;
   }
+  return true;
 }
 
 }  // namespace
@@ -547,7 +770,7 @@ int launch_grain_prepare_batch(const BatchWork &w, const StagePlan &sp, hipStrea
 size_t grain_apply_record_size() { return sizeof(ApplyRecord); }
 
 int grain_apply_plan(const GrainHdr &h, const GrainScratch &s, const PlanePtr in[3], const PlanePtr out[3],
-                     int shape, void *rec) {
+                     int bps, void *rec) {
   ApplyRecord &r = *static_cast<ApplyRecord *>(rec);
   r.h = h;
   r.s = s;
@@ -555,25 +778,38 @@ int grain_apply_plan(const GrainHdr &h, const GrainScratch &s, const PlanePtr in
     r.in[p] = in[p];
     r.out[p] = out[p];
   }
-  return apply_tiles(h, SHAPE_EX[shape < 0 || shape >= GRAIN_SHAPES ? 0 : shape]);
+  r.aligned = rows_aligned(h, in, out, bps);
+  return apply_tiles(h);
 }
 
 int launch_grain_apply_batch(const BatchWork &w, const StagePlan &sp, int bps, hipStream_t st) {
   if (w.total == 0) return 0;
   if (bps == 1)
-    launch_apply_shape<uint8_t>(w, sp, st);
+    grain_apply_batch<uint8_t><<<sp.grid, 256, 0, st>>>(w);
   else
-    launch_apply_shape<uint16_t>(w, sp, st);
+    grain_apply_batch<uint16_t><<<sp.grid, 256, 0, st>>>(w);
   return (int)hipGetLastError();
 }
 
 int launch_grain_apply(const GrainHdr &h, const GrainScratch &s, const PlanePtr in[3],
                        PlanePtr out[3], int bps, hipStream_t st) {
-  const int grid = apply_tiles(h, 64);
-  if (bps == 1)
-    grain_apply_kernel<uint8_t, uint8_t><<<grid, 256, 0, st>>>(h, s, in[0], in[1], in[2], out[0], out[1], out[2]);
-  else
-    grain_apply_kernel<uint16_t, uint16_t><<<grid, 256, 0, st>>>(h, s, in[0], in[1], in[2], out[0], out[1], out[2]);
+  const int grid = apply_tiles(h);
+  const int kind = !rows_aligned(h, in, out, bps) ? 0 : (h.nplanes > 1 && !h.ssx ? 2 : 1);
+  if (bps == 1) {
+    if (kind == 1)
+      grain_apply_kernel<uint8_t, 1><<<grid, 256, 0, st>>>(h, s, in[0], in[1], in[2], out[0], out[1], out[2]);
+    else if (kind == 2)
+      grain_apply_kernel<uint8_t, 2><<<grid, 256, 0, st>>>(h, s, in[0], in[1], in[2], out[0], out[1], out[2]);
+    else
+      grain_apply_kernel<uint8_t, 0><<<grid, 256, 0, st>>>(h, s, in[0], in[1], in[2], out[0], out[1], out[2]);
+  } else {
+    if (kind == 1)
+      grain_apply_kernel<uint16_t, 1><<<grid, 256, 0, st>>>(h, s, in[0], in[1], in[2], out[0], out[1], out[2]);
+    else if (kind == 2)
+      grain_apply_kernel<uint16_t, 2><<<grid, 256, 0, st>>>(h, s, in[0], in[1], in[2], out[0], out[1], out[2]);
+    else
+      grain_apply_kernel<uint16_t, 0><<<grid, 256, 0, st>>>(h, s, in[0], in[1], in[2], out[0], out[1], out[2]);
+  }
   return (int)hipGetLastError();
 }
 

@@ -411,7 +411,7 @@ int ensure_grain_scratch(rvpf_ctx *ctx, int slot, int w, int h) {
   const int ms = stripes > g.max_stripes ? stripes : g.max_stripes;
   const int mb = blocks > g.max_blocks ? blocks : g.max_blocks;
   hipError_t e = hipSuccess;
-  if (!g.tmpl) e = hipMalloc(&g.tmpl, 3 * 73 * 82 * sizeof(int16_t));
+  if (!g.tmpl) e = hipMalloc(&g.tmpl, GRAIN_TMPL_BYTES);
   if (e == hipSuccess && !g.lut) e = hipMalloc(&g.lut, 3 * 256 * sizeof(int16_t));
   if (e == hipSuccess) e = hipMalloc(&g.offsets, (size_t)ms * mb);
   if (e != hipSuccess) {

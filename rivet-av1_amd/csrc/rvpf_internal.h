@@ -76,9 +76,18 @@ struct GrainHdr {
   int npoints[3];
 };
 
+/* Film grain templates in device scratch: per plane, 8 copies of the
+ * 73 x 82 template, copy s shifted left by s samples, rows padded to 88,
+ * so that any 8 consecutive template samples are one aligned 16-byte
+ * load (copy 0 is the template itself). */
+constexpr int GRAIN_TH = 73, GRAIN_TW = 82, GRAIN_TWS = 88;
+constexpr int GRAIN_TCOPY = GRAIN_TH * GRAIN_TWS;  /* samples per shifted copy */
+constexpr int GRAIN_TPLANE = 8 * GRAIN_TCOPY;      /* samples per plane */
+constexpr size_t GRAIN_TMPL_BYTES = 3 * (size_t)GRAIN_TPLANE * sizeof(int16_t);
+
 /* Device scratch for the grain stage. */
 struct GrainScratch {
-  int16_t *tmpl;    /* [3][73][82] */
+  int16_t *tmpl;    /* [3][8][73][88], see GRAIN_TPLANE */
   int16_t *lut;     /* [3][256] */
   uint8_t *offsets; /* [stripes][blocks]: (offset_x << 4) | offset_y */
   int max_stripes, max_blocks;
@@ -226,12 +235,12 @@ int launch_grain_prepare_batch(const BatchWork &w, const StagePlan &sp, hipStrea
 
 size_t grain_apply_record_size();
 int grain_apply_plan(const GrainHdr &h, const GrainScratch &s, const PlanePtr in[3], const PlanePtr out[3],
-                     int shape, void *rec);
+                     int bps, void *rec);
 int launch_grain_apply_batch(const BatchWork &w, const StagePlan &sp, int bps, hipStream_t st);
 
 /* Number of tile shapes of each shape-tunable stage. */
 constexpr int LF_SHAPES = 3;    /* 64x4, 32x8, 16x16 edges per workgroup */
-constexpr int GRAIN_SHAPES = 3; /* 64x4, 32x8, 16x16 chroma positions per workgroup */
+constexpr int GRAIN_SHAPES = 1; /* 64 luma columns of a noise stripe per workgroup */
 constexpr int COPY_SHAPES = 1;
 
 /* Differentiable loop restoration (lrdiff.hip). */

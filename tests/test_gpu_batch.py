@@ -97,8 +97,8 @@ def test_batch_replays(ctx, entries):
 
 
 TILES = [
-    {"deblock": (1, 4, 1), "grain_apply": (1, 4, 1)},
-    {"deblock": (2, 2, 4), "grain_apply": (2, 2, 4), "copy": (0, 4, 4)},
+    {"deblock": (1, 4, 1), "grain_apply": (0, 4, 1)},
+    {"deblock": (2, 2, 4), "grain_apply": (0, 2, 4), "copy": (0, 4, 4)},
     {"deblock": (0, 0, 1), "cdef": (0, 0, 1), "lr": (0, 0, 1), "grain_prepare": (0, 0, 1),
      "grain_apply": (0, 0, 1), "copy": (0, 0, 1)},
     {"cdef": (0, 1, 2), "lr": (0, 1, 2), "grain_prepare": (0, 2, 1)},

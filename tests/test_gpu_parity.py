@@ -106,11 +106,15 @@ GRAIN_LAYOUTS = {"420": dict(ssx=1, ssy=1), "422": dict(ssx=1, ssy=0), "444": di
 @pytest.mark.parametrize("lag", [0, 1, 2, 3])
 @pytest.mark.parametrize("layout", sorted(GRAIN_LAYOUTS))
 @pytest.mark.parametrize("bd", [8, 10, 12])
-def test_grain_lags(ctx, lag, layout, bd):
+@pytest.mark.parametrize("rows", ["w98", "w128", "w98pad"])
+def test_grain_lags(ctx, lag, layout, bd, rows):
     """Film grain alone (templates, LUTs, offsets, blend) at every AR lag,
-    chroma layout and bit depth, on random planes."""
-    _say("grain lag", lag, layout, bd)
-    f = common.make_frame(width=98, height=70, bit_depth=bd, qindex=120, lr=0, grain=True, grain_lag=lag,
+    chroma layout and bit depth, on random planes: rows of 98 samples
+    (sample-by-sample kernel), of 128 (8-sample segments) and of 98 inside
+    64-aligned rows (segments, the last one partial)."""
+    _say("grain lag", lag, layout, bd, rows)
+    width = 128 if rows == "w128" else 98
+    f = common.make_frame(width=width, height=70, bit_depth=bd, qindex=120, lr=0, grain=True, grain_lag=lag,
                           seed=100 + 10 * lag + bd, **GRAIN_LAYOUTS[layout])
     g = f.params.grain
     if f.num_planes > 1 and not g.chroma_scaling_from_luma:
@@ -130,6 +134,15 @@ def test_grain_lags(ctx, lag, layout, bd):
     df = api.DeviceFrame(f)
     src = df.upload_planes(planes)
     disp = df.alloc_display()
+    if rows == "w98pad":
+        import torch
+        def padded(t):
+            if t is None:
+                return None
+            big = torch.zeros((t.shape[0], (t.shape[1] + 63) // 64 * 64), dtype=t.dtype, device=t.device)
+            big[:, :t.shape[1]] = t
+            return big[:, :t.shape[1]]
+        src, disp = [padded(t) for t in src], [padded(t) for t in disp]
     api.grain(ctx, df, src, disp)
     ctx.sync()
-    common.assert_planes(api.to_host(disp), want, f, "crop", "grain lag %d %s %d-bit" % (lag, layout, bd))
+    common.assert_planes(api.to_host(disp), want, f, "crop", "grain lag %d %s %d-bit %s" % (lag, layout, bd, rows))
