@@ -20,6 +20,8 @@
  * these footprints never overlap another thread's reads or writes and the
  * result equals the sequential one sample for sample.
  */
+#include <string.h>
+
 #include "av1_tables.h"
 #include "rvpf_internal.h"
 
@@ -29,21 +31,39 @@ namespace {
 
 __device__ __forceinline__ int clip3(int lo, int hi, int x) { return min(max(x, lo), hi); }
 
-/* filter_strength, src/decoder/postfilter.rs:207-264: lvl of the block. */
-__device__ __forceinline__ int lf_level(const LfHdr &h, const rvpf_mi &m, int i) {
+/* Everything the edge decision looks up per lane, built on the host and
+ * staged in LDS once per workgroup: log2 of the transform width / height
+ * of each TX_SIZE, log2 of the width / height of each block size as seen
+ * by luma [0] and chroma [1] (0xff where the subsampled size is invalid,
+ * i.e. every position is a block edge), and the level parameters. */
+struct LfTables {
+  uint8_t tx_log2[2][20];   /* [pass][tx size] */
+  uint8_t blk_log2[2][2][22]; /* [luma / chroma][pass][mi_size] */
+  int8_t ref_deltas[8];
+  int8_t mode_deltas[2];
+  uint8_t seg_lf_enabled[8];
+  int8_t seg_lf_data[8][4];
+  uint8_t level[4];
+  uint8_t pad[2];
+};
+static_assert(sizeof(LfTables) % 4 == 0, "LfTables is staged in words");
+
+/* filter_strength, src/decoder/postfilter.rs:207-264: lvl of the block
+ * (header values from the LDS tables t). */
+__device__ __forceinline__ int lf_level(const LfHdr &h, const LfTables &t, const rvpf_mi &m, int i) {
   const int seg = (m.seg_skip >> 1) & 7;
   const int rf = m.ref_frame0;
   const int mode = m.y_mode;
   const int mode_type = (mode >= 14 && mode != 16 && mode != 24); /* >= NEARESTMV, not GLOBALMV / GLOBAL_GLOBALMV */
   const int dlf = h.delta_lf_multi ? m.delta_lf[i] : m.delta_lf[0];
   int lvl = clip3(0, 63, dlf + h.level[i]);
-  if (h.seg_enabled && ((h.seg_lf_enabled[seg] >> i) & 1)) lvl = clip3(0, 63, h.seg_lf_data[seg][i] + lvl);
+  if (h.seg_enabled && ((t.seg_lf_enabled[seg] >> i) & 1)) lvl = clip3(0, 63, t.seg_lf_data[seg][i] + lvl);
   if (h.delta_enabled) {
     const int ns = 1 << (lvl >> 5);
     if (rf == 0)
-      lvl += h.ref_deltas[0] * ns;
+      lvl += t.ref_deltas[0] * ns;
     else if (rf > 0)
-      lvl += (h.ref_deltas[rf & 7] + h.mode_deltas[mode_type]) * ns;
+      lvl += (t.ref_deltas[rf & 7] + t.mode_deltas[mode_type]) * ns;
     lvl = clip3(0, 63, lvl);
   }
   return lvl;
@@ -143,6 +163,14 @@ template <typename T>
 struct Io;
 template <>
 struct Io<uint16_t> {
+  typedef uint2 W; /* 4 samples */
+  static __device__ __forceinline__ W ldw(const uint16_t *p) { return *reinterpret_cast<const uint2 *>(p); }
+  static __device__ __forceinline__ void unpack(W u, int *o) {
+    o[0] = u.x & 0xffff;
+    o[1] = u.x >> 16;
+    o[2] = u.y & 0xffff;
+    o[3] = u.y >> 16;
+  }
   /* 4 consecutive samples at p (8-byte aligned) */
   static __device__ __forceinline__ void ld4(const uint16_t *p, int *o) {
     const uint2 u = *reinterpret_cast<const uint2 *>(p);
@@ -164,6 +192,14 @@ struct Io<uint16_t> {
 };
 template <>
 struct Io<uint8_t> {
+  typedef uint32_t W; /* 4 samples */
+  static __device__ __forceinline__ W ldw(const uint8_t *p) { return *reinterpret_cast<const uint32_t *>(p); }
+  static __device__ __forceinline__ void unpack(W u, int *o) {
+    o[0] = u & 0xff;
+    o[1] = (u >> 8) & 0xff;
+    o[2] = (u >> 16) & 0xff;
+    o[3] = u >> 24;
+  }
   static __device__ __forceinline__ void ld4(const uint8_t *p, int *o) {
     const uint32_t u = *reinterpret_cast<const uint32_t *>(p);
     o[0] = u & 0xff;
@@ -183,6 +219,7 @@ struct Io<uint8_t> {
 struct LfLaunch {
   Geom g;
   LfHdr h;
+  LfTables t;
   LfSide s;
   PlanePtr planes[3];
   int plane_id[3];   /* planes filtered, in grid order */
@@ -192,48 +229,59 @@ struct LfLaunch {
   int nseg;
 };
 
-/* Edge decision of edge_loop_filter (postfilter.rs:115-172). Returns the
- * filter size (0: no filter) and the strength. */
-__device__ __forceinline__ int edge_decision(const LfLaunch &a, int plane, int pass, int ex, int ey,
-                                             int *limit_bd, int *blimit_bd, int *thresh_bd) {
+/* Mode info and transform sizes an edge decision reads, loaded together
+ * with the samples: the block of the edge (m), the block before it (pm,
+ * used when m's level is 0), the transform sizes of both sides. */
+struct EdgeSide {
+  rvpf_mi m, pm;
+  int tx, ptx;
+};
+
+__device__ __forceinline__ EdgeSide load_side(const LfLaunch &a, int plane, int pass, int ex, int ey) {
   const Geom &g = a.g;
   const int ssx = plane ? g.ssx : 0, ssy = plane ? g.ssy : 0;
-  const int row = (ey << ssy), col = (ex << ssx);
-  const int x = col * 4, y = row * 4;
-  if (x >= g.width || y >= g.height) return 0;
-  if (pass == 0 ? x == 0 : y == 0) return 0;
-  const int xp = x >> ssx, yp = y >> ssy;
-  const int rowo = row | ssy, colo = col | ssx;
+  const int rowo = (ey << ssy) | ssy, colo = (ex << ssx) | ssx;
   const int ms = a.s.mi_stride;
-  const rvpf_mi *mi = gptr(a.s.mi);
-  const rvpf_mi m = mi[(int64_t)rowo * ms + colo];
+  const uint2 *mi = gptr(reinterpret_cast<const uint2 *>(a.s.mi));
+  const int prow = pass == 0 ? rowo : rowo - (1 << ssy);
+  const int pcol = pass == 0 ? colo - (1 << ssx) : colo;
   const uint8_t *tx = gptr(a.s.tx[plane]);
-  const int tx_sz = min((int)tx[(int64_t)ey * ms + ex], 18);
-  const int prev_tx_sz = min((int)(pass == 0 ? tx[(int64_t)ey * ms + ex - 1] : tx[(int64_t)(ey - 1) * ms + ex]), 18);
-  const int plane_size = AV1_SUBSAMPLED_SIZE[min((int)m.mi_size, 21)][ssx][ssy];
-  bool is_block_edge = true; /* BLOCK_INVALID: a block edge everywhere */
-  if (plane_size < 22)
-    is_block_edge = pass == 0 ? (xp & (4 * AV1_NUM_4X4_BLOCKS_WIDE[plane_size] - 1)) == 0
-                              : (yp & (4 * AV1_NUM_4X4_BLOCKS_HIGH[plane_size] - 1)) == 0;
-  const bool is_tx_edge = pass == 0 ? (xp & (AV1_TX_WIDTH[tx_sz] - 1)) == 0
-                                    : (yp & (AV1_TX_HEIGHT[tx_sz] - 1)) == 0;
+  EdgeSide e;
+  e.m = __builtin_bit_cast(rvpf_mi, mi[(int64_t)rowo * ms + colo]);
+  e.pm = __builtin_bit_cast(rvpf_mi, mi[(int64_t)prow * ms + pcol]);
+  e.tx = tx[(int64_t)ey * ms + ex];
+  e.ptx = pass == 0 ? tx[(int64_t)ey * ms + ex - 1] : tx[(int64_t)(ey - 1) * ms + ex];
+  return e;
+}
+
+/* Edge decision of edge_loop_filter (postfilter.rs:115-172). Returns the
+ * filter size (0: no filter) and the strength. t: the LDS tables. */
+__device__ __forceinline__ int edge_decision(const LfLaunch &a, const LfTables &t, const EdgeSide &e, int plane,
+                                             int pass, int ex, int ey, int *limit_bd, int *blimit_bd,
+                                             int *thresh_bd) {
+  const Geom &g = a.g;
+  const int ssx = plane ? g.ssx : 0, ssy = plane ? g.ssy : 0;
+  const int x = (ex << ssx) * 4, y = (ey << ssy) * 4;
+  if (x >= g.width || y >= g.height) return 0;
+  const int pos = pass == 0 ? x >> ssx : y >> ssy;
+  const rvpf_mi &m = e.m;
+  const int tx_sz = min(e.tx, 18), prev_tx_sz = min(e.ptx, 18);
+  /* BLOCK_INVALID (log2 0xff): a block edge everywhere */
+  const int bl = t.blk_log2[plane != 0][pass][min((int)m.mi_size, 21)];
+  const bool is_block_edge = bl == 0xff || (pos & ((1 << bl) - 1)) == 0;
+  const int tl = t.tx_log2[pass][tx_sz], ptl = t.tx_log2[pass][prev_tx_sz];
+  const bool is_tx_edge = (pos & ((1 << tl) - 1)) == 0;
   const bool skip = m.seg_skip & 1;
   const bool is_intra = m.ref_frame0 <= 0;
   if (!(is_tx_edge && (is_block_edge || !skip || is_intra))) return 0;
-  const int base_size = pass == 0 ? min(AV1_TX_WIDTH[prev_tx_sz], AV1_TX_WIDTH[tx_sz])
-                                  : min(AV1_TX_HEIGHT[prev_tx_sz], AV1_TX_HEIGHT[tx_sz]);
+  const int base_size = 1 << min(tl, ptl);
   int filter_size = min(plane == 0 ? 16 : 8, base_size);
   /* A 16-wide edge of legal side info is at least 16 samples from the
    * plane's edge; keep the reads in bounds for side info that is not. */
-  if (filter_size == 16 && (pass == 0 ? xp : yp) < 8) filter_size = 8;
+  if (filter_size == 16 && pos < 8) filter_size = 8;
   const int li = plane == 0 ? pass : plane + 1;
-  int lvl = lf_level(a.h, m, li);
-  if (lvl == 0) {
-    const int prow = pass == 0 ? rowo : rowo - (1 << ssy);
-    const int pcol = pass == 0 ? colo - (1 << ssx) : colo;
-    const rvpf_mi pm = mi[(int64_t)prow * ms + pcol];
-    lvl = lf_level(a.h, pm, li);
-  }
+  int lvl = lf_level(a.h, t, m, li);
+  if (lvl == 0) lvl = lf_level(a.h, t, e.pm, li);
   if (lvl == 0) return 0;
   const int sharp = a.h.sharpness;
   const int shift = sharp > 4 ? 2 : (sharp > 0 ? 1 : 0);
@@ -245,27 +293,32 @@ __device__ __forceinline__ int edge_decision(const LfLaunch &a, int plane, int p
   return filter_size;
 }
 
+constexpr int LF_TWORDS = sizeof(LfTables) / 4;
+
 /* Block lb of grid plane gp: EX x (256 / EX) edges, one per thread. */
 template <typename T, int PASS, int EX>
-__device__ __forceinline__ void lf_block(const LfLaunch &a, int gp, int lb) {
+__device__ __forceinline__ void lf_block(const LfLaunch &a, int gp, int lb, LfTables *tsm) {
   const int plane = a.plane_id[gp];
   const int ex = (lb % a.blk_x[gp]) * EX + (threadIdx.x % EX);
   const int ey = (lb / a.blk_x[gp]) * (256 / EX) + (threadIdx.x / EX);
   const Geom &g = a.g;
   const int ssx = plane ? g.ssx : 0, ssy = plane ? g.ssy : 0;
-  if ((ey << ssy) >= g.mi_rows || (ex << ssx) >= g.mi_cols) return;
   const int xp = ex * 4, yp = ey * 4;
-  /* the edge is not filtered at the frame's left / top border */
-  if (PASS == 0 ? xp == 0 : yp == 0) return;
+  /* inside the mode-info area; the edge is not filtered at the frame's
+   * left / top border */
+  const bool valid = (ey << ssy) < g.mi_rows && (ex << ssx) < g.mi_cols && (PASS == 0 ? xp != 0 : yp != 0);
   const bool chroma = plane != 0;
   T *base = gptr(reinterpret_cast<T *>(a.planes[plane].data));
   const int64_t stride = a.planes[plane].stride;
-  /* The samples are loaded before the edge decision reads the mode info,
-   * so both memory round trips overlap; edges that turn out unfiltered
+  /* Every load of the edge is issued at once, before the workgroup stages
+   * its tables: the samples (raw words, unpacked after the barrier), the
+   * mode info and the transform sizes; edges that turn out unfiltered
    * only cost reads of a plane that the pass reads anyway. */
-  int s[4][16];
-  int v[4][14];
-  if (PASS == 0) {
+  const uint32_t tword = threadIdx.x < LF_TWORDS ? reinterpret_cast<const uint32_t *>(&a.t)[threadIdx.x] : 0u;
+  typename Io<T>::W raw[16];
+  EdgeSide side;
+  if (!valid) {
+  } else if (PASS == 0) {
     /* vertical edge at column xp: each lane is a row. The outer quads
      * (x - 8, x + 4) are only used by 16-wide edges, which have x >= 8 */
     T *row0 = base + (int64_t)yp * stride + xp;
@@ -273,11 +326,12 @@ __device__ __forceinline__ void lf_block(const LfLaunch &a, int gp, int lb) {
 #pragma unroll
     for (int l = 0; l < 4; l++) {
       T *row = row0 + (int64_t)l * stride;
-      Io<T>::ld4(row + lo8, s[l]);
-      Io<T>::ld4(row - 4, s[l] + 4);
-      Io<T>::ld4(row, s[l] + 8);
-      Io<T>::ld4(row + 4, s[l] + 12);
+      raw[4 * l] = Io<T>::ldw(row + lo8);
+      raw[4 * l + 1] = Io<T>::ldw(row - 4);
+      raw[4 * l + 2] = Io<T>::ldw(row);
+      raw[4 * l + 3] = Io<T>::ldw(row + 4);
     }
+    side = load_side(a, plane, PASS, ex, ey);
   } else {
     /* horizontal edge at row yp: each lane is a column. Rows beyond +-4
      * are only used by 16-wide edges, which have y >= 8 (rows above 0
@@ -287,14 +341,32 @@ __device__ __forceinline__ void lf_block(const LfLaunch &a, int gp, int lb) {
     for (int k = 0; k < 14; k++) {
       const int d = k - 7;
       const int dd = (d < -4 && yp < 8) ? -4 : d;
+      raw[k] = Io<T>::ldw(col + (int64_t)dd * stride);
+    }
+    side = load_side(a, plane, PASS, ex, ey);
+  }
+  if (threadIdx.x < LF_TWORDS) reinterpret_cast<uint32_t *>(tsm)[threadIdx.x] = tword;
+  __syncthreads();
+  const LfTables &t = *tsm;
+  if (!valid) return;
+  int s[4][16];
+  int v[4][14];
+  if (PASS == 0) {
+#pragma unroll
+    for (int l = 0; l < 4; l++)
+#pragma unroll
+      for (int q = 0; q < 4; q++) Io<T>::unpack(raw[4 * l + q], s[l] + 4 * q);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 14; k++) {
       int q[4];
-      Io<T>::ld4(col + (int64_t)dd * stride, q);
+      Io<T>::unpack(raw[k], q);
 #pragma unroll
       for (int l = 0; l < 4; l++) v[l][k] = q[l];
     }
   }
   int limit_bd, blimit_bd, thresh_bd;
-  const int fs = edge_decision(a, plane, PASS, ex, ey, &limit_bd, &blimit_bd, &thresh_bd);
+  const int fs = edge_decision(a, t, side, plane, PASS, ex, ey, &limit_bd, &blimit_bd, &thresh_bd);
   if (fs == 0) return;
   if (PASS == 0) {
     T *row0 = base + (int64_t)yp * stride + xp;
@@ -342,22 +414,51 @@ __device__ __forceinline__ void lf_block(const LfLaunch &a, int gp, int lb) {
 
 template <typename T, int PASS>
 __global__ __launch_bounds__(256) void lf_pass_kernel(LfLaunch a) {
+  __shared__ LfTables tsm;
   int lb;
   const int gp = xcd_segment(blockIdx.x, a.blk_start, a.nseg, &lb);
   if (gp >= a.nseg || lb >= a.blk_n[gp]) return; /* padding */
-  lf_block<T, PASS, 64>(a, gp, lb);
+  lf_block<T, PASS, 64>(a, gp, lb, &tsm);
 }
 
 /* Batched: item j of a frame is block j of its (unpadded) grid. */
 template <typename T, int PASS, int EX>
 __global__ __launch_bounds__(256) void lf_pass_batch(BatchWork w) {
+  __shared__ LfTables tsm;
   batch_loop(w, [&](int f, int j) {
     const LfLaunch &a = batch_table<LfLaunch>(w.rec, f);
     int gp = 0;
     while (gp + 1 < a.nseg && j >= a.blk_start[gp + 1]) gp++;
     const int lb = j - a.blk_start[gp];
-    if (lb < a.blk_n[gp]) lf_block<T, PASS, EX>(a, gp, lb);
+    if (lb < a.blk_n[gp]) lf_block<T, PASS, EX>(a, gp, lb, &tsm);
   });
+}
+
+int log2i(int v) {
+  int l = 0;
+  while ((1 << l) < v) l++;
+  return l;
+}
+
+void fill_tables(const Geom &g, const LfHdr &h, LfTables &t) {
+  memset(&t, 0, sizeof t);
+  for (int tx = 0; tx < 19; tx++) {
+    t.tx_log2[0][tx] = (uint8_t)log2i(AV1_TX_WIDTH[tx]);
+    t.tx_log2[1][tx] = (uint8_t)log2i(AV1_TX_HEIGHT[tx]);
+  }
+  for (int c = 0; c < 2; c++)
+    for (int b = 0; b < 22; b++) {
+      const int ps = AV1_SUBSAMPLED_SIZE[b][c ? g.ssx : 0][c ? g.ssy : 0];
+      t.blk_log2[c][0][b] = ps < 22 ? (uint8_t)log2i(4 * AV1_NUM_4X4_BLOCKS_WIDE[ps]) : 0xff;
+      t.blk_log2[c][1][b] = ps < 22 ? (uint8_t)log2i(4 * AV1_NUM_4X4_BLOCKS_HIGH[ps]) : 0xff;
+    }
+  for (int i = 0; i < 8; i++) {
+    t.ref_deltas[i] = (int8_t)h.ref_deltas[i];
+    t.seg_lf_enabled[i] = (uint8_t)h.seg_lf_enabled[i];
+    for (int j = 0; j < 4; j++) t.seg_lf_data[i][j] = (int8_t)h.seg_lf_data[i][j];
+  }
+  for (int i = 0; i < 2; i++) t.mode_deltas[i] = (int8_t)h.mode_deltas[i];
+  for (int i = 0; i < 4; i++) t.level[i] = (uint8_t)h.level[i];
 }
 
 /* Fills the launch record; blocks of EX x (256 / EX) edges, each plane's
@@ -368,6 +469,7 @@ int plan(const Geom &g, const LfHdr &h, const LfSide &s, const PlanePtr planes[3
   a.g = g;
   a.h = h;
   a.s = s;
+  fill_tables(g, h, a.t);
   int n = 0, total = 0;
   const int ey = 256 / ex;
   for (int p = 0; p < 3; p++) a.planes[p] = planes[p];
