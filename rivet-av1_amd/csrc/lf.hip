@@ -69,94 +69,106 @@ __device__ __forceinline__ int lf_level(const LfHdr &h, const LfTables &t, const
   return lvl;
 }
 
-/* 7.14.6 for one lane: v[k] is the k-th sample across the edge (q0 at 7).
- * Returns the filter applied: 0 none, 1 narrow, 2 wide-8 (6 for chroma),
- * 3 wide-16. Same arithmetic as filter_lanes, lane by lane. */
-__device__ __forceinline__ int filter_lane(int v[14], int limit_bd, int blimit_bd, int thresh_bd,
-                                           int filter_size, bool chroma, int bd) {
+/* 7.14.6 on one line of samples across an edge, for one filter class:
+ *   C = 0: filter size 4 (narrow filter only),      v = p1 p0 | q0 q1
+ *   C = 1: chroma size 8 (6-tap or narrow),           v = p2 .. p0 | q0 .. q2
+ *   C = 2: luma size 8 (8-tap or narrow),             v = p3 .. p0 | q0 .. q3
+ *   C = 3: size 16 (16-wide, 8-tap or narrow),        v = p6 .. p0 | q0 .. q6
+ * Every candidate output is computed and the one the masks select is kept,
+ * so all lanes of a class run the same instructions. Returns whether the
+ * line was filtered (v modified in place). Same arithmetic as
+ * dsp::lf::filter_lanes (src/dsp/lf.rs:58-211). */
+template <int C>
+struct LfClass {
+  static constexpr int N = C == 0 ? 4 : (C == 1 ? 6 : (C == 2 ? 8 : 14));
+};
+
+template <int C>
+__device__ __forceinline__ bool filter_class(int *v, int limit, int blimit, int thresh, int bd) {
+  constexpr int N = LfClass<C>::N, H = N / 2;
+  const int p0 = v[H - 1], p1 = v[H - 2], q0 = v[H], q1 = v[H + 1];
+  bool mask = abs(p0 - q0) * 2 + (abs(p1 - q1) >> 1) > blimit || abs(p1 - p0) > limit || abs(q1 - q0) > limit;
+  int p2 = 0, q2 = 0, p3 = 0, q3 = 0;
+  if (C >= 1) {
+    p2 = v[H - 3];
+    q2 = v[H + 2];
+    mask = mask || abs(p2 - p1) > limit || abs(q2 - q1) > limit;
+  }
+  if (C >= 2) {
+    p3 = v[H - 4];
+    q3 = v[H + 3];
+    mask = mask || abs(p3 - p2) > limit || abs(q3 - q2) > limit;
+  }
+  if (mask) return false;
   const int sh = bd - 8;
-  const int p0 = v[6], p1 = v[5], p2 = v[4], p3 = v[3];
-  const int q0 = v[7], q1 = v[8], q2 = v[9], q3 = v[10];
-  const int filter_len = filter_size == 4 ? 4 : (chroma ? 6 : (filter_size == 8 ? 8 : 16));
-  const bool hev = abs(p1 - p0) > thresh_bd || abs(q1 - q0) > thresh_bd;
-  bool mask = abs(p0 - q0) * 2 + (abs(p1 - q1) >> 1) > blimit_bd || abs(p1 - p0) > limit_bd ||
-              abs(q1 - q0) > limit_bd;
-  if (filter_len >= 6) mask = mask || abs(p2 - p1) > limit_bd || abs(q2 - q1) > limit_bd;
-  if (filter_len >= 8) mask = mask || abs(p3 - p2) > limit_bd || abs(q3 - q2) > limit_bd;
-  if (mask) return 0;
   const int t = 1 << sh;
-  bool flat = false, flat2 = false;
-  if (filter_size >= 8) {
-    bool m = abs(p1 - p0) > t || abs(q1 - q0) > t || abs(p2 - p0) > t || abs(q2 - q0) > t;
-    if (filter_len >= 8) m = m || abs(p3 - p0) > t || abs(q3 - q0) > t;
-    flat = !m;
+  bool flat = false;
+  if (C >= 1) {
+    flat = abs(p1 - p0) <= t && abs(q1 - q0) <= t && abs(p2 - p0) <= t && abs(q2 - q0) <= t;
+    if (C >= 2) flat = flat && abs(p3 - p0) <= t && abs(q3 - q0) <= t;
   }
-  if (filter_size >= 16) {
-    flat2 = !(abs(v[0] - p0) > t || abs(v[13] - q0) > t || abs(v[1] - p0) > t ||
-              abs(v[12] - q0) > t || abs(v[2] - p0) > t || abs(v[11] - q0) > t);
+  /* narrow filter (7.14.6.3) */
+  const bool hev = abs(p1 - p0) > thresh || abs(q1 - q0) > thresh;
+  const int lo = -(1 << (bd - 1)), hi = (1 << (bd - 1)) - 1;
+  const int off = 0x80 << sh;
+  const int ps1 = p1 - off, ps0 = p0 - off, qs0 = q0 - off, qs1 = q1 - off;
+  int fl = hev ? clip3(lo, hi, ps1 - qs1) : 0;
+  fl = clip3(lo, hi, fl + 3 * (qs0 - ps0));
+  const int filter1 = clip3(lo, hi, fl + 4) >> 3;
+  const int filter2 = clip3(lo, hi, fl + 3) >> 3;
+  const int f = (filter1 + 1) >> 1;
+  const int n_q0 = clip3(lo, hi, qs0 - filter1) + off;
+  const int n_p0 = clip3(lo, hi, ps0 + filter2) + off;
+  const int n_q1 = hev ? q1 : clip3(lo, hi, qs1 - f) + off;
+  const int n_p1 = hev ? p1 : clip3(lo, hi, ps1 + f) + off;
+  if (C == 0 || !flat) {
+    v[H - 2] = n_p1;
+    v[H - 1] = n_p0;
+    v[H] = n_q0;
+    v[H + 1] = n_q1;
+    return true;
   }
-  if (filter_size == 4 || !flat) {
-    /* narrow filter (7.14.6.3) */
-    const int lo = -(1 << (bd - 1)), hi = (1 << (bd - 1)) - 1;
-    const int off = 0x80 << sh;
-    const int ps1 = p1 - off, ps0 = p0 - off, qs0 = q0 - off, qs1 = q1 - off;
-    int fl = hev ? clip3(lo, hi, ps1 - qs1) : 0;
-    fl = clip3(lo, hi, fl + 3 * (qs0 - ps0));
-    const int filter1 = clip3(lo, hi, fl + 4) >> 3;
-    const int filter2 = clip3(lo, hi, fl + 3) >> 3;
-    v[7] = clip3(lo, hi, qs0 - filter1) + off;
-    v[6] = clip3(lo, hi, ps0 + filter2) + off;
-    if (!hev) {
-      const int f = (filter1 + 1) >> 1;
-      v[8] = clip3(lo, hi, qs1 - f) + off;
-      v[5] = clip3(lo, hi, ps1 + f) + off;
-    }
-    return 1;
+  if (C == 1) {
+    /* log2 size 3, n = 2, taps |j| <= 1 doubled: 6-tap chroma filter */
+    v[H - 2] = (3 * p2 + 2 * p1 + 2 * p0 + q0 + 4) >> 3;
+    v[H - 1] = (p2 + 2 * p1 + 2 * p0 + 2 * q0 + q1 + 4) >> 3;
+    v[H] = (p1 + 2 * p0 + 2 * q0 + 2 * q1 + q2 + 4) >> 3;
+    v[H + 1] = (p0 + 2 * q0 + 2 * q1 + 3 * q2 + 4) >> 3;
+    return true;
   }
-  if (filter_size == 8 || !flat2) {
-    if (!chroma) {
-      /* log2 size 3, n = 3, centre tap doubled: 8-tap flat filter */
-      const int o4 = (3 * p3 + 2 * p2 + p1 + p0 + q0 + 4) >> 3;
-      const int o5 = (2 * p3 + p2 + 2 * p1 + p0 + q0 + q1 + 4) >> 3;
-      const int o6 = (p3 + p2 + p1 + 2 * p0 + q0 + q1 + q2 + 4) >> 3;
-      const int o7 = (p2 + p1 + p0 + 2 * q0 + q1 + q2 + q3 + 4) >> 3;
-      const int o8 = (p1 + p0 + q0 + 2 * q1 + q2 + 2 * q3 + 4) >> 3;
-      const int o9 = (p0 + q0 + q1 + 2 * q2 + 3 * q3 + 4) >> 3;
-      v[4] = o4;
-      v[5] = o5;
-      v[6] = o6;
-      v[7] = o7;
-      v[8] = o8;
-      v[9] = o9;
-    } else {
-      /* log2 size 3, n = 2, taps |j| <= 1 doubled: 6-tap chroma filter */
-      const int o5 = (3 * p2 + 2 * p1 + 2 * p0 + q0 + 4) >> 3;
-      const int o6 = (p2 + 2 * p1 + 2 * p0 + 2 * q0 + q1 + 4) >> 3;
-      const int o7 = (p1 + 2 * p0 + 2 * q0 + 2 * q1 + q2 + 4) >> 3;
-      const int o8 = (p0 + 2 * q0 + 2 * q1 + 3 * q2 + 4) >> 3;
-      v[5] = o5;
-      v[6] = o6;
-      v[7] = o7;
-      v[8] = o8;
-    }
-    return 2;
+  bool flat2 = false;
+  if (C == 3) {
+    flat2 = abs(v[0] - p0) <= t && abs(v[13] - q0) <= t && abs(v[1] - p0) <= t && abs(v[12] - q0) <= t &&
+            abs(v[2] - p0) <= t && abs(v[11] - q0) <= t;
   }
-  /* log2 size 4, n = 6, taps |j| <= 1 doubled: 16-wide filter; output i
-   * (-6..5) sums v[clamp(i + j, -7, 6)] over j = -6..6. */
-  int o[12];
+  if (C == 2 || !flat2) {
+    /* log2 size 3, n = 3, centre tap doubled: 8-tap flat filter */
+    v[H - 3] = (3 * p3 + 2 * p2 + p1 + p0 + q0 + 4) >> 3;
+    v[H - 2] = (2 * p3 + p2 + 2 * p1 + p0 + q0 + q1 + 4) >> 3;
+    v[H - 1] = (p3 + p2 + p1 + 2 * p0 + q0 + q1 + q2 + 4) >> 3;
+    v[H] = (p2 + p1 + p0 + 2 * q0 + q1 + q2 + q3 + 4) >> 3;
+    v[H + 1] = (p1 + p0 + q0 + 2 * q1 + q2 + 2 * q3 + 4) >> 3;
+    v[H + 2] = (p0 + q0 + q1 + 2 * q2 + 3 * q3 + 4) >> 3;
+    return true;
+  }
+  /* log2 size 4, n = 6, taps |j| <= 1 doubled: 16-wide filter. Output i
+   * (-6..5) sums v[7 + clamp(i + j, -7, 6)] over j = -6..6; the window
+   * sums slide from one output to the next. */
+  int x[14];
+#pragma unroll
+  for (int k = 0; k < 14; k++) x[k] = v[k];
+  int wide = 0, mid = 0;
+#pragma unroll
+  for (int j = -6; j <= 6; j++) wide += x[7 + min(max(-6 + j, -7), 6)];
+#pragma unroll
+  for (int j = -1; j <= 1; j++) mid += x[7 + min(max(-6 + j, -7), 6)];
 #pragma unroll
   for (int i = -6; i < 6; i++) {
-    int acc = 0;
-#pragma unroll
-    for (int j = -6; j <= 6; j++) {
-      const int pp = min(max(i + j, -7), 6);
-      acc += v[7 + pp] * ((j >= -1 && j <= 1) ? 2 : 1);
-    }
-    o[i + 6] = (acc + 8) >> 4;
+    v[7 + i] = (wide + mid + 8) >> 4;
+    wide += x[7 + min(i + 7, 6)] - x[7 + max(i - 6, -7)];
+    mid += x[7 + min(i + 2, 6)] - x[7 + max(i - 1, -7)];
   }
-#pragma unroll
-  for (int i = 0; i < 12; i++) v[1 + i] = o[i];
-  return 3;
+  return true;
 }
 
 template <typename T>
@@ -286,135 +298,128 @@ __device__ __forceinline__ int edge_decision(const LfLaunch &a, const LfTables &
   const int sharp = a.h.sharpness;
   const int shift = sharp > 4 ? 2 : (sharp > 0 ? 1 : 0);
   const int limit = sharp > 0 ? clip3(1, 9 - sharp, lvl >> shift) : max(lvl >> shift, 1);
-  const int sh = g.bd - 8;
-  *limit_bd = limit << sh;
-  *blimit_bd = (2 * (lvl + 2) + limit) << sh;
-  *thresh_bd = (lvl >> 4) << sh;
+  /* unscaled (the filters shift them by bd - 8) */
+  *limit_bd = limit;
+  *blimit_bd = 2 * (lvl + 2) + limit;
+  *thresh_bd = lvl >> 4;
   return filter_size;
 }
 
 constexpr int LF_TWORDS = sizeof(LfTables) / 4;
 
-/* Block lb of grid plane gp: EX x (256 / EX) edges, one per thread. */
+/* LDS of a deblocking workgroup: the tables, and the filtered edges of
+ * the block sorted by filter class (local edge index | limit << 8 |
+ * blimit << 16 | thresh << 24, unscaled). */
+struct LfShared {
+  LfTables t;
+  int count[4];
+  uint32_t list[4][256];
+};
+
+/* One filter class of the block: one line of an edge per item (a row of
+ * 4-sample words in the vertical pass, a column in the horizontal one). */
+template <typename T, int PASS, int EX, int C>
+__device__ __forceinline__ void lf_class(const LfShared &sm, T *base, int64_t stride, int ex0, int ey0, int bd) {
+  constexpr int N = LfClass<C>::N, H = N / 2;
+  const int sh = bd - 8;
+  const int n = sm.count[C];
+  if (PASS == 0) {
+    for (int it = threadIdx.x; it < 4 * n; it += 256) {
+      const uint32_t e = sm.list[C][it >> 2];
+      const int li = e & 255;
+      const int xp = (ex0 + (li % EX)) * 4, yp = (ey0 + (li / EX)) * 4 + (it & 3);
+      T *row = base + (int64_t)yp * stride + xp;
+      int s[16];
+      if (C == 3) {
+        Io<T>::ld4(row - 8, s);
+        Io<T>::ld4(row + 4, s + 12);
+      }
+      Io<T>::ld4(row - 4, s + 4);
+      Io<T>::ld4(row, s + 8);
+      int v[N];
+#pragma unroll
+      for (int k = 0; k < N; k++) v[k] = s[8 - H + k];
+      if (!filter_class<C>(v, ((e >> 8) & 255) << sh, ((e >> 16) & 255) << sh, (e >> 24) << sh, bd)) continue;
+#pragma unroll
+      for (int k = 0; k < N; k++) s[8 - H + k] = v[k];
+      if (C <= 1) {
+        Io<T>::st2(row - 2, s[6], s[7]);
+        Io<T>::st2(row, s[8], s[9]);
+      } else {
+        if (C == 3) {
+          Io<T>::st4(row - 8, s);
+          Io<T>::st4(row + 4, s + 12);
+        }
+        Io<T>::st4(row - 4, s + 4);
+        Io<T>::st4(row, s + 8);
+      }
+    }
+  } else {
+    /* one column of an edge per item: the column's samples are 2-byte
+     * loads (the rows were just written by the vertical pass and sit in
+     * the caches), every lane filters one line as in the vertical pass */
+    for (int it = threadIdx.x; it < 4 * n; it += 256) {
+      const uint32_t e = sm.list[C][it >> 2];
+      const int li = e & 255;
+      const int xp = (ex0 + (li % EX)) * 4 + (it & 3), yp = (ey0 + (li / EX)) * 4;
+      T *col = base + (int64_t)yp * stride + xp;
+      int v[N];
+#pragma unroll
+      for (int k = 0; k < N; k++) v[k] = col[(int64_t)(k - H) * stride];
+      if (!filter_class<C>(v, ((e >> 8) & 255) << sh, ((e >> 16) & 255) << sh, (e >> 24) << sh, bd)) continue;
+      /* rows the filters modify: p1..q1, p2..q2 (8-tap), p5..q5 (16) */
+      constexpr int R = C <= 1 ? 2 : (C == 2 ? 3 : 6);
+#pragma unroll
+      for (int k = H - R; k < H + R; k++) col[(int64_t)(k - H) * stride] = (T)v[k];
+    }
+  }
+}
+
+/* Block lb of grid plane gp: EX x (256 / EX) edges. Each thread decides
+ * one edge (7.14.2-7.14.5); the filtered edges are gathered into one list
+ * per filter class, and each list is then filtered with every lane on the
+ * same code. */
 template <typename T, int PASS, int EX>
-__device__ __forceinline__ void lf_block(const LfLaunch &a, int gp, int lb, LfTables *tsm) {
+__device__ __forceinline__ void lf_block(const LfLaunch &a, int gp, int lb, LfShared *sm) {
   const int plane = a.plane_id[gp];
-  const int ex = (lb % a.blk_x[gp]) * EX + (threadIdx.x % EX);
-  const int ey = (lb / a.blk_x[gp]) * (256 / EX) + (threadIdx.x / EX);
+  const int ex0 = (lb % a.blk_x[gp]) * EX, ey0 = (lb / a.blk_x[gp]) * (256 / EX);
+  const int ex = ex0 + (threadIdx.x % EX), ey = ey0 + (threadIdx.x / EX);
   const Geom &g = a.g;
   const int ssx = plane ? g.ssx : 0, ssy = plane ? g.ssy : 0;
   const int xp = ex * 4, yp = ey * 4;
   /* inside the mode-info area; the edge is not filtered at the frame's
    * left / top border */
   const bool valid = (ey << ssy) < g.mi_rows && (ex << ssx) < g.mi_cols && (PASS == 0 ? xp != 0 : yp != 0);
-  const bool chroma = plane != 0;
+  const uint32_t tword = threadIdx.x < LF_TWORDS ? reinterpret_cast<const uint32_t *>(&a.t)[threadIdx.x] : 0u;
+  EdgeSide side;
+  if (valid) side = load_side(a, plane, PASS, ex, ey);
+  if (threadIdx.x < LF_TWORDS) reinterpret_cast<uint32_t *>(&sm->t)[threadIdx.x] = tword;
+  if (threadIdx.x < 4) sm->count[threadIdx.x] = 0;
+  __syncthreads();
+  if (valid) {
+    int limit, blimit, thresh;
+    const int fs = edge_decision(a, sm->t, side, plane, PASS, ex, ey, &limit, &blimit, &thresh);
+    if (fs) {
+      const int c = fs == 4 ? 0 : (fs == 16 ? 3 : (plane ? 1 : 2));
+      const int slot = atomicAdd(&sm->count[c], 1);
+      sm->list[c][slot] = threadIdx.x | (limit << 8) | (blimit << 16) | (thresh << 24);
+    }
+  }
+  __syncthreads();
   T *base = gptr(reinterpret_cast<T *>(a.planes[plane].data));
   const int64_t stride = a.planes[plane].stride;
-  /* Every load of the edge is issued at once, before the workgroup stages
-   * its tables: the samples (raw words, unpacked after the barrier), the
-   * mode info and the transform sizes; edges that turn out unfiltered
-   * only cost reads of a plane that the pass reads anyway. */
-  const uint32_t tword = threadIdx.x < LF_TWORDS ? reinterpret_cast<const uint32_t *>(&a.t)[threadIdx.x] : 0u;
-  typename Io<T>::W raw[16];
-  EdgeSide side;
-  if (!valid) {
-  } else if (PASS == 0) {
-    /* vertical edge at column xp: each lane is a row. The outer quads
-     * (x - 8, x + 4) are only used by 16-wide edges, which have x >= 8 */
-    T *row0 = base + (int64_t)yp * stride + xp;
-    const int lo8 = xp >= 8 ? -8 : -4;
-#pragma unroll
-    for (int l = 0; l < 4; l++) {
-      T *row = row0 + (int64_t)l * stride;
-      raw[4 * l] = Io<T>::ldw(row + lo8);
-      raw[4 * l + 1] = Io<T>::ldw(row - 4);
-      raw[4 * l + 2] = Io<T>::ldw(row);
-      raw[4 * l + 3] = Io<T>::ldw(row + 4);
-    }
-    side = load_side(a, plane, PASS, ex, ey);
+  lf_class<T, PASS, EX, 0>(*sm, base, stride, ex0, ey0, g.bd);
+  if (plane) {
+    lf_class<T, PASS, EX, 1>(*sm, base, stride, ex0, ey0, g.bd);
   } else {
-    /* horizontal edge at row yp: each lane is a column. Rows beyond +-4
-     * are only used by 16-wide edges, which have y >= 8 (rows above 0
-     * repeat row yp - 4 otherwise) */
-    const T *col = base + (int64_t)yp * stride + xp;
-#pragma unroll
-    for (int k = 0; k < 14; k++) {
-      const int d = k - 7;
-      const int dd = (d < -4 && yp < 8) ? -4 : d;
-      raw[k] = Io<T>::ldw(col + (int64_t)dd * stride);
-    }
-    side = load_side(a, plane, PASS, ex, ey);
-  }
-  if (threadIdx.x < LF_TWORDS) reinterpret_cast<uint32_t *>(tsm)[threadIdx.x] = tword;
-  __syncthreads();
-  const LfTables &t = *tsm;
-  if (!valid) return;
-  int s[4][16];
-  int v[4][14];
-  if (PASS == 0) {
-#pragma unroll
-    for (int l = 0; l < 4; l++)
-#pragma unroll
-      for (int q = 0; q < 4; q++) Io<T>::unpack(raw[4 * l + q], s[l] + 4 * q);
-  } else {
-#pragma unroll
-    for (int k = 0; k < 14; k++) {
-      int q[4];
-      Io<T>::unpack(raw[k], q);
-#pragma unroll
-      for (int l = 0; l < 4; l++) v[l][k] = q[l];
-    }
-  }
-  int limit_bd, blimit_bd, thresh_bd;
-  const int fs = edge_decision(a, t, side, plane, PASS, ex, ey, &limit_bd, &blimit_bd, &thresh_bd);
-  if (fs == 0) return;
-  if (PASS == 0) {
-    T *row0 = base + (int64_t)yp * stride + xp;
-#pragma unroll
-    for (int l = 0; l < 4; l++) {
-      T *row = row0 + (int64_t)l * stride;
-      int vr[14];
-#pragma unroll
-      for (int k = 0; k < 14; k++) vr[k] = s[l][k + 1]; /* vr[k] = sample at xp - 7 + k */
-      const int r = filter_lane(vr, limit_bd, blimit_bd, thresh_bd, fs, chroma, g.bd);
-      if (r == 0) continue;
-#pragma unroll
-      for (int k = 0; k < 14; k++) s[l][k + 1] = vr[k];
-      if (fs == 4) {
-        Io<T>::st2(row - 2, s[l][6], s[l][7]);
-        Io<T>::st2(row, s[l][8], s[l][9]);
-      } else if (fs == 8) {
-        Io<T>::st4(row - 4, s[l] + 4);
-        Io<T>::st4(row, s[l] + 8);
-      } else {
-        Io<T>::st4(row - 8, s[l]);
-        Io<T>::st4(row - 4, s[l] + 4);
-        Io<T>::st4(row, s[l] + 8);
-        Io<T>::st4(row + 4, s[l] + 12);
-      }
-    }
-  } else {
-    T *col = base + (int64_t)yp * stride + xp;
-    int any = 0;
-#pragma unroll
-    for (int l = 0; l < 4; l++) any |= filter_lane(v[l], limit_bd, blimit_bd, thresh_bd, fs, chroma, g.bd);
-    if (!any) return;
-    const int lo = fs == 4 ? -2 : (fs == 8 ? -4 : -7);
-    const int hi = fs == 4 ? 2 : (fs == 8 ? 4 : 7);
-#pragma unroll
-    for (int k = 0; k < 14; k++) {
-      const int d = k - 7;
-      if (d >= lo && d < hi) {
-        int q[4] = {v[0][k], v[1][k], v[2][k], v[3][k]};
-        Io<T>::st4(col + (int64_t)d * stride, q);
-      }
-    }
+    lf_class<T, PASS, EX, 2>(*sm, base, stride, ex0, ey0, g.bd);
+    lf_class<T, PASS, EX, 3>(*sm, base, stride, ex0, ey0, g.bd);
   }
 }
 
 template <typename T, int PASS>
 __global__ __launch_bounds__(256) void lf_pass_kernel(LfLaunch a) {
-  __shared__ LfTables tsm;
+  __shared__ LfShared tsm;
   int lb;
   const int gp = xcd_segment(blockIdx.x, a.blk_start, a.nseg, &lb);
   if (gp >= a.nseg || lb >= a.blk_n[gp]) return; /* padding */
@@ -424,7 +429,7 @@ __global__ __launch_bounds__(256) void lf_pass_kernel(LfLaunch a) {
 /* Batched: item j of a frame is block j of its (unpadded) grid. */
 template <typename T, int PASS, int EX>
 __global__ __launch_bounds__(256) void lf_pass_batch(BatchWork w) {
-  __shared__ LfTables tsm;
+  __shared__ LfShared tsm;
   batch_loop(w, [&](int f, int j) {
     const LfLaunch &a = batch_table<LfLaunch>(w.rec, f);
     int gp = 0;
