@@ -142,7 +142,7 @@ int plan_frame(const rvpf_batch *b, int k, const FrameIn &x, void *rec) {
     }
     case K_LR:
       if (!lr_on(p)) return 0;
-      return lr_plan(g, make_lr(p, &x.s), rec_, x.cdef, ref, rec);
+      return lr_plan(g, make_lr(p, &x.s), rec_, x.cdef, ref, x.recon.bytes_per_sample, rec);
     case K_APPLY:
       if (!x.shown || !grain_on(p)) return 0;
       return grain_apply_plan(make_grain(p), x.gs, ref, disp, x.recon.bytes_per_sample, rec);

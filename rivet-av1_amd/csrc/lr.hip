@@ -36,7 +36,10 @@ namespace {
 
 constexpr int NT = 256;        /* threads per workgroup */
 constexpr int MAXW = 64;       /* max slice width */
-constexpr int WS = MAXW + 8;   /* window row stride (samples) */
+constexpr int WS = MAXW + 16;  /* window row stride (samples) */
+/* Window column c (sample x0 - 3 + c) sits at LDS column c + WOFF, so the
+ * 8-sample chunk starting at x0 - 8 + 8k lands on an aligned column 8k. */
+constexpr int WOFF = 5;
 constexpr int WR = 64 + 6;     /* window rows: stripe + 3 above and below */
 constexpr int ABW = MAXW + 2;  /* A/B row stride: positions -1..w */
 constexpr int ABR = 64 + 2;    /* A/B rows: -1..h */
@@ -50,6 +53,7 @@ struct LrLaunch {
   int nseg;
   int nslices[3];
   int nstripes;
+  int wide[3]; /* rows of the plane's deblocked, CDEF and output planes are 8-sample aligned */
 };
 
 template <typename T>
@@ -139,6 +143,50 @@ __device__ __forceinline__ void copy_column(T *out, int64_t os, const T *in, int
   }
 }
 
+/* 8 consecutive samples as one 16-byte (u16) or 8-byte (u8) access. */
+template <typename T>
+struct Chunk;
+template <>
+struct Chunk<uint16_t> {
+  typedef uint4 V;
+  static __device__ __forceinline__ uint4 to_u16(V v) { return v; }
+};
+template <>
+struct Chunk<uint8_t> {
+  typedef uint2 V;
+  static __device__ __forceinline__ uint4 to_u16(V v) {
+    const uint32_t a = v.x, b = v.y;
+    return make_uint4((a & 0xff) | ((a << 8) & 0xff0000), ((a >> 16) & 0xff) | ((a >> 8) & 0xff0000),
+                      (b & 0xff) | ((b << 8) & 0xff0000), ((b >> 16) & 0xff) | ((b >> 8) & 0xff0000));
+  }
+};
+
+/* out = in over rows [ya, yb] x columns [x0, x0 + wc) in 8-sample chunks
+ * (rows 8-sample aligned); every load of a thread before its stores. The
+ * columns past the last whole chunk are copied sample by sample. */
+template <typename T>
+__device__ __forceinline__ void copy_rect_wide(T *out, int64_t os, const T *in, int64_t is, int x0, int wc, int ya,
+                                               int yb) {
+  typedef typename Chunk<T>::V V;
+  const int nch = wc >> 3, rows = yb - ya + 1, n = nch * rows;
+  /* three chunks per thread and step, loaded before they are stored */
+  auto chunk = [&](int it, int64_t st_) { const int r = it / nch; return (int64_t)(ya + r) * st_ + x0 + (it - r * nch) * 8; };
+  for (int i0 = threadIdx.x; i0 < n; i0 += 3 * NT) {
+    const int i1 = i0 + NT, i2 = i0 + 2 * NT;
+    const V v0 = *reinterpret_cast<const V *>(in + chunk(i0, is));
+    const V v1 = *reinterpret_cast<const V *>(in + chunk(min(i1, n - 1), is));
+    const V v2 = *reinterpret_cast<const V *>(in + chunk(min(i2, n - 1), is));
+    *reinterpret_cast<V *>(out + chunk(i0, os)) = v0;
+    if (i1 < n) *reinterpret_cast<V *>(out + chunk(i1, os)) = v1;
+    if (i2 < n) *reinterpret_cast<V *>(out + chunk(i2, os)) = v2;
+  }
+  const int rem = wc - nch * 8;
+  for (int it = threadIdx.x; it < rem * rows; it += NT) {
+    const int r = it / rem, c = x0 + nch * 8 + (it - r * rem);
+    out[(int64_t)(ya + r) * os + c] = in[(int64_t)(ya + r) * is + c];
+  }
+}
+
 /* Tile (plane, stripe st, slice sl) with slice width SW (64, or 32 for
  * 32-sample units). */
 template <typename T, int SW>
@@ -180,6 +228,10 @@ __device__ __forceinline__ void lr_tile(const LrLaunch &a, int plane, int st, in
 
   /* copy what is not restored: everything (no filter), else the rows
    * below the frame and the columns right of it inside the mode-info area */
+  if (type == 0 && a.wide[plane]) {
+    copy_rect_wide<T>(out, os, cdef, cs_, x0, x1c - x0 + 1, y0, y1c);
+    return;
+  }
   {
     const int x = x0 + col;
     const int ya = type ? y1 + 1 : y0;
@@ -194,32 +246,59 @@ __device__ __forceinline__ void lr_tile(const LrLaunch &a, int plane, int st, in
   {
     const T *deb = cpp<T>(a.deb[plane]);
     const int64_t ds = a.deb[plane].stride;
-    const int xa = min(max(x0 + col - 3, 0), pex);
-    const int xb = min(max(x0 + col + SW - 3, 0), pex);
-    const bool second = col + SW < ww;
-    /* LOADS rows per batch: every load is issued (rows past the window
-     * repeat its last row) before the LDS stores */
-    constexpr int LOADS = 6;
-    for (int r0 = band; r0 < wh; r0 += LOADS * NB) {
-      uint16_t va[LOADS], vb[LOADS];
+    /* source row of window row r (rows outside the stripe: the deblocked
+     * frame, at most 2 out) */
+    auto src_row = [&](int r) -> const T * {
+      const int yc = min(max(y0 + r - 3, 0), pey);
+      const bool above = yc < stripe_start, below = yc > stripe_end;
+      const int y = above ? max(stripe_start - 2, yc) : (below ? min(stripe_end + 2, yc) : yc);
+      return (above || below) ? deb + (int64_t)y * ds : cdef + (int64_t)y * cs_;
+    };
+    constexpr int NCH = (SW + 16) / 8; /* chunks from x0 - 8 to x0 + SW + 8 */
+    if (a.wide[plane] && x0 >= 8 && x0 + SW + 8 <= pex + 1) {
+      /* interior tile: aligned 8-sample chunks, every load of a thread
+       * before its LDS stores */
+      typedef typename Chunk<T>::V V;
+      const int n = NCH * wh;
+      constexpr int PER = (NCH * WR + NT - 1) / NT;
+      V v[PER];
 #pragma unroll
-      for (int k = 0; k < LOADS; k++) {
-        const int r = min(r0 + k * NB, wh - 1);
-        const int yc = min(max(y0 + r - 3, 0), pey);
-        /* rows outside the stripe: the deblocked frame, at most 2 out */
-        const bool above = yc < stripe_start, below = yc > stripe_end;
-        const int y = above ? max(stripe_start - 2, yc) : (below ? min(stripe_end + 2, yc) : yc);
-        const T *src = (above || below) ? deb + (int64_t)y * ds : cdef + (int64_t)y * cs_;
-        va[k] = src[xa];
-        vb[k] = src[xb];
+      for (int k = 0; k < PER; k++) {
+        const int it = min((int)threadIdx.x + k * NT, n - 1);
+        const int r = it / NCH, c = it - r * NCH;
+        v[k] = *reinterpret_cast<const V *>(src_row(r) + x0 - 8 + 8 * c);
       }
 #pragma unroll
-      for (int k = 0; k < LOADS; k++) {
-        const int r = r0 + k * NB;
-        if (r < wh) {
-          uint16_t *wr = &sm.win[r * WS + col];
-          wr[0] = va[k];
-          if (second) wr[SW] = vb[k];
+      for (int k = 0; k < PER; k++) {
+        const int it = threadIdx.x + k * NT;
+        if (it < n) {
+          const int r = it / NCH, c = it - r * NCH;
+          *reinterpret_cast<uint4 *>(&sm.win[r * WS + 8 * c]) = Chunk<T>::to_u16(v[k]);
+        }
+      }
+    } else {
+      const int xa = min(max(x0 + col - 3, 0), pex);
+      const int xb = min(max(x0 + col + SW - 3, 0), pex);
+      const bool second = col + SW < ww;
+      /* LOADS rows per batch: every load is issued (rows past the window
+       * repeat its last row) before the LDS stores */
+      constexpr int LOADS = 6;
+      for (int r0 = band; r0 < wh; r0 += LOADS * NB) {
+        uint16_t va[LOADS], vb[LOADS];
+#pragma unroll
+        for (int k = 0; k < LOADS; k++) {
+          const T *src = src_row(min(r0 + k * NB, wh - 1));
+          va[k] = src[xa];
+          vb[k] = src[xb];
+        }
+#pragma unroll
+        for (int k = 0; k < LOADS; k++) {
+          const int r = r0 + k * NB;
+          if (r < wh) {
+            uint16_t *wr = &sm.win[r * WS + WOFF + col];
+            wr[0] = va[k];
+            if (second) wr[SW] = vb[k];
+          }
         }
       }
     }
@@ -239,7 +318,7 @@ __device__ __forceinline__ void lr_tile(const LrLaunch &a, int plane, int st, in
     const int limit = (1 << (bd + 1 + 7 - round0)) - 1;
     if (col < w)
       for (int r = band; r < wh; r += NB) {
-        const uint16_t *s = &sm.win[r * WS + col];
+        const uint16_t *s = &sm.win[r * WS + WOFF + col];
         const int sum = __mul24(h0, s[0] + s[6]) + __mul24(h1, s[1] + s[5]) + __mul24(h2, s[2] + s[4]) +
                         __mul24(h3, (int)s[3]);
         sm.hbuf[r * MAXW + col] = (int16_t)min(max(round2(sum, round0), -offset), limit - offset);
@@ -287,7 +366,7 @@ __device__ __forceinline__ void lr_tile(const LrLaunch &a, int plane, int st, in
     if (j <= w) {
       uint32_t h5a = 0, h5b = 0, h5c = 0, h5d = 0, q5a = 0, q5b = 0, q5c = 0, q5d = 0;
       uint32_t h3a = 0, h3b = 0, q3a = 0, q3b = 0;
-      const uint16_t *wp = &sm.win[(lo + 1) * WS + j + 1];
+      const uint16_t *wp = &sm.win[(lo + 1) * WS + WOFF + j + 1];
       for (int R = lo + 1; R <= hi + 4; R++, wp += WS) {
         const uint32_t w1 = wp[0], w2 = wp[1], w3 = wp[2], w4 = wp[3], w5 = wp[4];
         const uint32_t h3n = w2 + w3 + w4, h5n = h3n + w1 + w5;
@@ -319,7 +398,7 @@ __device__ __forceinline__ void lr_tile(const LrLaunch &a, int plane, int st, in
       uint32_t s5 = 0, sq5 = 0, s3 = 0, sq3 = 0;
       for (int dy = 0; dy < 5; dy++)
         for (int dx = 0; dx < 5; dx++) {
-          const uint32_t v = sm.win[(i + 1 + dy) * WS + jj + 1 + dx];
+          const uint32_t v = sm.win[(i + 1 + dy) * WS + WOFF + jj + 1 + dx];
           s5 += v;
           sq5 += __umul24(v, v);
           if (dy >= 1 && dy <= 3 && dx >= 1 && dx <= 3) {
@@ -367,7 +446,7 @@ __device__ __forceinline__ void lr_tile(const LrLaunch &a, int plane, int st, in
     }
     T *o = out + (int64_t)(y0 + lo) * os + x0 + j;
     for (int i = lo; i < hi; i++, o += os) {
-      const int uu = sm.win[(i + 3) * WS + j + 3];
+      const int uu = sm.win[(i + 3) * WS + WOFF + j + 3];
       const int ue = uu << 4;
       int f0 = ue, f1 = ue;
       if (rad0) {
@@ -439,14 +518,20 @@ __global__ __launch_bounds__(NT) void lr_batch(BatchWork w) {
   });
 }
 
+bool rows_aligned8(const PlanePtr &q, int bps) {
+  return q.data && ((uintptr_t)q.data % (8 * (uintptr_t)bps)) == 0 && q.stride % 8 == 0;
+}
+
 int plan(const Geom &g, const LrHdr &h, const PlanePtr deblocked[3], const PlanePtr cdef[3],
-         const PlanePtr out[3], bool pad8, LrLaunch &a) {
+         const PlanePtr out[3], int bps, bool pad8, LrLaunch &a) {
   a.g = g;
   a.h = h;
   for (int p = 0; p < 3; p++) {
     a.deb[p] = deblocked[p];
     a.cdef[p] = cdef[p];
     a.out[p] = out[p];
+    a.wide[p] = p < g.nplanes && rows_aligned8(deblocked[p], bps) && rows_aligned8(cdef[p], bps) &&
+                rows_aligned8(out[p], bps);
   }
   a.nstripes = (g.height + 8 + 63) / 64;
   int n = 0, total = 0;
@@ -476,8 +561,8 @@ int plan(const Geom &g, const LrHdr &h, const PlanePtr deblocked[3], const Plane
 size_t lr_record_size() { return sizeof(LrLaunch); }
 
 int lr_plan(const Geom &g, const LrHdr &h, const PlanePtr deblocked[3], const PlanePtr cdef[3],
-            const PlanePtr out[3], void *rec) {
-  return plan(g, h, deblocked, cdef, out, false, *static_cast<LrLaunch *>(rec));
+            const PlanePtr out[3], int bps, void *rec) {
+  return plan(g, h, deblocked, cdef, out, bps, false, *static_cast<LrLaunch *>(rec));
 }
 
 int launch_lr_batch(const BatchWork &w, const StagePlan &sp, int bps, hipStream_t st) {
@@ -492,7 +577,7 @@ int launch_lr_batch(const BatchWork &w, const StagePlan &sp, int bps, hipStream_
 int launch_lr(const Geom &g, const LrHdr &h, const PlanePtr deblocked[3], const PlanePtr cdef[3],
               PlanePtr out[3], int bps, hipStream_t st) {
   LrLaunch a;
-  const int total = plan(g, h, deblocked, cdef, out, true, a);
+  const int total = plan(g, h, deblocked, cdef, out, bps, true, a);
   if (total == 0) return 0;
   if (bps == 1)
     lr_kernel<uint8_t><<<total, NT, 0, st>>>(a);

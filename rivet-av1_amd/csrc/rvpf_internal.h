@@ -221,7 +221,7 @@ int launch_cdef_batch(const BatchWork &w, const StagePlan &sp, int bps, hipStrea
 
 size_t lr_record_size();
 int lr_plan(const Geom &g, const LrHdr &h, const PlanePtr deblocked[3], const PlanePtr cdef[3],
-            const PlanePtr out[3], void *rec);
+            const PlanePtr out[3], int bps, void *rec);
 int launch_lr_batch(const BatchWork &w, const StagePlan &sp, int bps, hipStream_t st);
 
 size_t copy_record_size();
