@@ -19,13 +19,12 @@
  * Unfiltered blocks (cdef_idx -1, all four 4x4 skipped, CDEF off) are
  * copied, so `out` receives the whole mode-info area.
  *
- * Unavailable taps: NA_S = 0x7000 exceeds every sample by more than 24576,
- * so constrain() of it is 0 for every legal strength/damping (the
- * magnitude test |d| >> adj > strength holds for adj <= 10), it never wins
- * min(), and (v & 0xfff) maps it to 0 for max(); the reference skips it.
- * The clamp to [min, max] only matters when both strengths are non-zero:
- * with one kind of tap the total weight is 12/16, so the rounded update
- * never leaves [min, max]; the kernel clamps only then.
+ * Unavailable taps: NA_S = 0xA000, i.e. -24576 as a signed and 40960 as an
+ * unsigned 16-bit value. Its difference to any sample (0..4095) lies in
+ * [-28671, -24576] without wrapping, so constrain() of it is 0 for every
+ * legal strength/damping (the magnitude test |d| >> adj > strength holds
+ * for adj <= 10); a signed max() and an unsigned min() never pick it. The
+ * reference skips such taps, which therefore gives the same result.
  */
 #include "av1_tables.h"
 #include "rvpf_internal.h"
@@ -34,7 +33,7 @@ namespace rvpf {
 
 namespace {
 
-constexpr int NA_S = 0x7000;
+constexpr int NA_S = 0xA000;
 constexpr int TS = 68;                /* tile stride in samples: 64 + 2 * 2 */
 constexpr int SOFF = 68 * (TS / 2);   /* word offset of the shifted tile S */
 
@@ -106,7 +105,7 @@ __device__ __forceinline__ int mul16(int a, int b) { return (int)(int16_t)a * (i
 
 /* Two horizontally adjacent samples filtered at once in packed 16-bit
  * arithmetic (dsp::cdef::filter_scalar, src/dsp/cdef.rs:61-88). Every
- * quantity fits in int16: samples <= 4095, the sentinel 0x7000, their
+ * quantity fits in int16: samples <= 4095, the sentinel 0xA000, their
  * differences, constrain() results <= 240 and |sum| <= 24 * 240. */
 typedef short s16x2 __attribute__((ext_vector_type(2)));
 typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
@@ -189,15 +188,16 @@ __device__ __forceinline__ uint32_t filter_pair(const uint32_t *w, int base, con
 #pragma unroll
     for (int t = 0; t < 8; t++) sw[t] = w[base + k.so[t]];
   }
-  s16x2 sum = splat(0), mn = x, mx = x;
+  s16x2 sum = splat(0), mx = x;
+  u16x2 mn = __builtin_bit_cast(u16x2, x);
   if (PRI) {
     s16x2 c[4];
 #pragma unroll
     for (int t = 0; t < 4; t++) c[t] = constrain2(as_s2(pw[t]) - x, k.pri, k.pri_adj);
 #pragma unroll
     for (int t = 0; t < 4; t++) {
-      mn = __builtin_elementwise_min(mn, as_s2(pw[t]));
-      mx = __builtin_elementwise_max(mx, as_s2(pw[t] & 0x0fff0fffu));
+      mn = __builtin_elementwise_min(mn, __builtin_bit_cast(u16x2, pw[t]));
+      mx = __builtin_elementwise_max(mx, as_s2(pw[t]));
     }
     sum = k.pt0 * (c[0] + c[1]) + k.pt1 * (c[2] + c[3]);
   }
@@ -208,15 +208,15 @@ __device__ __forceinline__ uint32_t filter_pair(const uint32_t *w, int base, con
     for (int t = 0; t < 8; t++) c[t] = constrain2(as_s2(sw[t]) - x, k.sec, k.sec_adj);
 #pragma unroll
     for (int t = 0; t < 8; t++) {
-      mn = __builtin_elementwise_min(mn, as_s2(sw[t]));
-      mx = __builtin_elementwise_max(mx, as_s2(sw[t] & 0x0fff0fffu));
+      mn = __builtin_elementwise_min(mn, __builtin_bit_cast(u16x2, sw[t]));
+      mx = __builtin_elementwise_max(mx, as_s2(sw[t]));
     }
     const s16x2 s0 = (c[0] + c[1]) + (c[2] + c[3]), s1 = (c[4] + c[5]) + (c[6] + c[7]);
     sum += s0 + s0 + s1;
   }
-  /* x + ((8 + sum - (sum < 0)) >> 4) */
+  /* x + ((8 + sum - (sum < 0)) >> 4), clamped to [min, max] of the taps */
   s16x2 v = x + ((sum + splat(8) + (sum >> splat(15))) >> splat(4));
-  v = __builtin_elementwise_min(__builtin_elementwise_max(v, mn), mx);
+  v = __builtin_elementwise_min(__builtin_elementwise_max(v, __builtin_bit_cast(s16x2, mn)), mx);
   return __builtin_bit_cast(uint32_t, v);
 }
 
