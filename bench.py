@@ -47,7 +47,7 @@ def make_frames(n, w, h, bd, seed):
 # from the calibrated FETCH_SIZE / WRITE_SIZE counter passes summarised here
 PMC_SUMMARY = "profiles/chain_4k10b_pmc.md"
 STAGE_KERNELS = {"deblock": ["lf_pass_kernel<u16,V>", "lf_pass_kernel<u16,H>"], "cdef": ["cdef_kernel<u16>"],
-                 "lr": ["lr_kernel<u16>"], "grain_apply": ["grain_apply_kernel<u16,u16>"]}
+                 "lr": ["lr_kernel<u16>"], "grain_apply": ["grain_apply_kernel<u16>"]}
 
 
 def measured_traffic(stage):
