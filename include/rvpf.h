@@ -26,7 +26,10 @@
  * the outputs equal rivet-av1's CPU filters sample for sample over
  *   - deblocking: the whole plane buffer;
  *   - CDEF / LR / chain reference planes: the mode-info-aligned area
- *     ((mi_cols*4) >> ssx) x ((mi_rows*4) >> ssy) of each plane;
+ *     ((mi_cols*4) >> ssx) x ((mi_rows*4) >> ssy) of each plane; with
+ *     super-resolution, the LR and chain reference planes over the
+ *     upscaled_width x frame_height crop;
+ *   - super-resolution upscaling: the upscaled_width x frame_height crop;
  *   - display planes: the cropped upscaled_width x frame_height picture.
  */
 #ifndef RVPF_H
@@ -42,7 +45,7 @@ extern "C" {
 /* Status codes (the reference's Error kinds, src/lib.rs:60-73). */
 #define RVPF_OK 0
 #define RVPF_E_INVALID (-1)     /* geometry, strides or parameters out of range */
-#define RVPF_E_UNSUPPORTED (-2) /* e.g. use_superres, bytes_per_sample 1 above 8 bits */
+#define RVPF_E_UNSUPPORTED (-2) /* e.g. bytes_per_sample 1 above 8 bits, super-resolution frames in a batch */
 #define RVPF_E_DEVICE (-3)      /* a HIP runtime error */
 #define RVPF_E_NOMEM (-4)       /* device allocation failed */
 
@@ -118,7 +121,8 @@ typedef struct rvpf_frame_params {
   int32_t film_grain_params_present;
   /* frame geometry */
   int32_t frame_width, frame_height;
-  int32_t upscaled_width; /* must equal frame_width (super-res unsupported) */
+  int32_t upscaled_width; /* frame_width, or with use_superres the width frame_width is upscaled to:
+                           * frame_width = (upscaled_width * 8 + d / 2) / d for a denominator d in 9..16 */
   int32_t use_superres;
   int32_t mi_rows, mi_cols;
   int32_t coded_lossless, allow_intrabc;
@@ -197,7 +201,8 @@ const char *rvpf_ctx_last_error(const rvpf_ctx *ctx);
 #define RVPF_STAGE_GRAIN_PREPARE 3
 #define RVPF_STAGE_GRAIN_APPLY 4
 #define RVPF_STAGE_COPY 5
-#define RVPF_STAGE_COUNT 6
+#define RVPF_STAGE_UPSCALE 6
+#define RVPF_STAGE_COUNT 7
 typedef struct rvpf_profile {
   float ms[RVPF_STAGE_COUNT];       /* summed kernel time per stage */
   int32_t launches[RVPF_STAGE_COUNT]; /* stage executions measured */
@@ -229,11 +234,27 @@ int rvpf_cdef(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_side_info *s
               const rvpf_planes *deblocked, rvpf_planes *out, void *stream);
 
 /*
+ * Super-resolution upscaling (7.16) of `in` (frame_width wide, the
+ * mode-info area readable) into `out` (upscaled_width wide): rows
+ * 0 .. frame_height - 1 and columns 0 .. upscaled_width - 1 of each plane
+ * (subsampled for chroma) are written, nothing else (replaces upscale,
+ * src/decoder/postfilter.rs:521-566, called on the CDEF output and on the
+ * deblocked frame by decode_frame_wrapup, src/decoder/mod.rs:752-757).
+ * Requires use_superres.
+ */
+int rvpf_upscale(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_planes *in, rvpf_planes *out,
+                 void *stream);
+
+/*
  * Loop restoration of the CDEF output into `out`, reading the deblocked
  * frame above and below each 64-row stripe (replaces loop_restoration,
  * src/decoder/postfilter.rs:608-636, with lr_stripe :640-661,
  * restore_stripe :677-716, restore_rect_from_window :781-853 and
  * box_filter_rect :901-991). Samples outside restored units are copied.
+ * With use_superres the three frames are the upscaled ones (as the
+ * reference's, from rvpf_upscale): their width is upscaled_width rounded
+ * up to 8 (subsampled for chroma), and only the upscaled_width x
+ * frame_height crop of `out` is defined.
  */
 int rvpf_lr(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_side_info *s,
             const rvpf_planes *deblocked, const rvpf_planes *cdef, rvpf_planes *out,
@@ -250,8 +271,10 @@ int rvpf_grain(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_planes *ref
 
 /*
  * The whole chain of Decoder::decode_frame_wrapup (src/decoder/mod.rs:747-816)
- * and Decoder::output: deblock in place on `recon`, CDEF, loop
- * restoration into `ref_out` (the reference frame), then crop + grain into
+ * and Decoder::output: deblock in place on `recon`, CDEF, super-resolution
+ * upscaling when use_superres, loop restoration into `ref_out` (the
+ * reference frame; upscaled with super-resolution, when only its
+ * upscaled_width x frame_height crop is defined), then crop + grain into
  * `display` (may be NULL when the frame is not shown).
  */
 int rvpf_postfilter_frame(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_side_info *s,

@@ -44,6 +44,7 @@ def lib():
         L.orc_grain_apply.argtypes = [P(abi.RvpfFrameParams), C.c_int, C.c_int, C.c_void_p * 3]
         L.orc_postfilter_frame.argtypes = [P(abi.RvpfFrameParams), P(abi.RvpfSideInfo), P(OrcPlanes),
                                            P(OrcPlanes), C.c_void_p]
+        L.orc_upscale.argtypes = [P(abi.RvpfFrameParams), P(OrcPlanes), P(OrcPlanes)]
         L.orc_grain_rng_step.argtypes = [C.c_uint32]
         L.orc_grain_rng_step.restype = C.c_uint32
         OP = P(OrcPlanes)
@@ -106,6 +107,15 @@ def cdef(frame, deblocked, threads=1):
     return out
 
 
+def upscale(frame, planes):
+    """Super-resolution upscaling (7.16) of frame-width planes into zeroed
+    planes of the reference-frame shape."""
+    out = [None if a is None else np.zeros(frame.ref_shape(p), dtype=np.uint16) for p, a in enumerate(planes)]
+    pi, po = planes_struct(planes), planes_struct(out)
+    lib().orc_upscale(C.byref(frame.params), C.byref(pi), C.byref(po))
+    return out
+
+
 def lr(frame, deblocked, cdef_planes, threads=1):
     out = [None if a is None else np.zeros_like(a) for a in cdef_planes]
     s = frame.host_side_info()
@@ -144,7 +154,7 @@ def rng_step(state):
 def postfilter_frame(frame, threads=1, recon=None):
     """Runs the chain on copies of frame.rec. Returns (deblocked, ref, display)."""
     rec = [None if a is None else a.copy() for a in (recon or frame.rec)]
-    ref = [None if a is None else np.zeros_like(a) for a in rec]
+    ref = [None if a is None else np.zeros(frame.ref_shape(p), dtype=np.uint16) for p, a in enumerate(rec)]
     disp = []
     arr = (C.c_void_p * 3)()
     for p in range(3):

@@ -530,6 +530,65 @@ static void alloc_extent(const rvpf_frame_params *p, int plane, int *w, int *h) 
   *h = (ah >> sy) + 32;
 }
 
+/* The extent of an upscaled plane (super-resolution): the upscaled width
+ * plus 32, the rows of the plane it was made from (postfilter.rs:548). */
+static void up_extent(const rvpf_frame_params *p, int plane, int *w, int *h) {
+  int sx, sy;
+  plane_ss(p, plane, &sx, &sy);
+  alloc_extent(p, plane, w, h);
+  *w = round2(p->upscaled_width, sx) + 32;
+}
+
+/* The extent loop restoration reads and writes: its input is the upscaled
+ * frame with super-resolution. */
+static void lr_extent(const rvpf_frame_params *p, int plane, int *w, int *h) {
+  if (p->use_superres)
+    up_extent(p, plane, w, h);
+  else
+    alloc_extent(p, plane, w, h);
+}
+
+/* ---------------------------------------------------------------------
+ * Super-resolution (7.16), src/decoder/postfilter.rs:521-566
+ * ------------------------------------------------------------------- */
+
+/* The upscaling process: rows 0 .. round2(frame_height, ssy) - 1 and
+ * columns 0 .. round2(upscaled_width, ssx) - 1 of each plane of out
+ * (the rest of out is left as it is). */
+int orc_upscale(const rvpf_frame_params *p, const orc_planes *in, orc_planes *out) {
+  enum { SCALE_BITS = 14, EXTRA_BITS = 8, TAPS = 8, OFFSET = 3 };
+  const int64_t mask = ((int64_t)1 << SCALE_BITS) - 1;
+  const int maxv = (1 << p->bit_depth) - 1;
+  for (int pl = 0; pl < num_planes(p); pl++) {
+    int sx, sy;
+    plane_ss(p, pl, &sx, &sy);
+    const int64_t dw = round2(p->frame_width, sx), uw = round2(p->upscaled_width, sx);
+    const int ph = round2(p->frame_height, sy);
+    const int64_t step = ((dw << SCALE_BITS) + uw / 2) / uw;
+    const int64_t err = uw * step - (dw << SCALE_BITS);
+    int64_t init = (-((uw - dw) << (SCALE_BITS - 1)) + uw / 2) / uw + (1 << (EXTRA_BITS - 1)) - err / 2;
+    init &= mask;
+    const int max_x = (p->mi_cols >> sx) * 4 - 1;
+    for (int y = 0; y < ph; y++) {
+      const uint16_t *row = in->data[pl] + (int64_t)y * in->stride[pl];
+      uint16_t *o = out->data[pl] + (int64_t)y * out->stride[pl];
+      for (int64_t x = 0; x < uw; x++) {
+        const int64_t src_x = -((int64_t)1 << SCALE_BITS) + init + x * step;
+        const int64_t px = src_x >> SCALE_BITS;
+        const int sub = (int)((src_x & mask) >> EXTRA_BITS);
+        int sum = 0;
+        for (int k = 0; k < TAPS; k++) {
+          int64_t sxp = px + k - OFFSET;
+          sxp = sxp < 0 ? 0 : (sxp > max_x ? max_x : sxp);
+          sum += row[sxp] * AV1_UPSCALE_FILTER[sub][k];
+        }
+        o[x] = (uint16_t)clip3(0, maxv, round2(sum, 7));
+      }
+    }
+  }
+  return 0;
+}
+
 /* src/decoder/postfilter.rs:273-301. */
 int orc_cdef(const rvpf_frame_params *p, const rvpf_side_info *s, const orc_planes *in,
              orc_planes *out, int threads) {
@@ -745,7 +804,7 @@ int orc_lr(const rvpf_frame_params *p, const rvpf_side_info *s, const orc_planes
            const orc_planes *cdef, orc_planes *out, int threads) {
   for (int pl = 0; pl < num_planes(p); pl++) {
     int w, h;
-    alloc_extent(p, pl, &w, &h);
+    lr_extent(p, pl, &w, &h);
     copy_plane_rows(out->data[pl], out->stride[pl], cdef->data[pl], cdef->stride[pl], w, h);
   }
   int uses_lr = 0;
@@ -1403,7 +1462,32 @@ int orc_postfilter_frame(const rvpf_frame_params *p, const rvpf_side_info *s, or
     cdef.data[pl] = (uint16_t *)malloc(sizeof(uint16_t) * (size_t)w * h);
   }
   orc_cdef(p, s, recon, &cdef, threads);
-  orc_lr(p, s, recon, &cdef, ref_out, threads);
+  if (p->use_superres) {
+    /* decode_frame_wrapup, mod.rs:752-758: loop restoration reads the
+     * upscaled CDEF output and the upscaled deblocked frame (both zero
+     * outside the rows and columns the upscaling writes) */
+    orc_planes up_cdef, up_cur;
+    for (int pl = 0; pl < 3; pl++) {
+      up_cdef.data[pl] = up_cur.data[pl] = NULL;
+      up_cdef.stride[pl] = up_cur.stride[pl] = 0;
+    }
+    for (int pl = 0; pl < num_planes(p); pl++) {
+      int w, h;
+      up_extent(p, pl, &w, &h);
+      up_cdef.stride[pl] = up_cur.stride[pl] = w;
+      up_cdef.data[pl] = (uint16_t *)calloc((size_t)w * h, sizeof(uint16_t));
+      up_cur.data[pl] = (uint16_t *)calloc((size_t)w * h, sizeof(uint16_t));
+    }
+    orc_upscale(p, &cdef, &up_cdef);
+    orc_upscale(p, recon, &up_cur);
+    orc_lr(p, s, &up_cur, &up_cdef, ref_out, threads);
+    for (int pl = 0; pl < 3; pl++) {
+      free(up_cdef.data[pl]);
+      free(up_cur.data[pl]);
+    }
+  } else {
+    orc_lr(p, s, recon, &cdef, ref_out, threads);
+  }
   for (int pl = 0; pl < 3; pl++) free(cdef.data[pl]);
   if (!display) return 0;
   const int w = p->upscaled_width, h = p->frame_height;

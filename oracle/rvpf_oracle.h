@@ -34,6 +34,11 @@ typedef struct orc_planes {
   int64_t stride[3];
 } orc_planes;
 
+/* Super-resolution upscaling (7.16, src/decoder/postfilter.rs:521-566):
+ * in (frame_width wide) -> out (upscaled_width wide), rows of the frame
+ * height; the rest of out is left as it is. */
+int orc_upscale(const rvpf_frame_params *p, const orc_planes *in, orc_planes *out);
+
 /* 7.14.6 on four lanes, v[k][lane], q0 at k = 7 (src/dsp/lf.rs:58-211). */
 void orc_lf_filter_lanes(int32_t v[14][4], int limit, int blimit, int thresh,
                          int filter_size, int chroma, int bit_depth);

@@ -461,7 +461,11 @@ int rvpf_batch_create(rvpf_ctx *ctx, const rvpf_batch_frame *frames, int32_t n, 
     x.shown = in.display.data[0] != nullptr;
     const rvpf_frame_params *p = &x.p;
     int r;
-    if ((r = validate_params(p, msg)) || (r = validate_side(p, &x.s, msg)) ||
+    if (!(r = validate_params(p, msg)) && p->use_superres) {
+      snprintf(msg, sizeof msg, "super-resolution frames are not batched (use rvpf_postfilter_frame)");
+      r = RVPF_E_UNSUPPORTED;
+    }
+    if (r || (r = validate_side(p, &x.s, msg)) ||
         (r = validate_planes(p, &x.recon, "recon", 0, lf_on(p) ? 8 : 0, msg)) ||
         (r = validate_planes(p, &x.ref, "ref_out", 0, 0, msg, true)) ||
         (x.shown && (r = validate_planes(p, &x.disp, "display", 1, 0, msg)))) {

@@ -36,6 +36,8 @@ struct rvpf_ctx {
   int grain_slot;
   void *cdef_buf;
   size_t cdef_bytes;
+  void *up_buf; /* upscaled CDEF output and deblocked frame (super-resolution) */
+  size_t up_bytes;
   rvpf::GrainScratch grain[2];
   std::vector<TuneEntry> tune_cache;
   char err[256];

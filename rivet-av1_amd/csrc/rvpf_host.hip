@@ -6,8 +6,9 @@
  * loop filter only if loop_filter_level[0] or [1] is non-zero, CDEF
  * (identity when coded_lossless, allow_intrabc or !enable_cdef), loop
  * restoration (identity when no plane uses it), then Decoder::output's crop
- * and film grain (src/decoder/mod.rs:846-917). Super-resolution is not
- * supported (RVPF_E_UNSUPPORTED).
+ * and film grain (src/decoder/mod.rs:846-917). With super-resolution the
+ * CDEF output and the deblocked frame are upscaled (mod.rs:752-757) and loop
+ * restoration runs on the upscaled frames.
  */
 #include <stdio.h>
 #include <string.h>
@@ -94,6 +95,17 @@ void crop(const rvpf_frame_params *p, int pl, int *w, int *h) {
   *h = (p->frame_height + sy) >> sy;
 }
 
+/* the area loop restoration reads and writes: the mode-info area, or with
+ * super-resolution the upscaled frame's (upscaled_width rounded up to 8) */
+void lr_area(const rvpf_frame_params *p, int pl, int *w, int *h) {
+  area(p, pl, w, h);
+  if (p->use_superres) {
+    int sx, sy;
+    plane_ss(p, pl, &sx, &sy);
+    *w = (2 * ((p->upscaled_width + 7) >> 3) * 4) >> sx;
+  }
+}
+
 int validate_params(const rvpf_frame_params *p, char *msg) {
 #define BAD(...)                          \
   do {                                    \
@@ -101,9 +113,14 @@ int validate_params(const rvpf_frame_params *p, char *msg) {
     return RVPF_E_INVALID;                \
   } while (0)
   if (!p) BAD("params is NULL");
-  if (p->use_superres || p->upscaled_width != p->frame_width) {
-    snprintf(msg, 256, "super-resolution is not supported");
-    return RVPF_E_UNSUPPORTED;
+  if (p->use_superres) {
+    /* superres_params (7.21): frame_width = (upscaled_width * 8 + d / 2) / d, d in 9..16 */
+    bool ok = false;
+    for (int d = 9; d <= 16; d++) ok |= p->frame_width == (p->upscaled_width * 8 + d / 2) / d;
+    if (!ok) BAD("super-resolution: frame_width %d is not upscaled_width %d * 8 / d for d in 9..16", p->frame_width,
+                 p->upscaled_width);
+  } else if (p->upscaled_width != p->frame_width) {
+    BAD("upscaled_width %d differs from frame_width %d without super-resolution", p->upscaled_width, p->frame_width);
   }
   if (p->bit_depth != 8 && p->bit_depth != 10 && p->bit_depth != 12) BAD("bit_depth %d", p->bit_depth);
   if (p->ssx < 0 || p->ssx > 1 || p->ssy < 0 || p->ssy > 1 || (p->ssx == 0 && p->ssy == 1))
@@ -219,8 +236,10 @@ int validate_planes(const rvpf_frame_params *p, const rvpf_planes *pl, const cha
   }
   for (int i = 0; i < nplanes_of(p); i++) {
     int w, h;
-    if (use_crop)
+    if (use_crop == 1)
       crop(p, i, &w, &h);
+    else if (use_crop == 2)
+      lr_area(p, i, &w, &h);
     else
       area(p, i, &w, &h);
     w += margin;
@@ -257,6 +276,17 @@ Geom make_geom(const rvpf_frame_params *p) {
   g.height = p->frame_height;
   g.mi_rows = p->mi_rows;
   g.mi_cols = p->mi_cols;
+  return g;
+}
+
+/* The geometry loop restoration runs on: the upscaled frame with
+ * super-resolution (its width, and a mode-info width covering it). */
+Geom make_lr_geom(const rvpf_frame_params *p) {
+  Geom g = make_geom(p);
+  if (p->use_superres) {
+    g.width = p->upscaled_width;
+    g.mi_cols = 2 * ((p->upscaled_width + 7) >> 3);
+  }
   return g;
 }
 
@@ -377,18 +407,45 @@ GrainHdr make_grain(const rvpf_frame_params *p) {
   return h;
 }
 
-int ensure_cdef_buf(rvpf_ctx *ctx, size_t bytes) {
-  if (ctx->cdef_bytes >= bytes) return RVPF_OK;
-  if (ctx->cdef_buf) (void)hipFree(ctx->cdef_buf);
-  ctx->cdef_buf = nullptr;
-  ctx->cdef_bytes = 0;
-  hipError_t e = hipMalloc(&ctx->cdef_buf, bytes);
+/* grows a scratch buffer of the context to at least `bytes` */
+int ensure_buf(rvpf_ctx *ctx, void **buf, size_t *cap, size_t bytes, const char *what) {
+  if (*cap >= bytes) return RVPF_OK;
+  if (*buf) {
+    /* pending work may still use the old buffer */
+    hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) return hip_fail(ctx, e, "synchronize");
+    (void)hipFree(*buf);
+  }
+  *buf = nullptr;
+  *cap = 0;
+  hipError_t e = hipMalloc(buf, bytes);
   if (e != hipSuccess) {
     (void)hipGetLastError();
-    return fail(ctx, RVPF_E_NOMEM, "cannot allocate the CDEF scratch planes");
+    *buf = nullptr;
+    char msg[128];
+    snprintf(msg, sizeof msg, "cannot allocate the %s", what);
+    return fail(ctx, RVPF_E_NOMEM, msg);
   }
-  ctx->cdef_bytes = bytes;
+  *cap = bytes;
   return RVPF_OK;
+}
+
+/* Lays out planes of w[i] x h[i] samples (planes with w[i] == 0 get none)
+ * one after another from `base`, rows padded to 64 samples and planes to
+ * 256 bytes; returns the bytes needed. With base NULL only counts. */
+size_t layout_planes(const int w[3], const int h[3], int bps, void *base, PlanePtr out[3]) {
+  size_t bytes = 0;
+  for (int i = 0; i < 3; i++) {
+    if (w[i] == 0) continue;
+    const int64_t stride = (w[i] + 63) & ~63;
+    if (base) {
+      out[i].data = (char *)base + bytes;
+      out[i].stride = stride;
+    }
+    bytes += (size_t)stride * h[i] * bps;
+    bytes = (bytes + 255) & ~(size_t)255;
+  }
+  return bytes;
 }
 
 void grain_offsets_dims(int w, int h, int *stripes, int *blocks) {
@@ -517,6 +574,77 @@ int run_output(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_planes *ref
   return check_launch(ctx, launch_copy_planes(g, in, out, w, h, ref->bytes_per_sample, st), "display copy");
 }
 
+/* The chain after deblocking with super-resolution (decode_frame_wrapup,
+ * src/decoder/mod.rs:752-758): CDEF of every plane into scratch, the CDEF
+ * output upscaled (into ref_out for planes without loop restoration), the
+ * deblocked frame upscaled for the restored planes, then loop restoration on
+ * the upscaled frames into ref_out. The upscaled planes are written over the
+ * whole area loop restoration reads and writes (upscaled width rounded up to
+ * 8, mode-info rows), so ref_out holds no stale samples there. */
+int superres_tail(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_side_info *s, const PlanePtr rec[3],
+                  PlanePtr ref[3], int bps, hipStream_t st) {
+  const Geom g = make_geom(p);
+  const int np = g.nplanes;
+  int rc;
+  PlanePtr cdef[3];
+  for (int i = 0; i < 3; i++) cdef[i] = rec[i];
+  if (cdef_on(p)) {
+    int w[3] = {0, 0, 0}, h[3] = {0, 0, 0};
+    for (int i = 0; i < np; i++) area(p, i, &w[i], &h[i]);
+    if ((rc = ensure_buf(ctx, &ctx->cdef_buf, &ctx->cdef_bytes, layout_planes(w, h, bps, nullptr, nullptr),
+                         "CDEF scratch planes")))
+      return rc;
+    layout_planes(w, h, bps, ctx->cdef_buf, cdef);
+    StageMark m(ctx, RVPF_STAGE_CDEF, st);
+    if ((rc = check_launch(ctx,
+                           launch_cdef(g, make_cdef(p), s->mi, s->mi_stride, s->cdef_idx, s->cdef_stride, 1, rec,
+                                       cdef, bps, st),
+                           "cdef")))
+      return rc;
+  }
+  /* upscaled planes: the CDEF output, then (distinct only when CDEF ran)
+   * the deblocked frame, for the restored planes */
+  int uw[3] = {0, 0, 0}, uh[3] = {0, 0, 0}, lw[6] = {0, 0, 0, 0, 0, 0}, lh[6] = {0, 0, 0, 0, 0, 0};
+  const bool two = cdef_on(p);
+  for (int i = 0; i < np; i++) {
+    lr_area(p, i, &uw[i], &uh[i]);
+    if (!lr_plane_on(p, i)) continue;
+    lw[i] = uw[i];
+    lh[i] = uh[i];
+    if (two) {
+      lw[3 + i] = uw[i];
+      lh[3 + i] = uh[i];
+    }
+  }
+  PlanePtr up_cdef[3], up_deb[3];
+  for (int i = 0; i < 3; i++) up_cdef[i] = ref[i];
+  if (lr_on(p)) {
+    const size_t b0 = layout_planes(lw, lh, bps, nullptr, nullptr);
+    const size_t b1 = layout_planes(lw + 3, lh + 3, bps, nullptr, nullptr);
+    if ((rc = ensure_buf(ctx, &ctx->up_buf, &ctx->up_bytes, b0 + b1, "upscaled scratch planes"))) return rc;
+    PlanePtr tmp[3];
+    layout_planes(lw, lh, bps, ctx->up_buf, tmp);
+    for (int i = 0; i < np; i++)
+      if (lr_plane_on(p, i)) up_cdef[i] = tmp[i];
+    for (int i = 0; i < 3; i++) up_deb[i] = up_cdef[i];
+    if (two) layout_planes(lw + 3, lh + 3, bps, (char *)ctx->up_buf + b0, up_deb);
+  }
+  {
+    StageMark m(ctx, RVPF_STAGE_UPSCALE, st);
+    if ((rc = check_launch(ctx, launch_upscale(g, p->frame_width, p->upscaled_width, cdef, up_cdef, uw, uh, bps, st),
+                           "upscale")))
+      return rc;
+    if (two && lr_on(p) &&
+        (rc = check_launch(ctx, launch_upscale(g, p->frame_width, p->upscaled_width, rec, up_deb, lw + 3, lh + 3, bps,
+                                               st),
+                           "upscale")))
+      return rc;
+  }
+  if (!lr_on(p)) return RVPF_OK;
+  StageMark m(ctx, RVPF_STAGE_LR, st);
+  return check_launch(ctx, launch_lr(make_lr_geom(p), make_lr(p, s), up_deb, up_cdef, ref, bps, st), "lr");
+}
+
 }  // namespace host
 }  // namespace rvpf
 
@@ -553,6 +681,7 @@ void rvpf_ctx_destroy(rvpf_ctx *ctx) {
   if (!ctx) return;
   if (ctx->side) (void)hipStreamSynchronize(ctx->side);
   if (ctx->cdef_buf) (void)hipFree(ctx->cdef_buf);
+  if (ctx->up_buf) (void)hipFree(ctx->up_buf);
   (void)hipDeviceSynchronize();
   for (auto &m : ctx->marks) {
     (void)hipEventDestroy(m.a);
@@ -660,24 +789,45 @@ int rvpf_cdef(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_side_info *s
                       "cdef");
 }
 
+int rvpf_upscale(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_planes *in, rvpf_planes *out, void *stream) {
+  if (!ctx) return fail(nullptr, RVPF_E_INVALID, "ctx is NULL");
+  int rc;
+  if ((rc = validate_params(p, ctx->err))) return rc;
+  if (!p->use_superres) return fail(ctx, RVPF_E_INVALID, "use_superres is not set");
+  if ((rc = validate_planes(p, in, "in", 0, 0, ctx->err)) || (rc = validate_planes(p, out, "out", 1, 0, ctx->err)))
+    return rc;
+  if (in->bytes_per_sample != out->bytes_per_sample) return fail(ctx, RVPF_E_INVALID, "in and out sample sizes differ");
+  PlanePtr i_[3], o[3];
+  to_ptrs(in, i_);
+  to_ptrs(out, o);
+  int w[3] = {0, 0, 0}, h[3] = {0, 0, 0};
+  for (int i = 0; i < nplanes_of(p); i++) crop(p, i, &w[i], &h[i]);
+  hipStream_t st = (hipStream_t)stream;
+  StageMark m(ctx, RVPF_STAGE_UPSCALE, st);
+  return check_launch(ctx,
+                      launch_upscale(make_geom(p), p->frame_width, p->upscaled_width, i_, o, w, h,
+                                     out->bytes_per_sample, st),
+                      "upscale");
+}
+
 int rvpf_lr(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_side_info *s, const rvpf_planes *deblocked,
             const rvpf_planes *cdef, rvpf_planes *out, void *stream) {
   if (!ctx) return fail(nullptr, RVPF_E_INVALID, "ctx is NULL");
   int rc;
   if ((rc = validate_params(p, ctx->err)) || (rc = validate_side(p, s, ctx->err)) ||
-      (rc = validate_planes(p, deblocked, "deblocked", 0, 0, ctx->err)) ||
-      (rc = validate_planes(p, cdef, "cdef", 0, 0, ctx->err)) || (rc = validate_planes(p, out, "out", 0, 0, ctx->err)))
+      (rc = validate_planes(p, deblocked, "deblocked", 2, 0, ctx->err)) ||
+      (rc = validate_planes(p, cdef, "cdef", 2, 0, ctx->err)) || (rc = validate_planes(p, out, "out", 2, 0, ctx->err)))
     return rc;
   if (deblocked->bytes_per_sample != out->bytes_per_sample || cdef->bytes_per_sample != out->bytes_per_sample)
     return fail(ctx, RVPF_E_INVALID, "plane sample sizes differ");
-  const Geom g = make_geom(p);
+  const Geom g = make_lr_geom(p);
   PlanePtr d[3], c[3], o[3];
   to_ptrs(deblocked, d);
   to_ptrs(cdef, c);
   to_ptrs(out, o);
   int w[3] = {0, 0, 0}, h[3] = {0, 0, 0};
   for (int i = 0; i < g.nplanes; i++)
-    if (!lr_plane_on(p, i)) area(p, i, &w[i], &h[i]);
+    if (!lr_plane_on(p, i)) lr_area(p, i, &w[i], &h[i]);
   hipStream_t st = (hipStream_t)stream;
   {
     StageMark m(ctx, RVPF_STAGE_COPY, st);
@@ -708,7 +858,7 @@ int rvpf_postfilter_frame(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_
   int rc;
   if ((rc = validate_params(p, ctx->err)) || (rc = validate_side(p, s, ctx->err)) ||
       (rc = validate_planes(p, recon, "recon", 0, lf_on(p) ? 8 : 0, ctx->err)) ||
-      (rc = validate_planes(p, ref_out, "ref_out", 0, 0, ctx->err, true)))
+      (rc = validate_planes(p, ref_out, "ref_out", 2, 0, ctx->err, true)))
     return rc;
   if (display && (rc = validate_planes(p, display, "display", 1, 0, ctx->err))) return rc;
   const int bps = recon->bytes_per_sample;
@@ -732,53 +882,44 @@ int rvpf_postfilter_frame(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_
     StageMark m(ctx, RVPF_STAGE_DEBLOCK, st);
     if ((rc = check_launch(ctx, launch_deblock(g, make_lf(p), side, rec, bps, st), "deblock"))) return rc;
   }
-  /* 2. CDEF: planes restored next go to scratch, the others straight to ref_out */
   const bool any_lr = lr_on(p);
-  PlanePtr cdef[3];
-  if (cdef_on(p)) {
-    if (any_lr) {
-      size_t off[3], bytes = 0;
-      int64_t stride[3];
-      for (int i = 0; i < g.nplanes; i++) {
-        int w, h;
-        area(p, i, &w, &h);
-        stride[i] = (w + 63) & ~63;
-        off[i] = bytes;
-        bytes += (size_t)stride[i] * h * bps;
-        bytes = (bytes + 255) & ~(size_t)255;
-      }
-      if ((rc = ensure_cdef_buf(ctx, bytes))) return rc;
-      for (int i = 0; i < 3; i++) {
-        if (i < g.nplanes && lr_plane_on(p, i)) {
-          cdef[i].data = (char *)ctx->cdef_buf + off[i];
-          cdef[i].stride = stride[i];
-        } else {
-          cdef[i] = ref[i];
-        }
-      }
-    } else {
-      for (int i = 0; i < 3; i++) cdef[i] = ref[i];
-    }
-    StageMark m(ctx, RVPF_STAGE_CDEF, st);
-    if ((rc = check_launch(ctx,
-                           launch_cdef(g, make_cdef(p), s->mi, s->mi_stride, s->cdef_idx, s->cdef_stride, 1, rec,
-                                       cdef, bps, st),
-                           "cdef")))
-      return rc;
+  if (p->use_superres) {
+    if ((rc = superres_tail(ctx, p, s, rec, ref, bps, st))) return rc;
   } else {
-    /* CDEF is the identity: planes not restored are copied to ref_out */
-    int w[3] = {0, 0, 0}, h[3] = {0, 0, 0};
-    for (int i = 0; i < g.nplanes; i++) {
-      cdef[i] = rec[i];
-      if (!lr_plane_on(p, i)) area(p, i, &w[i], &h[i]);
+    /* 2. CDEF: planes restored next go to scratch, the others straight to ref_out */
+    PlanePtr cdef[3];
+    if (cdef_on(p)) {
+      for (int i = 0; i < 3; i++) cdef[i] = ref[i];
+      if (any_lr) {
+        int w[3] = {0, 0, 0}, h[3] = {0, 0, 0};
+        for (int i = 0; i < g.nplanes; i++)
+          if (lr_plane_on(p, i)) area(p, i, &w[i], &h[i]);
+        if ((rc = ensure_buf(ctx, &ctx->cdef_buf, &ctx->cdef_bytes, layout_planes(w, h, bps, nullptr, nullptr),
+                             "CDEF scratch planes")))
+          return rc;
+        layout_planes(w, h, bps, ctx->cdef_buf, cdef);
+      }
+      StageMark m(ctx, RVPF_STAGE_CDEF, st);
+      if ((rc = check_launch(ctx,
+                             launch_cdef(g, make_cdef(p), s->mi, s->mi_stride, s->cdef_idx, s->cdef_stride, 1, rec,
+                                         cdef, bps, st),
+                             "cdef")))
+        return rc;
+    } else {
+      /* CDEF is the identity: planes not restored are copied to ref_out */
+      int w[3] = {0, 0, 0}, h[3] = {0, 0, 0};
+      for (int i = 0; i < g.nplanes; i++) {
+        cdef[i] = rec[i];
+        if (!lr_plane_on(p, i)) area(p, i, &w[i], &h[i]);
+      }
+      StageMark m(ctx, RVPF_STAGE_COPY, st);
+      if ((rc = check_launch(ctx, launch_copy_planes(g, rec, ref, w, h, bps, st), "copy"))) return rc;
     }
-    StageMark m(ctx, RVPF_STAGE_COPY, st);
-    if ((rc = check_launch(ctx, launch_copy_planes(g, rec, ref, w, h, bps, st), "copy"))) return rc;
-  }
-  /* 3. loop restoration of the restored planes into ref_out */
-  if (any_lr) {
-    StageMark m(ctx, RVPF_STAGE_LR, st);
-    if ((rc = check_launch(ctx, launch_lr(g, make_lr(p, s), rec, cdef, ref, bps, st), "lr"))) return rc;
+    /* 3. loop restoration of the restored planes into ref_out */
+    if (any_lr) {
+      StageMark m(ctx, RVPF_STAGE_LR, st);
+      if ((rc = check_launch(ctx, launch_lr(g, make_lr(p, s), rec, cdef, ref, bps, st), "lr"))) return rc;
+    }
   }
   /* 4. display */
   if (!display) return RVPF_OK;

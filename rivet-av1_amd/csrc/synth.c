@@ -57,7 +57,10 @@ int synth_geometry(const synth_cfg *c, synth_geom *g) {
   if (c->width < 8 || c->height < 8 || c->width > 16384 || c->height > 16384) return -1;
   if (c->bit_depth != 8 && c->bit_depth != 10 && c->bit_depth != 12) return -1;
   if (c->sb_size != 64 && c->sb_size != 128) return -1;
-  g->mi_cols = 2 * ((c->width + 7) >> 3);
+  if (c->superres_denom && (c->superres_denom < 9 || c->superres_denom > 16)) return -1;
+  /* 7.21 (superres_params): the coded width from the upscaled one */
+  g->coded_width = c->superres_denom ? (c->width * 8 + c->superres_denom / 2) / c->superres_denom : c->width;
+  g->mi_cols = 2 * ((g->coded_width + 7) >> 3);
   g->mi_rows = 2 * ((c->height + 7) >> 3);
   g->mi_stride = g->mi_cols + 32;
   g->mi_alloc_rows = g->mi_rows + 32;
@@ -71,6 +74,10 @@ int synth_geometry(const synth_cfg *c, synth_geom *g) {
     g->plane_w[p] = (aw >> sx) + 32;
     g->plane_h[p] = (ah >> sy) + 32;
     g->crop_w[p] = (c->width + sx) >> sx;
+    {
+      const int awu = (2 * ((c->width + 7) >> 3) * 4 + 127) & ~127;
+      g->ref_w[p] = c->superres_denom ? (awu >> sx) + 32 : g->plane_w[p];
+    }
     g->crop_h[p] = (c->height + sy) >> sy;
     const int us = p == 0 ? unit : ((c->ssx && c->ssy && unit > 32) ? unit >> 1 : unit);
     g->lr_unit_rows[p] = count_units(us, (c->height + sy) >> sy);
@@ -101,6 +108,8 @@ typedef struct {
 
 static void gen_source(const synth_cfg *c, int plane, uint16_t *dst, int pw, int ph) {
   const int sx = plane ? c->ssx : 0, sy = plane ? c->ssy : 0;
+  /* the coded picture (narrower with super-resolution) */
+  const int cw = c->superres_denom ? (c->width * 8 + c->superres_denom / 2) / c->superres_denom : c->width;
   rng_t r = rng_make(c->seed, 1000 + (uint64_t)plane, 0);
   sinus_t s[4];
   for (int k = 0; k < 4; k++) {
@@ -113,9 +122,9 @@ static void gen_source(const synth_cfg *c, int plane, uint16_t *dst, int pw, int
   }
   rect_t rc[3];
   for (int k = 0; k < 3; k++) {
-    rc[k].w = (c->width >> sx) * (0.1 + 0.25 * rng_unit(&r));
+    rc[k].w = (cw >> sx) * (0.1 + 0.25 * rng_unit(&r));
     rc[k].h = (c->height >> sy) * (0.1 + 0.25 * rng_unit(&r));
-    rc[k].x = (c->width >> sx) * rng_unit(&r);
+    rc[k].x = (cw >> sx) * rng_unit(&r);
     rc[k].y = (c->height >> sy) * rng_unit(&r);
     rc[k].vx = (rng_unit(&r) * 8.0 - 4.0) / (1 << sx);
     rc[k].vy = (rng_unit(&r) * 8.0 - 4.0) / (1 << sy);
@@ -334,9 +343,10 @@ int synth_frame(const synth_cfg *c, rvpf_frame_params *p, rvpf_mi *mi, uint8_t *
   P.matrix_coefficients = c->mc_identity ? 0 : 1;
   P.enable_cdef = c->cdef;
   P.film_grain_params_present = c->grain;
-  P.frame_width = c->width;
+  P.frame_width = g.coded_width;
   P.frame_height = c->height;
   P.upscaled_width = c->width;
+  P.use_superres = c->superres_denom != 0;
   P.mi_rows = g.mi_rows;
   P.mi_cols = g.mi_cols;
   if (c->lf) {

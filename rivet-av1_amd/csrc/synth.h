@@ -41,6 +41,7 @@ typedef struct synth_cfg {
   int32_t grain_lag;     /* ar_coeff_lag */
   int32_t grain_y_points;
   int32_t mc_identity;   /* matrix_coefficients = 0 */
+  int32_t superres_denom; /* 0: off; 9..16: coded width = (width * 8 + denom / 2) / denom, upscaled to width */
 } synth_cfg;
 
 typedef struct synth_geom {
@@ -51,6 +52,8 @@ typedef struct synth_geom {
   int32_t crop_w[3], crop_h[3];   /* display plane sizes */
   int32_t lr_unit_rows[3], lr_unit_cols[3];
   int32_t num_planes;
+  int32_t coded_width;            /* frame_width: width, or the super-resolution coded width */
+  int32_t ref_w[3];               /* allocated width of a reference-frame (upscaled) plane */
 } synth_geom;
 
 int synth_geometry(const synth_cfg *c, synth_geom *g);

@@ -155,6 +155,7 @@ class SynthCfg(C.Structure):
         ("grain_lag", C.c_int32),
         ("grain_y_points", C.c_int32),
         ("mc_identity", C.c_int32),
+        ("superres_denom", C.c_int32),
     ]
 
 
@@ -173,6 +174,8 @@ class SynthGeom(C.Structure):
         ("lr_unit_rows", C.c_int32 * 3),
         ("lr_unit_cols", C.c_int32 * 3),
         ("num_planes", C.c_int32),
+        ("coded_width", C.c_int32),
+        ("ref_w", C.c_int32 * 3),
     ]
 
 
@@ -209,13 +212,13 @@ class RvpfLrdItem(C.Structure):
 
 
 TUNE_STAGES = ["deblock", "cdef", "lr", "grain_prepare", "grain_apply", "copy"]
-TUNE_SHAPES = [3, 1, 1, 1, 3, 1]
+TUNE_SHAPES = [3, 1, 1, 1, 1, 1]
 
-STAGES = ["deblock", "cdef", "lr", "grain_prepare", "grain_apply", "copy"]
+STAGES = ["deblock", "cdef", "lr", "grain_prepare", "grain_apply", "copy", "upscale"]
 
 
 class RvpfProfile(C.Structure):
-    _fields_ = [("ms", C.c_float * 6), ("launches", C.c_int32 * 6)]
+    _fields_ = [("ms", C.c_float * 7), ("launches", C.c_int32 * 7)]
 
 
 STRUCT_SIZES = [
@@ -246,6 +249,7 @@ RVPF_EXPORTS = [
     "rvpf_validate",
     "rvpf_deblock",
     "rvpf_cdef",
+    "rvpf_upscale",
     "rvpf_lr",
     "rvpf_grain",
     "rvpf_postfilter_frame",

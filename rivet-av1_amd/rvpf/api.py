@@ -44,6 +44,7 @@ def lib():
         L.rvpf_lr.argtypes = [C.c_void_p, P(abi.RvpfFrameParams), P(abi.RvpfSideInfo), P(abi.RvpfPlanes),
                               P(abi.RvpfPlanes), P(abi.RvpfPlanes), C.c_void_p]
         L.rvpf_grain.argtypes = [C.c_void_p, P(abi.RvpfFrameParams), P(abi.RvpfPlanes), P(abi.RvpfPlanes), C.c_void_p]
+        L.rvpf_upscale.argtypes = [C.c_void_p, P(abi.RvpfFrameParams), P(abi.RvpfPlanes), P(abi.RvpfPlanes), C.c_void_p]
         L.rvpf_postfilter_frame.argtypes = [C.c_void_p, P(abi.RvpfFrameParams), P(abi.RvpfSideInfo),
                                             P(abi.RvpfPlanes), P(abi.RvpfPlanes), P(abi.RvpfPlanes), C.c_void_p]
         L.rvpf_batch_create.argtypes = [C.c_void_p, P(abi.RvpfBatchFrame), C.c_int32, P(C.c_void_p)]
@@ -187,6 +188,12 @@ class DeviceFrame:
         return [None if a is None else torch.zeros(a.shape, dtype=self.dtype, device=self.device)
                 for a in self.rec_host]
 
+    def alloc_ref(self):
+        """Reference-frame planes (upscaled with super-resolution)."""
+        torch = _torch()
+        return [torch.zeros(self.frame.ref_shape(p), dtype=self.dtype, device=self.device)
+                if p < self.frame.num_planes else None for p in range(3)]
+
     def alloc_display(self):
         torch = _torch()
         return [torch.zeros(self.frame.crop_shape(p), dtype=self.dtype, device=self.device)
@@ -221,6 +228,14 @@ def lr(ctx, df, deblocked, cdef_planes, out, stream=None):
     pd, pc, po = planes_struct(deblocked, df.bps), planes_struct(cdef_planes, df.bps), planes_struct(out, df.bps)
     ctx.check(lib().rvpf_lr(ctx.h, C.byref(df.frame.params), C.byref(df.side), C.byref(pd), C.byref(pc),
                             C.byref(po), _stream_handle(stream)))
+
+
+def upscale(ctx, df, planes, out, stream=None):
+    """Super-resolution upscaling of `planes` (the coded width) into `out`
+    (the upscaled width); the frame must have use_superres."""
+    pi, po = planes_struct(planes, df.bps), planes_struct(out, df.bps)
+    ctx.check(lib().rvpf_upscale(ctx.h, C.byref(df.frame.params), C.byref(pi), C.byref(po),
+                                 _stream_handle(stream)))
 
 
 def grain(ctx, df, ref, display, stream=None):

@@ -47,7 +47,7 @@ class Frame:
     def __init__(self, width, height, bit_depth=8, ssx=1, ssy=1, mono=False, sb_size=64,
                  qindex=128, seed=1, frame_index=0, pan_px=0.0, lf=True, lf_extras=False,
                  cdef=True, cdef_bits=3, lr=2, lr_unit_size=64, grain=False, grain_lag=3,
-                 grain_y_points=8, mc_identity=False, with_source=True):
+                 grain_y_points=8, mc_identity=False, with_source=True, superres_denom=0):
         cfg = abi.SynthCfg()
         cfg.width, cfg.height, cfg.bit_depth = width, height, bit_depth
         cfg.ssx, cfg.ssy, cfg.mono = (1, 1, 1) if mono else (ssx, ssy, 0)
@@ -57,6 +57,7 @@ class Frame:
         cfg.lr, cfg.lr_unit_size = lr, lr_unit_size
         cfg.grain, cfg.grain_lag, cfg.grain_y_points = int(grain), grain_lag, grain_y_points
         cfg.mc_identity = int(mc_identity)
+        cfg.superres_denom = superres_denom
         self.cfg = cfg
         g = abi.SynthGeom()
         if synth_lib().synth_geometry(C.byref(cfg), C.byref(g)) != 0:
@@ -93,6 +94,16 @@ class Frame:
     def crop_shape(self, p):
         return (self.geom.crop_h[p], self.geom.crop_w[p])
 
+    def ref_shape(self, p):
+        """Allocated shape of a reference-frame plane (upscaled with
+        super-resolution, else the reconstruction's)."""
+        return (self.geom.plane_h[p], self.geom.ref_w[p])
+
+    def ref_area(self, p):
+        """(rows, cols) of a reference-frame plane that the chain defines:
+        the mode-info area, or the displayed size with super-resolution."""
+        return self.crop_shape(p) if self.params.use_superres else self.mi_area(p)
+
     def mi_area(self, p):
         """(rows, cols) of the mode-info-aligned area of plane p."""
         sx = self.params.ssx if p else 0
@@ -112,4 +123,4 @@ class Frame:
         return s
 
     def luma_pixels(self):
-        return self.params.frame_width * self.params.frame_height
+        return self.params.upscaled_width * self.params.frame_height

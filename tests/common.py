@@ -49,6 +49,8 @@ def assert_planes(got, want, frame, region, what):
             rows, cols = want[p].shape
         elif region == "mi":
             rows, cols = frame.mi_area(p)
+        elif region == "ref":
+            rows, cols = frame.ref_area(p)
         else:
             rows, cols = frame.crop_shape(p)
         msg = first_diff(got[p], want[p], rows, cols)

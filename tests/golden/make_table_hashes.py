@@ -13,7 +13,7 @@ REF = sys.argv[1] if len(sys.argv) > 1 else "/root/reference/src/tables.rs"
 NAMES = ["CDEF_DIRECTIONS", "CDEF_PRI_TAPS", "CDEF_SEC_TAPS", "CDEF_UV_DIR", "DIV_TABLE",
          "NUM_4X4_BLOCKS_WIDE", "NUM_4X4_BLOCKS_HIGH", "SUBSAMPLED_SIZE", "TX_WIDTH", "TX_HEIGHT",
          "SGR_PARAMS", "WIENER_TAPS_MIN", "WIENER_TAPS_MAX", "WIENER_TAPS_MID", "SGRPROJ_XQD_MIN",
-         "SGRPROJ_XQD_MAX", "SGRPROJ_XQD_MID", "AC_QLOOKUP", "GAUSSIAN_SEQUENCE"]
+         "SGRPROJ_XQD_MAX", "SGRPROJ_XQD_MID", "AC_QLOOKUP", "GAUSSIAN_SEQUENCE", "UPSCALE_FILTER"]
 
 
 def values_hash(vals):

@@ -43,8 +43,8 @@ def test_validate_accepts_synthetic_frames():
 
 @pytest.mark.parametrize("field,value,code", [
     ("bit_depth", 9, abi.RVPF_E_INVALID),
-    ("use_superres", 1, abi.RVPF_E_UNSUPPORTED),
-    ("upscaled_width", 100, abi.RVPF_E_UNSUPPORTED),
+    ("use_superres", 1, abi.RVPF_E_INVALID),  # frame_width is not upscaled_width * 8 / d
+    ("upscaled_width", 100, abi.RVPF_E_INVALID),  # differs from frame_width without super-resolution
     ("mi_cols", 23, abi.RVPF_E_INVALID),
     ("cdef_damping", 7, abi.RVPF_E_INVALID),
     ("loop_filter_sharpness", 8, abi.RVPF_E_INVALID),
@@ -54,6 +54,17 @@ def test_validate_rejects(field, value, code):
     setattr(f.params, field, value)
     rc, msg = api.validate(f.params, f.host_side_info())
     assert rc == code, msg
+
+
+@pytest.mark.parametrize("denom", [9, 12, 16])
+def test_validate_accepts_superres(denom):
+    f = _frame(width=200, superres_denom=denom)
+    assert f.params.use_superres == 1 and f.params.upscaled_width == 200
+    assert f.params.frame_width == (200 * 8 + denom // 2) // denom
+    rc, msg = api.validate(f.params, f.host_side_info())
+    assert rc == abi.RVPF_OK, msg
+    f.params.frame_width += 1  # no denominator in 9..16 gives this width
+    assert api.validate(f.params, f.host_side_info())[0] == abi.RVPF_E_INVALID
 
 
 def test_validate_rejects_bad_arrays():

@@ -28,6 +28,18 @@ def test_tables_match_reference():
         assert values_hash(got[n]) == want[n]["sha256"], n
 
 
+RANGES = {"int8_t": (-128, 127), "uint8_t": (0, 255), "int16_t": (-32768, 32767), "uint16_t": (0, 65535),
+          "int32_t": (-2 ** 31, 2 ** 31 - 1)}
+
+
+def test_table_types_hold_their_values():
+    src = open(HDR).read()
+    for m in re.finditer(r"AV1_TABLE (\w+) AV1_(\w+)((?:\[\d+\])+) = (\{.*?\});\n", src, re.S):
+        lo, hi = RANGES[m.group(1)]
+        vals = [int(v) for v in re.findall(r"-?\d+", m.group(4))]
+        assert all(lo <= v <= hi for v in vals), m.group(2)
+
+
 def test_derived_tables():
     t = header_tables()
     # Div_Table is 840 / k (7.15.2)
@@ -35,3 +47,6 @@ def test_derived_tables():
     # Gaussian sequence: 2048 entries, multiples of 4 within 12 bits
     g = t["GAUSSIAN_SEQUENCE"]
     assert len(g) == 2048 and all(v % 4 == 0 and -2048 <= v < 2048 for v in g)
+    # every super-resolution filter phase sums to 128 (7.16)
+    u = t["UPSCALE_FILTER"]
+    assert len(u) == 512 and all(sum(u[8 * i:8 * i + 8]) == 128 for i in range(64))
