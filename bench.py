@@ -32,14 +32,14 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make_frames(n, w, h, bd, seed):
+def make_frames(n, w, h, bd, seed, superres_denom=0):
     from rvpf import workload
     frames = []
     for i in range(n):
         frames.append(workload.Frame(w, h, bit_depth=bd, ssx=1, ssy=1, sb_size=128, qindex=QINDEX[i % 4],
                                      seed=seed, frame_index=i, pan_px=1.5, lf=True, cdef=True, cdef_bits=3,
                                      lr=2, lr_unit_size=64, grain=True, grain_lag=3, grain_y_points=8,
-                                     with_source=False))
+                                     with_source=False, superres_denom=superres_denom))
     return frames
 
 
@@ -73,6 +73,15 @@ def samples_per_frame(f):
     return sum(f.crop_shape(p)[0] * f.crop_shape(p)[1] for p in range(f.num_planes))
 
 
+def coded_samples_per_frame(f):
+    """Samples of the coded frame (before super-resolution upscaling)."""
+    n = 0
+    for p in range(f.num_planes):
+        sx = f.params.ssx if p else 0
+        n += f.crop_shape(p)[0] * ((f.params.frame_width + sx) >> sx)
+    return n
+
+
 def cpu_baseline(frames, seconds):
     """The oracle chain on the host, threaded like the reference's decoder."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -89,7 +98,10 @@ def cpu_baseline(frames, seconds):
             break
     px = sum(frames[i % len(frames)].luma_pixels() for i in range(n))
     return {"value": round(px / el / 1e6, 2), "unit": "Mpix/s", "cores": threads, "kind": "port",
-            "sample": "%d frames of the same 4K 10-bit 4:2:0 chain (%.1f s), C oracle, %d threads" % (n, el, threads)}
+            "sample": "%d frames of the same %dx%d %d-bit 4:2:0 chain%s (%.1f s), C oracle, %d threads"
+                      % (n, frames[0].params.upscaled_width, frames[0].params.frame_height,
+                         frames[0].params.bit_depth,
+                         " with super-resolution" if frames[0].params.use_superres else "", el, threads)}
 
 
 BATCH_SIZES = [(640, 360), (1280, 720), (1920, 1080), (2560, 1440), (3840, 2160)]
@@ -202,7 +214,7 @@ def bench_batch(args):
         got_ref, got_disp = api.to_host(it["ref"]), api.to_host(it["disp"])
         bitexact = True
         for p in range(f.num_planes):
-            r, c = f.mi_area(p)
+            r, c = f.ref_area(p)
             bitexact &= bool(np.array_equal(got_ref[p][:r, :c], ref[p][:r, :c]))
             bitexact &= bool(np.array_equal(got_disp[p], disp[p]))
         log("bit-exact vs oracle (smallest frame):", bitexact)
@@ -383,6 +395,8 @@ def main():
     ap.add_argument("--width", type=int, default=3840)
     ap.add_argument("--height", type=int, default=2160)
     ap.add_argument("--bit-depth", type=int, default=10)
+    ap.add_argument("--superres-denom", type=int, default=0,
+                    help="code the frames at width*8/d and upscale (super-resolution), d in 9..16")
     ap.add_argument("--seed", type=int, default=2160)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -419,7 +433,7 @@ def main():
         torch.cuda.synchronize()
         del a_, b_
     log("generating %d frames %dx%d %d-bit" % (args.frames, args.width, args.height, args.bit_depth))
-    frames = make_frames(args.frames, args.width, args.height, args.bit_depth, args.seed)
+    frames = make_frames(args.frames, args.width, args.height, args.bit_depth, args.seed, args.superres_denom)
     ctx = api.Context()
     dfs = [api.DeviceFrame(f) for f in frames]
     nsteps = args.steps + args.warmup
@@ -427,7 +441,7 @@ def main():
     pristine = [df.upload_planes(f.rec) for df, f in zip(dfs, frames)]
     recs = [[[None if t is None else t.clone() for t in pristine[i]] for i in range(len(frames))]
             for _ in range(nsteps)]
-    refs = [df.alloc_like_recon() for df in dfs]
+    refs = [df.alloc_ref() for df in dfs]
     disps = [df.alloc_display() for df in dfs]
     torch.cuda.synchronize()
     stream = torch.cuda.current_stream()
@@ -465,7 +479,7 @@ def main():
         got_ref, got_disp = api.to_host(refs[0]), api.to_host(disps[0])
         bitexact = True
         for p in range(f.num_planes):
-            r, c = f.mi_area(p)
+            r, c = f.ref_area(p)
             bitexact &= bool(np.array_equal(got_ref[p][:r, :c], ref[p][:r, :c]))
             bitexact &= bool(np.array_equal(got_disp[p], disp[p]))
         log("bit-exact vs oracle (frame 0):", bitexact)
@@ -477,7 +491,12 @@ def main():
     stages = {}
     if not args.no_stages:
         spf = samples_per_frame(frames[0])
-        alg = {"deblock": 2 * b, "cdef": 2 * b, "lr": 2 * b + b * 4 / 64, "grain_apply": 2 * b}
+        cpf = coded_samples_per_frame(frames[0])
+        # bytes per frame: deblock and CDEF run on the coded frame; the
+        # upscaling reads coded and writes upscaled samples twice (CDEF
+        # output and deblocked frame, every plane being restored here)
+        alg = {"deblock": 2 * b * cpf, "cdef": 2 * b * cpf, "lr": (2 * b + b * 4 / 64) * spf,
+               "grain_apply": 2 * b * spf, "upscale": 2 * b * (cpf + spf)}
         for s_ in range(nsteps):
             for i in range(len(frames)):
                 for t, src in zip(recs[s_][i], pristine[i]):
@@ -497,8 +516,8 @@ def main():
             avg = ms / n
             st = {"ms": round(avg, 4), "launches": n}
             if k in alg:
-                gbs = spf * alg[k] / (avg / 1e3) / 1e9
-                st.update({"alg_bytes": int(spf * alg[k]), "GB_s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)})
+                gbs = alg[k] / (avg / 1e3) / 1e9
+                st.update({"alg_bytes": int(alg[k]), "GB_s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)})
                 log("stage %-13s %.4f ms  %.1f GB/s algorithmic (%.1f%% of HBM peak)" % (k, avg, gbs, 100 * gbs / HBM_PEAK_GBS))
             else:
                 log("stage %-13s %.4f ms (side stream, overlapped)" % (k, avg))
@@ -509,12 +528,14 @@ def main():
     if main:
         dom = max(main, key=lambda k: stages[k]["ms"])
         s = stages[dom]
-        default_workload = (args.width, args.height, args.bit_depth, args.frames) == (3840, 2160, 10, 4)
+        default_workload = (args.width, args.height, args.bit_depth, args.frames, args.superres_denom) == \
+            (3840, 2160, 10, 4, 0)
         roofline = {"bound": "hbm", "achieved": s["GB_s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": s["frac"], "traffic": measured_traffic(dom) if default_workload else None,
                     "kernel": dom, "alg_bytes_per_launch": s["alg_bytes"], "ms_per_launch": s["ms"],
                     "traffic_source": PMC_SUMMARY if default_workload else None}
-    chain_alg = samples_per_frame(frames[0]) * (8 * b + b * 4 / 64) * len(frames)
+    spf, cpf = samples_per_frame(frames[0]), coded_samples_per_frame(frames[0])
+    chain_alg = (4 * b * cpf + (4 * b + b * 4 / 64) * spf + (2 * b * (cpf + spf) if cpf != spf else 0)) * len(frames)
     chain_gbs = chain_alg / (ms_per_step / 1e3) / 1e9
 
     cpu = None
@@ -537,9 +558,12 @@ def main():
         "dtype": "u16" if b == 2 else "u8",
         "data": "synthetic",
         "config": {
-            "workload": "%dx%d %d-bit 4:2:0, %d-frame GOP (qindex %s), deblock+CDEF+LR(Wiener/SGR, 64-px units)"
-                        "+film grain, per-frame drop-in" % (args.width, args.height, args.bit_depth, len(frames),
-                                                           "/".join(str(q) for q in QINDEX[:len(frames)])),
+            "workload": "%dx%d %d-bit 4:2:0, %d-frame GOP (qindex %s), deblock+CDEF+%sLR(Wiener/SGR, 64-px units)"
+                        "+film grain, per-frame drop-in" % (
+                            args.width, args.height, args.bit_depth, len(frames),
+                            "/".join(str(q) for q in QINDEX[:len(frames)]),
+                            "super-resolution from width %d (denominator %d)+" % (
+                                frames[0].params.frame_width, args.superres_denom) if args.superres_denom else ""),
             "frames_per_step": len(frames),
             "bitexact_frame0": bitexact,
             "chain_alg_GB_s": round(chain_gbs, 1),

@@ -617,7 +617,7 @@ int superres_tail(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_side_inf
     }
   }
   PlanePtr up_cdef[3], up_deb[3];
-  for (int i = 0; i < 3; i++) up_cdef[i] = ref[i];
+  for (int i = 0; i < 3; i++) up_cdef[i] = up_deb[i] = ref[i];
   if (lr_on(p)) {
     const size_t b0 = layout_planes(lw, lh, bps, nullptr, nullptr);
     const size_t b1 = layout_planes(lw + 3, lh + 3, bps, nullptr, nullptr);
@@ -630,14 +630,16 @@ int superres_tail(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_side_inf
     if (two) layout_planes(lw + 3, lh + 3, bps, (char *)ctx->up_buf + b0, up_deb);
   }
   {
+    /* one launch: every plane of the CDEF output, and the deblocked planes
+     * the restoration reads */
+    UpscaleJob jobs[6];
+    int nj = 0;
+    for (int i = 0; i < np; i++) jobs[nj++] = UpscaleJob{cdef[i], up_cdef[i], i, uw[i], uh[i]};
+    if (two)
+      for (int i = 0; i < np; i++)
+        if (lw[3 + i]) jobs[nj++] = UpscaleJob{rec[i], up_deb[i], i, lw[3 + i], lh[3 + i]};
     StageMark m(ctx, RVPF_STAGE_UPSCALE, st);
-    if ((rc = check_launch(ctx, launch_upscale(g, p->frame_width, p->upscaled_width, cdef, up_cdef, uw, uh, bps, st),
-                           "upscale")))
-      return rc;
-    if (two && lr_on(p) &&
-        (rc = check_launch(ctx, launch_upscale(g, p->frame_width, p->upscaled_width, rec, up_deb, lw + 3, lh + 3, bps,
-                                               st),
-                           "upscale")))
+    if ((rc = check_launch(ctx, launch_upscale(g, p->frame_width, p->upscaled_width, jobs, nj, bps, st), "upscale")))
       return rc;
   }
   if (!lr_on(p)) return RVPF_OK;
@@ -800,14 +802,17 @@ int rvpf_upscale(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_planes *i
   PlanePtr i_[3], o[3];
   to_ptrs(in, i_);
   to_ptrs(out, o);
-  int w[3] = {0, 0, 0}, h[3] = {0, 0, 0};
-  for (int i = 0; i < nplanes_of(p); i++) crop(p, i, &w[i], &h[i]);
+  UpscaleJob jobs[3];
+  const int np = nplanes_of(p);
+  for (int i = 0; i < np; i++) {
+    jobs[i] = UpscaleJob{i_[i], o[i], i, 0, 0};
+    crop(p, i, &jobs[i].cols, &jobs[i].rows);
+  }
   hipStream_t st = (hipStream_t)stream;
   StageMark m(ctx, RVPF_STAGE_UPSCALE, st);
-  return check_launch(ctx,
-                      launch_upscale(make_geom(p), p->frame_width, p->upscaled_width, i_, o, w, h,
-                                     out->bytes_per_sample, st),
-                      "upscale");
+  return check_launch(
+      ctx, launch_upscale(make_geom(p), p->frame_width, p->upscaled_width, jobs, np, out->bytes_per_sample, st),
+      "upscale");
 }
 
 int rvpf_lr(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_side_info *s, const rvpf_planes *deblocked,

@@ -270,11 +270,19 @@ int launch_grain_prepare(const GrainHdr &h, const GrainScratch &s, hipStream_t s
 int launch_grain_apply(const GrainHdr &h, const GrainScratch &s, const PlanePtr in[3],
                        PlanePtr out[3], int bps, hipStream_t st);
 
-/* Super-resolution upscaling (upscale.hip) of rows 0..rows[p]-1 and columns
- * 0..cols[p]-1 of each plane with cols[p] > 0; g is the coded (downscaled)
- * frame's geometry, whose mode-info width bounds the columns read. */
-int launch_upscale(const Geom &g, int frame_width, int upscaled_width, const PlanePtr in[3], PlanePtr out[3],
-                   const int cols[3], const int rows[3], int bps, hipStream_t st);
+/* One plane of a super-resolution upscaling launch: rows 0..rows-1 and
+ * columns 0..cols-1 of `out` from `in` (plane index 0..2). */
+struct UpscaleJob {
+  PlanePtr in, out;
+  int plane;
+  int cols, rows;
+};
+
+/* Super-resolution upscaling (upscale.hip) of up to six plane jobs in one
+ * launch; g is the coded (downscaled) frame's geometry, whose mode-info
+ * width bounds the columns read. */
+int launch_upscale(const Geom &g, int frame_width, int upscaled_width, const UpscaleJob *jobs, int njobs, int bps,
+                   hipStream_t st);
 
 /* Constant-table upload (once per device). */
 int upload_tables();
