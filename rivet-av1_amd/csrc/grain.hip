@@ -22,10 +22,13 @@
  *   block's offset, the horizontal blend with the block to the left, then
  *   the vertical blend with the stripe above (whose value is itself
  *   horizontally blended), exactly the order in which the reference fills
- *   its noise stripes and noise image. A lane owns a column of the stripe,
- *   so the block offsets and template origins are set up once and every
- *   row's template and plane accesses are coalesced. Chroma reads the
- *   un-noised luma of the input, as the reference blends chroma first.
+ *   its noise stripes and noise image. A thread takes 8-sample row
+ *   segments that lie in one noise block, so their noise is 8 consecutive
+ *   template samples: one aligned 16-byte load from one of 8 copies of the
+ *   templates, each shifted by one more sample, that the prepare kernel
+ *   writes. Chroma reads the un-noised luma of the input, as the reference
+ *   blends chroma first. Planes whose rows are not aligned for segment
+ *   accesses take a sample-by-sample variant.
  */
 #include "av1_tables.h"
 #include "rvpf_internal.h"

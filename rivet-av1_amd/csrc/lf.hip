@@ -5,20 +5,28 @@
  * filter_strength (src/decoder/postfilter.rs:67-264) and the sample filter
  * dsp::lf::filter_lanes (src/dsp/lf.rs:58-211).
  *
- * One thread per 4-sample edge of a plane; one launch per pass (vertical
- * edges, then horizontal edges), all planes flattened into one grid.
- * In place. The reference walks the edges of a row in order and writes
- * back p2..q2 (p5..q5) whatever filter ran; a parallel pass must not, since
- * a 4-wide edge's neighbour 4 samples on modifies those samples. Each
- * thread therefore stores only its filter's footprint:
+ * One launch per pass (vertical edges, then horizontal edges), all planes
+ * flattened into one grid, in place. A workgroup takes a block of 256
+ * 4-sample edges (64 x 4 in the per-frame launch): each thread decides one
+ * edge (7.14.2-7.14.5; the level tables sit in LDS), the filtered edges
+ * are gathered into one LDS list per filter class (4, chroma 6-tap, luma
+ * 8-tap, 16), and each list is then filtered with every lane on the same
+ * branch-free class filter, one line of an edge per lane (a row of the
+ * edge in the vertical pass, a column in the horizontal one).
+ *
+ * The reference walks the edges of a row in order and writes back p2..q2
+ * (p5..q5) whatever filter ran; a parallel pass must not, since a 4-wide
+ * edge's neighbour 4 samples on modifies those samples. Each line
+ * therefore stores only its filter's footprint:
  *   filter size 4            -> p1..q1
  *   filter size 8 (6/8-tap)  -> p3..q3 (p3, q3, and for chroma p2, q2, are
  *                               written back unchanged)
  *   filter size 16           -> p7..q7 (p7, p6, q6, q7 unchanged)
  * For side info whose transform blocks are aligned to their size (every
  * AV1 stream), a size-s edge at x has no other edge in (x - s, x + s), so
- * these footprints never overlap another thread's reads or writes and the
- * result equals the sequential one sample for sample.
+ * these footprints never overlap another line's reads or writes and the
+ * result equals the sequential one sample for sample, whatever order the
+ * lists run in.
  */
 #include <string.h>
 
