@@ -270,12 +270,43 @@ int rvpf_grain(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_planes *ref
                rvpf_planes *display, void *stream);
 
 /*
+ * The packed picture Decoder::output returns (src/decoder/mod.rs:846-917,
+ * the Frame layout of src/frame.rs:131-190): Y, then U, then V (Y alone
+ * for monochrome), each tightly packed (stride = width), one byte per
+ * sample at 8 bits, else little-endian uint16 with the value in the low
+ * bits. Offsets are in bytes from the start of the picture.
+ */
+typedef struct rvpf_packed_layout {
+  int64_t offset[3];
+  int32_t width[3], height[3];
+  int32_t num_planes;
+  int32_t bytes_per_sample; /* 1 at 8 bits, else 2 */
+  int64_t bytes;            /* the whole picture */
+} rvpf_packed_layout;
+
+/* Host only: the packed picture's layout for `p` (Frame::new, src/frame.rs:161-190). */
+int rvpf_packed_layout_of(const rvpf_frame_params *p, rvpf_packed_layout *out);
+
+/*
+ * Output stage into the packed picture: the crop of `ref` (16-bit or, at
+ * 8 bits, 8-bit sample planes), with film grain as rvpf_grain, written to
+ * the device buffer `packed` of at least `bytes` bytes in the layout of
+ * rvpf_packed_layout_of (replaces Decoder::output, src/decoder/mod.rs:846-917,
+ * crop, grain and pack in one pass). The grain templates are built on the
+ * context's side stream, so they overlap work queued before this call.
+ */
+int rvpf_output_packed(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_planes *ref, void *packed,
+                       int64_t bytes, void *stream);
+
+/*
  * The whole chain of Decoder::decode_frame_wrapup (src/decoder/mod.rs:747-816)
  * and Decoder::output: deblock in place on `recon`, CDEF, super-resolution
  * upscaling when use_superres, loop restoration into `ref_out` (the
  * reference frame; upscaled with super-resolution, when only its
  * upscaled_width x frame_height crop is defined), then crop + grain into
- * `display` (may be NULL when the frame is not shown).
+ * `display` (may be NULL when the frame is not shown). At 8 bits the
+ * display planes may hold bytes while the other planes hold 16-bit samples
+ * (also for rvpf_grain).
  */
 int rvpf_postfilter_frame(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_side_info *s,
                           rvpf_planes *recon, rvpf_planes *ref_out, rvpf_planes *display,

@@ -16,8 +16,9 @@ struct CopyLaunch {
   int blk_start[4];
 };
 
-/* Block b: 64 x 4 samples; rows of 64 lanes -> coalesced. */
-template <typename T>
+/* Block b: 64 x 4 samples; rows of 64 lanes -> coalesced. Samples T are
+ * stored as TO (8-bit output from 16-bit planes: the values fit). */
+template <typename T, typename TO = T>
 __device__ __forceinline__ void copy_block(const CopyLaunch &a, int b) {
   int p = 0;
   if (b >= a.blk_start[1]) p = 1;
@@ -26,13 +27,13 @@ __device__ __forceinline__ void copy_block(const CopyLaunch &a, int b) {
   const int x = (lb % a.blk_x[p]) * 64 + (threadIdx.x & 63);
   const int y = (lb / a.blk_x[p]) * 4 + (threadIdx.x >> 6);
   if (x >= a.w[p] || y >= a.h[p]) return;
-  gptr(reinterpret_cast<T *>(a.out[p].data))[(int64_t)y * a.out[p].stride + x] =
-      gptr(reinterpret_cast<const T *>(a.in[p].data))[(int64_t)y * a.in[p].stride + x];
+  gptr(reinterpret_cast<TO *>(a.out[p].data))[(int64_t)y * a.out[p].stride + x] =
+      (TO)gptr(reinterpret_cast<const T *>(a.in[p].data))[(int64_t)y * a.in[p].stride + x];
 }
 
-template <typename T>
+template <typename T, typename TO>
 __global__ __launch_bounds__(256) void copy_kernel(CopyLaunch a) {
-  copy_block<T>(a, blockIdx.x);
+  copy_block<T, TO>(a, blockIdx.x);
 }
 
 template <typename T>
@@ -76,14 +77,18 @@ int launch_copy_batch(const BatchWork &w, const StagePlan &sp, int bps, hipStrea
 }
 
 int launch_copy_planes(const Geom &g, const PlanePtr in[3], PlanePtr out[3], const int w[3],
-                       const int h[3], int bps, hipStream_t st) {
+                       const int h[3], int bps, hipStream_t st, int out_bps) {
+  if (out_bps == 0) out_bps = bps;
+  if (out_bps > bps) return (int)hipErrorInvalidValue;
   CopyLaunch a;
   const int total = plan(g, in, out, w, h, a);
   if (total == 0) return 0;
   if (bps == 1)
-    copy_kernel<uint8_t><<<total, 256, 0, st>>>(a);
+    copy_kernel<uint8_t, uint8_t><<<total, 256, 0, st>>>(a);
+  else if (out_bps == 1)
+    copy_kernel<uint16_t, uint8_t><<<total, 256, 0, st>>>(a);
   else
-    copy_kernel<uint16_t><<<total, 256, 0, st>>>(a);
+    copy_kernel<uint16_t, uint16_t><<<total, 256, 0, st>>>(a);
   return (int)hipGetLastError();
 }
 

@@ -507,7 +507,7 @@ struct Seg8<uint8_t> {
  * scaling tables are staged. The planes' rows must be 16-byte (u16) or
  * 8-byte (u8) aligned. Chroma reads the un-noised luma of the input, as
  * the reference blends chroma first. */
-template <typename T, int CI>
+template <typename T, typename TO, int CI>
 __device__ __forceinline__ void grain_apply_tile(const GrainHdr &h, const GrainScratch &sc, const PlanePtr &i0,
                                                  const PlanePtr &i1, const PlanePtr &i2, const PlanePtr &o0,
                                                  const PlanePtr &o1, const PlanePtr &o2, int tile_id, int16_t *lut) {
@@ -595,7 +595,7 @@ __device__ __forceinline__ void grain_apply_tile(const GrainHdr &h, const GrainS
         }
       }
       const PlanePtr &op = p == 1 ? o1 : o2;
-      Seg8<T>::store(gptr(reinterpret_cast<T *>(op.data)) + (int64_t)cy[u] * op.stride + cx[u], cw - cx[u], v);
+      Seg8<TO>::store(gptr(reinterpret_cast<TO *>(op.data)) + (int64_t)cy[u] * op.stride + cx[u], cw - cx[u], v);
     }
   }
   if (lon) {
@@ -608,14 +608,14 @@ __device__ __forceinline__ void grain_apply_tile(const GrainHdr &h, const GrainS
         v[k] = clip3(h.min_value, h.max_luma, lorig[k] + sn);
       }
     }
-    Seg8<T>::store(gptr(reinterpret_cast<T *>(o0.data)) + (int64_t)ly * o0.stride + lx, h.w - lx, v);
+    Seg8<TO>::store(gptr(reinterpret_cast<TO *>(o0.data)) + (int64_t)ly * o0.stride + lx, h.w - lx, v);
   }
 }
 
 /* Sample-by-sample variant for planes whose rows are not aligned for the
  * segment accesses: thread t of the tile takes samples t, t + 256, ... of
  * the tile's luma (64 x 32) and then of each chroma plane. */
-template <typename T>
+template <typename T, typename TO>
 __device__ __forceinline__ void grain_apply_tile_unaligned(const GrainHdr &h, const GrainScratch &sc,
                                                            const PlanePtr &i0, const PlanePtr &i1, const PlanePtr &i2,
                                                            const PlanePtr &o0, const PlanePtr &o1, const PlanePtr &o2,
@@ -666,7 +666,7 @@ __device__ __forceinline__ void grain_apply_tile_unaligned(const GrainHdr &h, co
         const int sn = round2(scale_lut(lut + p * 256, bd, merged) * z[cx & 7], h.scaling_shift);
         v = clip3(h.min_value, h.max_chroma, orig + sn);
       }
-      gptr(reinterpret_cast<T *>(op.data))[(int64_t)cy * op.stride + cx] = (T)v;
+      gptr(reinterpret_cast<TO *>(op.data))[(int64_t)cy * op.stride + cx] = (TO)v;
     }
   }
   for (int t = threadIdx.x; t < 64 * 32; t += 256) {
@@ -680,7 +680,7 @@ __device__ __forceinline__ void grain_apply_tile_unaligned(const GrainHdr &h, co
       const int sn = round2(scale_lut(lut, bd, orig) * z[x & 7], h.scaling_shift);
       v = clip3(h.min_value, h.max_luma, orig + sn);
     }
-    gptr(reinterpret_cast<T *>(o0.data))[(int64_t)y * o0.stride + x] = (T)v;
+    gptr(reinterpret_cast<TO *>(o0.data))[(int64_t)y * o0.stride + x] = (TO)v;
   }
 }
 
@@ -690,16 +690,17 @@ __global__ __launch_bounds__(PREP_THREADS) void grain_prepare_kernel(GrainHdr h,
 }
 
 /* ALIGNED: 1 (segments, one chroma segment per thread), 2 (segments,
- * two: 4:4:4), 0 (sample by sample). */
-template <typename T, int ALIGNED>
+ * two: 4:4:4), 0 (sample by sample). Input samples T, output samples TO
+ * (8-bit output from 16-bit planes for the packed 8-bit frame). */
+template <typename T, typename TO, int ALIGNED>
 __global__ __launch_bounds__(256) void grain_apply_kernel(GrainHdr h, GrainScratch sc, PlanePtr i0, PlanePtr i1,
                                                           PlanePtr i2, PlanePtr o0, PlanePtr o1, PlanePtr o2) {
   __shared__ int16_t lut[3 * 256];
   const int tile = xcd_remap(blockIdx.x, gridDim.x);
   if constexpr (ALIGNED != 0)
-    grain_apply_tile<T, ALIGNED>(h, sc, i0, i1, i2, o0, o1, o2, tile, lut);
+    grain_apply_tile<T, TO, ALIGNED>(h, sc, i0, i1, i2, o0, o1, o2, tile, lut);
   else
-    grain_apply_tile_unaligned<T>(h, sc, i0, i1, i2, o0, o1, o2, tile, lut);
+    grain_apply_tile_unaligned<T, TO>(h, sc, i0, i1, i2, o0, o1, o2, tile, lut);
 }
 
 struct PrepRecord {
@@ -728,9 +729,9 @@ __global__ __launch_bounds__(256) void grain_apply_batch(BatchWork w) {
   batch_loop(w, [&](int f, int j) {
     const ApplyRecord &r = batch_table<ApplyRecord>(w.rec, f);
     if (r.aligned)
-      grain_apply_tile<T, 1>(r.h, r.s, r.in[0], r.in[1], r.in[2], r.out[0], r.out[1], r.out[2], j, lut);
+      grain_apply_tile<T, T, 1>(r.h, r.s, r.in[0], r.in[1], r.in[2], r.out[0], r.out[1], r.out[2], j, lut);
     else
-      grain_apply_tile_unaligned<T>(r.h, r.s, r.in[0], r.in[1], r.in[2], r.out[0], r.out[1], r.out[2], j, lut);
+      grain_apply_tile_unaligned<T, T>(r.h, r.s, r.in[0], r.in[1], r.in[2], r.out[0], r.out[1], r.out[2], j, lut);
   });
 }
 
@@ -738,12 +739,10 @@ __global__ __launch_bounds__(256) void grain_apply_batch(BatchWork w) {
 int apply_tiles(const GrainHdr &h) { return ((h.w + 63) / 64) * ((h.h + 31) / 32); }
 
 /* Every plane row in and out starts on a segment boundary (8 samples). */
-bool rows_aligned(const GrainHdr &h, const PlanePtr in[3], const PlanePtr out[3], int bps) {
-  const uintptr_t a = 8 * (uintptr_t)bps;
+bool rows_aligned(const GrainHdr &h, const PlanePtr in[3], const PlanePtr out[3], int in_bps, int out_bps) {
   for (int p = 0; p < h.nplanes; p++) {
-    const PlanePtr *q[2] = {&in[p], &out[p]};
-    for (const PlanePtr *x : q)
-      if (((uintptr_t)x->data % a) != 0 || (x->stride % 8) != 0) return false;
+    if (((uintptr_t)in[p].data % (8 * (uintptr_t)in_bps)) != 0 || (in[p].stride % 8) != 0) return false;
+    if (((uintptr_t)out[p].data % (8 * (uintptr_t)out_bps)) != 0 || (out[p].stride % 8) != 0) return false;
   }
   return true;
 }
@@ -781,7 +780,7 @@ int grain_apply_plan(const GrainHdr &h, const GrainScratch &s, const PlanePtr in
     r.in[p] = in[p];
     r.out[p] = out[p];
   }
-  r.aligned = rows_aligned(h, in, out, bps);
+  r.aligned = rows_aligned(h, in, out, bps, bps);
   return apply_tiles(h);
 }
 
@@ -794,25 +793,29 @@ int launch_grain_apply_batch(const BatchWork &w, const StagePlan &sp, int bps, h
   return (int)hipGetLastError();
 }
 
-int launch_grain_apply(const GrainHdr &h, const GrainScratch &s, const PlanePtr in[3],
-                       PlanePtr out[3], int bps, hipStream_t st) {
+template <typename T, typename TO>
+void apply_launch(int kind, int grid, const GrainHdr &h, const GrainScratch &s, const PlanePtr in[3],
+                  PlanePtr out[3], hipStream_t st) {
+  if (kind == 1)
+    grain_apply_kernel<T, TO, 1><<<grid, 256, 0, st>>>(h, s, in[0], in[1], in[2], out[0], out[1], out[2]);
+  else if (kind == 2)
+    grain_apply_kernel<T, TO, 2><<<grid, 256, 0, st>>>(h, s, in[0], in[1], in[2], out[0], out[1], out[2]);
+  else
+    grain_apply_kernel<T, TO, 0><<<grid, 256, 0, st>>>(h, s, in[0], in[1], in[2], out[0], out[1], out[2]);
+}
+
+int launch_grain_apply(const GrainHdr &h, const GrainScratch &s, const PlanePtr in[3], PlanePtr out[3], int bps,
+                       hipStream_t st, int out_bps) {
+  if (out_bps == 0) out_bps = bps;
+  if (out_bps > bps) return (int)hipErrorInvalidValue;
   const int grid = apply_tiles(h);
-  const int kind = !rows_aligned(h, in, out, bps) ? 0 : (h.nplanes > 1 && !h.ssx ? 2 : 1);
-  if (bps == 1) {
-    if (kind == 1)
-      grain_apply_kernel<uint8_t, 1><<<grid, 256, 0, st>>>(h, s, in[0], in[1], in[2], out[0], out[1], out[2]);
-    else if (kind == 2)
-      grain_apply_kernel<uint8_t, 2><<<grid, 256, 0, st>>>(h, s, in[0], in[1], in[2], out[0], out[1], out[2]);
-    else
-      grain_apply_kernel<uint8_t, 0><<<grid, 256, 0, st>>>(h, s, in[0], in[1], in[2], out[0], out[1], out[2]);
-  } else {
-    if (kind == 1)
-      grain_apply_kernel<uint16_t, 1><<<grid, 256, 0, st>>>(h, s, in[0], in[1], in[2], out[0], out[1], out[2]);
-    else if (kind == 2)
-      grain_apply_kernel<uint16_t, 2><<<grid, 256, 0, st>>>(h, s, in[0], in[1], in[2], out[0], out[1], out[2]);
-    else
-      grain_apply_kernel<uint16_t, 0><<<grid, 256, 0, st>>>(h, s, in[0], in[1], in[2], out[0], out[1], out[2]);
-  }
+  const int kind = !rows_aligned(h, in, out, bps, out_bps) ? 0 : (h.nplanes > 1 && !h.ssx ? 2 : 1);
+  if (bps == 1)
+    apply_launch<uint8_t, uint8_t>(kind, grid, h, s, in, out, st);
+  else if (out_bps == 1)
+    apply_launch<uint16_t, uint8_t>(kind, grid, h, s, in, out, st);
+  else
+    apply_launch<uint16_t, uint16_t>(kind, grid, h, s, in, out, st);
   return (int)hipGetLastError();
 }
 

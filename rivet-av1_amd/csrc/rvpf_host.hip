@@ -548,6 +548,26 @@ int grain_prepare(rvpf_ctx *ctx, const rvpf_frame_params *p, hipStream_t prep, i
   return RVPF_OK;
 }
 
+/* the display planes hold the reference planes' sample size, or bytes
+ * from 16-bit planes at 8 bits */
+bool display_bps_ok(const rvpf_frame_params *p, int ref_bps, int disp_bps) {
+  return disp_bps == ref_bps || (p->bit_depth == 8 && ref_bps == 2 && disp_bps == 1);
+}
+
+/* Frame::new (src/frame.rs:161-190): Y, U, V tightly packed, one after another */
+void packed_layout(const rvpf_frame_params *p, rvpf_packed_layout *l) {
+  memset(l, 0, sizeof *l);
+  l->num_planes = nplanes_of(p);
+  l->bytes_per_sample = p->bit_depth > 8 ? 2 : 1;
+  int64_t off = 0;
+  for (int i = 0; i < l->num_planes; i++) {
+    crop(p, i, &l->width[i], &l->height[i]);
+    l->offset[i] = off;
+    off += (int64_t)l->width[i] * l->height[i] * l->bytes_per_sample;
+  }
+  l->bytes = off;
+}
+
 /* display output: crop (+ grain with the templates of `slot`) of `ref` */
 int run_output(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_planes *ref, rvpf_planes *disp,
                hipStream_t st, int slot) {
@@ -561,7 +581,9 @@ int run_output(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_planes *ref
     if ((e = hipStreamWaitEvent(st, ctx->ev_ready[slot], 0)) != hipSuccess) return hip_fail(ctx, e, "stream wait");
     {
       StageMark m(ctx, RVPF_STAGE_GRAIN_APPLY, st);
-      if ((rc = check_launch(ctx, launch_grain_apply(make_grain(p), ctx->grain[slot], in, out, ref->bytes_per_sample, st),
+      if ((rc = check_launch(ctx,
+                             launch_grain_apply(make_grain(p), ctx->grain[slot], in, out, ref->bytes_per_sample, st,
+                                                disp->bytes_per_sample),
                              "grain apply")))
         return rc;
     }
@@ -571,7 +593,8 @@ int run_output(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_planes *ref
   int w[3] = {0, 0, 0}, h[3] = {0, 0, 0};
   for (int i = 0; i < g.nplanes; i++) crop(p, i, &w[i], &h[i]);
   StageMark m(ctx, RVPF_STAGE_COPY, st);
-  return check_launch(ctx, launch_copy_planes(g, in, out, w, h, ref->bytes_per_sample, st), "display copy");
+  return check_launch(
+      ctx, launch_copy_planes(g, in, out, w, h, ref->bytes_per_sample, st, disp->bytes_per_sample), "display copy");
 }
 
 /* The chain after deblocking with super-resolution (decode_frame_wrapup,
@@ -849,12 +872,51 @@ int rvpf_grain(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_planes *ref
   if ((rc = validate_params(p, ctx->err)) || (rc = validate_planes(p, ref, "ref", 1, 0, ctx->err)) ||
       (rc = validate_planes(p, display, "display", 1, 0, ctx->err)))
     return rc;
-  if (ref->bytes_per_sample != display->bytes_per_sample)
-    return fail(ctx, RVPF_E_INVALID, "ref and display sample sizes differ");
+  if (!display_bps_ok(p, ref->bytes_per_sample, display->bytes_per_sample))
+    return fail(ctx, RVPF_E_INVALID, "ref and display sample sizes differ (bytes from 16-bit samples only at 8 bits)");
   hipStream_t st = (hipStream_t)stream;
   int slot = 0;
   if (grain_on(p) && (rc = grain_prepare(ctx, p, st, &slot))) return rc;
   return run_output(ctx, p, ref, display, st, slot);
+}
+
+int rvpf_packed_layout_of(const rvpf_frame_params *p, rvpf_packed_layout *out) {
+  int rc = validate_params(p, g_err);
+  if (rc) return rc;
+  if (!out) return fail(nullptr, RVPF_E_INVALID, "out is NULL");
+  packed_layout(p, out);
+  return RVPF_OK;
+}
+
+int rvpf_output_packed(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_planes *ref, void *packed,
+                       int64_t bytes, void *stream) {
+  if (!ctx) return fail(nullptr, RVPF_E_INVALID, "ctx is NULL");
+  int rc;
+  if ((rc = validate_params(p, ctx->err)) || (rc = validate_planes(p, ref, "ref", 1, 0, ctx->err))) return rc;
+  rvpf_packed_layout l;
+  packed_layout(p, &l);
+  if (!packed || bytes < l.bytes) {
+    snprintf(ctx->err, sizeof ctx->err, "packed picture: needs %lld bytes, has %lld", (long long)l.bytes,
+             (long long)bytes);
+    return RVPF_E_INVALID;
+  }
+  if (((uintptr_t)packed & (l.bytes_per_sample - 1)) != 0)
+    return fail(ctx, RVPF_E_INVALID, "packed picture not aligned to its sample size");
+  if (!display_bps_ok(p, ref->bytes_per_sample, l.bytes_per_sample))
+    return fail(ctx, RVPF_E_INVALID, "ref and packed sample sizes differ");
+  rvpf_planes disp;
+  memset(&disp, 0, sizeof disp);
+  disp.bytes_per_sample = l.bytes_per_sample;
+  for (int i = 0; i < l.num_planes; i++) {
+    disp.data[i] = (char *)packed + l.offset[i];
+    disp.stride[i] = l.width[i];
+    disp.width[i] = l.width[i];
+    disp.height[i] = l.height[i];
+  }
+  hipStream_t st = (hipStream_t)stream;
+  int slot = 0;
+  if (grain_on(p) && (rc = grain_prepare(ctx, p, ctx->side, &slot))) return rc;
+  return run_output(ctx, p, ref, &disp, st, slot);
 }
 
 int rvpf_postfilter_frame(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_side_info *s, rvpf_planes *recon,
@@ -867,7 +929,7 @@ int rvpf_postfilter_frame(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_
     return rc;
   if (display && (rc = validate_planes(p, display, "display", 1, 0, ctx->err))) return rc;
   const int bps = recon->bytes_per_sample;
-  if (ref_out->bytes_per_sample != bps || (display && display->bytes_per_sample != bps))
+  if (ref_out->bytes_per_sample != bps || (display && !display_bps_ok(p, bps, display->bytes_per_sample)))
     return fail(ctx, RVPF_E_INVALID, "plane sample sizes differ");
   hipStream_t st = (hipStream_t)stream;
   const Geom g = make_geom(p);

@@ -264,11 +264,13 @@ int launch_cdef(const Geom &g, const CdefHdr &h, const rvpf_mi *mi, int mi_strid
                 PlanePtr out[3], int bps, hipStream_t st);
 int launch_lr(const Geom &g, const LrHdr &h, const PlanePtr deblocked[3], const PlanePtr cdef[3],
               PlanePtr out[3], int bps, hipStream_t st);
+/* out_bps: bytes per output sample when it differs from the input's
+ * (1 from 2: 8-bit frames written as bytes); 0 = the same. */
 int launch_copy_planes(const Geom &g, const PlanePtr in[3], PlanePtr out[3], const int w[3],
-                       const int h[3], int bps, hipStream_t st);
+                       const int h[3], int bps, hipStream_t st, int out_bps = 0);
 int launch_grain_prepare(const GrainHdr &h, const GrainScratch &s, hipStream_t st);
 int launch_grain_apply(const GrainHdr &h, const GrainScratch &s, const PlanePtr in[3],
-                       PlanePtr out[3], int bps, hipStream_t st);
+                       PlanePtr out[3], int bps, hipStream_t st, int out_bps = 0);
 
 /* One plane of a super-resolution upscaling launch: rows 0..rows-1 and
  * columns 0..cols-1 of `out` from `in` (plane index 0..2). */

@@ -211,6 +211,17 @@ class RvpfLrdItem(C.Structure):
     ]
 
 
+class RvpfPackedLayout(C.Structure):
+    _fields_ = [
+        ("offset", C.c_int64 * 3),
+        ("width", C.c_int32 * 3),
+        ("height", C.c_int32 * 3),
+        ("num_planes", C.c_int32),
+        ("bytes_per_sample", C.c_int32),
+        ("bytes", C.c_int64),
+    ]
+
+
 TUNE_STAGES = ["deblock", "cdef", "lr", "grain_prepare", "grain_apply", "copy"]
 TUNE_SHAPES = [3, 1, 1, 1, 1, 1]
 
@@ -231,6 +242,7 @@ STRUCT_SIZES = [
     C.sizeof(RvpfBatchFrame),
     C.sizeof(RvpfTileConfig),
     C.sizeof(RvpfLrdItem),
+    C.sizeof(RvpfPackedLayout),
 ]
 
 
@@ -252,6 +264,8 @@ RVPF_EXPORTS = [
     "rvpf_upscale",
     "rvpf_lr",
     "rvpf_grain",
+    "rvpf_packed_layout_of",
+    "rvpf_output_packed",
     "rvpf_postfilter_frame",
     "rvpf_batch_create",
     "rvpf_batch_destroy",

@@ -45,6 +45,9 @@ def lib():
                               P(abi.RvpfPlanes), P(abi.RvpfPlanes), C.c_void_p]
         L.rvpf_grain.argtypes = [C.c_void_p, P(abi.RvpfFrameParams), P(abi.RvpfPlanes), P(abi.RvpfPlanes), C.c_void_p]
         L.rvpf_upscale.argtypes = [C.c_void_p, P(abi.RvpfFrameParams), P(abi.RvpfPlanes), P(abi.RvpfPlanes), C.c_void_p]
+        L.rvpf_packed_layout_of.argtypes = [P(abi.RvpfFrameParams), P(abi.RvpfPackedLayout)]
+        L.rvpf_output_packed.argtypes = [C.c_void_p, P(abi.RvpfFrameParams), P(abi.RvpfPlanes), C.c_void_p,
+                                         C.c_int64, C.c_void_p]
         L.rvpf_postfilter_frame.argtypes = [C.c_void_p, P(abi.RvpfFrameParams), P(abi.RvpfSideInfo),
                                             P(abi.RvpfPlanes), P(abi.RvpfPlanes), P(abi.RvpfPlanes), C.c_void_p]
         L.rvpf_batch_create.argtypes = [C.c_void_p, P(abi.RvpfBatchFrame), C.c_int32, P(C.c_void_p)]
@@ -238,15 +241,58 @@ def upscale(ctx, df, planes, out, stream=None):
                                  _stream_handle(stream)))
 
 
-def grain(ctx, df, ref, display, stream=None):
-    pr, pdsp = planes_struct(ref, df.bps), planes_struct(display, df.bps)
+def grain(ctx, df, ref, display, stream=None, display_bps=None):
+    """display_bps: 1 for byte display planes from 16-bit planes (8 bits)."""
+    pr, pdsp = planes_struct(ref, df.bps), planes_struct(display, display_bps or df.bps)
     ctx.check(lib().rvpf_grain(ctx.h, C.byref(df.frame.params), C.byref(pr), C.byref(pdsp),
                                _stream_handle(stream)))
 
 
-def postfilter_frame(ctx, df, recon, ref_out, display, stream=None):
+def packed_layout(params):
+    """rvpf_packed_layout_of: where each plane sits in the packed picture
+    of Decoder::output (host only)."""
+    lay = abi.RvpfPackedLayout()
+    rc = lib().rvpf_packed_layout_of(C.byref(params), C.byref(lay))
+    if rc != 0:
+        raise RvpfError(rc, lib().rvpf_ctx_last_error(None).decode())
+    return lay
+
+
+def alloc_packed(df):
+    """A device byte buffer for the frame's packed picture."""
+    torch = _torch()
+    return torch.zeros(packed_layout(df.frame.params).bytes, dtype=torch.uint8, device=df.device)
+
+
+def output_packed(ctx, df, ref, packed, stream=None):
+    """Crop + film grain of the reference planes into the packed picture
+    (a 1-D uint8 device tensor)."""
+    pr = planes_struct(ref, df.bps)
+    assert packed.dim() == 1 and packed.is_contiguous()
+    ctx.check(lib().rvpf_output_packed(ctx.h, C.byref(df.frame.params), C.byref(pr), C.c_void_p(packed.data_ptr()),
+                                       packed.numel() * packed.element_size(), _stream_handle(stream)))
+
+
+def frame_md5(packed, params):
+    """MD5 of a packed picture (bytes) as the AOM vectors' .md5 files hash
+    it: the packed planes, and for a monochrome frame two neutral (mid-grey)
+    4:2:0 chroma planes after them (tests/common/mod.rs:101-119 of the
+    reference)."""
+    import hashlib
+    m = hashlib.md5()
+    m.update(bytes(packed))
+    if params.mono_chrome:
+        cw, ch = (params.upscaled_width + 1) // 2, (params.frame_height + 1) // 2
+        mid = 1 << (params.bit_depth - 1)
+        plane = (mid.to_bytes(2, "little") if params.bit_depth > 8 else bytes([mid])) * (cw * ch)
+        m.update(plane)
+        m.update(plane)
+    return m.hexdigest()
+
+
+def postfilter_frame(ctx, df, recon, ref_out, display, stream=None, display_bps=None):
     pr, po = planes_struct(recon, df.bps), planes_struct(ref_out, df.bps)
-    pdsp = planes_struct(display, df.bps) if display is not None else None
+    pdsp = planes_struct(display, display_bps or df.bps) if display is not None else None
     ctx.check(lib().rvpf_postfilter_frame(ctx.h, C.byref(df.frame.params), C.byref(df.side), C.byref(pr),
                                           C.byref(po), C.byref(pdsp) if pdsp is not None else None,
                                           _stream_handle(stream)))
