@@ -313,6 +313,55 @@ int rvpf_postfilter_frame(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_
                           void *stream);
 
 /*
+ * Encoder filter searches: the reference's search_lf
+ * (src/encoder/mod.rs:1460-1490) and cdef::search (src/encoder/cdef.rs:52-253),
+ * which choose a frame's loop filter levels and CDEF strengths on the first
+ * pass's reconstruction against the source. The filtering and error sums
+ * run on the GPU with the decoder's kernels; the calls are synchronous
+ * (they wait for `stream`) and return host results.
+ */
+
+/*
+ * Loop filter levels: `recon` (not modified) deblocked with the start
+ * levels p->loop_filter_level scaled by 0, 0.5, 0.75, 1, 1.25 and 1.5
+ * (rounded half away from zero, at most 63), each plane's squared error
+ * against `source` over the frame, and for each plane the first scale with
+ * the least error; chroma is off when luma is. levels = {luma, luma, cb,
+ * cr}. The other loop filter fields of p (sharpness, deltas, segment
+ * features) are used as they are.
+ */
+int rvpf_lf_search(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_side_info *s,
+                   const rvpf_planes *recon, const rvpf_planes *source, int32_t levels[4], void *stream);
+
+/* The CDEF parameters cdef::search chooses (cdef_params() of the header). */
+typedef struct rvpf_cdef_search_result {
+  int32_t damping_minus_3; /* min(base_q_idx >> 6, 3) */
+  int32_t bits;            /* cdef_bits: 1 << bits strength pairs */
+  int32_t y_pri[8], y_sec[8]; /* as coded: secondary 3 means strength 4 */
+  int32_t uv_pri[8], uv_sec[8];
+} rvpf_cdef_search_result;
+
+/*
+ * CDEF strengths: every candidate (luma: primary {0,1,2,4,6,9,12}, or
+ * {0,1,2,3,4,5,6,8,10,12,15} when `thorough`, x secondary {0,1,2,3};
+ * chroma: {0,1,2,4,7} or {0,1,2,3,4,6,8,11} x {0,1,2}) filters every 8x8
+ * block CDEF would filter in `deblocked`, with the decoder's direction
+ * search and filter, and its squared error against `source` (inside the
+ * frame) is summed per 64x64 block. Up to eight (luma, chroma) pairs are
+ * then chosen greedily, for 0..3 bits, minimising the total error plus
+ * lambda * bits per block, and each block takes the pair with its least
+ * error. The 64x64 blocks that code an index are those with s->cdef_idx
+ * >= 0 (device memory, layout (r >> 4) * cdef_stride + (c >> 4));
+ * `cdef_idx_out` (host memory, the same layout and size) receives them
+ * with their chosen index, the other entries as in s->cdef_idx.
+ * p's CDEF fields are not read.
+ */
+int rvpf_cdef_search(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_side_info *s,
+                     const rvpf_planes *deblocked, const rvpf_planes *source, int32_t base_q_idx,
+                     double lambda, int32_t thorough, rvpf_cdef_search_result *out, int8_t *cdef_idx_out,
+                     void *stream);
+
+/*
  * Batched multi-stream path (no counterpart in the reference, whose
  * decoder filters one frame at a time; for servers and transcoders that
  * decode many streams at once).

@@ -56,6 +56,13 @@ def lib():
         L.orc_unit_sse.restype = C.c_uint64
         L.orc_lr_search_unit.argtypes = [P(abi.RvpfFrameParams), OP, OP, OP, C.c_int, C.c_int, C.c_int, C.c_double,
                                          C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
+        L.orc_cdef_search.argtypes = [P(abi.RvpfFrameParams), P(abi.RvpfSideInfo), OP, OP, C.c_int, C.c_double,
+                                      C.c_int, C.c_int, P(abi.RvpfCdefSearchResult), C.c_void_p, C.c_void_p,
+                                      C.c_void_p, C.c_void_p]
+        L.orc_cdef_candidates.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                          C.c_void_p]
+        L.orc_lf_search.argtypes = [P(abi.RvpfFrameParams), P(abi.RvpfSideInfo), OP, OP, C.c_int, C.c_void_p,
+                                    C.c_void_p]
         _lib = L
     return _lib
 
@@ -214,3 +221,44 @@ class LrSearchFrame:
         lib().orc_lr_search_unit(*self._args(), plane, ur, uc, lam, st.ctypes.data, len(st), step, out.ctypes.data,
                                  C.byref(cost))
         return out, cost.value
+
+
+def cdef_candidates(thorough):
+    """([(pri, sec)] luma, [(pri, sec)] chroma) of the encoder's CDEF search."""
+    a = [np.zeros(44, dtype=np.int32), np.zeros(44, dtype=np.int32), np.zeros(24, dtype=np.int32),
+         np.zeros(24, dtype=np.int32)]
+    ny, nuv = C.c_int(), C.c_int()
+    lib().orc_cdef_candidates(int(thorough), *[x.ctypes.data for x in a], C.byref(ny), C.byref(nuv))
+    return ([(int(a[0][i]), int(a[1][i])) for i in range(ny.value)],
+            [(int(a[2][i]), int(a[3][i])) for i in range(nuv.value)])
+
+
+def cdef_search(frame, deblocked, source, base_q_idx, lam, thorough, threads=1):
+    """cdef::search: (RvpfCdefSearchResult, cdef_idx table, per-block luma
+    errors [nb, ny], chroma errors [nb, nuv])."""
+    ycand, uvcand = cdef_candidates(thorough)
+    g = frame.geom
+    maxb = ((g.mi_rows + 15) // 16) * ((g.mi_cols + 15) // 16)
+    ys = np.zeros(maxb * len(ycand), dtype=np.uint64)
+    uvs = np.zeros(maxb * len(uvcand), dtype=np.uint64)
+    table = frame.cdef_idx.copy()  # rows past the 64x64 grid keep their value
+    res = abi.RvpfCdefSearchResult()
+    nb = C.c_int()
+    s = frame.host_side_info()
+    pd, ps = planes_struct(deblocked), planes_struct(source)
+    lib().orc_cdef_search(C.byref(frame.params), C.byref(s), C.byref(pd), C.byref(ps), base_q_idx, lam,
+                          int(thorough), threads, C.byref(res), table.ctypes.data, ys.ctypes.data, uvs.ctypes.data,
+                          C.byref(nb))
+    n = nb.value
+    return res, table, ys[:n * len(ycand)].reshape(n, len(ycand)), uvs[:n * len(uvcand)].reshape(n, len(uvcand))
+
+
+def lf_search(frame, recon, source, threads=1):
+    """search_lf: (levels [4], squared errors [6 scales, 3 planes])."""
+    out = np.zeros(4, dtype=np.int32)
+    sse = np.zeros((6, 3), dtype=np.uint64)
+    s = frame.host_side_info()
+    pr, ps = planes_struct(recon), planes_struct(source)
+    lib().orc_lf_search(C.byref(frame.params), C.byref(s), C.byref(pr), C.byref(ps), threads, out.ctypes.data,
+                        sse.ctypes.data)
+    return [int(v) for v in out], sse

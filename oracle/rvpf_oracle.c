@@ -10,6 +10,7 @@
  */
 #include "rvpf_oracle.h"
 
+#include <float.h>
 #include <math.h>
 #include <pthread.h>
 #include <stdatomic.h>
@@ -1149,6 +1150,280 @@ int orc_lr_search_unit(const rvpf_frame_params *f, const orc_planes *deb, const 
     }
   }
   if (cost_out) *cost_out = best;
+  return 0;
+}
+
+/* ---------------------------------------------------------------------
+ * Encoder CDEF search, src/encoder/cdef.rs:52-253, and loop filter level
+ * search, src/encoder/mod.rs:1460-1490
+ * ------------------------------------------------------------------- */
+
+/* The candidate strengths (cdef.rs:60-84), secondary as coded. */
+int orc_cdef_candidates(int thorough, int32_t ypri[44], int32_t ysec[44], int32_t uvpri[24], int32_t uvsec[24],
+                        int *ny, int *nuv) {
+  static const int yp_n[7] = {0, 1, 2, 4, 6, 9, 12}, yp_t[11] = {0, 1, 2, 3, 4, 5, 6, 8, 10, 12, 15};
+  static const int up_n[5] = {0, 1, 2, 4, 7}, up_t[8] = {0, 1, 2, 3, 4, 6, 8, 11};
+  const int *yp = thorough ? yp_t : yp_n, *up = thorough ? up_t : up_n;
+  const int nyp = thorough ? 11 : 7, nup = thorough ? 8 : 5;
+  int k = 0;
+  for (int i = 0; i < nyp; i++)
+    for (int j = 0; j < 4; j++, k++) {
+      ypri[k] = yp[i];
+      ysec[k] = j;
+    }
+  *ny = k;
+  k = 0;
+  for (int i = 0; i < nup; i++)
+    for (int j = 0; j < 3; j++, k++) {
+      uvpri[k] = up[i];
+      uvsec[k] = j;
+    }
+  *nuv = k;
+  return 0;
+}
+
+/* The filtered w x h block at (x0, y0) of a plane into res (row stride w),
+ * as postfilter.rs:449-513 computes it. */
+static void cdef_block_res(const uint16_t *src, int64_t stride, int x0, int y0, int w, int h, int xlim, int ylim,
+                           int pri, int sec, int damping, int dir, int coeff_shift, int32_t *res) {
+  int32_t win[CDEF_WS * CDEF_WS];
+  for (int i = 0; i < CDEF_WS * CDEF_WS; i++) win[i] = CDEF_NA;
+  for (int i = 0; i < h + 4; i++) {
+    const int y = y0 + i - 2;
+    if (y < 0 || y >= ylim) continue;
+    for (int j = 0; j < w + 4; j++) {
+      const int x = x0 + j - 2;
+      if (x >= 0 && x < xlim) win[i * CDEF_WS + j] = src[(int64_t)y * stride + x];
+    }
+  }
+  orc_cdef_filter_window(win, w, h, pri, sec, damping, dir, coeff_shift, res);
+}
+
+/* block_sse (cdef.rs:262-284): squared error inside the frame. */
+static uint64_t res_sse(const int32_t *res, int w, int h, const uint16_t *src, int64_t stride, int x0, int y0,
+                        int fw, int fh) {
+  uint64_t s = 0;
+  const int hh = imin(h, imax(fh - y0, 0)), ww = imin(w, imax(fw - x0, 0));
+  for (int i = 0; i < hh; i++)
+    for (int j = 0; j < ww; j++) {
+      const int64_t d = (int64_t)res[i * w + j] - src[(int64_t)(y0 + i) * stride + x0 + j];
+      s += (uint64_t)(d * d);
+    }
+  return s;
+}
+
+typedef struct {
+  const rvpf_frame_params *p;
+  const rvpf_side_info *s;
+  const orc_planes *deb, *src;
+  const int *blocks; /* fr * fb_cols + fc */
+  int fb_cols, ny, nuv, damping;
+  const int32_t *ypri, *ysec, *uvpri, *uvsec;
+  uint64_t *ysse, *uvsse;
+} csearch_job;
+
+static inline int coded_sec(int s) { return s == 3 ? 4 : s; }
+
+/* One 64x64 block's error for every candidate (cdef.rs:108-197). */
+static void csearch_block(void *arg, int bi) {
+  const csearch_job *j = (const csearch_job *)arg;
+  const rvpf_frame_params *f = j->p;
+  const int fr = j->blocks[bi] / j->fb_cols, fc = j->blocks[bi] % j->fb_cols;
+  const int cs = f->bit_depth - 8;
+  const int xlim = f->mi_cols * 4, ylim = f->mi_rows * 4;
+  int32_t res[64];
+  for (int r = fr * 16; r < imin(fr * 16 + 16, f->mi_rows); r += 2)
+    for (int c = fc * 16; c < imin(fc * 16 + 16, f->mi_cols); c += 2) {
+#define SK(rr, cc) (mi_at(j->s, rr, cc)->seg_skip & 1)
+      if (SK(r, c) && SK(r + 1, c) && SK(r, c + 1) && SK(r + 1, c + 1)) continue;
+#undef SK
+      int32_t var;
+      const int y_dir = orc_cdef_direction(j->deb->data[0], j->deb->stride[0], c * 4, r * 4, f->bit_depth, &var);
+      const int x0 = c * 4, y0 = r * 4;
+      for (int k = 0; k < j->ny; k++) {
+        int pri = j->ypri[k] << cs;
+        const int sec = coded_sec(j->ysec[k]) << cs;
+        const int dir = pri == 0 ? 0 : y_dir;
+        const int var_str = (var >> 6) != 0 ? imin(floor_log2((unsigned)(var >> 6)), 12) : 0;
+        pri = var != 0 ? (pri * (4 + var_str) + 8) >> 4 : 0;
+        cdef_block_res(j->deb->data[0], j->deb->stride[0], x0, y0, 8, 8, xlim, ylim, pri, sec, j->damping + cs, dir,
+                       cs, res);
+        j->ysse[(int64_t)bi * j->ny + k] +=
+            res_sse(res, 8, 8, j->src->data[0], j->src->stride[0], x0, y0, f->frame_width, f->frame_height);
+      }
+      if (num_planes(f) == 1) continue;
+      const int w = 8 >> f->ssx, h = 8 >> f->ssy, cx0 = x0 >> f->ssx, cy0 = y0 >> f->ssy;
+      const int fw = (f->frame_width + f->ssx) >> f->ssx, fh = (f->frame_height + f->ssy) >> f->ssy;
+      for (int k = 0; k < j->nuv; k++) {
+        const int pri = j->uvpri[k] << cs, sec = coded_sec(j->uvsec[k]) << cs;
+        const int dir = pri == 0 ? 0 : AV1_CDEF_UV_DIR[f->ssx][f->ssy][y_dir];
+        uint64_t total = 0;
+        for (int pl = 1; pl < 3; pl++) {
+          cdef_block_res(j->deb->data[pl], j->deb->stride[pl], cx0, cy0, w, h, xlim >> f->ssx, ylim >> f->ssy, pri,
+                         sec, j->damping + cs - 1, dir, cs, res);
+          total += res_sse(res, w, h, j->src->data[pl], j->src->stride[pl], cx0, cy0, fw, fh);
+        }
+        j->uvsse[(int64_t)bi * j->nuv + k] += total;
+      }
+    }
+}
+
+/* best_for (cdef.rs:201-216): the total error of the set and each block's
+ * first best pair in it. */
+static double cdef_best_for(const uint64_t *ys, const uint64_t *uvs, int nb, int ny, int nuv, const int *sy,
+                            const int *su, int n, int *pick) {
+  double total = 0;
+  for (int b = 0; b < nb; b++) {
+    uint64_t best = UINT64_MAX;
+    for (int i = 0; i < n; i++) {
+      const uint64_t v = ys[(int64_t)b * ny + sy[i]] + uvs[(int64_t)b * nuv + su[i]];
+      if (v < best) {
+        best = v;
+        if (pick) pick[b] = i;
+      }
+    }
+    total += (double)best;
+  }
+  return total;
+}
+
+int orc_cdef_search(const rvpf_frame_params *p, const rvpf_side_info *s, const orc_planes *deb,
+                    const orc_planes *src, int base_q_idx, double lambda, int thorough, int threads,
+                    rvpf_cdef_search_result *out, int8_t *table, uint64_t *ysse_out, uint64_t *uvsse_out,
+                    int *nb_out) {
+  int32_t ypri[44], ysec[44], uvpri[24], uvsec[24];
+  int ny, nuv;
+  orc_cdef_candidates(thorough, ypri, ysec, uvpri, uvsec, &ny, &nuv);
+  memset(out, 0, sizeof *out);
+  const int dm3 = imin(base_q_idx >> 6, 3); /* damping_for, cdef.rs:256-258 */
+  const int fb_rows = (p->mi_rows + 15) / 16, fb_cols = (p->mi_cols + 15) / 16;
+  for (int fr = 0; fr < fb_rows; fr++)
+    for (int fc = 0; fc < s->cdef_stride; fc++)
+      table[fr * s->cdef_stride + fc] = s->cdef_idx[fr * s->cdef_stride + fc];
+  int *blocks = (int *)malloc(sizeof(int) * (size_t)fb_rows * fb_cols);
+  int nb = 0;
+  for (int fr = 0; fr < fb_rows; fr++)
+    for (int fc = 0; fc < fb_cols; fc++)
+      if (s->cdef_idx[fr * s->cdef_stride + fc] >= 0) blocks[nb++] = fr * fb_cols + fc;
+  if (nb_out) *nb_out = nb;
+  if (nb == 0) { /* CdefParams::off */
+    free(blocks);
+    return 0;
+  }
+  uint64_t *ys = (uint64_t *)calloc((size_t)nb * ny, sizeof(uint64_t));
+  uint64_t *uvs = (uint64_t *)calloc((size_t)nb * nuv, sizeof(uint64_t));
+  csearch_job j = {p, s, deb, src, blocks, fb_cols, ny, nuv, dm3 + 3, ypri, ysec, uvpri, uvsec, ys, uvs};
+  parallel_map(nb, threads, csearch_block, &j);
+  /* the greedy choice (cdef.rs:217-236) */
+  int sy[8], su[8], n = 0, best_n = 0;
+  int *pick = (int *)calloc((size_t)nb, sizeof(int)), *best_pick = (int *)calloc((size_t)nb, sizeof(int));
+  double best_cost = DBL_MAX;
+  for (int bits = 0; bits <= 3; bits++) {
+    while (n < (1 << bits)) {
+      double add_t = DBL_MAX;
+      int ay = 0, au = 0;
+      for (int yk = 0; yk < ny; yk++)
+        for (int uk = 0; uk < nuv; uk++) {
+          int in = 0;
+          for (int i = 0; i < n; i++) in |= sy[i] == yk && su[i] == uk;
+          if (in) continue;
+          sy[n] = yk;
+          su[n] = uk;
+          const double t = cdef_best_for(ys, uvs, nb, ny, nuv, sy, su, n + 1, NULL);
+          if (t < add_t) {
+            add_t = t;
+            ay = yk;
+            au = uk;
+          }
+        }
+      sy[n] = ay;
+      su[n] = au;
+      n++;
+    }
+    const double t = cdef_best_for(ys, uvs, nb, ny, nuv, sy, su, n, pick);
+    const double cost = t + lambda * (double)bits * (double)nb;
+    if (cost < best_cost) {
+      best_cost = cost;
+      best_n = n;
+      memcpy(best_pick, pick, sizeof(int) * (size_t)nb);
+    }
+  }
+  out->damping_minus_3 = dm3;
+  out->bits = best_n == 1 ? 0 : (best_n == 2 ? 1 : (best_n == 4 ? 2 : 3));
+  for (int i = 0; i < best_n; i++) {
+    out->y_pri[i] = ypri[sy[i]];
+    out->y_sec[i] = ysec[sy[i]];
+    out->uv_pri[i] = uvpri[su[i]];
+    out->uv_sec[i] = uvsec[su[i]];
+  }
+  for (int b = 0; b < nb; b++)
+    table[(blocks[b] / fb_cols) * s->cdef_stride + blocks[b] % fb_cols] = (int8_t)best_pick[b];
+  if (ysse_out) memcpy(ysse_out, ys, sizeof(uint64_t) * (size_t)nb * ny);
+  if (uvsse_out) memcpy(uvsse_out, uvs, sizeof(uint64_t) * (size_t)nb * nuv);
+  free(blocks);
+  free(ys);
+  free(uvs);
+  free(pick);
+  free(best_pick);
+  return 0;
+}
+
+/* plane_sse (mod.rs:1492-1505): squared error over the frame. */
+static uint64_t plane_sse(const rvpf_frame_params *p, const orc_planes *a, const orc_planes *b, int pl) {
+  int sx, sy;
+  plane_ss(p, pl, &sx, &sy);
+  const int w = (p->frame_width + sx) >> sx, h = (p->frame_height + sy) >> sy;
+  uint64_t s = 0;
+  for (int y = 0; y < h; y++)
+    for (int x = 0; x < w; x++) {
+      const int64_t d = (int64_t)a->data[pl][(int64_t)y * a->stride[pl] + x] - b->data[pl][(int64_t)y * b->stride[pl] + x];
+      s += (uint64_t)(d * d);
+    }
+  return s;
+}
+
+int orc_lf_search(const rvpf_frame_params *p, const rvpf_side_info *s, const orc_planes *recon,
+                  const orc_planes *src, int threads, int32_t out[4], uint64_t sse[6][3]) {
+  static const double scales[6] = {0.0, 0.5, 0.75, 1.0, 1.25, 1.5};
+  uint64_t best_e[3] = {UINT64_MAX, UINT64_MAX, UINT64_MAX};
+  int best_l[3] = {0, 0, 0};
+  orc_planes work;
+  for (int pl = 0; pl < 3; pl++) {
+    work.data[pl] = NULL;
+    work.stride[pl] = 0;
+  }
+  int ext_w[3] = {0, 0, 0}, ext_h[3] = {0, 0, 0};
+  for (int pl = 0; pl < num_planes(p); pl++) {
+    alloc_extent(p, pl, &ext_w[pl], &ext_h[pl]);
+    work.stride[pl] = ext_w[pl];
+    work.data[pl] = (uint16_t *)malloc(sizeof(uint16_t) * (size_t)ext_w[pl] * ext_h[pl]);
+  }
+  rvpf_frame_params q = *p;
+  for (int k = 0; k < 6; k++) {
+    for (int i = 0; i < 4; i++) {
+      const int l = (int)round((double)p->loop_filter_level[i] * scales[k]);
+      q.loop_filter_level[i] = imin(l, 63);
+    }
+    for (int pl = 0; pl < num_planes(p); pl++)
+      copy_plane_rows(work.data[pl], work.stride[pl], recon->data[pl], recon->stride[pl], ext_w[pl], ext_h[pl]);
+    if (q.loop_filter_level[0] != 0 || q.loop_filter_level[1] != 0) orc_loop_filter(&q, s, &work, threads);
+    for (int pl = 0; pl < num_planes(p); pl++) {
+      const uint64_t e = plane_sse(p, &work, src, pl);
+      if (sse) sse[k][pl] = e;
+      if (e < best_e[pl]) {
+        best_e[pl] = e;
+        best_l[pl] = pl == 0 ? q.loop_filter_level[0] : q.loop_filter_level[1 + pl];
+      }
+    }
+  }
+  for (int pl = 0; pl < 3; pl++) free(work.data[pl]);
+  if (best_l[0] == 0) {
+    out[0] = out[1] = out[2] = out[3] = 0;
+  } else {
+    out[0] = out[1] = best_l[0];
+    out[2] = best_l[1];
+    out[3] = best_l[2];
+  }
   return 0;
 }
 

@@ -94,6 +94,25 @@ int orc_lr_search_unit(const rvpf_frame_params *f, const orc_planes *deb, const 
                        const orc_planes *src, int plane, int ur, int uc, double lambda, const int *sets,
                        int nsets, int step, rvpf_lr_unit *out, double *cost_out);
 
+/* The encoder's CDEF search candidates (cdef.rs:60-84): luma (pri, sec)
+ * and chroma (pri, sec), secondary as coded (3 = strength 4). */
+int orc_cdef_candidates(int thorough, int32_t ypri[44], int32_t ysec[44], int32_t uvpri[24], int32_t uvsec[24],
+                        int *ny, int *nuv);
+/* cdef::search (src/encoder/cdef.rs:52-253) on the deblocked frame `deb`
+ * against `src`: the chosen parameters, the cdef_idx table (layout of
+ * s->cdef_idx, fb_rows x cdef_stride) and, when not NULL, the per-block
+ * error tables (coded blocks in raster order, ny / nuv per block) and the
+ * number of coded blocks. */
+int orc_cdef_search(const rvpf_frame_params *p, const rvpf_side_info *s, const orc_planes *deb,
+                    const orc_planes *src, int base_q_idx, double lambda, int thorough, int threads,
+                    rvpf_cdef_search_result *out, int8_t *table, uint64_t *ysse_out, uint64_t *uvsse_out,
+                    int *nb_out);
+/* search_lf (src/encoder/mod.rs:1460-1490): levels from the start levels
+ * p->loop_filter_level; sse[scale][plane] when not NULL. recon is not
+ * modified. */
+int orc_lf_search(const rvpf_frame_params *p, const rvpf_side_info *s, const orc_planes *recon,
+                  const orc_planes *src, int threads, int32_t out[4], uint64_t sse[6][3]);
+
 /* Film grain templates (7.18.3.3), grain.rs:40-135: luma 73x82 and
  * cb/cr (rows x 82 used up to the chroma template size), row stride 82. */
 void orc_grain_templates(const rvpf_frame_params *p, int32_t luma[73 * 82], int32_t cb[73 * 82],

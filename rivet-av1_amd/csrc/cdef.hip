@@ -268,6 +268,54 @@ __device__ void dir_costs3(const uint16_t *t, int b0, int rs, int cs, int sh, in
   *c2 = __mul24(c, AV1_DIV_TABLE[8]);
 }
 
+/* Direction and variance (7.15.2) of the 8x8 block of lane tid (block
+ * tid / 4 of the tile at block row by, column bx): its four lanes each run
+ * dir_costs3 through one symmetry of the block, and one shuffle gathers
+ * the eight costs; the first largest wins. Lanes whose block is not
+ * filtered get direction 0 and variance 0 (all four lanes of a block
+ * agree on filt). */
+__device__ __forceinline__ void block_direction(const uint16_t *tile, int tid, bool filt, int by, int bx, int cs,
+                                                int *ydir, int32_t *var) {
+  const int q = tid & 3;
+  int32_t c0 = 0, c1 = 0, c2 = 0;
+  if (filt) {
+    const int base = (by * 8 + 2) * TS + bx * 8 + 2;
+    /* q: 0 identity, 1 horizontal flip, 2 transpose, 3 transpose + flip */
+    const int rs = q == 0 || q == 1 ? TS : (q == 2 ? 1 : -1);
+    const int csx = q == 0 ? 1 : (q == 1 ? -1 : TS);
+    const int b0 = base + (q == 1 || q == 3 ? 7 : 0);
+    dir_costs3(tile, b0, rs, csx, cs, &c0, &c1, &c2);
+  }
+  /* lane q of the block: q0 -> costs 0,1,2; q1 -> 4,3,2; q2 -> 0,7,6; q3 -> 4,5,6 */
+  const int lb = (tid & 63) & ~3;
+  const int32_t k0 = __shfl(c0, lb + 0, 64), k1 = __shfl(c1, lb + 0, 64), k2 = __shfl(c2, lb + 0, 64);
+  const int32_t k4 = __shfl(c0, lb + 1, 64), k3 = __shfl(c1, lb + 1, 64);
+  const int32_t k7 = __shfl(c1, lb + 2, 64), k6 = __shfl(c2, lb + 2, 64);
+  const int32_t k5 = __shfl(c1, lb + 3, 64);
+  *ydir = 0;
+  *var = 0;
+  if (filt) {
+    const int32_t cost[8] = {k0, k1, k2, k3, k4, k5, k6, k7};
+    int32_t best = 0;
+    int d = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+      if (cost[k] > best) {
+        best = cost[k];
+        d = k;
+      }
+    *ydir = d;
+    *var = (best - cost[(d + 4) & 7]) >> 10;
+  }
+}
+
+/* The luma primary strength after the variance adjustment of cdef_block
+ * (postfilter.rs:318-326); pri is already shifted by coeff_shift. */
+__device__ __forceinline__ int adjust_pri(int pri, int32_t var) {
+  const int var_str = (var >> 6) != 0 ? min(floor_log2(var >> 6), 12) : 0;
+  return var != 0 ? (pri * (4 + var_str) + 8) >> 4 : 0;
+}
+
 /* Filters this lane's pairs of one block: rows r0 .. r0 + nrows - 1, the
  * column pair starting at block column c (even); t0 = the block's top-left
  * tile sample (even), o = the block's top-left output sample. Lanes whose
@@ -362,36 +410,10 @@ __device__ __forceinline__ void cdef_tile(const CdefLaunch &a, int tile_id, uint
            mi[(int64_t)r * ms + c + 1].seg_skip & mi[(int64_t)(r + 1) * ms + c + 1].seg_skip & 1;
   }
   const bool filt = inside && !skip;
-  int32_t c0 = 0, c1 = 0, c2 = 0;
-  if (filt) {
-    const int base = (by * 8 + 2) * TS + bx * 8 + 2;
-    /* q: 0 identity, 1 horizontal flip, 2 transpose, 3 transpose + flip */
-    const int rs = q == 0 || q == 1 ? TS : (q == 2 ? 1 : -1);
-    const int csx = q == 0 ? 1 : (q == 1 ? -1 : TS);
-    const int b0 = base + (q == 1 || q == 3 ? 7 : 0);
-    dir_costs3(tile, b0, rs, csx, cs, &c0, &c1, &c2);
-  }
-  /* lane q of the block: q0 -> costs 0,1,2; q1 -> 4,3,2; q2 -> 0,7,6; q3 -> 4,5,6 */
-  const int lb = (tid & 63) & ~3;
-  const int32_t k0 = __shfl(c0, lb + 0, 64), k1 = __shfl(c1, lb + 0, 64), k2 = __shfl(c2, lb + 0, 64);
-  const int32_t k4 = __shfl(c0, lb + 1, 64), k3 = __shfl(c1, lb + 1, 64);
-  const int32_t k7 = __shfl(c1, lb + 2, 64), k6 = __shfl(c2, lb + 2, 64);
-  const int32_t k5 = __shfl(c1, lb + 3, 64);
-  int ydir = 0, pri = 0;
-  if (filt) {
-    const int32_t cost[8] = {k0, k1, k2, k3, k4, k5, k6, k7};
-    int32_t best = 0;
-#pragma unroll
-    for (int k = 0; k < 8; k++)
-      if (cost[k] > best) {
-        best = cost[k];
-        ydir = k;
-      }
-    const int32_t var = (best - cost[(ydir + 4) & 7]) >> 10;
-    pri = a.h.y_pri[idx] << cs;
-    const int var_str = (var >> 6) != 0 ? min(floor_log2(var >> 6), 12) : 0;
-    pri = var != 0 ? (pri * (4 + var_str) + 8) >> 4 : 0;
-  }
+  int ydir;
+  int32_t var;
+  block_direction(tile, tid, filt, by, bx, cs, &ydir, &var);
+  const int pri = filt ? adjust_pri(a.h.y_pri[idx] << cs, var) : 0;
 
   /* luma: lane q filters the column pair 2q, 2q + 1 of its block */
   if (inside) {
@@ -438,6 +460,177 @@ __global__ __launch_bounds__(256) void cdef_batch(BatchWork w) {
   batch_loop(w, [&](int f, int j) { cdef_tile<T>(batch_table<CdefLaunch>(w.rec, f), j, words); });
 }
 
+/* ---- the encoder's strength search (src/encoder/cdef.rs:52-197) ---- */
+
+constexpr int MAX_YC = 44, MAX_UVC = 24;
+
+struct CdefSearchLaunch {
+  Geom g;
+  const rvpf_mi *mi;
+  int mi_stride;
+  const int8_t *cdef_idx;
+  int cdef_stride;
+  PlanePtr in[3], src[3];
+  int fb_cols;
+  int damping; /* cdef_damping (damping_minus_3 + 3) */
+  int ny, nuv;
+  int8_t ypri[MAX_YC], ysec[MAX_YC], uvpri[MAX_UVC], uvsec[MAX_UVC]; /* strengths (secondary 3 -> 4) */
+  uint64_t *ysse, *uvsse; /* [64x64 block][candidate] */
+  int8_t *coded;          /* [64x64 block]: 1 when it codes an index */
+};
+
+/* Two horizontally adjacent samples (even column) as a 16-bit pair. */
+template <typename T>
+__device__ __forceinline__ uint32_t load_pair(const T *p) {
+  if constexpr (sizeof(T) == 2) {
+    return *reinterpret_cast<const uint32_t *>(p);
+  } else {
+    const uint32_t w = *reinterpret_cast<const uint16_t *>(p);
+    return (w & 0xffu) | ((w & 0xff00u) << 8);
+  }
+}
+
+/* The squared error of this lane's filtered pairs (rows r0 .. r0 + n - 1
+ * of the column pair c of the block at tile sample t0) against the source
+ * pairs sp[], components outside the frame masked off. Differences fit 16
+ * bits and 16 squares fit 31, so a 16-bit dot product accumulates them. */
+template <bool PRI, bool SEC>
+__device__ __forceinline__ int pairs_sse(const uint32_t *w, int t0, int c, int r0, int n, const Taps &k,
+                                         const uint32_t *sp, const uint32_t *mask) {
+  int e = 0;
+#pragma unroll 2
+  for (int r = 0; r < n; r++) {
+    const uint32_t f = filter_pair<PRI, SEC>(w, (t0 + (r0 + r) * TS + c) >> 1, k);
+    const uint32_t d = __builtin_bit_cast(uint32_t, as_s2(f) - as_s2(sp[r])) & mask[r];
+    e = __builtin_amdgcn_sdot2(as_s2(d), as_s2(d), e, false);
+  }
+  return e;
+}
+
+__device__ __forceinline__ int pairs_sse_any(bool pri, bool sec, const uint32_t *w, int t0, int c, int r0, int n,
+                                             const Taps &k, const uint32_t *sp, const uint32_t *mask) {
+  if (pri && sec) return pairs_sse<true, true>(w, t0, c, r0, n, k, sp, mask);
+  if (pri) return pairs_sse<true, false>(w, t0, c, r0, n, k, sp, mask);
+  if (sec) return pairs_sse<false, true>(w, t0, c, r0, n, k, sp, mask);
+  return pairs_sse<false, false>(w, t0, c, r0, n, k, sp, mask);
+}
+
+/* Adds the wave's sum of v to *acc (LDS). */
+__device__ __forceinline__ void wave_add(uint64_t *acc, uint32_t v) {
+  uint64_t x = v;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+  if ((threadIdx.x & 63) == 0) atomicAdd(reinterpret_cast<unsigned long long *>(acc), (unsigned long long)x);
+}
+
+/* One 64x64 block: every luma candidate on its 8x8 blocks, then every
+ * chroma candidate on both chroma planes, each candidate's squared error
+ * summed over the block. The direction search, the skip rule and the
+ * filter are the decoder's (cdef_tile); the source pairs of a lane are
+ * loaded once per plane and reused by every candidate. */
+template <typename T>
+__device__ void cdef_search_tile(const CdefSearchLaunch &a, int tile_id, uint32_t *words, uint64_t *acc) {
+  uint16_t *tile = reinterpret_cast<uint16_t *>(words);
+  uint16_t *shifted = reinterpret_cast<uint16_t *>(words + SOFF);
+  const Geom &g = a.g;
+  const int tid = threadIdx.x;
+  const int fbx = tile_id % a.fb_cols, fby = tile_id / a.fb_cols;
+  const int cs = g.bd - 8;
+  if (gptr(a.cdef_idx)[(int64_t)fby * a.cdef_stride + fbx] < 0) {
+    if (tid == 0) gptr(a.coded)[tile_id] = 0;
+    return;
+  }
+  const int nacc = a.ny + a.nuv;
+  if (tid < nacc) acc[tid] = 0;
+  const int xlim = g.mi_cols * 4, ylim = g.mi_rows * 4;
+  const int x0 = fbx * 64, y0 = fby * 64;
+  const int bw = min(64, xlim - x0), bh = min(64, ylim - y0);
+  stage_tile<T>(tile, shifted, pp<T>(a.in[0]), a.in[0].stride, x0, y0, 64, 64, xlim, ylim);
+  __syncthreads();
+
+  const int b = tid >> 2, q = tid & 3;
+  const int by = b >> 3, bx = b & 7;
+  const bool inside = by * 8 < bh && bx * 8 < bw;
+  bool skip = true;
+  if (inside) {
+    const int r = (y0 >> 2) + by * 2, c = (x0 >> 2) + bx * 2;
+    const int ms = a.mi_stride;
+    const rvpf_mi *mi = gptr(a.mi);
+    skip = mi[(int64_t)r * ms + c].seg_skip & mi[(int64_t)(r + 1) * ms + c].seg_skip &
+           mi[(int64_t)r * ms + c + 1].seg_skip & mi[(int64_t)(r + 1) * ms + c + 1].seg_skip & 1;
+  }
+  const bool filt = inside && !skip;
+  int ydir;
+  int32_t var;
+  block_direction(tile, tid, filt, by, bx, cs, &ydir, &var);
+
+  uint32_t sp[8], mask[8];
+  {
+    /* this lane's luma pairs: columns 2q, 2q + 1, rows 0..7 of its block */
+    const int fw = g.width, fh = g.height;
+    const int px = x0 + bx * 8 + 2 * q, py = y0 + by * 8;
+    const T *src = pp<T>(a.src[0]);
+    const int64_t ss = a.src[0].stride;
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+      /* clamped into the mode-info area, which the source covers */
+      sp[r] = load_pair<T>(src + (int64_t)min(py + r, ylim - 1) * ss + min(px, xlim - 2));
+      mask[r] = !filt || py + r >= fh || px >= fw ? 0u : (px + 1 < fw ? 0xffffffffu : 0xffffu);
+    }
+  }
+  const int t0 = (by * 8 + 2) * TS + bx * 8 + 2;
+  for (int k = 0; k < a.ny; k++) {
+    const int pri0 = a.ypri[k] << cs, sec = a.ysec[k] << cs;
+    const Taps tk = make_taps(filt ? adjust_pri(pri0, var) : 0, sec, a.damping + cs, pri0 ? ydir : 0, cs);
+    wave_add(&acc[k], (uint32_t)pairs_sse_any(pri0 != 0, sec != 0, words, t0, 2 * q, 0, 8, tk, sp, mask));
+  }
+
+  if (g.nplanes > 1) {
+    const int ssx = g.ssx, ssy = g.ssy;
+    const int cw = 64 >> ssx, ch = 64 >> ssy;
+    const int bws = 8 >> ssx, bhs = 8 >> ssy;
+    const int ncp = bws >> 1;
+    const int nrows = (bhs * ncp) >> 2;
+    const int cpair = q % ncp, r0 = (q / ncp) * nrows;
+    const int cdir = AV1_CDEF_UV_DIR[ssx][ssy][ydir];
+    const int fw = (g.width + ssx) >> ssx, fh = (g.height + ssy) >> ssy;
+    const int cxl = xlim >> ssx, cyl = ylim >> ssy;
+    const int ct0 = (by * bhs + 2) * TS + bx * bws + 2;
+    for (int p = 1; p < 3; p++) {
+      __syncthreads();
+      stage_tile<T>(tile, shifted, pp<T>(a.in[p]), a.in[p].stride, x0 >> ssx, y0 >> ssy, cw, ch, cxl, cyl);
+      __syncthreads();
+      const int px = (x0 >> ssx) + bx * bws + 2 * cpair, py = (y0 >> ssy) + by * bhs + r0;
+      const T *src = pp<T>(a.src[p]);
+      const int64_t ss = a.src[p].stride;
+#pragma unroll
+      for (int r = 0; r < 8; r++) {
+        sp[r] = load_pair<T>(src + (int64_t)min(py + r, cyl - 1) * ss + min(px, cxl - 2));
+        mask[r] = !filt || r >= nrows || py + r >= fh || px >= fw ? 0u : (px + 1 < fw ? 0xffffffffu : 0xffffu);
+      }
+      for (int k = 0; k < a.nuv; k++) {
+        const int pri = a.uvpri[k] << cs, sec = a.uvsec[k] << cs;
+        const Taps tk = make_taps(pri, sec, a.damping + cs - 1, pri ? cdir : 0, cs);
+        wave_add(&acc[a.ny + k], (uint32_t)pairs_sse_any(pri != 0, sec != 0, words, ct0, 2 * cpair, r0, nrows, tk,
+                                                         sp, mask));
+      }
+    }
+  }
+  __syncthreads();
+  if (tid < a.ny)
+    gptr(a.ysse)[(int64_t)tile_id * a.ny + tid] = acc[tid];
+  else if (tid < nacc)
+    gptr(a.uvsse)[(int64_t)tile_id * a.nuv + tid - a.ny] = acc[tid];
+  if (tid == 0) gptr(a.coded)[tile_id] = 1;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void cdef_search_kernel(CdefSearchLaunch a) {
+  __shared__ uint32_t words[2 * SOFF];
+  __shared__ uint64_t acc[MAX_YC + MAX_UVC];
+  cdef_search_tile<T>(a, xcd_remap(blockIdx.x, gridDim.x), words, acc);
+}
+
 int plan(const Geom &g, const CdefHdr &h, const rvpf_mi *mi, int mi_stride, const int8_t *cdef_idx,
          int cdef_stride, int enabled, const PlanePtr in[3], const PlanePtr out[3], CdefLaunch &a) {
   a.g = g;
@@ -482,6 +675,44 @@ int launch_cdef(const Geom &g, const CdefHdr &h, const rvpf_mi *mi, int mi_strid
     cdef_kernel<uint8_t><<<n, 256, 0, st>>>(a);
   else
     cdef_kernel<uint16_t><<<n, 256, 0, st>>>(a);
+  return (int)hipGetLastError();
+}
+
+int launch_cdef_search(const Geom &g, const rvpf_mi *mi, int mi_stride, const int8_t *cdef_idx, int cdef_stride,
+                       const PlanePtr in[3], const PlanePtr src[3], int damping, const int8_t *ypri,
+                       const int8_t *ysec, int ny, const int8_t *uvpri, const int8_t *uvsec, int nuv, uint64_t *ysse,
+                       uint64_t *uvsse, int8_t *coded, int bps, hipStream_t st) {
+  if (ny < 1 || ny > MAX_YC || nuv < 1 || nuv > MAX_UVC) return (int)hipErrorInvalidValue;
+  CdefSearchLaunch a;
+  a.g = g;
+  a.mi = mi;
+  a.mi_stride = mi_stride;
+  a.cdef_idx = cdef_idx;
+  a.cdef_stride = cdef_stride;
+  for (int p = 0; p < 3; p++) {
+    a.in[p] = in[p];
+    a.src[p] = src[p];
+  }
+  a.fb_cols = (g.mi_cols + 15) / 16;
+  a.damping = damping;
+  a.ny = ny;
+  a.nuv = nuv;
+  for (int k = 0; k < MAX_YC; k++) {
+    a.ypri[k] = k < ny ? ypri[k] : 0;
+    a.ysec[k] = k < ny ? ysec[k] : 0;
+  }
+  for (int k = 0; k < MAX_UVC; k++) {
+    a.uvpri[k] = k < nuv ? uvpri[k] : 0;
+    a.uvsec[k] = k < nuv ? uvsec[k] : 0;
+  }
+  a.ysse = ysse;
+  a.uvsse = uvsse;
+  a.coded = coded;
+  const int n = a.fb_cols * ((g.mi_rows + 15) / 16);
+  if (bps == 1)
+    cdef_search_kernel<uint8_t><<<n, 256, 0, st>>>(a);
+  else
+    cdef_search_kernel<uint16_t><<<n, 256, 0, st>>>(a);
   return (int)hipGetLastError();
 }
 

@@ -38,6 +38,8 @@ struct rvpf_ctx {
   size_t cdef_bytes;
   void *up_buf; /* upscaled CDEF output and deblocked frame (super-resolution) */
   size_t up_bytes;
+  void *search_buf; /* encoder searches: work planes, error tables */
+  size_t search_bytes;
   rvpf::GrainScratch grain[2];
   std::vector<TuneEntry> tune_cache;
   char err[256];
@@ -70,6 +72,13 @@ CdefHdr make_cdef(const rvpf_frame_params *p);
 LrHdr make_lr(const rvpf_frame_params *p, const rvpf_side_info *s);
 GrainHdr make_grain(const rvpf_frame_params *p);
 int check_launch(rvpf_ctx *ctx, int rc, const char *what);
+/* grows a scratch buffer of the context to at least `bytes` (waits for
+ * the device before freeing the old one) */
+int ensure_buf(rvpf_ctx *ctx, void **buf, size_t *cap, size_t bytes, const char *what);
+/* planes of w[i] x h[i] samples one after another from base (rows padded
+ * to 64 samples, planes to 256 bytes); returns the bytes; base NULL only
+ * counts */
+size_t layout_planes(const int w[3], const int h[3], int bps, void *base, PlanePtr out[3]);
 /* bytes of the offsets table of a grain scratch for a w x h frame */
 void grain_offsets_dims(int w, int h, int *stripes, int *blocks);
 

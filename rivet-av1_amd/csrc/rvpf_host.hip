@@ -707,6 +707,7 @@ void rvpf_ctx_destroy(rvpf_ctx *ctx) {
   if (ctx->side) (void)hipStreamSynchronize(ctx->side);
   if (ctx->cdef_buf) (void)hipFree(ctx->cdef_buf);
   if (ctx->up_buf) (void)hipFree(ctx->up_buf);
+  if (ctx->search_buf) (void)hipFree(ctx->search_buf);
   (void)hipDeviceSynchronize();
   for (auto &m : ctx->marks) {
     (void)hipEventDestroy(m.a);

@@ -272,6 +272,15 @@ int launch_grain_prepare(const GrainHdr &h, const GrainScratch &s, hipStream_t s
 int launch_grain_apply(const GrainHdr &h, const GrainScratch &s, const PlanePtr in[3],
                        PlanePtr out[3], int bps, hipStream_t st, int out_bps = 0);
 
+/* The encoder's CDEF search (cdef.hip): for every 64x64 block that codes
+ * an index (cdef_idx >= 0), the squared error against src of each luma
+ * candidate (ypri/ysec strengths, ny) and each chroma candidate (nuv),
+ * into ysse[block * ny + k] / uvsse[block * nuv + k]; coded[block] = 0/1. */
+int launch_cdef_search(const Geom &g, const rvpf_mi *mi, int mi_stride, const int8_t *cdef_idx, int cdef_stride,
+                       const PlanePtr in[3], const PlanePtr src[3], int damping, const int8_t *ypri,
+                       const int8_t *ysec, int ny, const int8_t *uvpri, const int8_t *uvsec, int nuv, uint64_t *ysse,
+                       uint64_t *uvsse, int8_t *coded, int bps, hipStream_t st);
+
 /* One plane of a super-resolution upscaling launch: rows 0..rows-1 and
  * columns 0..cols-1 of `out` from `in` (plane index 0..2). */
 struct UpscaleJob {

@@ -86,7 +86,7 @@ int synth_geometry(const synth_cfg *c, synth_geom *g) {
   return 0;
 }
 
-void synth_struct_sizes(int32_t out[10]) {
+void synth_struct_sizes(int32_t out[11]) {
   out[0] = (int32_t)sizeof(rvpf_frame_params);
   out[1] = (int32_t)sizeof(rvpf_side_info);
   out[2] = (int32_t)sizeof(rvpf_mi);
@@ -97,6 +97,7 @@ void synth_struct_sizes(int32_t out[10]) {
   out[7] = (int32_t)sizeof(rvpf_tile_config);
   out[8] = (int32_t)sizeof(rvpf_lrd_item);
   out[9] = (int32_t)sizeof(rvpf_packed_layout);
+  out[10] = (int32_t)sizeof(rvpf_cdef_search_result);
 }
 
 /* ------------------------------------------------------------- source */

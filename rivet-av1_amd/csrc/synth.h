@@ -67,7 +67,7 @@ int synth_frame(const synth_cfg *c, rvpf_frame_params *p, rvpf_mi *mi, uint8_t *
 
 /* Struct sizes, for binding checks: frame_params, side_info, mi, lr_unit,
  * grain_params, planes. */
-void synth_struct_sizes(int32_t out[10]);
+void synth_struct_sizes(int32_t out[11]);
 
 #ifdef __cplusplus
 }

@@ -222,6 +222,22 @@ class RvpfPackedLayout(C.Structure):
     ]
 
 
+class RvpfCdefSearchResult(C.Structure):
+    _fields_ = [
+        ("damping_minus_3", C.c_int32),
+        ("bits", C.c_int32),
+        ("y_pri", C.c_int32 * 8),
+        ("y_sec", C.c_int32 * 8),
+        ("uv_pri", C.c_int32 * 8),
+        ("uv_sec", C.c_int32 * 8),
+    ]
+
+    def as_tuple(self):
+        n = 1 << self.bits
+        return (self.damping_minus_3, self.bits, [(self.y_pri[i], self.y_sec[i]) for i in range(n)],
+                [(self.uv_pri[i], self.uv_sec[i]) for i in range(n)])
+
+
 TUNE_STAGES = ["deblock", "cdef", "lr", "grain_prepare", "grain_apply", "copy"]
 TUNE_SHAPES = [3, 1, 1, 1, 1, 1]
 
@@ -243,6 +259,7 @@ STRUCT_SIZES = [
     C.sizeof(RvpfTileConfig),
     C.sizeof(RvpfLrdItem),
     C.sizeof(RvpfPackedLayout),
+    C.sizeof(RvpfCdefSearchResult),
 ]
 
 
@@ -267,6 +284,8 @@ RVPF_EXPORTS = [
     "rvpf_packed_layout_of",
     "rvpf_output_packed",
     "rvpf_postfilter_frame",
+    "rvpf_lf_search",
+    "rvpf_cdef_search",
     "rvpf_batch_create",
     "rvpf_batch_destroy",
     "rvpf_batch_autotune",

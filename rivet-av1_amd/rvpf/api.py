@@ -46,6 +46,11 @@ def lib():
         L.rvpf_grain.argtypes = [C.c_void_p, P(abi.RvpfFrameParams), P(abi.RvpfPlanes), P(abi.RvpfPlanes), C.c_void_p]
         L.rvpf_upscale.argtypes = [C.c_void_p, P(abi.RvpfFrameParams), P(abi.RvpfPlanes), P(abi.RvpfPlanes), C.c_void_p]
         L.rvpf_packed_layout_of.argtypes = [P(abi.RvpfFrameParams), P(abi.RvpfPackedLayout)]
+        L.rvpf_lf_search.argtypes = [C.c_void_p, P(abi.RvpfFrameParams), P(abi.RvpfSideInfo), P(abi.RvpfPlanes),
+                                     P(abi.RvpfPlanes), C.c_void_p, C.c_void_p]
+        L.rvpf_cdef_search.argtypes = [C.c_void_p, P(abi.RvpfFrameParams), P(abi.RvpfSideInfo), P(abi.RvpfPlanes),
+                                       P(abi.RvpfPlanes), C.c_int32, C.c_double, C.c_int32,
+                                       P(abi.RvpfCdefSearchResult), C.c_void_p, C.c_void_p]
         L.rvpf_output_packed.argtypes = [C.c_void_p, P(abi.RvpfFrameParams), P(abi.RvpfPlanes), C.c_void_p,
                                          C.c_int64, C.c_void_p]
         L.rvpf_postfilter_frame.argtypes = [C.c_void_p, P(abi.RvpfFrameParams), P(abi.RvpfSideInfo),
@@ -288,6 +293,29 @@ def frame_md5(packed, params):
         m.update(plane)
         m.update(plane)
     return m.hexdigest()
+
+
+def lf_search(ctx, df, recon, source, stream=None):
+    """search_lf on the GPU: the loop filter levels [4] chosen from the
+    frame's start levels (recon is not modified)."""
+    pr, ps = planes_struct(recon, df.bps), planes_struct(source, df.bps)
+    out = (C.c_int32 * 4)()
+    ctx.check(lib().rvpf_lf_search(ctx.h, C.byref(df.frame.params), C.byref(df.side), C.byref(pr), C.byref(ps),
+                                   out, _stream_handle(stream)))
+    return list(out)
+
+
+def cdef_search(ctx, df, deblocked, source, base_q_idx, lam, thorough=False, stream=None):
+    """cdef::search on the GPU: (RvpfCdefSearchResult, cdef_idx table as
+    an int8 numpy array of the frame's cdef_idx size; entries past the
+    64x64 grid's rows are the frame's own)."""
+    pd, ps = planes_struct(deblocked, df.bps), planes_struct(source, df.bps)
+    res = abi.RvpfCdefSearchResult()
+    table = df.frame.cdef_idx.copy()
+    ctx.check(lib().rvpf_cdef_search(ctx.h, C.byref(df.frame.params), C.byref(df.side), C.byref(pd), C.byref(ps),
+                                     int(base_q_idx), float(lam), int(bool(thorough)), C.byref(res),
+                                     table.ctypes.data, _stream_handle(stream)))
+    return res, table
 
 
 def postfilter_frame(ctx, df, recon, ref_out, display, stream=None, display_bps=None):

@@ -32,7 +32,7 @@ def synth_lib():
 
 
 def struct_sizes():
-    out = (C.c_int32 * 10)()
+    out = (C.c_int32 * 11)()
     synth_lib().synth_struct_sizes(out)
     return list(out)
 
