@@ -380,13 +380,127 @@ def bench_lrdiff(args):
     ctx.close()
 
 
+def bench_encsearch(args):
+    """The encoder's loop filter level search and CDEF strength search on
+    the GPU (search_lf + cdef::search, normal effort) on 4K 10-bit frames:
+    the searches on each frame's first-pass reconstruction, wall clock per
+    frame (the searches return host results, so each call waits)."""
+    import numpy as np
+    import torch
+    from rvpf import api, workload
+    torch.cuda.set_device(0)
+    nf = args.frames
+    w, h, bd = args.width, args.height, args.bit_depth
+    log("generating %d frames %dx%d %d-bit" % (nf, w, h, bd))
+    frames = [workload.Frame(w, h, bit_depth=bd, ssx=1, ssy=1, sb_size=128, qindex=QINDEX[i % 4], seed=args.seed,
+                             frame_index=i, pan_px=1.5, lr=0, lf_extras=True) for i in range(nf)]
+    ctx = api.Context()
+    stream = torch.cuda.current_stream()
+    items = []
+    for f in frames:
+        df = api.DeviceFrame(f)
+        rec = df.upload_planes(f.rec)
+        src = df.upload_planes(f.src)
+        deb = [None if t is None else t.clone() for t in rec]
+        api.deblock(ctx, df, deb, stream)
+        items.append((f, df, rec, deb, src))
+    torch.cuda.synchronize()
+    lams = {q: 0.1 * (ac_q(bd, q) / 8.0) ** 2 for q in set(QINDEX)}
+
+    def run_all():
+        out = []
+        for f, df, rec, deb, src in items:
+            lv = api.lf_search(ctx, df, rec, src, stream)
+            res, table = api.cdef_search(ctx, df, deb, src, f.cfg.qindex, lams[f.cfg.qindex], False, stream)
+            out.append((lv, res, table))
+        return out
+
+    for _ in range(args.warmup):
+        run_all()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        results = run_all()
+    torch.cuda.synchronize()
+    ms_step = (time.perf_counter() - t0) * 1e3 / args.steps
+    px = sum(f.luma_pixels() for f in frames)
+    log("searches: %.3f ms per step (%d frames), %.2f ms/frame" % (ms_step, nf, ms_step / nf))
+    # per-search split on frame 0
+    f, df, rec, deb, src = items[0]
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        api.lf_search(ctx, df, rec, src, stream)
+    t2 = time.perf_counter()
+    for _ in range(args.steps):
+        api.cdef_search(ctx, df, deb, src, f.cfg.qindex, lams[f.cfg.qindex], False, stream)
+    t3 = time.perf_counter()
+    lf_ms, cdef_ms = (t2 - t1) * 1e3 / args.steps, (t3 - t2) * 1e3 / args.steps
+    log("frame 0: lf search %.3f ms, cdef search %.3f ms" % (lf_ms, cdef_ms))
+    check = None
+    cpu = None
+    if not args.no_check or not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+        threads = min(int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1, 16)
+        host_deb = api.to_host(deb)
+        tc = time.perf_counter()
+        want_lv, _ = pyoracle.lf_search(f, [None if a is None else a.copy() for a in f.rec], f.src, threads=threads)
+        want_res, want_table, _, _ = pyoracle.cdef_search(f, host_deb, f.src, f.cfg.qindex, lams[f.cfg.qindex],
+                                                          False, threads=threads)
+        el = time.perf_counter() - tc
+        lv, res, table = results[0]
+        g = f.geom
+        n = ((g.mi_rows + 15) // 16) * g.cdef_stride
+        check = bool(lv == want_lv and res.as_tuple() == want_res.as_tuple()
+                     and np.array_equal(table[:n], want_table[:n]))
+        log("equal to the oracle's choices (frame 0):", check)
+        if not args.no_cpu:
+            cpu = {"value": round(f.luma_pixels() / el / 1e6, 3), "unit": "Mpix/s", "cores": threads, "kind": "port",
+                   "sample": "frame 0 (%dx%d %d-bit): oracle search_lf + cdef::search (%.1f s), %d threads"
+                             % (w, h, bd, el, threads)}
+            log("cpu:", cpu)
+    out = {
+        "metric": "encoder filter search Mpix/s (search_lf + cdef::search, normal effort), %dx%d %d-bit" % (w, h, bd),
+        "value": round(px / (ms_step / 1e3) / 1e6, 1),
+        "unit": "Mpix/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u16",
+        "data": "synthetic",
+        "config": {
+            "workload": "%d frames %dx%d %d-bit 4:2:0 (qindex %s): loop filter level search (6 scales) + CDEF "
+                        "strength search (28 luma x 15 chroma candidates, greedy <= 8 pairs), wall clock"
+                        % (nf, w, h, bd, "/".join(str(q) for q in QINDEX[:nf])),
+            "frames_per_step": nf,
+            "lf_search_ms_frame0": round(lf_ms, 3),
+            "cdef_search_ms_frame0": round(cdef_ms, 3),
+            "equal_to_oracle_frame0": check,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+def ac_q(bd, q):
+    """AC_QLOOKUP[(bd - 8) >> 1][q] through the library (rvpf_ac_q)."""
+    from rvpf import api
+    return api.lib().rvpf_ac_q(bd, q)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=["gop4k", "batch32", "lrdiff"], default="gop4k",
+    ap.add_argument("--workload", choices=["gop4k", "batch32", "lrdiff", "encsearch"], default="gop4k",
                     help="gop4k: the per-frame chain on the 4K 10-bit GOP (default); "
                          "batch32: the batched path against the per-frame drop-in on 32 mixed streams; "
-                         "lrdiff: the encoder's restoration search on 1080p 10-bit frames")
+                         "lrdiff: the encoder's restoration search on 1080p 10-bit frames; "
+                         "encsearch: the encoder's loop filter and CDEF searches on the GOP's frames")
     ap.add_argument("--lr-frames", type=int, default=16)
     ap.add_argument("--streams", type=int, default=32)
     ap.add_argument("--stream-frames", type=int, default=4)
@@ -416,6 +530,9 @@ def main():
         return
     if args.workload == "lrdiff":
         bench_lrdiff(args)
+        return
+    if args.workload == "encsearch":
+        bench_encsearch(args)
         return
 
     import numpy as np
