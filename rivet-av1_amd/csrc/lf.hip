@@ -63,7 +63,11 @@ __device__ __forceinline__ int lf_level(const LfHdr &h, const LfTables &t, const
   const int rf = m.ref_frame0;
   const int mode = m.y_mode;
   const int mode_type = (mode >= 14 && mode != 16 && mode != 24); /* >= NEARESTMV, not GLOBALMV / GLOBAL_GLOBALMV */
-  const int dlf = h.delta_lf_multi ? m.delta_lf[i] : m.delta_lf[0];
+  /* delta_lf[i] (or [0]) by a shift of the four packed bytes, so the
+   * block's mode info stays in registers */
+  const uint32_t dw = (uint32_t)(uint8_t)m.delta_lf[0] | ((uint32_t)(uint8_t)m.delta_lf[1] << 8) |
+                      ((uint32_t)(uint8_t)m.delta_lf[2] << 16) | ((uint32_t)(uint8_t)m.delta_lf[3] << 24);
+  const int dlf = (int)(int8_t)(dw >> (8 * (h.delta_lf_multi ? i : 0)));
   int lvl = clip3(0, 63, dlf + h.level[i]);
   if (h.seg_enabled && ((t.seg_lf_enabled[seg] >> i) & 1)) lvl = clip3(0, 63, t.seg_lf_data[seg][i] + lvl);
   if (h.delta_enabled) {
@@ -315,41 +319,72 @@ __device__ __forceinline__ int edge_decision(const LfLaunch &a, const LfTables &
 
 constexpr int LF_TWORDS = sizeof(LfTables) / 4;
 
-/* LDS of a deblocking workgroup: the tables, and the filtered edges of
- * the block sorted by filter class (local edge index | limit << 8 |
- * blimit << 16 | thresh << 24, unscaled). */
+/* The block's samples staged in LDS (16-bit at every sample size): in the
+ * vertical pass rows [4 ey0, 4 ey0 + 4 EY) x columns [4 ex0 - 8,
+ * 4 ex0 + 4 EX + 8), in the horizontal pass rows [4 ey0 - 8,
+ * 4 ey0 + 4 EY + 8) x columns [4 ex0, 4 ex0 + 4 EX): every sample a
+ * filter of the block reads. */
+template <int PASS, int EX>
+struct LfTile {
+  static constexpr int EY = 256 / EX;
+  static constexpr int W = PASS == 0 ? 4 * EX + 16 : 4 * EX; /* samples per row */
+  static constexpr int R = PASS == 0 ? 4 * EY : 4 * EY + 16;
+  static constexpr int CW = W / 4; /* 4-sample chunks per row */
+  static constexpr int CHUNKS = R * CW;
+  static constexpr int PER = (CHUNKS + 255) / 256;
+};
+
+/* LDS of a deblocking workgroup: the staged samples, the tables, and the
+ * filtered edges of the block sorted by filter class (local edge index |
+ * limit << 8 | blimit << 16 | thresh << 24, unscaled). */
+template <int PASS, int EX>
 struct LfShared {
+  __align__(16) uint16_t tile[LfTile<PASS, EX>::R * LfTile<PASS, EX>::W];
   LfTables t;
   int count[4];
   uint32_t list[4][256];
 };
 
-/* One filter class of the block: one line of an edge per item (a row of
- * 4-sample words in the vertical pass, a column in the horizontal one). */
+/* 4 staged samples (8-byte aligned) */
+__device__ __forceinline__ void lds4(const uint16_t *p, int *o) {
+  const uint2 u = *reinterpret_cast<const uint2 *>(p);
+  o[0] = u.x & 0xffff;
+  o[1] = u.x >> 16;
+  o[2] = u.y & 0xffff;
+  o[3] = u.y >> 16;
+}
+
+/* One filter class of the block: one line of an edge per item (a row in
+ * the vertical pass, a column in the horizontal one), read from the staged
+ * samples; only the filter's footprint is stored to the plane. */
 template <typename T, int PASS, int EX, int C>
-__device__ __forceinline__ void lf_class(const LfShared &sm, T *base, int64_t stride, int ex0, int ey0, int bd) {
+__device__ __forceinline__ void lf_class(const LfShared<PASS, EX> &sm, T *base, int64_t stride, int ex0, int ey0,
+                                         int bd) {
+  typedef LfTile<PASS, EX> TL;
   constexpr int N = LfClass<C>::N, H = N / 2;
   const int sh = bd - 8;
   const int n = sm.count[C];
+  const int tx0 = PASS == 0 ? 4 * ex0 - 8 : 4 * ex0, ty0 = PASS == 0 ? 4 * ey0 : 4 * ey0 - 8;
   if (PASS == 0) {
     for (int it = threadIdx.x; it < 4 * n; it += 256) {
       const uint32_t e = sm.list[C][it >> 2];
       const int li = e & 255;
       const int xp = (ex0 + (li % EX)) * 4, yp = (ey0 + (li / EX)) * 4 + (it & 3);
-      T *row = base + (int64_t)yp * stride + xp;
+      const uint16_t *trow = sm.tile + (yp - ty0) * TL::W + (xp - tx0);
       int s[16];
       if (C == 3) {
-        Io<T>::ld4(row - 8, s);
-        Io<T>::ld4(row + 4, s + 12);
+        lds4(trow - 8, s);
+        lds4(trow + 4, s + 12);
       }
-      Io<T>::ld4(row - 4, s + 4);
-      Io<T>::ld4(row, s + 8);
+      lds4(trow - 4, s + 4);
+      lds4(trow, s + 8);
       int v[N];
 #pragma unroll
       for (int k = 0; k < N; k++) v[k] = s[8 - H + k];
       if (!filter_class<C>(v, ((e >> 8) & 255) << sh, ((e >> 16) & 255) << sh, (e >> 24) << sh, bd)) continue;
 #pragma unroll
       for (int k = 0; k < N; k++) s[8 - H + k] = v[k];
+      T *row = base + (int64_t)yp * stride + xp;
       if (C <= 1) {
         Io<T>::st2(row - 2, s[6], s[7]);
         Io<T>::st2(row, s[8], s[9]);
@@ -363,32 +398,38 @@ __device__ __forceinline__ void lf_class(const LfShared &sm, T *base, int64_t st
       }
     }
   } else {
-    /* one column of an edge per item: the column's samples are 2-byte
-     * loads (the rows were just written by the vertical pass and sit in
-     * the caches), every lane filters one line as in the vertical pass */
     for (int it = threadIdx.x; it < 4 * n; it += 256) {
       const uint32_t e = sm.list[C][it >> 2];
       const int li = e & 255;
       const int xp = (ex0 + (li % EX)) * 4 + (it & 3), yp = (ey0 + (li / EX)) * 4;
-      T *col = base + (int64_t)yp * stride + xp;
+      const uint16_t *tcol = sm.tile + (yp - ty0) * TL::W + (xp - tx0);
       int v[N];
 #pragma unroll
-      for (int k = 0; k < N; k++) v[k] = col[(int64_t)(k - H) * stride];
+      for (int k = 0; k < N; k++) v[k] = tcol[(k - H) * TL::W];
       if (!filter_class<C>(v, ((e >> 8) & 255) << sh, ((e >> 16) & 255) << sh, (e >> 24) << sh, bd)) continue;
       /* rows the filters modify: p1..q1, p2..q2 (8-tap), p5..q5 (16) */
       constexpr int R = C <= 1 ? 2 : (C == 2 ? 3 : 6);
+      T *col = base + (int64_t)yp * stride + xp;
 #pragma unroll
       for (int k = H - R; k < H + R; k++) col[(int64_t)(k - H) * stride] = (T)v[k];
     }
   }
 }
 
-/* Block lb of grid plane gp: EX x (256 / EX) edges. Each thread decides
- * one edge (7.14.2-7.14.5); the filtered edges are gathered into one list
- * per filter class, and each list is then filtered with every lane on the
- * same code. */
+/* Block lb of grid plane gp: EX x (256 / EX) edges. The block's samples
+ * are loaded (every load in flight together with the side info) and staged
+ * in LDS; each thread decides one edge (7.14.2-7.14.5); the filtered edges
+ * are gathered into one list per filter class, and each list is then
+ * filtered with every lane on the same code.
+ *
+ * Staging bounds: a filtered edge of the vertical pass reads columns
+ * x - 8 .. x + 7 of rows inside the mode-info area, one of the horizontal
+ * pass rows y - 8 .. y + 7 of columns inside it; x + 7 and y + 7 stay below
+ * the area plus the 8-sample margin every deblocked plane has. Chunks
+ * outside those bounds are loaded from a clamped position and never read. */
 template <typename T, int PASS, int EX>
-__device__ __forceinline__ void lf_block(const LfLaunch &a, int gp, int lb, LfShared *sm) {
+__device__ __forceinline__ void lf_block(const LfLaunch &a, int gp, int lb, LfShared<PASS, EX> *sm) {
+  typedef LfTile<PASS, EX> TL;
   const int plane = a.plane_id[gp];
   const int ex0 = (lb % a.blk_x[gp]) * EX, ey0 = (lb / a.blk_x[gp]) * (256 / EX);
   const int ex = ex0 + (threadIdx.x % EX), ey = ey0 + (threadIdx.x / EX);
@@ -401,8 +442,31 @@ __device__ __forceinline__ void lf_block(const LfLaunch &a, int gp, int lb, LfSh
   const uint32_t tword = threadIdx.x < LF_TWORDS ? reinterpret_cast<const uint32_t *>(&a.t)[threadIdx.x] : 0u;
   EdgeSide side;
   if (valid) side = load_side(a, plane, PASS, ex, ey);
+  T *base = gptr(reinterpret_cast<T *>(a.planes[plane].data));
+  const int64_t stride = a.planes[plane].stride;
+  const int area_w = (g.mi_cols * 4) >> ssx, area_h = (g.mi_rows * 4) >> ssy;
+  const int tx0 = PASS == 0 ? 4 * ex0 - 8 : 4 * ex0, ty0 = PASS == 0 ? 4 * ey0 : 4 * ey0 - 8;
+  const int cmax = PASS == 0 ? area_w + 4 : area_w - 4, rmax = PASS == 0 ? area_h - 1 : area_h + 7;
+  typename Io<T>::W v[TL::PER];
+#pragma unroll
+  for (int k = 0; k < TL::PER; k++) {
+    const int ch = min((int)threadIdx.x + 256 * k, TL::CHUNKS - 1);
+    const int r = ch / TL::CW, c = (ch - r * TL::CW) * 4;
+    const int y = min(max(ty0 + r, 0), rmax), x = min(max(tx0 + c, 0), cmax);
+    v[k] = Io<T>::ldw(base + (int64_t)y * stride + x);
+  }
   if (threadIdx.x < LF_TWORDS) reinterpret_cast<uint32_t *>(&sm->t)[threadIdx.x] = tword;
   if (threadIdx.x < 4) sm->count[threadIdx.x] = 0;
+#pragma unroll
+  for (int k = 0; k < TL::PER; k++) {
+    const int ch = threadIdx.x + 256 * k;
+    if (ch < TL::CHUNKS) {
+      int o[4];
+      Io<T>::unpack(v[k], o);
+      *reinterpret_cast<uint2 *>(&sm->tile[ch * 4]) =
+          make_uint2((uint32_t)o[0] | ((uint32_t)o[1] << 16), (uint32_t)o[2] | ((uint32_t)o[3] << 16));
+    }
+  }
   __syncthreads();
   if (valid) {
     int limit, blimit, thresh;
@@ -414,8 +478,6 @@ __device__ __forceinline__ void lf_block(const LfLaunch &a, int gp, int lb, LfSh
     }
   }
   __syncthreads();
-  T *base = gptr(reinterpret_cast<T *>(a.planes[plane].data));
-  const int64_t stride = a.planes[plane].stride;
   lf_class<T, PASS, EX, 0>(*sm, base, stride, ex0, ey0, g.bd);
   if (plane) {
     lf_class<T, PASS, EX, 1>(*sm, base, stride, ex0, ey0, g.bd);
@@ -427,7 +489,7 @@ __device__ __forceinline__ void lf_block(const LfLaunch &a, int gp, int lb, LfSh
 
 template <typename T, int PASS>
 __global__ __launch_bounds__(256) void lf_pass_kernel(LfLaunch a) {
-  __shared__ LfShared tsm;
+  __shared__ LfShared<PASS, 64> tsm;
   int lb;
   const int gp = xcd_segment(blockIdx.x, a.blk_start, a.nseg, &lb);
   if (gp >= a.nseg || lb >= a.blk_n[gp]) return; /* padding */
@@ -437,7 +499,7 @@ __global__ __launch_bounds__(256) void lf_pass_kernel(LfLaunch a) {
 /* Batched: item j of a frame is block j of its (unpadded) grid. */
 template <typename T, int PASS, int EX>
 __global__ __launch_bounds__(256) void lf_pass_batch(BatchWork w) {
-  __shared__ LfShared tsm;
+  __shared__ LfShared<PASS, EX> tsm;
   batch_loop(w, [&](int f, int j) {
     const LfLaunch &a = batch_table<LfLaunch>(w.rec, f);
     int gp = 0;
