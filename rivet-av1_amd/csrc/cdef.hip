@@ -113,18 +113,23 @@ typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ s16x2 as_s2(uint32_t w) { return __builtin_bit_cast(s16x2, w); }
 __device__ __forceinline__ s16x2 splat(int v) { return (s16x2){(short)v, (short)v}; }
 
-/* constrain() of 7.15.3: sign(d) * min(|d|, max(0, t - (|d| >> adj))),
- * i.e. d clamped to [-m, m] with m = max(0, t - (|d| >> adj)). */
-__device__ __forceinline__ s16x2 constrain2(s16x2 d, s16x2 t, u16x2 adj) {
-  const u16x2 ad = __builtin_bit_cast(u16x2, __builtin_elementwise_abs(d));
-  const s16x2 m = __builtin_elementwise_max(t - __builtin_bit_cast(s16x2, ad >> adj), splat(0));
-  return __builtin_elementwise_min(__builtin_elementwise_max(d, -m), m);
+/* constrain() of 7.15.3, sign(d) * min(|d|, max(0, t - (|d| >> adj))),
+ * of the tap p against the centre x, all unsigned: the positive and
+ * negative parts of d = p - x are saturating differences (one of them is
+ * 0, so their or is |d|), m = max(0, t - (|d| >> adj)) is a saturating
+ * difference too, and the result is min(pos, m) - min(neg, m). An
+ * unavailable tap (NA_S = 40960) has |d| >= 36865, which keeps m at 0. */
+__device__ __forceinline__ s16x2 constrain2(u16x2 p, u16x2 x, u16x2 t, u16x2 adj) {
+  const u16x2 pos = __builtin_elementwise_sub_sat(p, x), neg = __builtin_elementwise_sub_sat(x, p);
+  const u16x2 m = __builtin_elementwise_sub_sat(t, (pos | neg) >> adj);
+  return __builtin_bit_cast(s16x2, __builtin_elementwise_min(pos, m)) -
+         __builtin_bit_cast(s16x2, __builtin_elementwise_min(neg, m));
 }
 
 /* Per-lane filter set-up: strengths, damping shifts, primary tap weights
  * and the 32-bit word offsets of the 12 taps relative to the pair's word. */
 struct Taps {
-  s16x2 pri, sec;
+  u16x2 pri, sec;
   u16x2 pri_adj, sec_adj;
   s16x2 pt0, pt1;
   int po[4];  /* +d0, -d0, +d1, -d1 */
@@ -144,8 +149,8 @@ __device__ __forceinline__ Taps make_taps(int pri, int sec, int damping, int dir
   k.has_pri = pri != 0;
   k.has_sec = sec != 0;
   k.clamp = k.has_pri && k.has_sec;
-  k.pri = splat(pri);
-  k.sec = splat(sec);
+  k.pri = (u16x2){(unsigned short)pri, (unsigned short)pri};
+  k.sec = (u16x2){(unsigned short)sec, (unsigned short)sec};
   const int pa = pri ? max(damping - floor_log2(pri), 0) : 0;
   const int sa = sec ? max(damping - floor_log2(sec), 0) : 0;
   k.pri_adj = (u16x2){(unsigned short)pa, (unsigned short)pa};
@@ -179,6 +184,7 @@ __device__ __forceinline__ Taps make_taps(int pri, int sec, int damping, int dir
 template <bool PRI, bool SEC>
 __device__ __forceinline__ uint32_t filter_pair(const uint32_t *w, int base, const Taps &k) {
   const s16x2 x = as_s2(w[base]);
+  const u16x2 xu = __builtin_bit_cast(u16x2, w[base]);
   uint32_t pw[4], sw[8];
   if (PRI) {
 #pragma unroll
@@ -193,7 +199,7 @@ __device__ __forceinline__ uint32_t filter_pair(const uint32_t *w, int base, con
   if (PRI) {
     s16x2 c[4];
 #pragma unroll
-    for (int t = 0; t < 4; t++) c[t] = constrain2(as_s2(pw[t]) - x, k.pri, k.pri_adj);
+    for (int t = 0; t < 4; t++) c[t] = constrain2(__builtin_bit_cast(u16x2, pw[t]), xu, k.pri, k.pri_adj);
 #pragma unroll
     for (int t = 0; t < 4; t++) {
       mn = __builtin_elementwise_min(mn, __builtin_bit_cast(u16x2, pw[t]));
@@ -205,7 +211,7 @@ __device__ __forceinline__ uint32_t filter_pair(const uint32_t *w, int base, con
     /* Cdef_Sec_Taps is {2, 1} in both rows */
     s16x2 c[8];
 #pragma unroll
-    for (int t = 0; t < 8; t++) c[t] = constrain2(as_s2(sw[t]) - x, k.sec, k.sec_adj);
+    for (int t = 0; t < 8; t++) c[t] = constrain2(__builtin_bit_cast(u16x2, sw[t]), xu, k.sec, k.sec_adj);
 #pragma unroll
     for (int t = 0; t < 8; t++) {
       mn = __builtin_elementwise_min(mn, __builtin_bit_cast(u16x2, sw[t]));
