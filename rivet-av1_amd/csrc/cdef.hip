@@ -177,10 +177,12 @@ __device__ __forceinline__ Taps make_taps(int pri, int sec, int damping, int dir
 /* The filtered pair whose own word is w[base]. PRI / SEC: the tile has
  * primary / secondary taps at all (a lane's strength of 0 makes its taps
  * contribute nothing, since constrain() with t = 0 is 0). The clamp to
- * [min, max] of the available taps is applied always: with one kind of
- * tap the update already stays inside (total weight 12/16), and with no
- * taps it is the identity. Every tap is loaded before the arithmetic, and
- * the taps are independent, so the packed operations interleave. */
+ * [min, max] of the available taps is computed only with both kinds: with
+ * one kind the taps' weights total 12 and |constrain(d)| <= |d|, so
+ * (8 + sum - (sum < 0)) >> 4 lies between -max(x - p) and max(p - x) and
+ * the update stays inside by itself; with no taps it is the identity.
+ * Every tap is loaded before the arithmetic, and the taps are
+ * independent, so the packed operations interleave. */
 template <bool PRI, bool SEC>
 __device__ __forceinline__ uint32_t filter_pair(const uint32_t *w, int base, const Taps &k) {
   const s16x2 x = as_s2(w[base]);
@@ -194,16 +196,19 @@ __device__ __forceinline__ uint32_t filter_pair(const uint32_t *w, int base, con
 #pragma unroll
     for (int t = 0; t < 8; t++) sw[t] = w[base + k.so[t]];
   }
+  constexpr bool CLAMP = PRI && SEC;
   s16x2 sum = splat(0), mx = x;
   u16x2 mn = __builtin_bit_cast(u16x2, x);
   if (PRI) {
     s16x2 c[4];
 #pragma unroll
     for (int t = 0; t < 4; t++) c[t] = constrain2(__builtin_bit_cast(u16x2, pw[t]), xu, k.pri, k.pri_adj);
+    if (CLAMP) {
 #pragma unroll
-    for (int t = 0; t < 4; t++) {
-      mn = __builtin_elementwise_min(mn, __builtin_bit_cast(u16x2, pw[t]));
-      mx = __builtin_elementwise_max(mx, as_s2(pw[t]));
+      for (int t = 0; t < 4; t++) {
+        mn = __builtin_elementwise_min(mn, __builtin_bit_cast(u16x2, pw[t]));
+        mx = __builtin_elementwise_max(mx, as_s2(pw[t]));
+      }
     }
     sum = k.pt0 * (c[0] + c[1]) + k.pt1 * (c[2] + c[3]);
   }
@@ -212,17 +217,19 @@ __device__ __forceinline__ uint32_t filter_pair(const uint32_t *w, int base, con
     s16x2 c[8];
 #pragma unroll
     for (int t = 0; t < 8; t++) c[t] = constrain2(__builtin_bit_cast(u16x2, sw[t]), xu, k.sec, k.sec_adj);
+    if (CLAMP) {
 #pragma unroll
-    for (int t = 0; t < 8; t++) {
-      mn = __builtin_elementwise_min(mn, __builtin_bit_cast(u16x2, sw[t]));
-      mx = __builtin_elementwise_max(mx, as_s2(sw[t]));
+      for (int t = 0; t < 8; t++) {
+        mn = __builtin_elementwise_min(mn, __builtin_bit_cast(u16x2, sw[t]));
+        mx = __builtin_elementwise_max(mx, as_s2(sw[t]));
+      }
     }
     const s16x2 s0 = (c[0] + c[1]) + (c[2] + c[3]), s1 = (c[4] + c[5]) + (c[6] + c[7]);
     sum += s0 + s0 + s1;
   }
   /* x + ((8 + sum - (sum < 0)) >> 4), clamped to [min, max] of the taps */
   s16x2 v = x + ((sum + splat(8) + (sum >> splat(15))) >> splat(4));
-  v = __builtin_elementwise_min(__builtin_elementwise_max(v, __builtin_bit_cast(s16x2, mn)), mx);
+  if (CLAMP) v = __builtin_elementwise_min(__builtin_elementwise_max(v, __builtin_bit_cast(s16x2, mn)), mx);
   return __builtin_bit_cast(uint32_t, v);
 }
 
@@ -460,8 +467,11 @@ __global__ __launch_bounds__(256) void cdef_kernel(CdefLaunch a) {
   cdef_tile<T>(a, xcd_remap(blockIdx.x, gridDim.x), words);
 }
 
+/* Batched: 5 waves per SIMD, as the per-frame kernel gets by itself (inside
+ * the grab loop the compiler otherwise takes 192 VGPRs, 2 waves; the few
+ * spills the bound costs are cheaper than the lost latency hiding). */
 template <typename T>
-__global__ __launch_bounds__(256) void cdef_batch(BatchWork w) {
+__global__ __launch_bounds__(256, 5) void cdef_batch(BatchWork w) {
   __shared__ uint32_t words[2 * SOFF];
   batch_loop(w, [&](int f, int j) { cdef_tile<T>(batch_table<CdefLaunch>(w.rec, f), j, words); });
 }
