@@ -470,10 +470,10 @@ __global__ __launch_bounds__(256) void cdef_kernel(CdefLaunch a) {
 /* Batched: 5 waves per SIMD, as the per-frame kernel gets by itself (inside
  * the grab loop the compiler otherwise takes 192 VGPRs, 2 waves; the few
  * spills the bound costs are cheaper than the lost latency hiding). */
-template <typename T>
+template <typename T, bool DIRECT>
 __global__ __launch_bounds__(256, 5) void cdef_batch(BatchWork w) {
   __shared__ uint32_t words[2 * SOFF];
-  batch_loop(w, [&](int f, int j) { cdef_tile<T>(batch_table<CdefLaunch>(w.rec, f), j, words); });
+  batch_loop<DIRECT>(w, [&](int f, int j) { cdef_tile<T>(batch_table<CdefLaunch>(w.rec, f), j, words); });
 }
 
 /* ---- the encoder's strength search (src/encoder/cdef.rs:52-197) ---- */
@@ -676,9 +676,15 @@ int cdef_plan(const Geom &g, const CdefHdr &h, const rvpf_mi *mi, int mi_stride,
 int launch_cdef_batch(const BatchWork &w, const StagePlan &sp, int bps, hipStream_t st) {
   if (w.total == 0) return 0;
   if (bps == 1)
-    cdef_batch<uint8_t><<<sp.grid, 256, 0, st>>>(w);
+    if (batch_direct(w, sp.grid))
+      cdef_batch<uint8_t, true><<<sp.grid, 256, 0, st>>>(w);
+    else
+      cdef_batch<uint8_t, false><<<sp.grid, 256, 0, st>>>(w);
   else
-    cdef_batch<uint16_t><<<sp.grid, 256, 0, st>>>(w);
+    if (batch_direct(w, sp.grid))
+      cdef_batch<uint16_t, true><<<sp.grid, 256, 0, st>>>(w);
+    else
+      cdef_batch<uint16_t, false><<<sp.grid, 256, 0, st>>>(w);
   return (int)hipGetLastError();
 }
 

@@ -36,9 +36,9 @@ __global__ __launch_bounds__(256) void copy_kernel(CopyLaunch a) {
   copy_block<T, TO>(a, blockIdx.x);
 }
 
-template <typename T>
+template <typename T, bool DIRECT>
 __global__ __launch_bounds__(256) void copy_batch(BatchWork w) {
-  batch_loop(w, [&](int f, int j) { copy_block<T>(batch_table<CopyLaunch>(w.rec, f), j); });
+  batch_loop<DIRECT>(w, [&](int f, int j) { copy_block<T>(batch_table<CopyLaunch>(w.rec, f), j); });
 }
 
 int plan(const Geom &g, const PlanePtr in[3], const PlanePtr out[3], const int w[3], const int h[3],
@@ -70,9 +70,15 @@ int copy_plan(const Geom &g, const PlanePtr in[3], const PlanePtr out[3], const 
 int launch_copy_batch(const BatchWork &w, const StagePlan &sp, int bps, hipStream_t st) {
   if (w.total == 0) return 0;
   if (bps == 1)
-    copy_batch<uint8_t><<<sp.grid, 256, 0, st>>>(w);
+    if (batch_direct(w, sp.grid))
+      copy_batch<uint8_t, true><<<sp.grid, 256, 0, st>>>(w);
+    else
+      copy_batch<uint8_t, false><<<sp.grid, 256, 0, st>>>(w);
   else
-    copy_batch<uint16_t><<<sp.grid, 256, 0, st>>>(w);
+    if (batch_direct(w, sp.grid))
+      copy_batch<uint16_t, true><<<sp.grid, 256, 0, st>>>(w);
+    else
+      copy_batch<uint16_t, false><<<sp.grid, 256, 0, st>>>(w);
   return (int)hipGetLastError();
 }
 

@@ -715,19 +715,20 @@ struct ApplyRecord {
   int aligned;
 };
 
+template <bool DIRECT>
 __global__ __launch_bounds__(PREP_THREADS) void grain_prepare_batch(BatchWork w) {
   __shared__ PrepShared sm;
-  batch_loop(w, [&](int f, int) {
+  batch_loop<DIRECT>(w, [&](int f, int) {
     const PrepRecord &r = batch_table<PrepRecord>(w.rec, f);
     grain_prepare_body(r.h, r.s, sm);
   });
 }
 
 /* 5 waves per SIMD (96 VGPRs, no spills; unbounded it takes 108, 4 waves). */
-template <typename T>
+template <typename T, bool DIRECT>
 __global__ __launch_bounds__(256, 5) void grain_apply_batch(BatchWork w) {
   __shared__ int16_t lut[3 * 256];
-  batch_loop(w, [&](int f, int j) {
+  batch_loop<DIRECT>(w, [&](int f, int j) {
     const ApplyRecord &r = batch_table<ApplyRecord>(w.rec, f);
     if (r.aligned)
       grain_apply_tile<T, T, 1>(r.h, r.s, r.in[0], r.in[1], r.in[2], r.out[0], r.out[1], r.out[2], j, lut);
@@ -766,7 +767,10 @@ int grain_prep_plan(const GrainHdr &h, const GrainScratch &s, void *rec) {
 
 int launch_grain_prepare_batch(const BatchWork &w, const StagePlan &sp, hipStream_t st) {
   if (w.total == 0) return 0;
-  grain_prepare_batch<<<sp.grid, PREP_THREADS, 0, st>>>(w);
+  if (batch_direct(w, sp.grid))
+    grain_prepare_batch<true><<<sp.grid, PREP_THREADS, 0, st>>>(w);
+  else
+    grain_prepare_batch<false><<<sp.grid, PREP_THREADS, 0, st>>>(w);
   return (int)hipGetLastError();
 }
 
@@ -788,9 +792,15 @@ int grain_apply_plan(const GrainHdr &h, const GrainScratch &s, const PlanePtr in
 int launch_grain_apply_batch(const BatchWork &w, const StagePlan &sp, int bps, hipStream_t st) {
   if (w.total == 0) return 0;
   if (bps == 1)
-    grain_apply_batch<uint8_t><<<sp.grid, 256, 0, st>>>(w);
+    if (batch_direct(w, sp.grid))
+      grain_apply_batch<uint8_t, true><<<sp.grid, 256, 0, st>>>(w);
+    else
+      grain_apply_batch<uint8_t, false><<<sp.grid, 256, 0, st>>>(w);
   else
-    grain_apply_batch<uint16_t><<<sp.grid, 256, 0, st>>>(w);
+    if (batch_direct(w, sp.grid))
+      grain_apply_batch<uint16_t, true><<<sp.grid, 256, 0, st>>>(w);
+    else
+      grain_apply_batch<uint16_t, false><<<sp.grid, 256, 0, st>>>(w);
   return (int)hipGetLastError();
 }
 

@@ -487,20 +487,20 @@ __device__ __forceinline__ void lf_block(const LfLaunch &a, int gp, int lb, LfSh
   }
 }
 
-template <typename T, int PASS>
+template <typename T, int PASS, int EX>
 __global__ __launch_bounds__(256) void lf_pass_kernel(LfLaunch a) {
-  __shared__ LfShared<PASS, 64> tsm;
+  __shared__ LfShared<PASS, EX> tsm;
   int lb;
   const int gp = xcd_segment(blockIdx.x, a.blk_start, a.nseg, &lb);
   if (gp >= a.nseg || lb >= a.blk_n[gp]) return; /* padding */
-  lf_block<T, PASS, 64>(a, gp, lb, &tsm);
+  lf_block<T, PASS, EX>(a, gp, lb, &tsm);
 }
 
 /* Batched: item j of a frame is block j of its (unpadded) grid. */
-template <typename T, int PASS, int EX>
+template <typename T, int PASS, int EX, bool DIRECT>
 __global__ __launch_bounds__(256) void lf_pass_batch(BatchWork w) {
   __shared__ LfShared<PASS, EX> tsm;
-  batch_loop(w, [&](int f, int j) {
+  batch_loop<DIRECT>(w, [&](int f, int j) {
     const LfLaunch &a = batch_table<LfLaunch>(w.rec, f);
     int gp = 0;
     while (gp + 1 < a.nseg && j >= a.blk_start[gp + 1]) gp++;
@@ -573,10 +573,25 @@ int plan(const Geom &g, const LfHdr &h, const LfSide &s, const PlanePtr planes[3
 
 constexpr int SHAPE_EX[LF_SHAPES] = {64, 32, 16};
 
+/* Per-frame blocks: 16 x 16 edges (64 x 64 samples), whose 8-sample
+ * borders add a quarter to the staged samples in either pass (64 x 4
+ * edges add a sixteenth across and twice the rows down). */
+constexpr int LF_EX = 16;
+
 template <typename T, int EX>
 void launch_batch_t(const BatchWork &wv, const BatchWork &wh, const StagePlan &sp, hipStream_t st) {
-  if (wv.total > 0) lf_pass_batch<T, 0, EX><<<sp.grid, 256, 0, st>>>(wv);
-  if (wh.total > 0) lf_pass_batch<T, 1, EX><<<sp.grid, 256, 0, st>>>(wh);
+  if (wv.total > 0) {
+    if (batch_direct(wv, sp.grid))
+      lf_pass_batch<T, 0, EX, true><<<sp.grid, 256, 0, st>>>(wv);
+    else
+      lf_pass_batch<T, 0, EX, false><<<sp.grid, 256, 0, st>>>(wv);
+  }
+  if (wh.total > 0) {
+    if (batch_direct(wh, sp.grid))
+      lf_pass_batch<T, 1, EX, true><<<sp.grid, 256, 0, st>>>(wh);
+    else
+      lf_pass_batch<T, 1, EX, false><<<sp.grid, 256, 0, st>>>(wh);
+  }
 }
 
 template <typename T>
@@ -608,14 +623,14 @@ int launch_deblock_batch(const BatchWork &wv, const BatchWork &wh, const StagePl
 int launch_deblock(const Geom &g, const LfHdr &h, const LfSide &s, PlanePtr planes[3], int bps,
                    hipStream_t st) {
   LfLaunch a;
-  const int total = plan(g, h, s, planes, 64, true, a);
+  const int total = plan(g, h, s, planes, LF_EX, true, a);
   if (total == 0) return 0;
   if (bps == 1) {
-    lf_pass_kernel<uint8_t, 0><<<total, 256, 0, st>>>(a);
-    lf_pass_kernel<uint8_t, 1><<<total, 256, 0, st>>>(a);
+    lf_pass_kernel<uint8_t, 0, LF_EX><<<total, 256, 0, st>>>(a);
+    lf_pass_kernel<uint8_t, 1, LF_EX><<<total, 256, 0, st>>>(a);
   } else {
-    lf_pass_kernel<uint16_t, 0><<<total, 256, 0, st>>>(a);
-    lf_pass_kernel<uint16_t, 1><<<total, 256, 0, st>>>(a);
+    lf_pass_kernel<uint16_t, 0, LF_EX><<<total, 256, 0, st>>>(a);
+    lf_pass_kernel<uint16_t, 1, LF_EX><<<total, 256, 0, st>>>(a);
   }
   return (int)hipGetLastError();
 }

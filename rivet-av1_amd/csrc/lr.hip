@@ -507,10 +507,10 @@ __global__ __launch_bounds__(NT) void lr_kernel(LrLaunch a) {
 }
 
 /* Batched: item j of a frame is slice j of its (unpadded) grid. */
-template <typename T>
+template <typename T, bool DIRECT>
 __global__ __launch_bounds__(NT) void lr_batch(BatchWork w) {
   __shared__ LrShared sm;
-  batch_loop(w, [&](int f, int j) {
+  batch_loop<DIRECT>(w, [&](int f, int j) {
     const LrLaunch &a = batch_table<LrLaunch>(w.rec, f);
     int gp = 0;
     while (gp + 1 < a.nseg && j >= a.blk_start[gp + 1]) gp++;
@@ -568,9 +568,15 @@ int lr_plan(const Geom &g, const LrHdr &h, const PlanePtr deblocked[3], const Pl
 int launch_lr_batch(const BatchWork &w, const StagePlan &sp, int bps, hipStream_t st) {
   if (w.total == 0) return 0;
   if (bps == 1)
-    lr_batch<uint8_t><<<sp.grid, NT, 0, st>>>(w);
+    if (batch_direct(w, sp.grid))
+      lr_batch<uint8_t, true><<<sp.grid, NT, 0, st>>>(w);
+    else
+      lr_batch<uint8_t, false><<<sp.grid, NT, 0, st>>>(w);
   else
-    lr_batch<uint16_t><<<sp.grid, NT, 0, st>>>(w);
+    if (batch_direct(w, sp.grid))
+      lr_batch<uint16_t, true><<<sp.grid, NT, 0, st>>>(w);
+    else
+      lr_batch<uint16_t, false><<<sp.grid, NT, 0, st>>>(w);
   return (int)hipGetLastError();
 }
 

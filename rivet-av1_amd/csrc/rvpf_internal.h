@@ -171,26 +171,46 @@ __device__ __forceinline__ int batch_frame(const int *start, int n, int i) {
 }
 
 /* The grab loop of a batched kernel: body(f, j) runs item j of frame f
- * with the whole workgroup (f and j are wave-uniform). The queue is
- * re-armed by the last workgroup to finish, so the next launch of the same
- * work (a graph replay) starts from zero without a memset. Every workgroup
- * leaves once the counter passes the total. */
+ * with the whole workgroup (f and j are wave-uniform). A grid with a
+ * workgroup for every chunk (DIRECT, batch_direct() on the host; the
+ * kernel is compiled without the queue code, so it keeps the registers of
+ * the per-frame kernel) needs no queue: workgroup b takes chunk
+ * xcd_remap(b) (neighbouring chunks on one XCD, as the per-frame kernels
+ * map their tiles). A smaller (persistent) grid takes chunks from the
+ * queue, the next grab in flight while the current chunk runs, so the
+ * atomic's round trip is not waited for between chunks; a grabbed chunk
+ * below the total is always run. The queue is re-armed by the last
+ * workgroup to finish, so the next launch of the same work (a graph
+ * replay) starts from zero without a memset. Every workgroup leaves once
+ * its grab passes the total. */
 template <class Body>
+__device__ __forceinline__ void batch_run(const BatchWork &w, int i0, int i1, Body &body) {
+  for (int i = i0; i < i1; i++) {
+    const int f = __builtin_amdgcn_readfirstlane(batch_frame(w.start, w.n, i));
+    const int j = __builtin_amdgcn_readfirstlane(i - batch_table<int>(w.start, f));
+    body(f, j);
+    __syncthreads(); /* the next item reuses the LDS */
+  }
+}
+
+template <bool DIRECT, class Body>
 __device__ __forceinline__ void batch_loop(const BatchWork &w, Body &&body) {
+  if (DIRECT) {
+    const int i0 = xcd_remap(blockIdx.x, gridDim.x) * w.chunk;
+    batch_run(w, i0, min(i0 + w.chunk, w.total), body);
+    return;
+  }
   __shared__ int s_item;
+  int next = 0;
+  if (threadIdx.x == 0) next = atomicAdd(&w.queue[0], w.chunk);
   for (;;) {
-    if (threadIdx.x == 0) s_item = atomicAdd(&w.queue[0], w.chunk);
+    if (threadIdx.x == 0) s_item = next;
     __syncthreads();
     const int i0 = s_item;
     __syncthreads();
     if (i0 >= w.total) break;
-    const int i1 = min(i0 + w.chunk, w.total);
-    for (int i = i0; i < i1; i++) {
-      const int f = __builtin_amdgcn_readfirstlane(batch_frame(w.start, w.n, i));
-      const int j = __builtin_amdgcn_readfirstlane(i - batch_table<int>(w.start, f));
-      body(f, j);
-      __syncthreads(); /* the next item reuses the LDS */
-    }
+    if (threadIdx.x == 0) next = atomicAdd(&w.queue[0], w.chunk);
+    batch_run(w, i0, min(i0 + w.chunk, w.total), body);
   }
   if (threadIdx.x == 0) {
     __threadfence();
@@ -200,6 +220,9 @@ __device__ __forceinline__ void batch_loop(const BatchWork &w, Body &&body) {
     }
   }
 }
+
+/* Whether a launch of `grid` workgroups over w has one per chunk. */
+inline bool batch_direct(const BatchWork &w, int grid) { return (int64_t)grid * w.chunk >= w.total; }
 
 /* Tile shapes a batched stage can run with (the autotuner's choices). */
 struct StagePlan {
