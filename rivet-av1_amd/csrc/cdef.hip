@@ -467,11 +467,12 @@ __global__ __launch_bounds__(256) void cdef_kernel(CdefLaunch a) {
   cdef_tile<T>(a, xcd_remap(blockIdx.x, gridDim.x), words);
 }
 
-/* Batched: 5 waves per SIMD, as the per-frame kernel gets by itself (inside
- * the grab loop the compiler otherwise takes 192 VGPRs, 2 waves; the few
- * spills the bound costs are cheaper than the lost latency hiding). */
+/* Batched. The persistent form is held at 5 waves per SIMD, as the
+ * per-frame kernel gets by itself (inside the grab loop the compiler
+ * otherwise takes 192 VGPRs, 2 waves; the few spills the bound costs are
+ * cheaper than the lost latency hiding); the direct form needs no bound. */
 template <typename T, bool DIRECT>
-__global__ __launch_bounds__(256, 5) void cdef_batch(BatchWork w) {
+__global__ __launch_bounds__(256, DIRECT ? 1 : 5) void cdef_batch(BatchWork w) {
   __shared__ uint32_t words[2 * SOFF];
   batch_loop<DIRECT>(w, [&](int f, int j) { cdef_tile<T>(batch_table<CdefLaunch>(w.rec, f), j, words); });
 }

@@ -726,7 +726,7 @@ __global__ __launch_bounds__(PREP_THREADS) void grain_prepare_batch(BatchWork w)
 
 /* 5 waves per SIMD (96 VGPRs, no spills; unbounded it takes 108, 4 waves). */
 template <typename T, bool DIRECT>
-__global__ __launch_bounds__(256, 5) void grain_apply_batch(BatchWork w) {
+__global__ __launch_bounds__(256, DIRECT ? 1 : 5) void grain_apply_batch(BatchWork w) {
   __shared__ int16_t lut[3 * 256];
   batch_loop<DIRECT>(w, [&](int f, int j) {
     const ApplyRecord &r = batch_table<ApplyRecord>(w.rec, f);

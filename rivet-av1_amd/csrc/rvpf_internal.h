@@ -172,11 +172,13 @@ __device__ __forceinline__ int batch_frame(const int *start, int n, int i) {
 
 /* The grab loop of a batched kernel: body(f, j) runs item j of frame f
  * with the whole workgroup (f and j are wave-uniform). A grid with a
- * workgroup for every chunk (DIRECT, batch_direct() on the host; the
- * kernel is compiled without the queue code, so it keeps the registers of
- * the per-frame kernel) needs no queue: workgroup b takes chunk
- * xcd_remap(b) (neighbouring chunks on one XCD, as the per-frame kernels
- * map their tiles). A smaller (persistent) grid takes chunks from the
+ * workgroup for every item (DIRECT, batch_direct() on the host) needs no
+ * queue and no loop: workgroup b runs item xcd_remap(b) (neighbouring
+ * tiles on one XCD, as the per-frame kernels map theirs). Without a loop
+ * around the tile code the compiler keeps the per-frame kernel's register
+ * allocation (CDEF: 93 VGPRs; inside the loop it hoists the per-thread
+ * set-up out of the body and takes about twice as many). A smaller
+ * (persistent) grid, or chunks of several items, takes chunks from the
  * queue, the next grab in flight while the current chunk runs, so the
  * atomic's round trip is not waited for between chunks; a grabbed chunk
  * below the total is always run. The queue is re-armed by the last
@@ -196,8 +198,12 @@ __device__ __forceinline__ void batch_run(const BatchWork &w, int i0, int i1, Bo
 template <bool DIRECT, class Body>
 __device__ __forceinline__ void batch_loop(const BatchWork &w, Body &&body) {
   if (DIRECT) {
-    const int i0 = xcd_remap(blockIdx.x, gridDim.x) * w.chunk;
-    batch_run(w, i0, min(i0 + w.chunk, w.total), body);
+    const int i = xcd_remap(blockIdx.x, gridDim.x);
+    if (i < w.total) {
+      const int f = __builtin_amdgcn_readfirstlane(batch_frame(w.start, w.n, i));
+      const int j = __builtin_amdgcn_readfirstlane(i - batch_table<int>(w.start, f));
+      body(f, j);
+    }
     return;
   }
   __shared__ int s_item;
@@ -221,8 +227,8 @@ __device__ __forceinline__ void batch_loop(const BatchWork &w, Body &&body) {
   }
 }
 
-/* Whether a launch of `grid` workgroups over w has one per chunk. */
-inline bool batch_direct(const BatchWork &w, int grid) { return (int64_t)grid * w.chunk >= w.total; }
+/* Whether a launch of `grid` workgroups over w has one per item. */
+inline bool batch_direct(const BatchWork &w, int grid) { return w.chunk == 1 && grid >= w.total; }
 
 /* Tile shapes a batched stage can run with (the autotuner's choices). */
 struct StagePlan {
