@@ -42,7 +42,11 @@ constexpr int WS = MAXW + 16;  /* window row stride (samples) */
 constexpr int WOFF = 5;
 constexpr int WR = 64 + 6;     /* window rows: stripe + 3 above and below */
 constexpr int ABW = MAXW + 2;  /* A/B row stride: positions -1..w */
-constexpr int ABR = 64 + 2;    /* A/B rows: -1..h */
+/* The self-guided A/B are computed and used in two halves of the stripe
+ * (rows [0, 32) and [32, h)), so their buffers hold the positions of one
+ * half: r0 - 1 .. r1, at most 34 rows. */
+constexpr int SGR_HALF = 32;
+constexpr int ABR = SGR_HALF + 2; /* A/B rows of a half */
 
 struct LrLaunch {
   Geom g;
@@ -81,8 +85,8 @@ struct LrShared {
   union {
     int16_t hbuf[WR * MAXW]; /* Wiener: horizontal pass output */
     struct {
-      uint32_t ab0[(ABR / 2 + 1) * ABW]; /* pass 0 (r = 2): odd rows i at (i + 1) >> 1 */
-      uint32_t ab1[ABR * ABW];           /* pass 1 (r = 1): row i at i + 1 */
+      uint32_t ab0[(ABR / 2 + 1) * ABW]; /* pass 0 (r = 2): odd rows i at (i - r0 + 1) >> 1 */
+      uint32_t ab1[ABR * ABW];           /* pass 1 (r = 1): row i at i - r0 + 1 */
     } sgr;
   };
   uint16_t a2[256];
@@ -353,135 +357,144 @@ __device__ __forceinline__ void lr_tile(const LrLaunch &a, int plane, int st, in
   const SgrPass q0 = sgr_pass(2, rad0 ? AV1_SGR_PARAMS[set][1] : 1);
   const SgrPass q1 = sgr_pass(1, rad1 ? AV1_SGR_PARAMS[set][3] : 1);
   const uint16_t *a2t = sm.a2;
-  /* A/B at positions (i, j), i in [-1, h], j in [-1, w]; the box of
-   * radius 2 covers window rows i+1..i+5 and columns j+1..j+5. The walk
-   * of thread (band, col) covers column j = col - 1 (the columns past
-   * SW - 2 are done directly below). */
-  {
-    const int j = col - 1;
-    int lo, hi;
-    band_rows(h + 2, NB, band, lo, hi);
-    lo -= 1;
-    hi -= 1;
-    if (j <= w) {
-      uint32_t h5a = 0, h5b = 0, h5c = 0, h5d = 0, q5a = 0, q5b = 0, q5c = 0, q5d = 0;
-      uint32_t h3a = 0, h3b = 0, q3a = 0, q3b = 0;
-      const uint16_t *wp = &sm.win[(lo + 1) * WS + WOFF + j + 1];
-      for (int R = lo + 1; R <= hi + 4; R++, wp += WS) {
-        const uint32_t w1 = wp[0], w2 = wp[1], w3 = wp[2], w4 = wp[3], w5 = wp[4];
-        const uint32_t h3n = w2 + w3 + w4, h5n = h3n + w1 + w5;
-        const uint32_t q3n = __umul24(w2, w2) + __umul24(w3, w3) + __umul24(w4, w4);
-        const uint32_t q5n = q3n + __umul24(w1, w1) + __umul24(w5, w5);
-        const int i5 = R - 5, i3 = R - 4;
-        if (rad0 && i5 >= lo && ((y0 + i5) & 1))
-          sm.sgr.ab0[((i5 + 1) >> 1) * ABW + j + 1] =
-              sgr_ab(h5a + h5b + h5c + h5d + h5n, q5a + q5b + q5c + q5d + q5n, bd, q0, a2t);
-        if (rad1 && i3 >= lo && i3 < hi) sm.sgr.ab1[(i3 + 1) * ABW + j + 1] = sgr_ab(h3a + h3b + h3n, q3a + q3b + q3n, bd, q1, a2t);
-        h5a = h5b;
-        h5b = h5c;
-        h5c = h5d;
-        h5d = h5n;
-        q5a = q5b;
-        q5b = q5c;
-        q5c = q5d;
-        q5d = q5n;
-        h3a = h3b;
-        h3b = h3n;
-        q3a = q3b;
-        q3b = q3n;
+  for (int half = 0; half < 2; half++) {
+    /* output rows [r0, r1) of this half */
+    const int r0 = half ? min(h, SGR_HALF) : 0, r1 = half ? h : min(h, SGR_HALF);
+    if (r0 >= r1) break;
+    if (half) __syncthreads(); /* the first half's weighting has read A/B */
+    /* A/B at positions (i, j), i in [r0 - 1, r1], j in [-1, w]; the box of
+     * radius 2 covers window rows i+1..i+5 and columns j+1..j+5. The walk
+     * of thread (band, col) covers column j = col - 1 (the columns past
+     * SW - 2 are done directly below). */
+    {
+      const int j = col - 1;
+      const int np = r1 - r0 + 2;
+      int lo, hi;
+      band_rows(np, NB, band, lo, hi);
+      lo += r0 - 1;
+      hi += r0 - 1;
+      if (j <= w) {
+        uint32_t h5a = 0, h5b = 0, h5c = 0, h5d = 0, q5a = 0, q5b = 0, q5c = 0, q5d = 0;
+        uint32_t h3a = 0, h3b = 0, q3a = 0, q3b = 0;
+        const uint16_t *wp = &sm.win[(lo + 1) * WS + WOFF + j + 1];
+        for (int R = lo + 1; R <= hi + 4; R++, wp += WS) {
+          const uint32_t w1 = wp[0], w2 = wp[1], w3 = wp[2], w4 = wp[3], w5 = wp[4];
+          const uint32_t h3n = w2 + w3 + w4, h5n = h3n + w1 + w5;
+          const uint32_t q3n = __umul24(w2, w2) + __umul24(w3, w3) + __umul24(w4, w4);
+          const uint32_t q5n = q3n + __umul24(w1, w1) + __umul24(w5, w5);
+          const int i5 = R - 5, i3 = R - 4;
+          if (rad0 && i5 >= lo && ((y0 + i5) & 1))
+            sm.sgr.ab0[((i5 - r0 + 1) >> 1) * ABW + j + 1] =
+                sgr_ab(h5a + h5b + h5c + h5d + h5n, q5a + q5b + q5c + q5d + q5n, bd, q0, a2t);
+          if (rad1 && i3 >= lo && i3 < hi) sm.sgr.ab1[(i3 - r0 + 1) * ABW + j + 1] = sgr_ab(h3a + h3b + h3n, q3a + q3b + q3n, bd, q1, a2t);
+          h5a = h5b;
+          h5b = h5c;
+          h5c = h5d;
+          h5d = h5n;
+          q5a = q5b;
+          q5b = q5c;
+          q5c = q5d;
+          q5d = q5n;
+          h3a = h3b;
+          h3b = h3n;
+          q3a = q3b;
+          q3b = q3n;
+        }
+      }
+      /* columns SW - 1 .. w, every row, directly */
+      const int nl = w - SW + 2;
+      for (int t = threadIdx.x; t < nl * np; t += NT) {
+        const int jj = SW - 1 + t / np, i = r0 - 1 + t % np;
+        uint32_t s5 = 0, sq5 = 0, s3 = 0, sq3 = 0;
+        for (int dy = 0; dy < 5; dy++)
+          for (int dx = 0; dx < 5; dx++) {
+            const uint32_t v = sm.win[(i + 1 + dy) * WS + WOFF + jj + 1 + dx];
+            s5 += v;
+            sq5 += __umul24(v, v);
+            if (dy >= 1 && dy <= 3 && dx >= 1 && dx <= 3) {
+              s3 += v;
+              sq3 += __umul24(v, v);
+            }
+          }
+        if (rad0 && ((y0 + i) & 1)) sm.sgr.ab0[((i - r0 + 1) >> 1) * ABW + jj + 1] = sgr_ab(s5, sq5, bd, q0, a2t);
+        if (rad1) sm.sgr.ab1[(i - r0 + 1) * ABW + jj + 1] = sgr_ab(s3, sq3, bd, q1, a2t);
       }
     }
-    /* columns SW - 1 .. w, every row, directly */
-    const int nl = w - SW + 2;
-    for (int t = threadIdx.x; t < nl * (h + 2); t += NT) {
-      const int jj = SW - 1 + t / (h + 2), i = t % (h + 2) - 1;
-      uint32_t s5 = 0, sq5 = 0, s3 = 0, sq3 = 0;
-      for (int dy = 0; dy < 5; dy++)
-        for (int dx = 0; dx < 5; dx++) {
-          const uint32_t v = sm.win[(i + 1 + dy) * WS + WOFF + jj + 1 + dx];
-          s5 += v;
-          sq5 += __umul24(v, v);
-          if (dy >= 1 && dy <= 3 && dx >= 1 && dx <= 3) {
-            s3 += v;
-            sq3 += __umul24(v, v);
+    __syncthreads();
+    /* 3x3 weighting of both passes and the projection (postfilter.rs:823-851) */
+    if (col < w) {
+      const int j = col;
+      const int w0 = u->sgr_xqd[0], w1 = u->sgr_xqd[1];
+      const int w2 = 128 - w0 - w1;
+      int lo, hi;
+      band_rows(r1 - r0, NB, band, lo, hi);
+      lo += r0;
+      hi += r0;
+      /* per A/B row: E = A(j-1) + A(j+1), C = A(j) (and the same for B) */
+      const uint32_t *p1 = &sm.sgr.ab1[(lo - r0) * ABW + j];
+      int ea_m = 0, ca_m = 0, eb_m = 0, cb_m = 0, ea_c = 0, ca_c = 0, eb_c = 0, cb_c = 0;
+      if (rad1) {
+        uint32_t l = p1[0], c = p1[1], r = p1[2];
+        ea_m = (int)((l >> 20) + (r >> 20));
+        ca_m = (int)(c >> 20);
+        eb_m = (int)((l & 0xFFFFF) + (r & 0xFFFFF));
+        cb_m = (int)(c & 0xFFFFF);
+        l = p1[ABW];
+        c = p1[ABW + 1];
+        r = p1[ABW + 2];
+        ea_c = (int)((l >> 20) + (r >> 20));
+        ca_c = (int)(c >> 20);
+        eb_c = (int)((l & 0xFFFFF) + (r & 0xFFFFF));
+        cb_c = (int)(c & 0xFFFFF);
+      }
+      /* pass 0: 6*C + 5*E of the last odd row at or above i */
+      int ra_p = 0, rb_p = 0;
+      if (rad0) {
+        const int io = ((y0 + lo) & 1) ? lo : lo - 1;
+        const uint32_t *p0 = &sm.sgr.ab0[((io - r0 + 1) >> 1) * ABW + j];
+        const uint32_t l = p0[0], c = p0[1], r = p0[2];
+        ra_p = 6 * (int)(c >> 20) + 5 * (int)((l >> 20) + (r >> 20));
+        rb_p = 6 * (int)(c & 0xFFFFF) + 5 * (int)((l & 0xFFFFF) + (r & 0xFFFFF));
+      }
+      T *o = out + (int64_t)(y0 + lo) * os + x0 + j;
+      for (int i = lo; i < hi; i++, o += os) {
+        const int uu = sm.win[(i + 3) * WS + WOFF + j + 3];
+        const int ue = uu << 4;
+        int f0 = ue, f1 = ue;
+        if (rad0) {
+          if ((y0 + i) & 1) {
+            f0 = round2(__mul24(ra_p, uu) + rb_p, 8);
+          } else {
+            const uint32_t *p0 = &sm.sgr.ab0[((i + 2 - r0) >> 1) * ABW + j];
+            const uint32_t l = p0[0], c = p0[1], r = p0[2];
+            const int ra_n = 6 * (int)(c >> 20) + 5 * (int)((l >> 20) + (r >> 20));
+            const int rb_n = 6 * (int)(c & 0xFFFFF) + 5 * (int)((l & 0xFFFFF) + (r & 0xFFFFF));
+            f0 = round2(__mul24(ra_p + ra_n, uu) + rb_p + rb_n, 9);
+            ra_p = ra_n;
+            rb_p = rb_n;
           }
         }
-      if (rad0 && ((y0 + i) & 1)) sm.sgr.ab0[((i + 1) >> 1) * ABW + jj + 1] = sgr_ab(s5, sq5, bd, q0, a2t);
-      if (rad1) sm.sgr.ab1[(i + 1) * ABW + jj + 1] = sgr_ab(s3, sq3, bd, q1, a2t);
-    }
-  }
-  __syncthreads();
-  /* 3x3 weighting of both passes and the projection (postfilter.rs:823-851) */
-  if (col < w) {
-    const int j = col;
-    const int w0 = u->sgr_xqd[0], w1 = u->sgr_xqd[1];
-    const int w2 = 128 - w0 - w1;
-    int lo, hi;
-    band_rows(h, NB, band, lo, hi);
-    /* per A/B row: E = A(j-1) + A(j+1), C = A(j) (and the same for B) */
-    const uint32_t *p1 = &sm.sgr.ab1[lo * ABW + j];
-    int ea_m = 0, ca_m = 0, eb_m = 0, cb_m = 0, ea_c = 0, ca_c = 0, eb_c = 0, cb_c = 0;
-    if (rad1) {
-      uint32_t l = p1[0], c = p1[1], r = p1[2];
-      ea_m = (int)((l >> 20) + (r >> 20));
-      ca_m = (int)(c >> 20);
-      eb_m = (int)((l & 0xFFFFF) + (r & 0xFFFFF));
-      cb_m = (int)(c & 0xFFFFF);
-      l = p1[ABW];
-      c = p1[ABW + 1];
-      r = p1[ABW + 2];
-      ea_c = (int)((l >> 20) + (r >> 20));
-      ca_c = (int)(c >> 20);
-      eb_c = (int)((l & 0xFFFFF) + (r & 0xFFFFF));
-      cb_c = (int)(c & 0xFFFFF);
-    }
-    /* pass 0: 6*C + 5*E of the last odd row at or above i */
-    int ra_p = 0, rb_p = 0;
-    if (rad0) {
-      const int io = ((y0 + lo) & 1) ? lo : lo - 1;
-      const uint32_t *p0 = &sm.sgr.ab0[((io + 1) >> 1) * ABW + j];
-      const uint32_t l = p0[0], c = p0[1], r = p0[2];
-      ra_p = 6 * (int)(c >> 20) + 5 * (int)((l >> 20) + (r >> 20));
-      rb_p = 6 * (int)(c & 0xFFFFF) + 5 * (int)((l & 0xFFFFF) + (r & 0xFFFFF));
-    }
-    T *o = out + (int64_t)(y0 + lo) * os + x0 + j;
-    for (int i = lo; i < hi; i++, o += os) {
-      const int uu = sm.win[(i + 3) * WS + WOFF + j + 3];
-      const int ue = uu << 4;
-      int f0 = ue, f1 = ue;
-      if (rad0) {
-        if ((y0 + i) & 1) {
-          f0 = round2(__mul24(ra_p, uu) + rb_p, 8);
-        } else {
-          const uint32_t *p0 = &sm.sgr.ab0[((i + 2) >> 1) * ABW + j];
-          const uint32_t l = p0[0], c = p0[1], r = p0[2];
-          const int ra_n = 6 * (int)(c >> 20) + 5 * (int)((l >> 20) + (r >> 20));
-          const int rb_n = 6 * (int)(c & 0xFFFFF) + 5 * (int)((l & 0xFFFFF) + (r & 0xFFFFF));
-          f0 = round2(__mul24(ra_p + ra_n, uu) + rb_p + rb_n, 9);
-          ra_p = ra_n;
-          rb_p = rb_n;
+        if (rad1) {
+          const uint32_t *pn = &sm.sgr.ab1[(i + 2 - r0) * ABW + j];
+          const uint32_t l = pn[0], c = pn[1], r = pn[2];
+          const int ea_n = (int)((l >> 20) + (r >> 20)), ca_n = (int)(c >> 20);
+          const int eb_n = (int)((l & 0xFFFFF) + (r & 0xFFFFF)), cb_n = (int)(c & 0xFFFFF);
+          /* weights 3 4 3 / 4 4 4 / 3 4 3 */
+          const int av = 3 * (ea_m + ea_n) + 4 * (ca_m + ca_n + ea_c + ca_c);
+          const int bv = 3 * (eb_m + eb_n) + 4 * (cb_m + cb_n + eb_c + cb_c);
+          f1 = round2(__mul24(av, uu) + bv, 9);
+          ea_m = ea_c;
+          ca_m = ca_c;
+          eb_m = eb_c;
+          cb_m = cb_c;
+          ea_c = ea_n;
+          ca_c = ca_n;
+          eb_c = eb_n;
+          cb_c = cb_n;
         }
+        const int acc = __mul24(w1, ue) + __mul24(w0, f0) + __mul24(w2, f1);
+        *o = (T)min(max(round2(acc, 11), 0), maxv);
       }
-      if (rad1) {
-        const uint32_t *pn = &sm.sgr.ab1[(i + 2) * ABW + j];
-        const uint32_t l = pn[0], c = pn[1], r = pn[2];
-        const int ea_n = (int)((l >> 20) + (r >> 20)), ca_n = (int)(c >> 20);
-        const int eb_n = (int)((l & 0xFFFFF) + (r & 0xFFFFF)), cb_n = (int)(c & 0xFFFFF);
-        /* weights 3 4 3 / 4 4 4 / 3 4 3 */
-        const int av = 3 * (ea_m + ea_n) + 4 * (ca_m + ca_n + ea_c + ca_c);
-        const int bv = 3 * (eb_m + eb_n) + 4 * (cb_m + cb_n + eb_c + cb_c);
-        f1 = round2(__mul24(av, uu) + bv, 9);
-        ea_m = ea_c;
-        ca_m = ca_c;
-        eb_m = eb_c;
-        cb_m = cb_c;
-        ea_c = ea_n;
-        ca_c = ca_n;
-        eb_c = eb_n;
-        cb_c = cb_n;
-      }
-      const int acc = __mul24(w1, ue) + __mul24(w0, f0) + __mul24(w2, f1);
-      *o = (T)min(max(round2(acc, 11), 0), maxv);
     }
   }
 }
