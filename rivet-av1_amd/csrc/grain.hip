@@ -499,40 +499,53 @@ struct Seg8<uint8_t> {
 };
 
 /* One tile of the noise-applying stage: noise stripe ln (32 luma rows) and
- * luma columns [x0, x0 + 64), with the chroma samples they cover. A thread
- * takes one 8-sample row segment of luma (8 per row, 32 rows) and CI
- * chroma segments per pass (one pass, or two for 4:4:4 with CI = 1);
+ * luma columns [x0, x0 + 64 LS), with the chroma samples they cover. A
+ * thread takes LS 8-sample row segments of luma, 64 columns apart (8 per
+ * row and 64 columns, 32 rows), and CI chroma segments per pass (one pass,
+ * or two for 4:4:4 with CI = 1). LS = 2 for 4:2:0 gives every thread one
+ * chroma segment (with LS = 1 only half of them would have one, and their
+ * waves would carry all of the chroma work);
  * every segment lies in one 32-column noise block, so its noise is 8
  * consecutive template samples. The luma plane load is issued before the
  * scaling tables are staged. The planes' rows must be 16-byte (u16) or
  * 8-byte (u8) aligned. Chroma reads the un-noised luma of the input, as
  * the reference blends chroma first. */
-template <typename T, typename TO, int CI>
+template <typename T, typename TO, int CI, int LS>
 __device__ __forceinline__ void grain_apply_tile(const GrainHdr &h, const GrainScratch &sc, const PlanePtr &i0,
                                                  const PlanePtr &i1, const PlanePtr &i2, const PlanePtr &o0,
                                                  const PlanePtr &o1, const PlanePtr &o2, int tile_id, int16_t *lut) {
-  const int ntx = (h.w + 63) >> 6;
-  const int ln = tile_id / ntx, x0 = (tile_id - ln * ntx) * 64, y0 = ln * 32;
+  constexpr int TW = 64 * LS; /* luma columns per tile */
+  const int ntx = (h.w + TW - 1) / TW;
+  const int ln = tile_id / ntx, x0 = (tile_id - ln * ntx) * TW, y0 = ln * 32;
   const int bd = h.bd;
   const int gmin = -(128 << (bd - 8)), gmax = (256 << (bd - 8)) - 1 - (128 << (bd - 8));
   const bool ov = h.overlap;
   const int mb = sc.max_blocks;
   const T *lin = gptr(reinterpret_cast<const T *>(i0.data));
   const int64_t ls = i0.stride;
-  /* luma segment of this thread */
-  const int lx = x0 + (threadIdx.x & 7) * 8, li = threadIdx.x >> 3, ly = y0 + li;
-  const bool lon = lx < h.w && ly < h.h;
-  int lorig[8];
-  if (lon) Seg8<T>::load(lin + (int64_t)ly * ls + lx, h.w - lx, lorig);
+  /* luma segments of this thread */
+  const int lx0 = x0 + (threadIdx.x & 7) * 8, li = threadIdx.x >> 3, ly = y0 + li;
+  bool lon[LS];
+  int lorig[LS][8];
+#pragma unroll
+  for (int q = 0; q < LS; q++) {
+    const int lx = lx0 + 64 * q;
+    lon[q] = lx < h.w && ly < h.h;
+    if (lon[q]) Seg8<T>::load(lin + (int64_t)ly * ls + lx, h.w - lx, lorig[q]);
+  }
   for (int i = threadIdx.x; i < h.nplanes * 256; i += 256) lut[i] = sc.lut[i];
   __syncthreads();
-  int lz[8];
-  if (lon) seg_noise(sc.tmpl, sc.offsets, mb, ln, lx >> 5, lx & 31, li, 0, 0, ov, gmin, gmax, lz);
+  int lz[LS][8];
+#pragma unroll
+  for (int q = 0; q < LS; q++) {
+    const int lx = lx0 + 64 * q;
+    if (lon[q]) seg_noise(sc.tmpl, sc.offsets, mb, ln, lx >> 5, lx & 31, li, 0, 0, ov, gmin, gmax, lz[q]);
+  }
   /* chroma segments: items threadIdx.x + 256 (u + CI pass), u < CI */
   const bool chroma = h.nplanes > 1;
   const int ssx = chroma ? h.ssx : 0, ssy = chroma ? h.ssy : 0;
   const int cw = (h.w + ssx) >> ssx, ch = (h.h + ssy) >> ssy;
-  const int segs = 8 >> ssx, per = segs * (32 >> ssy);
+  const int segs = (8 * LS) >> ssx, per = segs * (32 >> ssy);
   const int maxv = (1 << bd) - 1;
   for (int pass = 0; chroma && pass * 256 * CI < 2 * per; pass++) {
     int cp[CI], cx[CI], cy[CI], ci[CI], corig[CI][8], cavg[CI][8], cz[CI][8];
@@ -598,14 +611,17 @@ __device__ __forceinline__ void grain_apply_tile(const GrainHdr &h, const GrainS
       Seg8<TO>::store(gptr(reinterpret_cast<TO *>(op.data)) + (int64_t)cy[u] * op.stride + cx[u], cw - cx[u], v);
     }
   }
-  if (lon) {
+#pragma unroll
+  for (int q = 0; q < LS; q++) {
+    if (!lon[q]) continue;
+    const int lx = lx0 + 64 * q;
     int v[8];
 #pragma unroll
     for (int k = 0; k < 8; k++) {
-      v[k] = lorig[k];
+      v[k] = lorig[q][k];
       if (h.num_y > 0) {
-        const int sn = round2(scale_lut(lut, bd, lorig[k]) * lz[k], h.scaling_shift);
-        v[k] = clip3(h.min_value, h.max_luma, lorig[k] + sn);
+        const int sn = round2(scale_lut(lut, bd, lorig[q][k]) * lz[q][k], h.scaling_shift);
+        v[k] = clip3(h.min_value, h.max_luma, lorig[q][k] + sn);
       }
     }
     Seg8<TO>::store(gptr(reinterpret_cast<TO *>(o0.data)) + (int64_t)ly * o0.stride + lx, h.w - lx, v);
@@ -689,16 +705,18 @@ __global__ __launch_bounds__(PREP_THREADS) void grain_prepare_kernel(GrainHdr h,
   grain_prepare_body(h, sc, sm);
 }
 
-/* ALIGNED: 1 (segments, one chroma segment per thread), 2 (segments,
- * two: 4:4:4), 0 (sample by sample). Input samples T, output samples TO
- * (8-bit output from 16-bit planes for the packed 8-bit frame). */
+/* KIND (apply_kind): 1 (segments, 64 luma columns per tile: 4:2:2, mono),
+ * 2 (segments, two chroma segments per thread: 4:4:4), 3 (segments, 128
+ * luma columns per tile: 4:2:0), 0 (sample by sample). Input samples T,
+ * output samples TO (8-bit output from 16-bit planes for the packed 8-bit
+ * frame). */
 template <typename T, typename TO, int ALIGNED>
 __global__ __launch_bounds__(256) void grain_apply_kernel(GrainHdr h, GrainScratch sc, PlanePtr i0, PlanePtr i1,
                                                           PlanePtr i2, PlanePtr o0, PlanePtr o1, PlanePtr o2) {
   __shared__ int16_t lut[3 * 256];
   const int tile = xcd_remap(blockIdx.x, gridDim.x);
   if constexpr (ALIGNED != 0)
-    grain_apply_tile<T, TO, ALIGNED>(h, sc, i0, i1, i2, o0, o1, o2, tile, lut);
+    grain_apply_tile<T, TO, ALIGNED == 2 ? 2 : 1, ALIGNED == 3 ? 2 : 1>(h, sc, i0, i1, i2, o0, o1, o2, tile, lut);
   else
     grain_apply_tile_unaligned<T, TO>(h, sc, i0, i1, i2, o0, o1, o2, tile, lut);
 }
@@ -712,7 +730,7 @@ struct ApplyRecord {
   GrainHdr h;
   GrainScratch s;
   PlanePtr in[3], out[3];
-  int aligned;
+  int kind; /* apply_kind */
 };
 
 template <bool DIRECT>
@@ -730,15 +748,21 @@ __global__ __launch_bounds__(256, DIRECT ? 1 : 5) void grain_apply_batch(BatchWo
   __shared__ int16_t lut[3 * 256];
   batch_loop<DIRECT>(w, [&](int f, int j) {
     const ApplyRecord &r = batch_table<ApplyRecord>(w.rec, f);
-    if (r.aligned)
-      grain_apply_tile<T, T, 1>(r.h, r.s, r.in[0], r.in[1], r.in[2], r.out[0], r.out[1], r.out[2], j, lut);
+    if (r.kind == 3)
+      grain_apply_tile<T, T, 1, 2>(r.h, r.s, r.in[0], r.in[1], r.in[2], r.out[0], r.out[1], r.out[2], j, lut);
+    else if (r.kind != 0)
+      grain_apply_tile<T, T, 1, 1>(r.h, r.s, r.in[0], r.in[1], r.in[2], r.out[0], r.out[1], r.out[2], j, lut);
     else
       grain_apply_tile_unaligned<T, T>(r.h, r.s, r.in[0], r.in[1], r.in[2], r.out[0], r.out[1], r.out[2], j, lut);
   });
 }
 
-/* Tiles of the noise-applying stage: 64 luma columns of one noise stripe. */
-int apply_tiles(const GrainHdr &h) { return ((h.w + 63) / 64) * ((h.h + 31) / 32); }
+/* Tiles of the noise-applying stage: 64 luma columns (128 for kind 3) of
+ * one noise stripe. */
+int apply_tiles(const GrainHdr &h, int kind) {
+  const int tw = kind == 3 ? 128 : 64;
+  return ((h.w + tw - 1) / tw) * ((h.h + 31) / 32);
+}
 
 /* Every plane row in and out starts on a segment boundary (8 samples). */
 bool rows_aligned(const GrainHdr &h, const PlanePtr in[3], const PlanePtr out[3], int in_bps, int out_bps) {
@@ -747,6 +771,15 @@ bool rows_aligned(const GrainHdr &h, const PlanePtr in[3], const PlanePtr out[3]
     if (((uintptr_t)out[p].data % (8 * (uintptr_t)out_bps)) != 0 || (out[p].stride % 8) != 0) return false;
   }
   return true;
+}
+
+/* Tile kind of a frame: 0 (rows not aligned: sample by sample), 2 (4:4:4),
+ * 3 (4:2:0), 1 (4:2:2 and mono). */
+int apply_kind(const GrainHdr &h, const PlanePtr in[3], const PlanePtr out[3], int in_bps, int out_bps) {
+  if (!rows_aligned(h, in, out, in_bps, out_bps)) return 0;
+  if (h.nplanes == 1) return 1;
+  if (!h.ssx) return 2;
+  return h.ssy ? 3 : 1;
 }
 
 }  // namespace
@@ -785,8 +818,11 @@ int grain_apply_plan(const GrainHdr &h, const GrainScratch &s, const PlanePtr in
     r.in[p] = in[p];
     r.out[p] = out[p];
   }
-  r.aligned = rows_aligned(h, in, out, bps, bps);
-  return apply_tiles(h);
+  r.kind = apply_kind(h, in, out, bps, bps);
+  /* the batched kernel runs kind 2 (4:4:4) with one chroma segment per
+   * thread and pass */
+  if (r.kind == 2) r.kind = 1;
+  return apply_tiles(h, r.kind);
 }
 
 int launch_grain_apply_batch(const BatchWork &w, const StagePlan &sp, int bps, hipStream_t st) {
@@ -809,6 +845,8 @@ void apply_launch(int kind, int grid, const GrainHdr &h, const GrainScratch &s, 
                   PlanePtr out[3], hipStream_t st) {
   if (kind == 1)
     grain_apply_kernel<T, TO, 1><<<grid, 256, 0, st>>>(h, s, in[0], in[1], in[2], out[0], out[1], out[2]);
+  else if (kind == 3)
+    grain_apply_kernel<T, TO, 3><<<grid, 256, 0, st>>>(h, s, in[0], in[1], in[2], out[0], out[1], out[2]);
   else if (kind == 2)
     grain_apply_kernel<T, TO, 2><<<grid, 256, 0, st>>>(h, s, in[0], in[1], in[2], out[0], out[1], out[2]);
   else
@@ -819,8 +857,8 @@ int launch_grain_apply(const GrainHdr &h, const GrainScratch &s, const PlanePtr 
                        hipStream_t st, int out_bps) {
   if (out_bps == 0) out_bps = bps;
   if (out_bps > bps) return (int)hipErrorInvalidValue;
-  const int grid = apply_tiles(h);
-  const int kind = !rows_aligned(h, in, out, bps, out_bps) ? 0 : (h.nplanes > 1 && !h.ssx ? 2 : 1);
+  const int kind = apply_kind(h, in, out, bps, out_bps);
+  const int grid = apply_tiles(h, kind);
   if (bps == 1)
     apply_launch<uint8_t, uint8_t>(kind, grid, h, s, in, out, st);
   else if (out_bps == 1)
