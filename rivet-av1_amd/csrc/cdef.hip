@@ -461,18 +461,20 @@ __device__ __forceinline__ void cdef_tile(const CdefLaunch &a, int tile_id, uint
   }
 }
 
+/* 6 waves per SIMD: 80 VGPRs without spills (unbounded the compiler takes
+ * 93, 5 waves; 6 waves run 3 % faster per 4K frame, 7 and 8 slower). */
 template <typename T>
-__global__ __launch_bounds__(256) void cdef_kernel(CdefLaunch a) {
+__global__ __launch_bounds__(256, 6) void cdef_kernel(CdefLaunch a) {
   __shared__ uint32_t words[2 * SOFF];
   cdef_tile<T>(a, xcd_remap(blockIdx.x, gridDim.x), words);
 }
 
-/* Batched. The persistent form is held at 5 waves per SIMD, as the
- * per-frame kernel gets by itself (inside the grab loop the compiler
- * otherwise takes 192 VGPRs, 2 waves; the few spills the bound costs are
- * cheaper than the lost latency hiding); the direct form needs no bound. */
+/* Batched. The persistent form is held at 5 waves per SIMD (inside the
+ * grab loop the compiler otherwise takes 192 VGPRs, 2 waves; the few
+ * spills the bound costs are cheaper than the lost latency hiding); the
+ * direct form at 6, as the per-frame kernel. */
 template <typename T, bool DIRECT>
-__global__ __launch_bounds__(256, DIRECT ? 1 : 5) void cdef_batch(BatchWork w) {
+__global__ __launch_bounds__(256, DIRECT ? 6 : 5) void cdef_batch(BatchWork w) {
   __shared__ uint32_t words[2 * SOFF];
   batch_loop<DIRECT>(w, [&](int f, int j) { cdef_tile<T>(batch_table<CdefLaunch>(w.rec, f), j, words); });
 }

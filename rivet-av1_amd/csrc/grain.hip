@@ -709,9 +709,10 @@ __global__ __launch_bounds__(PREP_THREADS) void grain_prepare_kernel(GrainHdr h,
  * 2 (segments, two chroma segments per thread: 4:4:4), 3 (segments, 128
  * luma columns per tile: 4:2:0), 0 (sample by sample). Input samples T,
  * output samples TO (8-bit output from 16-bit planes for the packed 8-bit
- * frame). */
+ * frame). Kind 3 is held at 5 waves per SIMD (96 VGPRs and a 12-byte
+ * spill; unbounded it takes 101, 4 waves, and runs 5 % slower). */
 template <typename T, typename TO, int ALIGNED>
-__global__ __launch_bounds__(256) void grain_apply_kernel(GrainHdr h, GrainScratch sc, PlanePtr i0, PlanePtr i1,
+__global__ __launch_bounds__(256, ALIGNED == 3 ? 5 : 1) void grain_apply_kernel(GrainHdr h, GrainScratch sc, PlanePtr i0, PlanePtr i1,
                                                           PlanePtr i2, PlanePtr o0, PlanePtr o1, PlanePtr o2) {
   __shared__ int16_t lut[3 * 256];
   const int tile = xcd_remap(blockIdx.x, gridDim.x);
@@ -742,9 +743,10 @@ __global__ __launch_bounds__(PREP_THREADS) void grain_prepare_batch(BatchWork w)
   });
 }
 
-/* 5 waves per SIMD (96 VGPRs, no spills; unbounded it takes 108, 4 waves). */
+/* 5 waves per SIMD (96 VGPRs, a few spills; unbounded it takes 108-112,
+ * 4 waves, and runs 9 % slower). */
 template <typename T, bool DIRECT>
-__global__ __launch_bounds__(256, DIRECT ? 1 : 5) void grain_apply_batch(BatchWork w) {
+__global__ __launch_bounds__(256, 5) void grain_apply_batch(BatchWork w) {
   __shared__ int16_t lut[3 * 256];
   batch_loop<DIRECT>(w, [&](int f, int j) {
     const ApplyRecord &r = batch_table<ApplyRecord>(w.rec, f);
