@@ -42,11 +42,11 @@ constexpr int WS = MAXW + 16;  /* window row stride (samples) */
 constexpr int WOFF = 5;
 constexpr int WR = 64 + 6;     /* window rows: stripe + 3 above and below */
 constexpr int ABW = MAXW + 2;  /* A/B row stride: positions -1..w */
-/* The self-guided A/B are computed and used in two halves of the stripe
- * (rows [0, 32) and [32, h)), so their buffers hold the positions of one
- * half: r0 - 1 .. r1, at most 34 rows. */
-constexpr int SGR_HALF = 32;
-constexpr int ABR = SGR_HALF + 2; /* A/B rows of a half */
+/* The self-guided A/B are computed and used in parts of SGR_PART rows of
+ * the stripe (rows [r0, r1)), so their buffers hold the positions of one
+ * part: r0 - 1 .. r1, at most SGR_PART + 2 rows. */
+constexpr int SGR_PART = 32;
+constexpr int ABR = SGR_PART + 2; /* A/B rows of a part */
 
 struct LrLaunch {
   Geom g;
@@ -87,9 +87,9 @@ struct LrShared {
     struct {
       uint32_t ab0[(ABR / 2 + 1) * ABW]; /* pass 0 (r = 2): odd rows i at (i - r0 + 1) >> 1 */
       uint32_t ab1[ABR * ABW];           /* pass 1 (r = 1): row i at i - r0 + 1 */
+      uint16_t a2[256];                  /* the a2 table of sgr_ab */
     } sgr;
   };
-  uint16_t a2[256];
 };
 
 /* Per-pass constants of the self-guided filter (postfilter.rs:915-935). */
@@ -350,18 +350,17 @@ __device__ __forceinline__ void lr_tile(const LrLaunch &a, int plane, int st, in
   }
 
   /* self-guided: box_filter_rect (postfilter.rs:901-991) for both passes */
-  for (int i = threadIdx.x; i < 256; i += NT) sm.a2[i] = SGR_A2.v[i];
+  for (int i = threadIdx.x; i < 256; i += NT) sm.sgr.a2[i] = SGR_A2.v[i];
   __syncthreads();
   const int set = u->sgr_set & 15;
   const int rad0 = AV1_SGR_PARAMS[set][0], rad1 = AV1_SGR_PARAMS[set][2];
   const SgrPass q0 = sgr_pass(2, rad0 ? AV1_SGR_PARAMS[set][1] : 1);
   const SgrPass q1 = sgr_pass(1, rad1 ? AV1_SGR_PARAMS[set][3] : 1);
-  const uint16_t *a2t = sm.a2;
-  for (int half = 0; half < 2; half++) {
-    /* output rows [r0, r1) of this half */
-    const int r0 = half ? min(h, SGR_HALF) : 0, r1 = half ? h : min(h, SGR_HALF);
-    if (r0 >= r1) break;
-    if (half) __syncthreads(); /* the first half's weighting has read A/B */
+  const uint16_t *a2t = sm.sgr.a2;
+  for (int r0 = 0; r0 < h; r0 += SGR_PART) {
+    /* output rows [r0, r1) of this part */
+    const int r1 = min(h, r0 + SGR_PART);
+    if (r0) __syncthreads(); /* the previous part's weighting has read A/B */
     /* A/B at positions (i, j), i in [r0 - 1, r1], j in [-1, w]; the box of
      * radius 2 covers window rows i+1..i+5 and columns j+1..j+5. The walk
      * of thread (band, col) covers column j = col - 1 (the columns past
