@@ -394,7 +394,7 @@ typedef struct rvpf_batch rvpf_batch;
 #define RVPF_TUNE_COUNT 6
 typedef struct rvpf_tile_config {
   int32_t shape;            /* tile shape: deblock 0..2 = 64x4, 32x8, 16x16 edges;
-                               grain apply 0..2 = 64x4, 32x8, 16x16 chroma samples; others 0 */
+                               other stages 0 (one shape each) */
   int32_t workgroups_per_cu;/* persistent grid = CUs x this; 0 = one workgroup per grab */
   int32_t chunk;            /* tiles taken per queue grab, >= 1 */
   float us;                 /* measured stage time of this choice (output; 0 = not measured) */

@@ -269,7 +269,7 @@ int launch_grain_apply_batch(const BatchWork &w, const StagePlan &sp, int bps, h
 
 /* Number of tile shapes of each shape-tunable stage. */
 constexpr int LF_SHAPES = 3;    /* 64x4, 32x8, 16x16 edges per workgroup */
-constexpr int GRAIN_SHAPES = 1; /* 64 luma columns of a noise stripe per workgroup */
+constexpr int GRAIN_SHAPES = 1; /* one: a noise stripe's 64 luma columns (128 at 4:2:0) per workgroup */
 constexpr int COPY_SHAPES = 1;
 
 /* Differentiable loop restoration (lrdiff.hip). */
