@@ -175,9 +175,9 @@ __device__ __forceinline__ int batch_frame(const int *start, int n, int i) {
  * workgroup for every item (DIRECT, batch_direct() on the host) needs no
  * queue and no loop: workgroup b runs item xcd_remap(b) (neighbouring
  * tiles on one XCD, as the per-frame kernels map theirs). Without a loop
- * around the tile code the compiler keeps the per-frame kernel's register
- * allocation (CDEF: 93 VGPRs; inside the loop it hoists the per-thread
- * set-up out of the body and takes about twice as many). A smaller
+ * around the tile code the compiler allocates registers as for the
+ * per-frame kernel (inside a loop it hoists each tile's per-thread set-up
+ * out of the body and takes about twice as many VGPRs). A smaller
  * (persistent) grid, or chunks of several items, takes chunks from the
  * queue, the next grab in flight while the current chunk runs, so the
  * atomic's round trip is not waited for between chunks; a grabbed chunk
