@@ -32,11 +32,15 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make_frames(n, w, h, bd, seed, superres_denom=0):
+CHROMA = {"420": dict(ssx=1, ssy=1), "422": dict(ssx=1, ssy=0), "444": dict(ssx=0, ssy=0), "mono": dict(mono=True)}
+CHROMA_NAME = {"420": "4:2:0", "422": "4:2:2", "444": "4:4:4", "mono": "monochrome"}
+
+
+def make_frames(n, w, h, bd, seed, superres_denom=0, chroma="420"):
     from rvpf import workload
     frames = []
     for i in range(n):
-        frames.append(workload.Frame(w, h, bit_depth=bd, ssx=1, ssy=1, sb_size=128, qindex=QINDEX[i % 4],
+        frames.append(workload.Frame(w, h, bit_depth=bd, **CHROMA[chroma], sb_size=128, qindex=QINDEX[i % 4],
                                      seed=seed, frame_index=i, pan_px=1.5, lf=True, cdef=True, cdef_bits=3,
                                      lr=2, lr_unit_size=64, grain=True, grain_lag=3, grain_y_points=8,
                                      with_source=False, superres_denom=superres_denom))
@@ -83,6 +87,13 @@ def coded_samples_per_frame(f):
     return n
 
 
+def frame_layout(f):
+    """Chroma layout of a frame, as a name."""
+    if f.num_planes == 1:
+        return "monochrome"
+    return {(1, 1): "4:2:0", (1, 0): "4:2:2", (0, 0): "4:4:4"}[(f.params.ssx, f.params.ssy)]
+
+
 def cpu_baseline(frames, seconds):
     """The oracle chain on the host, threaded like the reference's decoder."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -99,9 +110,9 @@ def cpu_baseline(frames, seconds):
             break
     px = sum(frames[i % len(frames)].luma_pixels() for i in range(n))
     return {"value": round(px / el / 1e6, 2), "unit": "Mpix/s", "cores": threads, "kind": "port",
-            "sample": "%d frames of the same %dx%d %d-bit 4:2:0 chain%s (%.1f s), C oracle, %d threads"
+            "sample": "%d frames of the same %dx%d %d-bit %s chain%s (%.1f s), C oracle, %d threads"
                       % (n, frames[0].params.upscaled_width, frames[0].params.frame_height,
-                         frames[0].params.bit_depth,
+                         frames[0].params.bit_depth, frame_layout(frames[0]),
                          " with super-resolution" if frames[0].params.use_superres else "", el, threads)}
 
 
@@ -510,6 +521,8 @@ def main():
     ap.add_argument("--width", type=int, default=3840)
     ap.add_argument("--height", type=int, default=2160)
     ap.add_argument("--bit-depth", type=int, default=10)
+    ap.add_argument("--chroma", choices=sorted(CHROMA), default="420",
+                    help="chroma layout of the gop4k frames (4:2:0 by default)")
     ap.add_argument("--superres-denom", type=int, default=0,
                     help="code the frames at width*8/d and upscale (super-resolution), d in 9..16")
     ap.add_argument("--seed", type=int, default=2160)
@@ -551,7 +564,8 @@ def main():
         torch.cuda.synchronize()
         del a_, b_
     log("generating %d frames %dx%d %d-bit" % (args.frames, args.width, args.height, args.bit_depth))
-    frames = make_frames(args.frames, args.width, args.height, args.bit_depth, args.seed, args.superres_denom)
+    frames = make_frames(args.frames, args.width, args.height, args.bit_depth, args.seed, args.superres_denom,
+                         args.chroma)
     ctx = api.Context()
     dfs = [api.DeviceFrame(f) for f in frames]
     nsteps = args.steps + args.warmup
@@ -648,8 +662,8 @@ def main():
     if main:
         dom = max(main, key=lambda k: stages[k]["ms"])
         s = stages[dom]
-        default_workload = (args.width, args.height, args.bit_depth, args.frames, args.superres_denom) == \
-            (3840, 2160, 10, 4, 0)
+        default_workload = (args.width, args.height, args.bit_depth, args.frames, args.superres_denom,
+                            args.chroma) == (3840, 2160, 10, 4, 0, "420")
         roofline = {"bound": "hbm", "achieved": s["GB_s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": s["frac"], "traffic": measured_traffic(dom) if default_workload else None,
                     "kernel": dom, "alg_bytes_per_launch": s["alg_bytes"], "ms_per_launch": s["ms"],
@@ -678,9 +692,9 @@ def main():
         "dtype": "u16" if b == 2 else "u8",
         "data": "synthetic",
         "config": {
-            "workload": "%dx%d %d-bit 4:2:0, %d-frame GOP (qindex %s), deblock+CDEF+%sLR(Wiener/SGR, 64-px units)"
+            "workload": "%dx%d %d-bit %s, %d-frame GOP (qindex %s), deblock+CDEF+%sLR(Wiener/SGR, 64-px units)"
                         "+film grain, per-frame drop-in" % (
-                            args.width, args.height, args.bit_depth, len(frames),
+                            args.width, args.height, args.bit_depth, CHROMA_NAME[args.chroma], len(frames),
                             "/".join(str(q) for q in QINDEX[:len(frames)]),
                             "super-resolution from width %d (denominator %d)+" % (
                                 frames[0].params.frame_width, args.superres_denom) if args.superres_denom else ""),
