@@ -468,14 +468,32 @@ __device__ __forceinline__ void lf_block(const LfLaunch &a, int gp, int lb, LfSh
     }
   }
   __syncthreads();
+  int cls = -1; /* filter class of this lane's edge, -1: not filtered */
+  uint32_t word = 0;
   if (valid) {
     int limit, blimit, thresh;
     const int fs = edge_decision(a, sm->t, side, plane, PASS, ex, ey, &limit, &blimit, &thresh);
     if (fs) {
-      const int c = fs == 4 ? 0 : (fs == 16 ? 3 : (plane ? 1 : 2));
-      const int slot = atomicAdd(&sm->count[c], 1);
-      sm->list[c][slot] = threadIdx.x | (limit << 8) | (blimit << 16) | (thresh << 24);
+      cls = fs == 4 ? 0 : (fs == 16 ? 3 : (plane ? 1 : 2));
+      word = threadIdx.x | (limit << 8) | (blimit << 16) | (thresh << 24);
     }
+  }
+  /* wave-aggregated append: a ballot per class gives the wave's edges of
+   * each class; lanes 0..3 reserve the wave's run of class 0..3 with one
+   * LDS atomic instruction, a shuffle hands each lane its class's base and
+   * its slot adds the number of the class's lanes below it (mbcnt). The
+   * order within a list does not matter: the edges of a class write
+   * disjoint footprints and read the staged samples. */
+  {
+    const uint64_t m0 = __ballot(cls == 0), m1 = __ballot(cls == 1), m2 = __ballot(cls == 2), m3 = __ballot(cls == 3);
+    const int lane = threadIdx.x & 63;
+    const uint64_t mk = lane == 0 ? m0 : (lane == 1 ? m1 : (lane == 2 ? m2 : m3));
+    int base = 0;
+    if (lane < 4 && mk != 0) base = atomicAdd(&sm->count[lane], __popcll(mk));
+    const uint64_t mc = cls == 0 ? m0 : (cls == 1 ? m1 : (cls == 2 ? m2 : m3));
+    const int below = __builtin_amdgcn_mbcnt_hi((uint32_t)(mc >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mc, 0u));
+    base = __shfl(base, cls < 0 ? 0 : cls, 64);
+    if (cls >= 0) sm->list[cls][base + below] = word;
   }
   __syncthreads();
   lf_class<T, PASS, EX, 0>(*sm, base, stride, ex0, ey0, g.bd);
