@@ -518,9 +518,11 @@ __global__ __launch_bounds__(NT) void lr_kernel(LrLaunch a) {
   lr_slice<T>(a, gp, lb, sm);
 }
 
-/* Batched: item j of a frame is slice j of its (unpadded) grid. */
+/* Batched: item j of a frame is slice j of its (unpadded) grid. The
+ * persistent form is held at 6 waves per SIMD, the occupancy the LDS
+ * allows (80 VGPRs and a 48-byte spill; unbounded it takes 110, 4 waves). */
 template <typename T, bool DIRECT>
-__global__ __launch_bounds__(NT) void lr_batch(BatchWork w) {
+__global__ __launch_bounds__(NT, DIRECT ? 1 : 6) void lr_batch(BatchWork w) {
   __shared__ LrShared sm;
   batch_loop<DIRECT>(w, [&](int f, int j) {
     const LrLaunch &a = batch_table<LrLaunch>(w.rec, f);
