@@ -94,12 +94,27 @@ def frame_layout(f):
     return {(1, 1): "4:2:0", (1, 0): "4:2:2", (0, 0): "4:4:4"}[(f.params.ssx, f.params.ssy)]
 
 
-def cpu_baseline(frames, seconds):
-    """The oracle chain on the host, threaded like the reference's decoder."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import pyoracle
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
-    threads = min(threads, 16)
+def host_threads():
+    """Host cores this process may run on (nproc)."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        return os.cpu_count() or 1
+
+
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), or None."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def _oracle_rate(frames, seconds, threads, pyoracle):
+    """Mpix/s of the oracle chain over the frames in turn for about `seconds`."""
     pyoracle.postfilter_frame(frames[0], threads=threads)  # warm-up
     n, t0 = 0, time.perf_counter()
     while True:
@@ -109,42 +124,47 @@ def cpu_baseline(frames, seconds):
         if el >= seconds or n >= 400:
             break
     px = sum(frames[i % len(frames)].luma_pixels() for i in range(n))
-    return {"value": round(px / el / 1e6, 2), "unit": "Mpix/s", "cores": threads, "kind": "port",
-            "sample": "%d frames of the same %dx%d %d-bit %s chain%s (%.1f s), C oracle, %d threads"
-                      % (n, frames[0].params.upscaled_width, frames[0].params.frame_height,
-                         frames[0].params.bit_depth, frame_layout(frames[0]),
-                         " with super-resolution" if frames[0].params.use_superres else "", el, threads)}
+    return px / el / 1e6, n, el
 
 
-BATCH_SIZES = [(640, 360), (1280, 720), (1920, 1080), (2560, 1440), (3840, 2160)]
+def cpu_baseline(frames, seconds):
+    """The oracle chain on the host, threaded like the reference's decoder
+    (set_threads(nproc), src/decoder/mod.rs:323-329), and on one thread."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    threads = host_threads()
+    what = "%dx%d %d-bit %s chain%s" % (frames[0].params.upscaled_width, frames[0].params.frame_height,
+                                        frames[0].params.bit_depth, frame_layout(frames[0]),
+                                        " with super-resolution" if frames[0].params.use_superres else "")
+    rate, n, el = _oracle_rate(frames, seconds, threads, pyoracle)
+    rate1, n1, el1 = _oracle_rate(frames, seconds, 1, pyoracle)
+    return {"value": round(rate, 2), "unit": "Mpix/s", "cores": threads, "kind": "port",
+            "sample": "%d frames of the same %s (%.1f s), C oracle, %d threads (nproc)" % (n, what, el, threads),
+            "cpu_model": cpu_model(),
+            "one_thread": {"value": round(rate1, 2), "unit": "Mpix/s", "cores": 1,
+                           "sample": "%d frames (%.1f s), 1 thread" % (n1, el1)}}
 
 
 def batch_streams(n_streams, frames_per_stream, seed):
-    """The 32-stream mixed workload: sizes uniform over BATCH_SIZES, half
-    8-bit (in 8-bit sample planes) and half 10-bit, 4:2:0 except four 4:4:4
-    streams, a random qindex in [48, 200] per stream, every filter on with
-    film grain, as the 1080p GOP configuration."""
-    import random
+    """The 32-stream mixed workload (rvpf.workload.batch_stream_configs):
+    [(bit depth, [Frame])]."""
     from rvpf import workload
-    rng = random.Random(seed)
-    streams = []
-    for s in range(n_streams):
-        w, h = BATCH_SIZES[rng.randrange(len(BATCH_SIZES))]
-        bd = 8 if s % 2 == 0 else 10
-        ss = 0 if s % 8 == 3 else 1
-        q = rng.randint(48, 200)
-        frames = [workload.Frame(w, h, bit_depth=bd, ssx=ss, ssy=ss, sb_size=128, qindex=q,
-                                 seed=seed * 1000 + s, frame_index=i, pan_px=1.5, lr=2, lr_unit_size=64,
-                                 grain=True, grain_lag=3, grain_y_points=8, with_source=False)
-                  for i in range(frames_per_stream)]
-        streams.append((bd, frames))
-    return streams
+    cfgs = workload.batch_stream_configs(n_streams, frames_per_stream, seed)
+    flat = workload.make_frames([kw for _, kws in cfgs for kw in kws])
+    out, k = [], 0
+    for bd, kws in cfgs:
+        out.append((bd, flat[k:k + len(kws)]))
+        k += len(kws)
+    return out
 
 
-def bench_batch(args):
-    """Batched path vs the plain per-frame drop-in on the same frames."""
+def run_batch(args, steps, warmup, check_all):
+    """Batched path vs the plain per-frame drop-in on the same frames of
+    the 32-stream workload; returns the figures as a dict. check_all: every
+    frame of the batched output against the oracle (else the smallest)."""
+    import numpy as np
     import torch
-    from rvpf import api
+    from rvpf import api, workload
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     log("generating %d streams x %d frames" % (args.streams, args.stream_frames))
@@ -155,7 +175,6 @@ def bench_batch(args):
         for f in frames:
             df = api.DeviceFrame(f, bytes_per_sample=1 if bd == 8 else 2)
             pristine = df.upload_planes(f.rec)
-            f.rec = None  # host copy no longer needed
             items.append({"f": f, "df": df, "pristine": pristine,
                           "rec": [None if t is None else t.clone() for t in pristine],
                           "ref": df.alloc_like_recon(), "disp": df.alloc_display()})
@@ -185,10 +204,10 @@ def bench_batch(args):
         for it in items:
             api.postfilter_frame(ctx, it["df"], it["rec"], it["ref"], it["disp"], stream)
 
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         refill()
         plain()
-    ms_plain = timed(plain, args.steps)
+    ms_plain = timed(plain, steps)
     plain_out = [[t.clone() if t is not None else None for t in it["ref"] + it["disp"]] for it in items]
     log("plain per-frame drop-in: %.3f ms per batch, %.1f Mpix/s" % (ms_plain, luma_px / ms_plain / 1e3))
 
@@ -202,66 +221,87 @@ def bench_batch(args):
         for t in it["ref"] + it["disp"]:
             if t is not None:
                 t.zero_()
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         refill()
         batch.run(stream)
-    ms_batch = timed(lambda: batch.run(stream), args.steps)
+    ms_batch = timed(lambda: batch.run(stream), steps)
     log("batched graph launch: %.3f ms per batch, %.1f Mpix/s" % (ms_batch, luma_px / ms_batch / 1e3))
     same = True
     for it, want in zip(items, plain_out):
         for t, w in zip(it["ref"] + it["disp"], want):
             if t is not None:
                 same &= bool(torch.equal(t, w))
+    del plain_out
     log("batched outputs equal the per-frame outputs:", same)
-    bitexact = None
+    bitexact, checked = None, 0
     if not args.no_check:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import pyoracle
-        import numpy as np
-        small = min(range(len(items)), key=lambda i: items[i]["f"].luma_pixels())
-        it = items[small]
-        f = it["f"]
-        f.rec = api.to_host(it["pristine"])
-        deb, ref, disp = pyoracle.postfilter_frame(f, threads=min(16, os.cpu_count() or 1))
-        got_ref, got_disp = api.to_host(it["ref"]), api.to_host(it["disp"])
-        bitexact = True
-        for p in range(f.num_planes):
-            r, c = f.ref_area(p)
-            bitexact &= bool(np.array_equal(got_ref[p][:r, :c], ref[p][:r, :c]))
-            bitexact &= bool(np.array_equal(got_disp[p], disp[p]))
-        log("bit-exact vs oracle (smallest frame):", bitexact)
+        from concurrent.futures import ThreadPoolExecutor
+        order = range(len(items)) if check_all else \
+            [min(range(len(items)), key=lambda i: items[i]["f"].luma_pixels())]
+
+        def check(i):
+            it = items[i]
+            f = it["f"]
+            deb, ref, disp = pyoracle.postfilter_frame(f, threads=4)
+            got_ref, got_disp = api.to_host(it["ref"]), api.to_host(it["disp"])
+            ok = True
+            for p in range(f.num_planes):
+                r, c = f.ref_area(p)
+                ok &= bool(np.array_equal(got_ref[p][:r, :c], ref[p][:r, :c]))
+                ok &= bool(np.array_equal(got_disp[p], disp[p]))
+            return ok
+        workers = max(1, host_threads() // 4)
+        with ThreadPoolExecutor(max_workers=workers) as ex:
+            oks = list(ex.map(check, order))
+        bitexact, checked = all(oks), len(oks)
+        log("bit-exact vs oracle (%d frames):" % checked, bitexact)
+    res = {
+        "workload": "%d streams x %d frames, sizes %s, half 8-bit (u8 planes) / half 10-bit, 4:2:0 + four "
+                    "4:4:4 streams, qindex U[48,200], all filters + film grain" %
+                    (args.streams, args.stream_frames, "/".join("%dx%d" % s for s in workload.BATCH_SIZES)),
+        "frames": len(items),
+        "mpix_per_batch": round(luma_px / 1e6, 2),
+        "steps": steps,
+        "plain_ms": round(ms_plain, 4),
+        "plain_mpix_s": round(luma_px / ms_plain / 1e3, 1),
+        "batched_ms": round(ms_batch, 4),
+        "batched_mpix_s": round(luma_px / ms_batch / 1e3, 1),
+        "batched_over_plain": round(ms_plain / ms_batch, 3),
+        "batched_equals_plain": same,
+        "bitexact_vs_oracle": bitexact,
+        "frames_checked_vs_oracle": checked,
+        "autotune_s": round(tune_s, 2),
+        "tiles": {k: {"shape": v[0], "workgroups_per_cu": v[1], "chunk": v[2], "us": round(v[3], 1)}
+                  for k, v in tiles.items()},
+    }
+    batch.close()
+    ctx.close()
+    del items
+    torch.cuda.empty_cache()
+    return res
+
+
+def bench_batch(args):
+    """--workload batch32: the batched path's own JSON line."""
+    r = run_batch(args, args.steps, args.warmup, check_all=True)
     out = {
         "metric": "batched post-filter Mpix/s, 32-stream mixed workload (one graph launch vs per-frame drop-in)",
-        "value": round(luma_px / ms_batch / 1e3, 1),
+        "value": r["batched_mpix_s"],
         "unit": "Mpix/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_batch, 4),
+        "ms_per_step": r["batched_ms"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u8/u16",
         "data": "synthetic",
-        "config": {
-            "workload": "%d streams x %d frames, sizes %s, half 8-bit (u8 planes) / half 10-bit, 4:2:0 + four "
-                        "4:4:4 streams, qindex U[48,200], all filters + film grain" %
-                        (args.streams, args.stream_frames, "/".join("%dx%d" % s for s in BATCH_SIZES)),
-            "frames": len(items),
-            "mpix_per_batch": round(luma_px / 1e6, 2),
-            "plain_ms": round(ms_plain, 4),
-            "plain_mpix_s": round(luma_px / ms_plain / 1e3, 1),
-            "batched_over_plain": round(ms_plain / ms_batch, 3),
-            "batched_equals_plain": same,
-            "bitexact_smallest_frame": bitexact,
-            "autotune_s": round(tune_s, 2),
-            "tiles": {k: {"shape": v[0], "workgroups_per_cu": v[1], "chunk": v[2], "us": round(v[3], 1)}
-                      for k, v in tiles.items()},
-        },
+        "config": r,
     }
     print(json.dumps(out), flush=True)
-    batch.close()
-    ctx.close()
 
 
 def bench_lrdiff(args):
@@ -452,7 +492,7 @@ def bench_encsearch(args):
     if not args.no_check or not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import pyoracle
-        threads = min(int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1, 16)
+        threads = host_threads()
         host_deb = api.to_host(deb)
         tc = time.perf_counter()
         want_lv, _ = pyoracle.lf_search(f, [None if a is None else a.copy() for a in f.rec], f.src, threads=threads)
@@ -530,6 +570,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-stages", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the 32-stream batched-vs-per-frame leg of the default run")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="first run a plain device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -607,7 +649,7 @@ def main():
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import pyoracle
         f = frames[0]
-        deb, ref, disp = pyoracle.postfilter_frame(f, threads=min(16, os.cpu_count() or 1))
+        deb, ref, disp = pyoracle.postfilter_frame(f, threads=host_threads())
         got_ref, got_disp = api.to_host(refs[0]), api.to_host(disps[0])
         bitexact = True
         for p in range(f.num_planes):
@@ -674,9 +716,16 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        log("cpu baseline (%.0f s budget)" % args.cpu_seconds)
+        log("cpu baseline (%.0f s budget per thread count)" % args.cpu_seconds)
         cpu = cpu_baseline(frames, args.cpu_seconds)
         log("cpu:", cpu)
+
+    batch32 = None
+    if not args.no_batch:
+        # the batched path against the per-frame drop-in on the 32-stream workload
+        del recs, refs, disps, pristine
+        torch.cuda.empty_cache()
+        batch32 = run_batch(args, steps=min(args.steps, 5), warmup=min(args.warmup, 2), check_all=True)
 
     out = {
         "metric": "post-filter Mpix/s (deblock+CDEF+LR+grain) 4K 10-bit, bit-exact; % HBM roofline",
@@ -703,6 +752,7 @@ def main():
             "chain_alg_GB_s": round(chain_gbs, 1),
             "chain_hbm_frac": round(chain_gbs / HBM_PEAK_GBS, 4),
             "stages": stages,
+            "batch32": batch32,
         },
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -180,7 +180,12 @@ typedef struct rvpf_planes {
 typedef struct rvpf_ctx rvpf_ctx;
 
 /* A context owns device scratch, uploaded tables and graph instances, on
- * the device current at creation. Calls on one context are not reentrant
+ * the device current at creation. Successive calls on one context may be
+ * enqueued on different streams: the context orders its own scratch
+ * buffers between them with events (a call's first kernel that writes a
+ * scratch buffer waits for the previous call's last kernel that read it),
+ * so each call's result is as if the calls ran one after another; the
+ * caller orders its own buffers. Calls on one context are not reentrant
  * across host threads; use one context per thread. */
 int rvpf_ctx_create(rvpf_ctx **out);
 void rvpf_ctx_destroy(rvpf_ctx *ctx);

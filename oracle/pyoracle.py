@@ -127,7 +127,8 @@ def lr(frame, deblocked, cdef_planes, threads=1):
     out = [None if a is None else np.zeros_like(a) for a in cdef_planes]
     s = frame.host_side_info()
     pd, pc, po = planes_struct(deblocked), planes_struct(cdef_planes), planes_struct(out)
-    lib().orc_lr(C.byref(frame.params), C.byref(s), C.byref(pd), C.byref(pc), C.byref(po), threads)
+    if lib().orc_lr(C.byref(frame.params), C.byref(s), C.byref(pd), C.byref(pc), C.byref(po), threads) != 0:
+        raise MemoryError("orc_lr: cannot allocate its job list")
     return out
 
 
@@ -173,7 +174,8 @@ def postfilter_frame(frame, threads=1, recon=None):
             disp.append(None)
     s = frame.host_side_info()
     pr, po = planes_struct(rec), planes_struct(ref)
-    lib().orc_postfilter_frame(C.byref(frame.params), C.byref(s), C.byref(pr), C.byref(po), arr, threads)
+    if lib().orc_postfilter_frame(C.byref(frame.params), C.byref(s), C.byref(pr), C.byref(po), arr, threads) != 0:
+        raise MemoryError("orc_postfilter_frame failed")
     return rec, ref, disp
 
 

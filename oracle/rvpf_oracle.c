@@ -765,7 +765,7 @@ typedef struct {
   const rvpf_side_info *s;
   const orc_planes *cur, *cdef;
   orc_planes *out;
-  int jobs[3 * 256][2];
+  int (*jobs)[2]; /* (plane, stripe) */
 } lr_job;
 
 /* lr_stripe + restore_stripe, src/decoder/postfilter.rs:640-716. */
@@ -811,23 +811,21 @@ int orc_lr(const rvpf_frame_params *p, const rvpf_side_info *s, const orc_planes
   int uses_lr = 0;
   for (int pl = 0; pl < num_planes(p); pl++) uses_lr |= p->frame_restoration_type[pl] != 0;
   if (!uses_lr) return 0;
-  lr_job *j = (lr_job *)malloc(sizeof(lr_job));
-  j->p = p;
-  j->s = s;
-  j->cur = deblocked;
-  j->cdef = cdef;
-  j->out = out;
   const int stripes = (p->frame_height + 8 + 63) / 64;
+  lr_job j = {p, s, deblocked, cdef, out, NULL};
+  /* one job per (plane, stripe), as the reference's job list (postfilter.rs:619-634) */
+  j.jobs = (int(*)[2])malloc(sizeof(int[2]) * (size_t)num_planes(p) * (size_t)stripes);
+  if (!j.jobs) return -1;
   int n = 0;
   for (int pl = 0; pl < num_planes(p); pl++)
     if (p->frame_restoration_type[pl] != RVPF_RESTORE_NONE)
-      for (int st = 0; st < stripes && n < 3 * 256; st++) {
-        j->jobs[n][0] = pl;
-        j->jobs[n][1] = st;
+      for (int st = 0; st < stripes; st++) {
+        j.jobs[n][0] = pl;
+        j.jobs[n][1] = st;
         n++;
       }
-  parallel_map(n, threads, lr_stripe_job, j);
-  free(j);
+  parallel_map(n, threads, lr_stripe_job, &j);
+  free(j.jobs);
   return 0;
 }
 
@@ -1737,6 +1735,7 @@ int orc_postfilter_frame(const rvpf_frame_params *p, const rvpf_side_info *s, or
     cdef.data[pl] = (uint16_t *)malloc(sizeof(uint16_t) * (size_t)w * h);
   }
   orc_cdef(p, s, recon, &cdef, threads);
+  int lr_rc;
   if (p->use_superres) {
     /* decode_frame_wrapup, mod.rs:752-758: loop restoration reads the
      * upscaled CDEF output and the upscaled deblocked frame (both zero
@@ -1755,15 +1754,16 @@ int orc_postfilter_frame(const rvpf_frame_params *p, const rvpf_side_info *s, or
     }
     orc_upscale(p, &cdef, &up_cdef);
     orc_upscale(p, recon, &up_cur);
-    orc_lr(p, s, &up_cur, &up_cdef, ref_out, threads);
+    lr_rc = orc_lr(p, s, &up_cur, &up_cdef, ref_out, threads);
     for (int pl = 0; pl < 3; pl++) {
       free(up_cdef.data[pl]);
       free(up_cur.data[pl]);
     }
   } else {
-    orc_lr(p, s, recon, &cdef, ref_out, threads);
+    lr_rc = orc_lr(p, s, recon, &cdef, ref_out, threads);
   }
   for (int pl = 0; pl < 3; pl++) free(cdef.data[pl]);
+  if (lr_rc) return lr_rc;
   if (!display) return 0;
   const int w = p->upscaled_width, h = p->frame_height;
   for (int pl = 0; pl < num_planes(p); pl++) {

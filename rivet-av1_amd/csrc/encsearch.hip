@@ -201,6 +201,16 @@ size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 using namespace rvpf;
 using namespace rvpf::host;
 
+namespace {
+/* Releases the search scratch on every return path, so a later call on
+ * another stream waits for whatever this call left in flight. */
+struct SearchGateGuard {
+  rvpf_ctx *ctx;
+  hipStream_t st;
+  ~SearchGateGuard() { (void)scratch_release(ctx, &ctx->search_gate, st); }
+};
+}  // namespace
+
 extern "C" {
 
 int rvpf_lf_search(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_side_info *s, const rvpf_planes *recon,
@@ -227,6 +237,8 @@ int rvpf_lf_search(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_side_in
   }
   const size_t plane_bytes = layout_planes(ww, wh, bps, nullptr, nullptr);
   if ((rc = ensure_buf(ctx, &ctx->search_buf, &ctx->search_bytes, plane_bytes + 256, "search scratch"))) return rc;
+  if ((rc = scratch_acquire(ctx, &ctx->search_gate, st))) return rc;
+  SearchGateGuard guard{ctx, st};
   PlanePtr work[3] = {}, rec[3], src[3];
   layout_planes(ww, wh, bps, ctx->search_buf, work);
   uint64_t *sse = reinterpret_cast<uint64_t *>((char *)ctx->search_buf + plane_bytes);
@@ -306,6 +318,8 @@ int rvpf_cdef_search(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_side_
                o_tot = align256(o_cur + (size_t)nfb * 8), o_coded = align256(o_tot + (size_t)npair * 8),
                o_pick = align256(o_coded + (size_t)nfb), bytes = align256(o_pick + (size_t)nfb);
   if ((rc = ensure_buf(ctx, &ctx->search_buf, &ctx->search_bytes, bytes, "search scratch"))) return rc;
+  if ((rc = scratch_acquire(ctx, &ctx->search_gate, st))) return rc;
+  SearchGateGuard guard{ctx, st};
   char *base = (char *)ctx->search_buf;
   uint64_t *d_ys = (uint64_t *)(base + o_ys), *d_uvs = (uint64_t *)(base + o_uvs), *d_cur = (uint64_t *)(base + o_cur),
            *d_tot = (uint64_t *)(base + o_tot);

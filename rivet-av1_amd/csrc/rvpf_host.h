@@ -21,6 +21,15 @@ struct TuneEntry {
   rvpf_tile_config cfg[RVPF_TUNE_COUNT];
 };
 
+/* A scratch buffer shared by calls that may run on different streams:
+ * `free` is recorded on the stream of the call that used the buffer last,
+ * after its last kernel that reads it, and the next call's stream waits for
+ * it before its first kernel writes the buffer. */
+struct ScratchGate {
+  hipEvent_t free;
+  int recorded;
+};
+
 /* Film grain scratch is double-buffered: the templates of frame n+1 are
  * built on the side stream while frame n still filters. Slot s is rebuilt
  * only after the apply that last read it (ev_done[s]); the apply waits for
@@ -38,8 +47,10 @@ struct rvpf_ctx {
   size_t cdef_bytes;
   void *up_buf; /* upscaled CDEF output and deblocked frame (super-resolution) */
   size_t up_bytes;
+  ScratchGate chain_gate; /* cdef_buf and up_buf */
   void *search_buf; /* encoder searches: work planes, error tables */
   size_t search_bytes;
+  ScratchGate search_gate;
   rvpf::GrainScratch grain[2];
   std::vector<TuneEntry> tune_cache;
   char err[256];
@@ -75,6 +86,11 @@ int check_launch(rvpf_ctx *ctx, int rc, const char *what);
 /* grows a scratch buffer of the context to at least `bytes` (waits for
  * the device before freeing the old one) */
 int ensure_buf(rvpf_ctx *ctx, void **buf, size_t *cap, size_t bytes, const char *what);
+/* orders a call's use of a shared scratch buffer on stream `st` after the
+ * previous call's (acquire before the first write, release after the last
+ * read) */
+int scratch_acquire(rvpf_ctx *ctx, ScratchGate *g, hipStream_t st);
+int scratch_release(rvpf_ctx *ctx, ScratchGate *g, hipStream_t st);
 /* planes of w[i] x h[i] samples one after another from base (rows padded
  * to 64 samples, planes to 256 bytes); returns the bytes; base NULL only
  * counts */

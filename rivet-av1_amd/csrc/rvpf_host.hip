@@ -430,6 +430,19 @@ int ensure_buf(rvpf_ctx *ctx, void **buf, size_t *cap, size_t bytes, const char 
   return RVPF_OK;
 }
 
+int scratch_acquire(rvpf_ctx *ctx, ScratchGate *g, hipStream_t st) {
+  if (!g->recorded) return RVPF_OK;
+  hipError_t e = hipStreamWaitEvent(st, g->free, 0);
+  return e == hipSuccess ? RVPF_OK : hip_fail(ctx, e, "stream wait (scratch)");
+}
+
+int scratch_release(rvpf_ctx *ctx, ScratchGate *g, hipStream_t st) {
+  hipError_t e = hipEventRecord(g->free, st);
+  if (e != hipSuccess) return hip_fail(ctx, e, "event record (scratch)");
+  g->recorded = 1;
+  return RVPF_OK;
+}
+
 /* Lays out planes of w[i] x h[i] samples (planes with w[i] == 0 get none)
  * one after another from `base`, rows padded to 64 samples and planes to
  * 256 bytes; returns the bytes needed. With base NULL only counts. */
@@ -609,6 +622,8 @@ int superres_tail(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_side_inf
   const Geom g = make_geom(p);
   const int np = g.nplanes;
   int rc;
+  /* cdef_buf and up_buf are written below: wait for the call that read them last */
+  if ((rc = scratch_acquire(ctx, &ctx->chain_gate, st))) return rc;
   PlanePtr cdef[3];
   for (int i = 0; i < 3; i++) cdef[i] = rec[i];
   if (cdef_on(p)) {
@@ -665,9 +680,12 @@ int superres_tail(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_side_inf
     if ((rc = check_launch(ctx, launch_upscale(g, p->frame_width, p->upscaled_width, jobs, nj, bps, st), "upscale")))
       return rc;
   }
-  if (!lr_on(p)) return RVPF_OK;
-  StageMark m(ctx, RVPF_STAGE_LR, st);
-  return check_launch(ctx, launch_lr(make_lr_geom(p), make_lr(p, s), up_deb, up_cdef, ref, bps, st), "lr");
+  if (lr_on(p)) {
+    StageMark m(ctx, RVPF_STAGE_LR, st);
+    if ((rc = check_launch(ctx, launch_lr(make_lr_geom(p), make_lr(p, s), up_deb, up_cdef, ref, bps, st), "lr")))
+      return rc;
+  }
+  return scratch_release(ctx, &ctx->chain_gate, st);
 }
 
 }  // namespace host
@@ -686,6 +704,8 @@ int rvpf_ctx_create(rvpf_ctx **out) {
   hipError_t e = hipGetDevice(&ctx->device);
   if (e == hipSuccess) e = hipDeviceGetAttribute(&ctx->cus, hipDeviceAttributeMultiprocessorCount, ctx->device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->chain_gate.free, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->search_gate.free, hipEventDisableTiming);
   for (int i = 0; i < 2; i++) {
     if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_ready[i], hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_done[i], hipEventDisableTiming);
@@ -721,6 +741,8 @@ void rvpf_ctx_destroy(rvpf_ctx *ctx) {
     if (ctx->ev_ready[i]) (void)hipEventDestroy(ctx->ev_ready[i]);
     if (ctx->ev_done[i]) (void)hipEventDestroy(ctx->ev_done[i]);
   }
+  if (ctx->chain_gate.free) (void)hipEventDestroy(ctx->chain_gate.free);
+  if (ctx->search_gate.free) (void)hipEventDestroy(ctx->search_gate.free);
   if (ctx->side) (void)hipStreamDestroy(ctx->side);
   delete ctx;
 }
@@ -956,6 +978,9 @@ int rvpf_postfilter_frame(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_
   } else {
     /* 2. CDEF: planes restored next go to scratch, the others straight to ref_out */
     PlanePtr cdef[3];
+    /* CDEF writes cdef_buf for the restored planes and LR reads it */
+    const bool scratch = cdef_on(p) && any_lr;
+    if (scratch && (rc = scratch_acquire(ctx, &ctx->chain_gate, st))) return rc;
     if (cdef_on(p)) {
       for (int i = 0; i < 3; i++) cdef[i] = ref[i];
       if (any_lr) {
@@ -988,6 +1013,7 @@ int rvpf_postfilter_frame(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_
       StageMark m(ctx, RVPF_STAGE_LR, st);
       if ((rc = check_launch(ctx, launch_lr(g, make_lr(p, s), rec, cdef, ref, bps, st), "lr"))) return rc;
     }
+    if (scratch && (rc = scratch_release(ctx, &ctx->chain_gate, st))) return rc;
   }
   /* 4. display */
   if (!display) return RVPF_OK;

@@ -124,3 +124,37 @@ class Frame:
 
     def luma_pixels(self):
         return self.params.upscaled_width * self.params.frame_height
+
+
+# Resolutions of the mixed multi-stream workload.
+BATCH_SIZES = [(640, 360), (1280, 720), (1920, 1080), (2560, 1440), (3840, 2160)]
+
+
+def batch_stream_configs(n_streams, frames_per_stream, seed):
+    """Frame configurations of the mixed multi-stream workload: per stream a
+    size drawn uniformly from BATCH_SIZES, half 8-bit and half 10-bit, 4:2:0
+    except every eighth stream (4:4:4), a random qindex in [48, 200], SB 128,
+    every filter on (restoration mixing Wiener / self-guided per 64-sample
+    unit) with film grain (lag 3, 8 luma points), frames of a GOP with a
+    1.5 pixel per frame pan. Returns [(bit depth, [Frame kwargs])]."""
+    import random
+    rng = random.Random(seed)
+    streams = []
+    for s in range(n_streams):
+        w, h = BATCH_SIZES[rng.randrange(len(BATCH_SIZES))]
+        bd = 8 if s % 2 == 0 else 10
+        ss = 0 if s % 8 == 3 else 1
+        q = rng.randint(48, 200)
+        streams.append((bd, [dict(width=w, height=h, bit_depth=bd, ssx=ss, ssy=ss, sb_size=128, qindex=q,
+                                  seed=seed * 1000 + s, frame_index=i, pan_px=1.5, lr=2, lr_unit_size=64,
+                                  grain=True, grain_lag=3, grain_y_points=8, with_source=False)
+                             for i in range(frames_per_stream)]))
+    return streams
+
+
+def make_frames(configs, workers=16):
+    """Frames of many configurations, generated on a pool of threads (the
+    generator runs in C without the interpreter lock)."""
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=max(1, workers)) as ex:
+        return list(ex.map(lambda kw: Frame(**kw), configs))

@@ -123,3 +123,105 @@ def test_chain_with_only_upscaling():
         ch, cw = f.crop_shape(p)
         assert np.array_equal(ref[p][:ch, :cw], up[p][:ch, :cw])
         assert np.array_equal(disp[p], up[p][:ch, :cw])
+
+
+def _tdiv(a, b):
+    """Integer division truncating toward zero (the spec's "/", section 4.6)."""
+    q = abs(a) // abs(b)
+    return q if (a >= 0) == (b >= 0) else -q
+
+
+def _spec_positions(dw, uw):
+    """(integer source position, 1/64 phase) of every output column, from
+    7.16's position arithmetic (SUPERRES_SCALE_BITS 14, EXTRA_BITS 8)."""
+    step = _tdiv((dw << 14) + _tdiv(uw, 2), uw)
+    err = uw * step - (dw << 14)
+    init = _tdiv(-((uw - dw) << 13) + _tdiv(uw, 2), uw) + (1 << 7) - _tdiv(err, 2)
+    init &= (1 << 14) - 1
+    pos = [-(1 << 14) + init + x * step for x in range(uw)]
+    return [p >> 14 for p in pos], [(p & ((1 << 14) - 1)) >> 8 for p in pos]
+
+
+def _upscale_filter():
+    """UPSCALE_FILTER of the library's tables (its rows hash-checked against
+    the reference's src/tables.rs in test_tables.py)."""
+    import os
+    import re
+    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "rivet-av1_amd", "csrc",
+                        "av1_tables.h")
+    text = open(path).read()
+    body = text[text.index("AV1_UPSCALE_FILTER[64][8]"):]
+    body = body[body.index("{") + 1:body.index("};")]
+    vals = [int(v) for v in re.findall(r"-?\d+", body)]
+    return np.array(vals, dtype=np.int64).reshape(64, 8)
+
+
+@pytest.mark.parametrize("w,denom,ssx", [(226, 12, 1), (333, 9, 1), (200, 16, 0), (97, 13, 1), (640, 11, 0),
+                                          (150, 10, 1), (777, 15, 0), (514, 14, 1)])
+def test_phase_and_offset_of_every_column_by_impulse_response(w, denom, ssx):
+    """Each output column's integer source position and 1/64 phase, read
+    back from the oracle's impulse responses, equal 7.16's position
+    arithmetic exactly, and the response is the filter row of that phase
+    with the taps clamped at column 0 and at the mode-info edge.
+
+    12-bit planes: a mid-grey background of 2048 with impulses of +1024
+    make the output 2048 + 8 * tap exactly (taps sum to 128), never clipped.
+    Row r holds impulses at the source columns congruent to r modulo 16, so
+    each output column sees at most one impulse per row and all 16 rows
+    together give its response to every source column."""
+    F = _upscale_filter()
+    f = workload.Frame(w, 32, bit_depth=12, ssx=ssx, ssy=1, qindex=100, lr=0, seed=7, superres_denom=denom,
+                       with_source=False)
+    planes = []
+    for p in range(3):
+        ph, pw = f.plane_shape(p)
+        a = np.full((ph, pw), 2048, dtype=np.uint16)
+        for r in range(ph):
+            a[r, (r % 16)::16] += 1024
+        planes.append(a)
+    up = pyoracle.upscale(f, planes)
+    for p in range(3):
+        sx = ssx if p else 0
+        dw, uw = (f.params.frame_width + sx) >> sx, (w + sx) >> sx
+        max_x = (f.params.mi_cols >> sx) * 4 - 1
+        px_spec, sub_spec = _spec_positions(dw, uw)
+        # measured response of column x to source column j (16 rows cover all j mod 16)
+        resp = np.zeros((uw, max_x + 1), dtype=np.int64)
+        for r in range(16):
+            d = up[p][r, :uw].astype(np.int64) - 2048
+            assert np.all(d % 8 == 0)
+            for x in range(uw):
+                # the impulses of row r that column x can reach, one at most
+                js = [j for j in range(r % 16, max_x + 1, 16) if abs(j - px_spec[x]) <= 8 or j in (0, max_x)]
+                hit = [j for j in js if max(0, min(max_x, j)) == j and
+                       any(max(0, min(max_x, px_spec[x] + k - 3)) == j for k in range(8))]
+                assert len(hit) <= 1
+                if hit:
+                    resp[x, hit[0]] = d[x] // 8
+                else:
+                    assert d[x] == 0, (p, x, r)
+        readout_checked = 0
+        for x in range(uw):
+            # the phase and offset the response itself shows (interior columns)
+            nz = np.nonzero(resp[x])[0]
+            if px_spec[x] - 3 >= 0 and px_spec[x] + 4 <= max_x:
+                matches = []
+                for sub in range(64):
+                    row = F[sub]
+                    k0 = int(np.nonzero(row)[0][0])
+                    px = int(nz[0]) + 3 - k0
+                    if 0 <= px - 3 and px + 4 <= max_x and np.array_equal(resp[x, px - 3:px + 5], row) and \
+                            np.count_nonzero(resp[x]) == np.count_nonzero(row):
+                        matches.append((px, sub))
+                assert matches == [(px_spec[x], sub_spec[x])], (p, x, matches, px_spec[x], sub_spec[x])
+                readout_checked += 1
+            # every column, edges included: the clamped filter row of its spec position
+            want = np.zeros(max_x + 1, dtype=np.int64)
+            for k in range(8):
+                want[max(0, min(max_x, px_spec[x] + k - 3))] += F[sub_spec[x]][k]
+            assert np.array_equal(resp[x], want), (p, x)
+        assert readout_checked > uw // 2
+        # the positions are the centre-aligned ones to within 1/32 sample
+        pos = np.array(px_spec) + np.array(sub_spec) / 64.0
+        ideal = (np.arange(uw) + 0.5) * dw / uw - 0.5
+        assert np.max(np.abs(pos - ideal)) < 1 / 32 + 1e-9, (p, np.max(np.abs(pos - ideal)))
