@@ -5,9 +5,9 @@
  *
  * grain_prepare_kernel: one workgroup of three waves per frame.
  *   - Gaussian templates (7.18.3.3): luma 73x82, cb/cr 38x44 (4:2:0) or
- *     larger. The 16-bit LFSR is linear over GF(2), so lane t jumps to the
- *     state after t*C steps with precomputed powers A^(2^k) of the step
- *     matrix and then generates its C values.
+ *     larger. The 16-bit LFSR is linear over GF(2), so the thread of a
+ *     template row starts from A^(y W) seed (a precomputed matrix per row)
+ *     and steps along its row.
  *   - The auto-regressive filter is a recursion: sample (y, x) reads rows
  *     y-lag..y and columns x-lag..x+lag. Processing it along the diagonals
  *     T = (lag+1)*y + x visits every dependency at an earlier T, so all
@@ -16,7 +16,10 @@
  *     are passed between lanes by DPP rotations and stay in registers.
  *     Luma runs on wave 0, cb and cr on waves 1 and 2, a fixed number of
  *     diagonals behind the luma samples they read.
- *   - Scaling LUTs (7.18.3.4) and the per-stripe random block offsets.
+ *   - The filtered templates into copy 0 of the scratch.
+ * grain_tables_kernel: TABLE_ITEMS workgroups per frame: the 7 shifted
+ *   copies of the templates, the scaling LUTs (7.18.3.4) and the per-stripe
+ *   random block offsets.
  * grain_apply_kernel: one workgroup per (noise stripe, 64 luma columns).
  *   The noise of a sample is rebuilt directly from the template: its 32x32
  *   block's offset, the horizontal blend with the block to the left, then
@@ -42,10 +45,6 @@ constexpr uint16_t lfsr_step(uint16_t r) {
   return (uint16_t)((r >> 1) | ((((r) ^ (r >> 1) ^ (r >> 3) ^ (r >> 12)) & 1) << 15));
 }
 
-struct JumpTable {
-  uint16_t col[16][16]; /* col[k][j] = A^(2^k) e_j */
-};
-
 constexpr uint16_t apply_cols(const uint16_t *cols, uint16_t s) {
   uint16_t r = 0;
   for (int j = 0; j < 16; j++)
@@ -53,26 +52,31 @@ constexpr uint16_t apply_cols(const uint16_t *cols, uint16_t s) {
   return r;
 }
 
-constexpr JumpTable make_jump_table() {
-  JumpTable t{};
-  for (int j = 0; j < 16; j++) t.col[0][j] = lfsr_step((uint16_t)(1u << j));
-  for (int k = 1; k < 16; k++)
-    for (int j = 0; j < 16; j++) t.col[k][j] = apply_cols(t.col[k - 1], t.col[k - 1][j]);
+/* LFSR states at the start of every template row: col[w][y][j] = A^(y W) e_j
+ * for rows of W = 82 (w = 0) or 44 (w = 1) samples, so the state before
+ * sample (y, 0) is one matrix-vector product with the seed. */
+struct RowJump {
+  uint16_t col[2][73][16];
+};
+
+constexpr RowJump make_row_jump() {
+  RowJump t{};
+  const int widths[2] = {82, 44};
+  for (int w = 0; w < 2; w++) {
+    uint16_t m1[16] = {}; /* A^W */
+    for (int j = 0; j < 16; j++) {
+      uint16_t v = (uint16_t)(1u << j);
+      for (int k = 0; k < widths[w]; k++) v = lfsr_step(v);
+      m1[j] = v;
+    }
+    for (int j = 0; j < 16; j++) t.col[w][0][j] = (uint16_t)(1u << j);
+    for (int y = 1; y < 73; y++)
+      for (int j = 0; j < 16; j++) t.col[w][y][j] = apply_cols(m1, t.col[w][y - 1][j]);
+  }
   return t;
 }
 
-constexpr JumpTable JUMP = make_jump_table();
-
-__device__ __forceinline__ uint32_t jump(uint32_t s, uint32_t n) {
-  for (int k = 0; k < 16 && n; k++, n >>= 1)
-    if (n & 1) {
-      uint32_t r = 0;
-#pragma unroll
-      for (int j = 0; j < 16; j++) r ^= ((s >> j) & 1) ? JUMP.col[k][j] : 0u;
-      s = r;
-    }
-  return s;
-}
+constexpr RowJump ROW_JUMP = make_row_jump();
 
 __device__ __forceinline__ uint32_t step(uint32_t r) {
   const uint32_t bit = (r ^ (r >> 1) ^ (r >> 3) ^ (r >> 12)) & 1;
@@ -85,25 +89,37 @@ __device__ __forceinline__ int clip3(int lo, int hi, int x) { return min(max(x, 
 constexpr int TW = 82, TH = 73;
 constexpr int PREP_THREADS = 192; /* three waves: luma, cb, cr */
 
-/* Gaussian values of the LFSR sequence from `seed` into t (rows x cols, stride TW). */
-__device__ void gaussian_fill(int16_t *t, const int16_t *gauss, uint32_t seed, int rows, int cols, int shift,
-                              bool on, int lane) {
-  const int n = rows * cols;
-  const int per = (n + 63) / 64;
-  const int i0 = lane * per, i1 = min(i0 + per, n);
-  int r = i0 / cols, c = i0 - r * cols;
-  uint32_t s = on ? jump(seed, (uint32_t)i0) : 0;
-  for (int i = i0; i < i1; i++) {
-    int v = 0;
-    if (on) {
-      s = step(s);
-      v = round2(gauss[(s >> 5) & 2047], shift);
+/* Row y of a Gaussian template (7.18.3.3, grain.rs:40-56): the LFSR
+ * sequence from `seed` runs over the template in raster order, so the row
+ * starts at state A^(y cols) seed (ROW_JUMP, wsel = 0 for 82 columns, 1 for
+ * 44). The LFSR's bit stream b obeys b[k + 16] = b[k] ^ b[k + 1] ^ b[k + 3]
+ * ^ b[k + 12], so 4 new bits at a time come from bits already known: 16
+ * steps are 4 such groups, after which a 32-bit window holds b[0 .. 31]
+ * and the state after step m is b[m .. m + 15]; its table index (state >> 5)
+ * & 2047 is the 11-bit field at m + 5. The 16 table reads are issued before
+ * the 16 template writes. Template rows that are off hold 0. */
+__device__ void fill_row(int16_t *t, const int16_t *gauss, uint32_t seed, int y, int cols, int wsel, int shift,
+                         bool on) {
+  uint32_t s = 0;
+#pragma unroll
+  for (int j = 0; j < 16; j++) s ^= (uint32_t)ROW_JUMP.col[wsel][y][j] & (0u - ((seed >> j) & 1u));
+  const int keep = on ? -1 : 0;
+  int16_t *row = t + y * TW;
+  for (int x0 = 0; x0 < cols; x0 += 16) {
+    uint32_t w = s;
+#pragma unroll
+    for (int g = 0; g < 4; g++) {
+      const int n = 4 * g;
+      const uint32_t nb = ((w >> n) ^ (w >> (n + 1)) ^ (w >> (n + 3)) ^ (w >> (n + 12))) & 15u;
+      w |= nb << (16 + n);
     }
-    t[r * TW + c] = (int16_t)v;
-    if (++c == cols) {
-      c = 0;
-      r++;
-    }
+    int v[16];
+#pragma unroll
+    for (int m = 1; m <= 16; m++) v[m - 1] = gauss[__builtin_amdgcn_ubfe(w, m + 5, 11)];
+#pragma unroll
+    for (int k = 0; k < 16; k++)
+      if (x0 + k < cols) row[x0 + k] = (int16_t)(round2(v[k], shift) & keep);
+    s = w >> 16;
   }
 }
 
@@ -127,7 +143,7 @@ __device__ __forceinline__ void chroma_lag0(int16_t *tcb, int16_t *tcr, const in
 
 /* Lane l receives the value of lane (l - 1) mod 64 (DPP wave_ror:1). */
 __device__ __forceinline__ int from_lane_below(int v) {
-  return __builtin_amdgcn_update_dpp(0, v, 0x13C, 0xF, 0xF, false);
+  return __builtin_amdgcn_mov_dpp(v, 0x13C, 0xF, 0xF, false);
 }
 
 /* Diagonals per round of the AR recursion (one workgroup barrier each). */
@@ -139,11 +155,160 @@ constexpr int AR_ROUND = 16;
  * (subsampled) or 16j+15 (4:4:4), so it runs in barrier round 2j+2 or j+1,
  * when every luma round it depends on is behind a barrier. */
 struct ArPlan {
-  int first, every, rounds;
+  int first;  /* barrier round of the first chunk */
+  int halves; /* barrier rounds per chunk of AR_ROUND diagonals (1 or 2) */
+  int rounds; /* barrier rounds in all */
 };
 
 __device__ __forceinline__ int ar_rounds(int K, int rows, int cols) {
   return (K * (rows - 1) + cols - 1) / AR_ROUND + 1;
+}
+
+typedef short short2_g __attribute__((ext_vector_type(2)));
+
+/* acc + a.lo * b.lo + a.hi * b.hi (signed 16-bit halves; v_dot2_i32_i16) */
+__device__ __forceinline__ int dot2(uint32_t a, uint32_t b, int acc) {
+  return __builtin_amdgcn_sdot2(__builtin_bit_cast(short2_g, a), __builtin_bit_cast(short2_g, b), acc, false);
+}
+
+/* clip3(lo, hi, x) as one v_med3_i32 (lo <= hi) */
+__device__ __forceinline__ int med3(int lo, int hi, int x) {
+  int r;
+  asm("v_med3_i32 %0, %1, %2, %3" : "=v"(r) : "v"(x), "v"(lo), "v"(hi));
+  return r;
+}
+
+/* (lo & 0xffff) | (hi << 16) */
+__device__ __forceinline__ uint32_t pack16(int lo, int hi) {
+  return __builtin_amdgcn_perm((uint32_t)hi, (uint32_t)lo, 0x05040100u);
+}
+
+/* The taps of the AR filter as dot products of packed pairs. Row d above
+ * (d = LAG .. 1) has the 2 LAG + 1 taps dc = -LAG .. LAG, the row itself
+ * (d = 0) the LAG taps dc = -LAG .. -1. Tap (d, dc) of the sample on
+ * diagonal T lies on diagonal T - (d K - dc), so the taps of a row are
+ * consecutive diagonals; they are paired from the latest one down, (dc - 1,
+ * dc), and an odd one out takes the upper half of a pair whose lower
+ * coefficient is 0. A pair (dc_lo, dc_lo + 1) is the packed history entry
+ * of diagonal T - m, m = d K - dc_lo. */
+template <int LAG>
+struct ArTaps {
+  static constexpr int K = LAG + 1;
+  static constexpr int NP = LAG * (LAG + 1) + (LAG + 1) / 2; /* dot products per sample */
+  int d[NP], m[NP], lo[NP], hi[NP];                        /* coefficient index of each half, -1 = 0 */
+  constexpr ArTaps() : d{}, m{}, lo{}, hi{} {
+    int n = 0;
+    /* oldest pairs first, so the latest taps come last in the sum */
+    for (int dd = LAG; dd >= 0; dd--) {
+      const int base = (LAG - dd) * (2 * LAG + 1);
+      const int first = -LAG, last = dd ? LAG : -1;
+      int q[2 * LAG + 2][2] = {}, nq = 0;
+      for (int dc = last; dc >= first; dc -= 2) {
+        q[nq][0] = dc - 1 >= first ? base + (dc - 1 + LAG) : -1;
+        q[nq][1] = base + (dc + LAG);
+        nq++;
+      }
+      for (int k = nq - 1; k >= 0; k--) {
+        const int dc_hi = last - 2 * k;
+        d[n] = dd;
+        m[n] = dd * K - (dc_hi - 1);
+        lo[n] = q[k][0];
+        hi[n] = q[k][1];
+        n++;
+      }
+    }
+  }
+};
+
+/* The auto-regressive filter of 7.18.3.3 (grain.rs:57-73 luma, 101-134
+ * chroma) over one template, on one wave. Samples are visited along the
+ * diagonals T = K*y + x (K = LAG + 1): every sample (y - d, x + dc) that
+ * (y, x) reads lies on an earlier diagonal, T - (d*K - dc).
+ *
+ * Lane l owns the template rows y = (l + 3) mod 64 and y + 64 and handles,
+ * on diagonal T, the sample of its row at x = T - K*y (the two rows of a
+ * lane are never both inside the template on one diagonal). Each lane
+ * emits one value per diagonal: the filtered sample inside the filtered
+ * area, the unfiltered template value elsewhere. It keeps its last 16
+ * emitted values as packed pairs of consecutive diagonals (PO[j & 15] =
+ * (value of j, value of j + 1)) and the same pairs of the lanes 1, 2 and 3
+ * below it (PN1, PN2, PN3, each filled by one lane rotation of the previous
+ * array per diagonal), which are exactly the rows above its row; the sum
+ * of a sample is then NP dot products of pairs with packed coefficients,
+ * in registers indexed by T mod 16 (the rounds of 16 diagonals are
+ * unrolled, so every index is a constant). The filtered value replaces the
+ * sample only inside the filtered area (a select, no branch).
+ *
+ * CHROMA adds the tap on the co-located luma average (read from the finished
+ * luma template in LDS). coef holds the 2 LAG K neighbour coefficients. */
+/* Diagonals T0 + S0 .. T0 + S1 - 1 of one template's AR filter (see
+ * ar_wave). Every sample of these diagonals is read before any is written:
+ * each template position is written only on its own diagonal. A lane
+ * outside the template on a diagonal reads and writes its spare sample; am
+ * is -1 where the sample is filtered, else 0. The luma tap of a chroma
+ * sample averages the co-located luma samples (clamped into the template
+ * for lanes whose sample is not filtered). */
+template <int LAG, bool CHROMA, int S0, int S1>
+__device__ __forceinline__ void ar_steps(int16_t *t, int rows, int cols, const uint32_t *cp, int luma_coef,
+                                         const int16_t *tl, int lsx, int lsy, bool luma_tap, int ars, int gmin,
+                                         int gmax, int T0, uint32_t (&PO)[AR_ROUND], uint32_t (&PN1)[AR_ROUND],
+                                         uint32_t (&PN2)[AR_ROUND], uint32_t (&PN3)[AR_ROUND], int &prev) {
+  constexpr int K = LAG + 1;
+  constexpr ArTaps<LAG> TP{};
+  constexpr int NP = ArTaps<LAG>::NP;
+  constexpr int N = S1 - S0;
+  const int lane = threadIdx.x & 63;
+  const int ya = (lane + 3) & 63;
+  const int rc = 1 << (ars - 1);
+  const int x0 = T0 - K * ya;
+  int tv[N], lv[N], addr[N], am[N];
+#pragma unroll
+  for (int k = 0; k < N; k++) {
+    int x = x0 + S0 + k, y = ya;
+    if (x >= cols) {
+      x -= 64 * K;
+      y += 64;
+    }
+    const bool valid = y < rows && x >= 0 && x < cols;
+    am[k] = -(int)(valid && y >= 3 && x >= 3 && x < cols - 3);
+    addr[k] = valid ? y * TW + x : TH * TW + lane;
+    tv[k] = t[addr[k]];
+    lv[k] = 0;
+    if (CHROMA && luma_tap) {
+      const int ly = min(max(((y - 3) << lsy) + 3, 0), TH - 1 - lsy), lx = min(max(((x - 3) << lsx) + 3, 0), TW - 1 - lsx);
+      const int16_t *q = tl + ly * TW + lx;
+      int l = q[0];
+      if (lsx) l += q[1];
+      if (lsy) l += q[TW];
+      if (lsx && lsy) l += q[TW + 1];
+      lv[k] = round2(l, lsx + lsy);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < N; k++) {
+    const int s = S0 + k;
+    /* three independent sums of the older pairs, then the two latest
+     * pairs (rows 0 and 1, diagonals T - 2 and T - 1) on two of them,
+     * so only one dot product and one add follow the latest values */
+    int acc[3] = {rc, 0, (CHROMA && luma_tap) ? __mul24(lv[k], luma_coef) : 0};
+#pragma unroll
+    for (int i = 0; i < NP; i++) {
+      const int idx = (s - TP.m[i]) & (AR_ROUND - 1);
+      const uint32_t pv = TP.d[i] == 0 ? PO[idx] : TP.d[i] == 1 ? PN1[idx] : TP.d[i] == 2 ? PN2[idx] : PN3[idx];
+      const int a = i >= NP - 2 ? i - (NP - 2) : i % 3;
+      acc[a] = dot2(pv, cp[i], acc[a]);
+    }
+    const int sum = acc[0] + acc[1] + acc[2];
+    const int val = med3(gmin, gmax, tv[k] + (sum >> ars));
+    const int out = (val & am[k]) | (tv[k] & ~am[k]);
+    t[addr[k]] = (int16_t)out;
+    const int j = (s - 1) & (AR_ROUND - 1);
+    PO[j] = pack16(prev, out);
+    prev = out;
+    if (LAG >= 1) PN1[j] = (uint32_t)from_lane_below((int)PO[j]);
+    if (LAG >= 2) PN2[j] = (uint32_t)from_lane_below((int)PN1[j]);
+    if (LAG >= 3) PN3[j] = (uint32_t)from_lane_below((int)PN2[j]);
+  }
 }
 
 /* The auto-regressive filter of 7.18.3.3 (grain.rs:57-73 luma, 101-134
@@ -155,83 +320,51 @@ __device__ __forceinline__ int ar_rounds(int K, int rows, int cols) {
  * on diagonal T, the sample of its row at x = T - K*y (the two rows of a
  * lane are never both inside the template on one diagonal). Each lane
  * emits one value per diagonal: the filtered sample inside the filtered
- * area, the unfiltered template value on the 3-sample border. It keeps the
- * last 16 values emitted by itself (O) and by the lanes 1, 2 and 3 below
- * it (N1, N2, N3, each filled by one lane rotation of the previous array
- * per diagonal), which are exactly the rows above its row: sample
- * (y - d, x + dc) is N_d[T - (d*K - dc)] (O for d = 0). The taps therefore
- * come from registers, in registers indexed by T mod 16 (the rounds of 16
- * diagonals are unrolled, so every index is a constant).
+ * area, the unfiltered template value elsewhere. It keeps its last 16
+ * emitted values as packed pairs of consecutive diagonals (PO[j & 15] =
+ * (value of j, value of j + 1)) and the same pairs of the lanes 1, 2 and 3
+ * below it (PN1, PN2, PN3, each filled by one lane rotation of the previous
+ * array per diagonal), which are exactly the rows above its row; the sum
+ * of a sample is then NP dot products of pairs with packed coefficients,
+ * in registers indexed by T mod 16 (every index is a constant of the
+ * unrolled steps). The filtered value replaces the sample only inside the
+ * filtered area (a select, no branch).
  *
- * CHROMA adds the tap on the co-located luma average (read from the finished
- * luma template in LDS). coef holds the NL neighbour coefficients (then the
- * luma one). */
+ * The diagonals run in chunks of AR_ROUND, a chunk over plan.halves barrier
+ * rounds (half a chunk per round for subsampled chroma, whose rounds then
+ * cost about what a luma round costs). CHROMA adds the tap on the
+ * co-located luma average (read from the finished luma template in LDS).
+ * coef holds the 2 LAG K neighbour coefficients. */
 template <int LAG, bool CHROMA>
 __device__ void ar_wave(int16_t *t, int rows, int cols, const int *coef, int luma_coef, const int16_t *tl, int lsx,
                         int lsy, bool luma_tap, int ars, int gmin, int gmax, ArPlan plan) {
-  constexpr int K = LAG + 1, NL = 2 * LAG * K;
-  const int lane = threadIdx.x & 63;
-  const int ya = (lane + 3) & 63;
-  int c[NL > 0 ? NL : 1];
+  constexpr int K = LAG + 1;
+  constexpr ArTaps<LAG> TP{};
+  constexpr int NP = ArTaps<LAG>::NP;
+  uint32_t cp[NP];
 #pragma unroll
-  for (int i = 0; i < NL; i++) c[i] = coef[i];
-  int O[AR_ROUND], N1[AR_ROUND], N2[AR_ROUND], N3[AR_ROUND];
+  for (int i = 0; i < NP; i++) cp[i] = pack16(TP.lo[i] >= 0 ? coef[TP.lo[i]] : 0, coef[TP.hi[i]]);
+  uint32_t PO[AR_ROUND], PN1[AR_ROUND], PN2[AR_ROUND], PN3[AR_ROUND];
 #pragma unroll
-  for (int i = 0; i < AR_ROUND; i++) O[i] = N1[i] = N2[i] = N3[i] = 0;
+  for (int i = 0; i < AR_ROUND; i++) PO[i] = PN1[i] = PN2[i] = PN3[i] = 0;
+  int prev = 0; /* value this lane emitted on the previous diagonal */
   const int nchunks = ar_rounds(K, rows, cols);
-  int chunk = 0;
   for (int r = 0; r < plan.rounds; r++) {
-    if (r >= plan.first && (r - plan.first) % plan.every == 0 && chunk < nchunks) {
-      const int T0 = chunk * AR_ROUND;
-      int tv[AR_ROUND], lv[AR_ROUND], addr[AR_ROUND];
-      bool act[AR_ROUND];
-      /* every sample of this round is read before any is written: each
-       * template position is written only on its own diagonal */
-#pragma unroll
-      for (int s = 0; s < AR_ROUND; s++) {
-        int x = T0 + s - K * ya, y = ya;
-        if (x >= cols) {
-          x -= 64 * K;
-          y += 64;
-        }
-        const bool valid = y < rows && x >= 0 && x < cols;
-        act[s] = valid && y >= 3 && x >= 3 && x < cols - 3;
-        addr[s] = valid ? y * TW + x : 0;
-        tv[s] = t[addr[s]];
-        lv[s] = 0;
-        if (CHROMA && luma_tap) {
-          const int ly = act[s] ? ((y - 3) << lsy) + 3 : 0, lx = act[s] ? ((x - 3) << lsx) + 3 : 0;
-          const int16_t *q = tl + ly * TW + lx;
-          int l = q[0];
-          if (lsx) l += q[1];
-          if (lsy) l += q[TW];
-          if (lsx && lsy) l += q[TW + 1];
-          lv[s] = round2(l, lsx + lsy);
-        }
+    const int q = r - plan.first;
+    if (q >= 0 && q < nchunks * plan.halves) {
+      if constexpr (!CHROMA) {
+        ar_steps<LAG, false, 0, AR_ROUND>(t, rows, cols, cp, luma_coef, tl, lsx, lsy, luma_tap, ars, gmin, gmax,
+                                          q * AR_ROUND, PO, PN1, PN2, PN3, prev);
+      } else if (plan.halves == 1) {
+        ar_steps<LAG, true, 0, AR_ROUND>(t, rows, cols, cp, luma_coef, tl, lsx, lsy, luma_tap, ars, gmin, gmax,
+                                         q * AR_ROUND, PO, PN1, PN2, PN3, prev);
+      } else if ((q & 1) == 0) {
+        ar_steps<LAG, true, 0, AR_ROUND / 2>(t, rows, cols, cp, luma_coef, tl, lsx, lsy, luma_tap, ars, gmin, gmax,
+                                             (q >> 1) * AR_ROUND, PO, PN1, PN2, PN3, prev);
+      } else {
+        ar_steps<LAG, true, AR_ROUND / 2, AR_ROUND>(t, rows, cols, cp, luma_coef, tl, lsx, lsy, luma_tap, ars, gmin,
+                                                    gmax, (q >> 1) * AR_ROUND, PO, PN1, PN2, PN3, prev);
       }
-#pragma unroll
-      for (int s = 0; s < AR_ROUND; s++) {
-        int acc[4] = {0, 0, 0, 0};
-        int pos = 0;
-#pragma unroll
-        for (int d = LAG; d >= 0; d--)
-#pragma unroll
-          for (int dc = -LAG; dc <= (d ? LAG : -1); dc++) {
-            const int idx = (s - (d * K - dc)) & (AR_ROUND - 1);
-            const int v = d == 0 ? O[idx] : d == 1 ? N1[idx] : d == 2 ? N2[idx] : N3[idx];
-            acc[pos & 3] += __mul24(v, c[pos]);
-            pos++;
-          }
-        int sum = (acc[0] + acc[1]) + (acc[2] + acc[3]);
-        if (CHROMA && luma_tap) sum += __mul24(lv[s], luma_coef);
-        const int out = act[s] ? clip3(gmin, gmax, tv[s] + round2(sum, ars)) : tv[s];
-        if (act[s]) t[addr[s]] = (int16_t)out;
-        O[s] = out;
-        N1[s] = from_lane_below(O[s]);
-        N2[s] = from_lane_below(N1[s]);
-        N3[s] = from_lane_below(N2[s]);
-      }
-      chunk++;
     }
     __syncthreads();
   }
@@ -261,8 +394,12 @@ __device__ void ar_all(int16_t *tl, int16_t *tcb, int16_t *tcr, const GrainHdr &
   const int sub = h.ssx | h.ssy;
   const int lr = ar_rounds(K, TH, TW);
   const int cr = chroma ? ar_rounds(K, ch, cw) : 0;
-  const ArPlan pl_luma = {0, 1, 0}, pl_chroma = {sub ? 2 : 1, sub ? 2 : 1, 0};
-  const int rounds = max(lr, chroma ? pl_chroma.first + pl_chroma.every * (cr - 1) + 1 : 0);
+  /* A chroma chunk c reads luma samples finished by diagonal 2 (16 c + 15)
+   * (subsampled) or 16 c + 15 (4:4:4): subsampled, its first half runs in
+   * round 2 c + 1 (luma done through 32 c + 15) and its second half in
+   * round 2 c + 2; 4:4:4, the chunk runs in round c + 1. */
+  const ArPlan pl_luma = {0, 1, 0}, pl_chroma = {1, sub ? 2 : 1, 0};
+  const int rounds = max(lr, chroma ? pl_chroma.first + pl_chroma.halves * cr : 0);
   if (wave == 0) {
     ArPlan p = pl_luma;
     p.rounds = rounds;
@@ -280,91 +417,128 @@ __device__ void ar_all(int16_t *tl, int16_t *tcb, int16_t *tcr, const GrainHdr &
   }
 }
 
+/* Each template is followed by 64 spare samples: the AR filter's lanes that
+ * are outside their template on a diagonal store there. */
+constexpr int TPAD = TH * TW + 64;
+
 struct PrepShared {
-  int16_t tl[TH * TW], tcb[TH * TW], tcr[TH * TW];
+  int16_t tl[TPAD], tcb[TPAD], tcr[TPAD];
   int16_t gauss[2048];
 };
 
-/* Templates, LUTs and block offsets of one frame (PREP_THREADS threads). */
-__device__ __forceinline__ void grain_prepare_body(const GrainHdr &h, const GrainScratch &sc, PrepShared &sm) {
-  int16_t *tl = sm.tl, *tcb = sm.tcb, *tcr = sm.tcr, *gauss = sm.gauss;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int i = threadIdx.x; i < 2048; i += PREP_THREADS) gauss[i] = AV1_GAUSSIAN_SEQUENCE[i];
+/* Phase 1 of the prepare stage: the Gaussian templates into LDS, one
+ * template row per thread. */
+__device__ __forceinline__ void prep_fill(const GrainHdr &h, PrepShared &sm) {
+  for (int i = threadIdx.x; i < 2048; i += PREP_THREADS) sm.gauss[i] = AV1_GAUSSIAN_SEQUENCE[i];
   __syncthreads();
-  const int bd = h.bd;
-  const int gmin = -(128 << (bd - 8)), gmax = (256 << (bd - 8)) - 1 - (128 << (bd - 8));
-  const int shift = 12 - bd + h.grain_scale_shift;
+  const int shift = 12 - h.bd + h.grain_scale_shift;
   const bool chroma = h.nplanes > 1;
   const int cw = h.ssx ? 44 : 82, ch = h.ssy ? 38 : 73;
-  if (wave == 0)
-    gaussian_fill(tl, gauss, h.seed, TH, TW, shift, h.num_y > 0, lane);
-  else if (wave == 1 && chroma)
-    gaussian_fill(tcb, gauss, h.seed ^ 0xb524, ch, cw, shift, h.num_cb > 0 || h.cfl, lane);
-  else if (wave == 2 && chroma)
-    gaussian_fill(tcr, gauss, h.seed ^ 0x49d8, ch, cw, shift, h.num_cr > 0 || h.cfl, lane);
-  __syncthreads();
-  switch (h.ar_lag) {
-    case 0: ar_all<0>(tl, tcb, tcr, h, chroma, cw, ch, gmin, gmax); break;
-    case 1: ar_all<1>(tl, tcb, tcr, h, chroma, cw, ch, gmin, gmax); break;
-    case 2: ar_all<2>(tl, tcb, tcr, h, chroma, cw, ch, gmin, gmax); break;
-    default: ar_all<3>(tl, tcb, tcr, h, chroma, cw, ch, gmin, gmax); break;
-  }
-  /* templates out: per plane the 8 shifted copies (GRAIN_TPLANE). A thread
-   * takes 16 samples of a row (8 aligned words of the LDS template) and
-   * writes the 8 shifted 8-sample chunks starting at its first sample. */
-  {
-    constexpr int Q = GRAIN_TWS / 8;
-    for (int i = threadIdx.x; i < 3 * TH * Q; i += PREP_THREADS) {
-      const int q = i % Q, r = i / Q;
-      const int y = r % TH, p = r / TH;
-      const int16_t *t = p == 0 ? tl : (p == 1 ? tcb : tcr);
-      const bool on = p == 0 || chroma;
-      uint32_t w[9];
-#pragma unroll
-      for (int k = 0; k < 9; k++) {
-        const int x = q * 8 + 2 * k;
-        w[k] = (on && x < TW) ? *reinterpret_cast<const uint32_t *>(t + y * TW + x) : 0u;
-      }
-      /* samples past the template row read as 0 */
-      int16_t *dst = sc.tmpl + (size_t)p * GRAIN_TPLANE + y * GRAIN_TWS + q * 8;
-#pragma unroll
-      for (int sft = 0; sft < 8; sft++) {
-        uint32_t c[4];
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-          const int a = (sft >> 1) + k;
-          c[k] = (sft & 1) ? __builtin_amdgcn_alignbyte(w[a + 1], w[a], 2) : w[a];
-        }
-        *reinterpret_cast<uint4 *>(dst + sft * GRAIN_TCOPY) = make_uint4(c[0], c[1], c[2], c[3]);
-      }
+  const int nrows = TH + (chroma ? 2 * ch : 0);
+  for (int r = threadIdx.x; r < nrows; r += PREP_THREADS) {
+    if (r < TH) {
+      fill_row(sm.tl, sm.gauss, h.seed, r, TW, 0, shift, h.num_y > 0);
+    } else {
+      const int pl = (r - TH) / ch, y = (r - TH) - pl * ch;
+      fill_row(pl ? sm.tcr : sm.tcb, sm.gauss, h.seed ^ (pl ? 0x49d8u : 0xb524u), y, cw, cw == 82 ? 0 : 1, shift,
+               (pl ? h.num_cr : h.num_cb) > 0 || h.cfl);
     }
   }
-  /* scaling LUTs (grain.rs:137-165) */
-  for (int p = 0; p < h.nplanes; p++) {
-    const int np = h.npoints[p];
-    const int *xs = h.points[p][0], *ys = h.points[p][1];
-    for (int x = threadIdx.x; x < 256; x += PREP_THREADS) {
-      int v = 0;
-      if (np > 0) {
-        if (x < xs[0]) {
-          v = ys[0];
-        } else if (x >= xs[np - 1]) {
-          v = ys[np - 1];
-        } else {
-          int i = 0;
-          while (i + 1 < np && xs[i + 1] <= x) i++;
-          const int dy = ys[i + 1] - ys[i], dx = xs[i + 1] - xs[i];
-          const int delta = dy * ((65536 + (dx >> 1)) / dx);
-          v = ys[i] + (((x - xs[i]) * delta + 32768) >> 16);
-        }
+  __syncthreads();
+}
+
+/* Phase 2: the auto-regressive filter of the three templates, in LDS. */
+__device__ __forceinline__ void prep_ar(const GrainHdr &h, PrepShared &sm) {
+  const int bd = h.bd;
+  const int gmin = -(128 << (bd - 8)), gmax = (256 << (bd - 8)) - 1 - (128 << (bd - 8));
+  const bool chroma = h.nplanes > 1;
+  const int cw = h.ssx ? 44 : 82, ch = h.ssy ? 38 : 73;
+  switch (h.ar_lag) {
+    case 0: ar_all<0>(sm.tl, sm.tcb, sm.tcr, h, chroma, cw, ch, gmin, gmax); break;
+    case 1: ar_all<1>(sm.tl, sm.tcb, sm.tcr, h, chroma, cw, ch, gmin, gmax); break;
+    case 2: ar_all<2>(sm.tl, sm.tcb, sm.tcr, h, chroma, cw, ch, gmin, gmax); break;
+    default: ar_all<3>(sm.tl, sm.tcb, sm.tcr, h, chroma, cw, ch, gmin, gmax); break;
+  }
+}
+
+/* Phase 3: the filtered templates into copy 0 of the scratch (the
+ * template itself, rows padded to GRAIN_TWS with zeros). */
+__device__ __forceinline__ void prep_out(const GrainHdr &h, const GrainScratch &sc, PrepShared &sm) {
+  const bool chroma = h.nplanes > 1;
+  constexpr int Q = GRAIN_TWS / 8;
+  for (int i = threadIdx.x; i < 3 * TH * Q; i += PREP_THREADS) {
+    const int q = i % Q, r = i / Q;
+    const int y = r % TH, p = r / TH;
+    const int16_t *t = p == 0 ? sm.tl : (p == 1 ? sm.tcb : sm.tcr);
+    const bool on = p == 0 || chroma;
+    uint32_t w[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int x = q * 8 + 2 * k;
+      w[k] = (on && x < TW) ? *reinterpret_cast<const uint32_t *>(t + y * TW + x) : 0u;
+    }
+    *reinterpret_cast<uint4 *>(sc.tmpl + (size_t)p * GRAIN_TPLANE + y * GRAIN_TWS + q * 8) =
+        make_uint4(w[0], w[1], w[2], w[3]);
+  }
+}
+
+/* Workgroups per frame of the tables stage. */
+constexpr int TABLE_ITEMS = 8;
+
+/* Item j of the tables stage of one frame (256 threads): the shifted
+ * copies 1..7 of the templates from copy 0 (copy s at x holds sample
+ * x + s, 0 past the row; each thread takes one 8-sample chunk of a row),
+ * the scaling LUTs (item 0) and the block offsets of the noise stripes
+ * j, j + TABLE_ITEMS, ... */
+__device__ __forceinline__ void grain_tables_item(const GrainHdr &h, const GrainScratch &sc, int j) {
+  constexpr int Q = GRAIN_TWS / 8;
+  for (int i = j * 256 + threadIdx.x; i < 3 * TH * Q; i += TABLE_ITEMS * 256) {
+    const int q = i % Q, r = i / Q;
+    const int y = r % TH, p = r / TH;
+    const int16_t *row = sc.tmpl + (size_t)p * GRAIN_TPLANE + y * GRAIN_TWS;
+    const uint4 a = *reinterpret_cast<const uint4 *>(row + q * 8);
+    const uint4 b = q + 1 < Q ? *reinterpret_cast<const uint4 *>(row + q * 8 + 8) : make_uint4(0, 0, 0, 0);
+    const uint32_t w[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+    int16_t *dst = sc.tmpl + (size_t)p * GRAIN_TPLANE + y * GRAIN_TWS + q * 8;
+#pragma unroll
+    for (int sft = 1; sft < 8; sft++) {
+      uint32_t c[4];
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const int a2 = (sft >> 1) + k;
+        c[k] = (sft & 1) ? __builtin_amdgcn_alignbyte(w[a2 + 1], w[a2], 2) : w[a2];
       }
-      sc.lut[p * 256 + x] = (int16_t)v;
+      *reinterpret_cast<uint4 *>(dst + sft * GRAIN_TCOPY) = make_uint4(c[0], c[1], c[2], c[3]);
+    }
+  }
+  if (j == 0) {
+  /* scaling LUTs (grain.rs:137-165) */
+    for (int p = 0; p < h.nplanes; p++) {
+      const int np = h.npoints[p];
+      const int *xs = h.points[p][0], *ys = h.points[p][1];
+      for (int x = threadIdx.x; x < 256; x += 256) {
+        int v = 0;
+        if (np > 0) {
+          if (x < xs[0]) {
+            v = ys[0];
+          } else if (x >= xs[np - 1]) {
+            v = ys[np - 1];
+          } else {
+            int i = 0;
+            while (i + 1 < np && xs[i + 1] <= x) i++;
+            const int dy = ys[i + 1] - ys[i], dx = xs[i + 1] - xs[i];
+            const int delta = dy * ((65536 + (dx >> 1)) / dx);
+            v = ys[i] + (((x - xs[i]) * delta + 32768) >> 16);
+          }
+        }
+        sc.lut[p * 256 + x] = (int16_t)v;
+      }
     }
   }
   /* block offsets per noise stripe (grain.rs:186-199) */
   const int nstripes = ((h.h + 1) / 2 + 15) / 16;
   const int nblocks = ((h.w + 1) / 2 + 15) / 16;
-  for (int ln = threadIdx.x; ln < nstripes; ln += PREP_THREADS) {
+  for (int ln = j * 256 + threadIdx.x; ln < nstripes; ln += TABLE_ITEMS * 256) {
     uint32_t r = h.seed;
     r ^= (uint32_t)(((ln * 37 + 178) & 255) << 8);
     r ^= (uint32_t)((ln * 173 + 105) & 255);
@@ -373,6 +547,13 @@ __device__ __forceinline__ void grain_prepare_body(const GrainHdr &h, const Grai
       sc.offsets[ln * sc.max_blocks + b] = (uint8_t)((r >> 8) & 255);
     }
   }
+}
+
+/* Templates, LUTs and block offsets of one frame (PREP_THREADS threads). */
+__device__ __forceinline__ void grain_prepare_body(const GrainHdr &h, const GrainScratch &sc, PrepShared &sm) {
+  prep_fill(h, sm);
+  prep_ar(h, sm);
+  prep_out(h, sc, sm);
 }
 
 __device__ __forceinline__ int scale_lut(const int16_t *lut, int bd, int index) {
@@ -705,6 +886,10 @@ __global__ __launch_bounds__(PREP_THREADS) void grain_prepare_kernel(GrainHdr h,
   grain_prepare_body(h, sc, sm);
 }
 
+__global__ __launch_bounds__(256) void grain_tables_kernel(GrainHdr h, GrainScratch sc) {
+  grain_tables_item(h, sc, blockIdx.x);
+}
+
 /* KIND (apply_kind): 1 (segments, 64 luma columns per tile: 4:2:2, mono),
  * 2 (segments, two chroma segments per thread: 4:4:4), 3 (segments, 128
  * luma columns per tile: 4:2:0), 0 (sample by sample). Input samples T,
@@ -741,6 +926,13 @@ __global__ __launch_bounds__(PREP_THREADS) void grain_prepare_batch(BatchWork w)
     const PrepRecord &r = batch_table<PrepRecord>(w.rec, f);
     grain_prepare_body(r.h, r.s, sm);
   });
+}
+
+/* The tables stage of every frame: TABLE_ITEMS workgroups per frame. */
+__global__ __launch_bounds__(256) void grain_tables_batch(BatchWork w) {
+  const int f = blockIdx.x / TABLE_ITEMS, j = blockIdx.x - f * TABLE_ITEMS;
+  const PrepRecord &r = batch_table<PrepRecord>(w.rec, f);
+  grain_tables_item(r.h, r.s, j);
 }
 
 /* 5 waves per SIMD (96 VGPRs, a few spills; unbounded it takes 108-112,
@@ -788,6 +980,7 @@ int apply_kind(const GrainHdr &h, const PlanePtr in[3], const PlanePtr out[3], i
 
 int launch_grain_prepare(const GrainHdr &h, const GrainScratch &s, hipStream_t st) {
   grain_prepare_kernel<<<1, PREP_THREADS, 0, st>>>(h, s);
+  grain_tables_kernel<<<TABLE_ITEMS, 256, 0, st>>>(h, s);
   return (int)hipGetLastError();
 }
 
@@ -806,6 +999,8 @@ int launch_grain_prepare_batch(const BatchWork &w, const StagePlan &sp, hipStrea
     grain_prepare_batch<true><<<sp.grid, PREP_THREADS, 0, st>>>(w);
   else
     grain_prepare_batch<false><<<sp.grid, PREP_THREADS, 0, st>>>(w);
+  /* one prepare item per frame: records are indexed by frame */
+  grain_tables_batch<<<w.n * TABLE_ITEMS, 256, 0, st>>>(w);
   return (int)hipGetLastError();
 }
 

@@ -142,6 +142,24 @@ def test_wiener_autograd_function(cases):
     assert torch.allclose(taps.grad.double(), want, rtol=1e-5, atol=1e-3)
 
 
+def test_sgr_autograd_function(cases):
+    """SgrSSE through autograd: the forward is the relaxed error per unit,
+    and backward scales the adjoint gradient by the incoming gradient."""
+    import torch
+    c = cases[0]
+    g = torch.Generator().manual_seed(11)
+    weights = torch.rand(c.s.U, generator=g).cuda()
+    for set_ in (0, 10, 14):
+        flt = c.s.sgr_filter(set_)
+        w = ((torch.rand((c.s.U, 2), generator=g) - 0.5) * 60).cuda().requires_grad_(True)
+        sse = lrdiff.SgrSSE.apply(w, c.s, flt)
+        want_sse, want_grad = c.s.sgr_value_grad(w.detach().double(), flt)
+        assert torch.allclose(sse.double(), want_sse.double(), rtol=1e-5, atol=1e-3), set_
+        (sse * weights).sum().backward()
+        assert torch.allclose(w.grad.double(), want_grad.double() * weights.double()[:, None], rtol=1e-4,
+                              atol=1e-2), set_
+
+
 def test_sgr_gradient_finite_differences(cases):
     import torch
     for c in cases:

@@ -34,11 +34,6 @@ def units_of(f, p):
     return a
 
 
-def box(ii, r, h, w):
-    """Sums over (2r+1)^2 boxes centred on positions (1..h+2, ...) of an integral image."""
-    return ii[2 + 2 * r + 1 - r - 1 + np.arange(h + 2)[:, None] + r + 1 - (r + 1), 0:0] if False else None
-
-
 def sgr_pass(win, r, eps, bd, h, w, y0, pass_):
     """A/B on (h+2)x(w+2) positions from integral images, then the weighted
     3x3 filter; returns flt (h x w) as int64."""
