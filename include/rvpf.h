@@ -275,6 +275,20 @@ int rvpf_grain(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_planes *ref
                rvpf_planes *display, void *stream);
 
 /*
+ * rvpf_grain with no device allocation. No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). It takes
+ * no context; the film grain scratch (noise templates, scaling LUTs, block
+ * offsets, 376 320 bytes) is a static device array of the library, one per
+ * GPU, held until the process ends. Frames up to 8192 x 8192 (upscaled_width
+ * x frame_height); larger ones return RVPF_E_INVALID before anything is
+ * enqueued. Calls that overlap on one GPU (on different streams or from
+ * several threads) race on that scratch; calls on one stream do not. Errors
+ * are read with rvpf_ctx_last_error(NULL).
+ */
+int rvpf_grain_noalloc(const rvpf_frame_params *p, const rvpf_planes *ref, rvpf_planes *display, void *stream);
+
+/*
  * The packed picture Decoder::output returns (src/decoder/mod.rs:846-917,
  * the Frame layout of src/frame.rs:131-190): Y, then U, then V (Y alone
  * for monochrome), each tightly packed (stride = width), one byte per

@@ -95,6 +95,9 @@ int scratch_release(rvpf_ctx *ctx, ScratchGate *g, hipStream_t st);
  * to 64 samples, planes to 256 bytes); returns the bytes; base NULL only
  * counts */
 size_t layout_planes(const int w[3], const int h[3], int bps, void *base, PlanePtr out[3]);
+/* the display planes hold the reference planes' sample size, or bytes
+ * from 16-bit planes at 8 bits */
+bool display_bps_ok(const rvpf_frame_params *p, int ref_bps, int disp_bps);
 /* bytes of the offsets table of a grain scratch for a w x h frame */
 void grain_offsets_dims(int w, int h, int *stripes, int *blocks);
 

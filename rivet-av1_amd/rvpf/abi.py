@@ -281,6 +281,7 @@ RVPF_EXPORTS = [
     "rvpf_upscale",
     "rvpf_lr",
     "rvpf_grain",
+    "rvpf_grain_noalloc",
     "rvpf_packed_layout_of",
     "rvpf_output_packed",
     "rvpf_postfilter_frame",

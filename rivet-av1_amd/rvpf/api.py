@@ -45,6 +45,7 @@ def lib():
                               P(abi.RvpfPlanes), P(abi.RvpfPlanes), C.c_void_p]
         L.rvpf_grain.argtypes = [C.c_void_p, P(abi.RvpfFrameParams), P(abi.RvpfPlanes), P(abi.RvpfPlanes), C.c_void_p]
         L.rvpf_upscale.argtypes = [C.c_void_p, P(abi.RvpfFrameParams), P(abi.RvpfPlanes), P(abi.RvpfPlanes), C.c_void_p]
+        L.rvpf_grain_noalloc.argtypes = [P(abi.RvpfFrameParams), P(abi.RvpfPlanes), P(abi.RvpfPlanes), C.c_void_p]
         L.rvpf_packed_layout_of.argtypes = [P(abi.RvpfFrameParams), P(abi.RvpfPackedLayout)]
         L.rvpf_lf_search.argtypes = [C.c_void_p, P(abi.RvpfFrameParams), P(abi.RvpfSideInfo), P(abi.RvpfPlanes),
                                      P(abi.RvpfPlanes), C.c_void_p, C.c_void_p]
@@ -251,6 +252,15 @@ def grain(ctx, df, ref, display, stream=None, display_bps=None):
     pr, pdsp = planes_struct(ref, df.bps), planes_struct(display, display_bps or df.bps)
     ctx.check(lib().rvpf_grain(ctx.h, C.byref(df.frame.params), C.byref(pr), C.byref(pdsp),
                                _stream_handle(stream)))
+
+
+def grain_noalloc(df, ref, display, stream=None, display_bps=None):
+    """rvpf_grain_noalloc: as grain(), with no device allocation (the
+    library's static scratch; frames up to 8192x8192). Raises RvpfError."""
+    pr, pdsp = planes_struct(ref, df.bps), planes_struct(display, display_bps or df.bps)
+    rc = lib().rvpf_grain_noalloc(C.byref(df.frame.params), C.byref(pr), C.byref(pdsp), _stream_handle(stream))
+    if rc != 0:
+        raise RvpfError(rc, lib().rvpf_ctx_last_error(None).decode())
 
 
 def packed_layout(params):
