@@ -8,7 +8,10 @@
  * One workgroup (256 lanes, 4 waves) per 64x64 filter block (one
  * cdef_idx):
  *   1. the luma tile plus a 2-sample border is staged in LDS; samples at or
- *      beyond the mode-info edge are the sentinel NA_S (unavailable);
+ *      beyond the mode-info edge are the sentinel NA_S (unavailable); one
+ *      wave lists the 8x8 blocks CDEF filters, then the skipped ones, and
+ *      the four lanes of slot s take block s of that list, so the waves
+ *      past the filtered blocks only copy theirs;
  *   2. the four lanes of an 8x8 block run its direction search, each
  *      through one symmetry of the block (no divergence), and one shuffle
  *      gathers the eight costs;
@@ -97,6 +100,46 @@ __device__ void stage_tile(uint16_t *t, uint16_t *sh, const T *src, int64_t stri
         if (tx == 0) sh[r * TS + tw - 1] = (uint16_t)NA_S;
       }
     }
+  }
+}
+
+/* A chroma tile of a 4:2:0 frame ((32 + 4) x (32 + 4) samples) loaded into
+ * registers, 9 rows per thread (rows ty, ty + 4, ...; column tx < 36),
+ * with the unavailable samples already NA_S, so that its loads can be in
+ * flight while the luma tile filters; stage_store writes it to P and S. */
+struct ChromaRegs {
+  uint32_t v[5]; /* row k in half k & 1 of word k >> 1 */
+};
+
+template <typename T>
+__device__ __forceinline__ void chroma_load(ChromaRegs &cr, const T *src, int64_t stride, int x0, int y0, int xlim,
+                                            int ylim) {
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const int xa = x0 + tx - 2;
+  const bool xa_ok = tx < 36 && xa >= 0 && xa < xlim;
+  const int xac = min(max(xa, 0), xlim - 1);
+  uint32_t v[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    const int y = y0 + ty + 4 * k - 2;
+    const bool yok = y >= 0 && y < ylim;
+    const int a = src[(int64_t)min(max(y, 0), ylim - 1) * stride + xac];
+    v[k] = yok && xa_ok ? (uint32_t)a : (uint32_t)NA_S;
+  }
+#pragma unroll
+  for (int k = 0; k < 5; k++) cr.v[k] = v[2 * k] | (2 * k + 1 < 9 ? v[2 * k + 1] << 16 : 0u);
+}
+
+__device__ __forceinline__ void chroma_store(const ChromaRegs &cr, uint16_t *t, uint16_t *sh) {
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  if (tx >= 36) return;
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    const int r = ty + 4 * k;
+    const uint16_t v = (uint16_t)(cr.v[k >> 1] >> (16 * (k & 1)));
+    t[r * TS + tx] = v;
+    if (tx > 0) sh[r * TS + tx - 1] = v;
+    if (tx == 0) sh[r * TS + 35] = (uint16_t)NA_S;
   }
 }
 
@@ -364,10 +407,22 @@ __device__ __forceinline__ void filter_block_any(bool pri, bool sec, const uint3
     filter_block_pairs<T, false, false>(w, t0, o, os, c, r0, nrows, k, false);
 }
 
+/* The 8x8 blocks of a filter block in the order lanes take them: the
+ * blocks CDEF filters first (nfilt of them), then the ones it copies
+ * (ncopy: all four 4x4 skipped); blocks past the mode-info edge are in
+ * neither. */
+struct CdefBlocks {
+  uint8_t list[64];
+  int nfilt, ncopy;
+};
+
 /* One 64x64 filter block (tile_id in raster order); words holds the
- * tile P, then the shifted tile S. */
+ * tile P, then the shifted tile S. The four lanes of slot s = tid / 4 take
+ * block list[s]: only the waves whose slots hold filtered blocks run the
+ * direction search and the filters, the rest copy their blocks from the
+ * staged tile. */
 template <typename T>
-__device__ __forceinline__ void cdef_tile(const CdefLaunch &a, int tile_id, uint32_t *words) {
+__device__ __forceinline__ void cdef_tile(const CdefLaunch &a, int tile_id, uint32_t *words, CdefBlocks &bl) {
   uint16_t *tile = reinterpret_cast<uint16_t *>(words);
   uint16_t *shifted = reinterpret_cast<uint16_t *>(words + SOFF);
   const Geom &g = a.g;
@@ -405,36 +460,62 @@ __device__ __forceinline__ void cdef_tile(const CdefLaunch &a, int tile_id, uint
     return;
   }
 
+  /* cdef_block's selection (postfilter.rs:304-310): wave 0, a lane per
+   * 8x8 block, lists the filtered blocks, then the skipped ones */
+  if (tid < 64) {
+    const int by = tid >> 3, bx = tid & 7;
+    const bool inside = by * 8 < bh && bx * 8 < bw;
+    bool skip = true;
+    if (inside) {
+      const int r = (y0 >> 2) + by * 2, c = (x0 >> 2) + bx * 2;
+      const int ms = a.mi_stride;
+      const rvpf_mi *mi = gptr(a.mi);
+      skip = mi[(int64_t)r * ms + c].seg_skip & mi[(int64_t)(r + 1) * ms + c].seg_skip &
+             mi[(int64_t)r * ms + c + 1].seg_skip & mi[(int64_t)(r + 1) * ms + c + 1].seg_skip & 1;
+    }
+    const bool filt = inside && !skip;
+    const uint64_t mf = __ballot(filt), mc = __ballot(inside && skip);
+    const int nf = __popcll(mf);
+    const uint32_t flo = (uint32_t)mf, fhi = (uint32_t)(mf >> 32), clo = (uint32_t)mc, chi = (uint32_t)(mc >> 32);
+    if (filt) bl.list[__builtin_amdgcn_mbcnt_hi(fhi, __builtin_amdgcn_mbcnt_lo(flo, 0u))] = (uint8_t)tid;
+    if (inside && skip) bl.list[nf + __builtin_amdgcn_mbcnt_hi(chi, __builtin_amdgcn_mbcnt_lo(clo, 0u))] = (uint8_t)tid;
+    if (tid == 0) {
+      bl.nfilt = nf;
+      bl.ncopy = __popcll(mc);
+    }
+  }
   stage_tile<T>(tile, shifted, pp<T>(a.in[0]), a.in[0].stride, x0, y0, 64, 64, xlim, ylim);
+  /* 4:2:0: both chroma tiles' loads go out now and land while luma filters */
+  const bool pre = g.nplanes > 1 && g.ssx && g.ssy;
+  ChromaRegs cpre[2];
+  if (pre)
+    for (int p = 1; p < 3; p++)
+      chroma_load<T>(cpre[p - 1], pp<T>(a.in[p]), a.in[p].stride, x0 >> 1, y0 >> 1, xlim >> 1, ylim >> 1);
   __syncthreads();
 
-  /* cdef_block (postfilter.rs:304-341): the 4 lanes of block b = tid / 4
-   * run its direction search, each through one symmetry of the block, and
-   * then filter its samples (lane q: columns 2q, 2q + 1 of luma). */
-  const int b = tid >> 2, q = tid & 3;
+  const int nf = bl.nfilt, nact = nf + bl.ncopy;
+  const int slot = tid >> 2, q = tid & 3;
+  const bool active = slot < nact, filt = slot < nf;
+  /* this wave's slots hold filtered blocks (wave-uniform) */
+  const bool wave_filters = 16 * (tid >> 6) < nf;
+  const int b = active ? bl.list[slot] : 0;
   const int by = b >> 3, bx = b & 7;
-  const bool inside = by * 8 < bh && bx * 8 < bw;
-  bool skip = true;
-  if (inside) {
-    const int r = (y0 >> 2) + by * 2, c = (x0 >> 2) + bx * 2;
-    const int ms = a.mi_stride;
-    const rvpf_mi *mi = gptr(a.mi);
-    skip = mi[(int64_t)r * ms + c].seg_skip & mi[(int64_t)(r + 1) * ms + c].seg_skip &
-           mi[(int64_t)r * ms + c + 1].seg_skip & mi[(int64_t)(r + 1) * ms + c + 1].seg_skip & 1;
-  }
-  const bool filt = inside && !skip;
-  int ydir;
-  int32_t var;
-  block_direction(tile, tid, filt, by, bx, cs, &ydir, &var);
+  int ydir = 0;
+  int32_t var = 0;
+  if (wave_filters) block_direction(tile, tid, filt, by, bx, cs, &ydir, &var);
   const int pri = filt ? adjust_pri(a.h.y_pri[idx] << cs, var) : 0;
 
-  /* luma: lane q filters the column pair 2q, 2q + 1 of its block */
-  if (inside) {
+  /* luma: lane q filters (or copies) the column pair 2q, 2q + 1 of its block */
+  if (active) {
     const int sec = a.h.y_sec[idx] << cs;
     const Taps k = make_taps(pri, sec, a.h.damping + cs, a.h.y_pri[idx] ? ydir : 0, cs);
     T *o = pp<T>(a.out[0]) + (int64_t)(y0 + by * 8) * a.out[0].stride + x0 + bx * 8;
-    filter_block_any<T>(a.h.y_pri[idx] != 0, sec != 0, words, (by * 8 + 2) * TS + bx * 8 + 2, o, a.out[0].stride,
-                        2 * q, 0, 8, k, filt && (pri | sec));
+    const int t0 = (by * 8 + 2) * TS + bx * 8 + 2;
+    if (wave_filters)
+      filter_block_any<T>(a.h.y_pri[idx] != 0, sec != 0, words, t0, o, a.out[0].stride, 2 * q, 0, 8, k,
+                          filt && (pri | sec));
+    else
+      filter_block_any<T>(false, false, words, t0, o, a.out[0].stride, 2 * q, 0, 8, k, false);
   }
   if (g.nplanes == 1) return;
 
@@ -450,23 +531,32 @@ __device__ __forceinline__ void cdef_tile(const CdefLaunch &a, int tile_id, uint
   const Taps k = make_taps(upri, usec, a.h.damping + cs - 1, upri ? AV1_CDEF_UV_DIR[ssx][ssy][ydir] : 0, cs);
   for (int p = 1; p < 3; p++) {
     __syncthreads();
-    stage_tile<T>(tile, shifted, pp<T>(a.in[p]), a.in[p].stride, x0 >> ssx, y0 >> ssy, cw, ch, xlim >> ssx,
-                  ylim >> ssy);
+    if (pre)
+      chroma_store(cpre[p - 1], tile, shifted);
+    else
+      stage_tile<T>(tile, shifted, pp<T>(a.in[p]), a.in[p].stride, x0 >> ssx, y0 >> ssy, cw, ch, xlim >> ssx,
+                    ylim >> ssy);
     __syncthreads();
-    if (inside) {
+    if (active) {
       T *o = pp<T>(a.out[p]) + (int64_t)((y0 >> ssy) + by * bhs) * a.out[p].stride + (x0 >> ssx) + bx * bws;
-      filter_block_any<T>(upri != 0, usec != 0, words, (by * bhs + 2) * TS + bx * bws + 2, o, a.out[p].stride,
-                          2 * cpair, r0, nrows, k, filt && (upri | usec));
+      const int t0 = (by * bhs + 2) * TS + bx * bws + 2;
+      if (wave_filters)
+        filter_block_any<T>(upri != 0, usec != 0, words, t0, o, a.out[p].stride, 2 * cpair, r0, nrows, k,
+                            filt && (upri | usec));
+      else
+        filter_block_any<T>(false, false, words, t0, o, a.out[p].stride, 2 * cpair, r0, nrows, k, false);
     }
   }
 }
 
-/* 6 waves per SIMD: 80 VGPRs without spills (unbounded the compiler takes
- * 93, 5 waves; 6 waves run 3 % faster per 4K frame, 7 and 8 slower). */
+/* 5 waves per SIMD: 96 VGPRs, no spills (at 6 waves, 80 VGPRs, the
+ * prefetched 4:2:0 chroma tiles spill and the kernel is 6 % slower per 4K
+ * frame). */
 template <typename T>
-__global__ __launch_bounds__(256, 6) void cdef_kernel(CdefLaunch a) {
+__global__ __launch_bounds__(256, 5) void cdef_kernel(CdefLaunch a) {
   __shared__ uint32_t words[2 * SOFF];
-  cdef_tile<T>(a, xcd_remap(blockIdx.x, gridDim.x), words);
+  __shared__ CdefBlocks bl;
+  cdef_tile<T>(a, xcd_remap(blockIdx.x, gridDim.x), words, bl);
 }
 
 /* Batched. The persistent form is held at 5 waves per SIMD (inside the
@@ -476,7 +566,8 @@ __global__ __launch_bounds__(256, 6) void cdef_kernel(CdefLaunch a) {
 template <typename T, bool DIRECT>
 __global__ __launch_bounds__(256, DIRECT ? 6 : 5) void cdef_batch(BatchWork w) {
   __shared__ uint32_t words[2 * SOFF];
-  batch_loop<DIRECT>(w, [&](int f, int j) { cdef_tile<T>(batch_table<CdefLaunch>(w.rec, f), j, words); });
+  __shared__ CdefBlocks bl;
+  batch_loop<DIRECT>(w, [&](int f, int j) { cdef_tile<T>(batch_table<CdefLaunch>(w.rec, f), j, words, bl); });
 }
 
 /* ---- the encoder's strength search (src/encoder/cdef.rs:52-197) ---- */
