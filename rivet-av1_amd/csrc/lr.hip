@@ -12,18 +12,24 @@
  * The workgroup stages its window (slice + 3-sample border) in LDS with the
  * stripe rule of 7.17.6 (rows above / below the stripe come from the
  * deblocked frame, at most 2 rows out). Threads form SW-wide lane groups,
- * one per band of rows; a thread owns one column and walks down its band,
- * so every vertical stencil slides through registers and reads one new row
- * per output row:
- *   Wiener: horizontal 7-tap pass into an int16 LDS buffer (the clip3
- *           range fits 16 bits at 8/10/12 bits), then the vertical 7-tap
- *           pass over a 7-row register window.
- *   SGR:    one walk per column computes the 5- and 3-row sums of the
- *           5- and 3-sample horizontal sums (sum and sum of squares) and
- *           from them A and B of both passes (pass 0, r = 2, on odd rows
- *           only), each packed as (A << 20) | B; a second walk applies
- *           the 3x3 weights (6/5 on odd rows for r = 2, 4/3 for r = 1)
- *           from a 3-row register window and the projection.
+ * one per band of rows. Every LDS read of the window is dword-aligned (the
+ * LDS's unaligned path stalls its command queue), and pairs of 16-bit
+ * samples go through the packed dot products:
+ *   Wiener: a horizontal 7-tap pass writes pairs of columns into an int16
+ *           LDS buffer (the clip3 range fits 16 bits at 8/10/12 bits), each
+ *           output a sum of packed sample pairs times packed tap pairs;
+ *           then a vertical 7-tap pass down each column's band over a
+ *           7-row register window.
+ *   SGR:    per part of 32 output rows, a walk down each column computes,
+ *           two rows at a time, the 5- and 3-sample horizontal sums (sum
+ *           and sum of squares) of the new window rows, their 5- and 3-row
+ *           sums, and A and B of both passes (pass 0, r = 2, on odd rows
+ *           only), packed as (256 - A) << 21 | B so that the sum of two
+ *           neighbours is one add; the 24-bit multiplies use per-set
+ *           constants from a table. A second walk applies the 3x3 weights
+ *           (6/5 on odd rows for r = 2, 4/3 for r = 1) over row pairs and
+ *           the projection. The set's passes select one of three code
+ *           paths (both, r = 1 alone, r = 2 alone).
  * Samples outside restored units, beyond the frame but inside the
  * mode-info area, are copied from the CDEF output.
  */
@@ -65,20 +71,6 @@ __device__ __forceinline__ const T *cpp(const PlanePtr &p) {
   return gptr(reinterpret_cast<const T *>(p.data));
 }
 
-__device__ __forceinline__ int round2(int x, int n) { return (x + ((1 << n) >> 1)) >> n; }
-
-/* SGR_A2 (postfilter.rs:884-898): 256 at 255 and above, 1 at 0, else
- * ((z << 8) + z / 2) / (z + 1). */
-struct A2Table {
-  uint16_t v[256];
-};
-constexpr A2Table make_a2() {
-  A2Table t{};
-  for (int z = 0; z < 256; z++) t.v[z] = (uint16_t)(z >= 255 ? 256 : (z == 0 ? 1 : ((z << 8) + z / 2) / (z + 1)));
-  return t;
-}
-constexpr A2Table SGR_A2 = make_a2();
-
 /* LDS of one workgroup. */
 struct LrShared {
   uint16_t win[WR * WS];
@@ -87,43 +79,10 @@ struct LrShared {
     struct {
       uint32_t ab0[(ABR / 2 + 1) * ABW]; /* pass 0 (r = 2): odd rows i at (i - r0 + 1) >> 1 */
       uint32_t ab1[ABR * ABW];           /* pass 1 (r = 1): row i at i - r0 + 1 */
-      uint16_t a2[256];                  /* the a2 table of sgr_ab */
+      uint8_t t[256];                    /* 256 - SGR_A2, for sgr_ab */
     } sgr;
   };
 };
-
-/* Per-pass constants of the self-guided filter (postfilter.rs:915-935). */
-struct SgrPass {
-  uint32_t n, s, one_over_n, p_lim;
-};
-
-__device__ __forceinline__ SgrPass sgr_pass(int r, int eps) {
-  SgrPass q;
-  const int n = (2 * r + 1) * (2 * r + 1);
-  const int n2e = n * n * eps;
-  q.n = (uint32_t)n;
-  q.s = ((1u << 20) + (uint32_t)(n2e / 2)) / (uint32_t)n2e;
-  q.one_over_n = ((1u << 12) + (uint32_t)(n / 2)) / (uint32_t)n;
-  /* p * s fits in 32 bits up to this bound; beyond it z saturates */
-  q.p_lim = (0xFFFFFFFFu - (1u << 19)) / q.s;
-  return q;
-}
-
-/* A and B at one position from the box sum and sum of squares
- * (postfilter.rs:940-965), packed as (A << 20) | B: A <= 256 and
- * B <= (255 * 25 * 4095 * 164 + 2048) >> 12 < 2^20. Every product fits
- * 32 bits unsigned at 8/10/12 bits. */
-__device__ __forceinline__ uint32_t sgr_ab(uint32_t sum, uint32_t sq, int bd, const SgrPass &q, const uint16_t *a2t) {
-  const int sh = bd - 8;
-  const uint32_t av = (sq + ((1u << (2 * sh)) >> 1)) >> (2 * sh);
-  const uint32_t d = (sum + ((1u << sh) >> 1)) >> sh;
-  const uint32_t an = __umul24(av, q.n), dd = __umul24(d, d);
-  const uint32_t p = an > dd ? an - dd : 0;
-  const uint32_t z = p > q.p_lim ? 255u : min((p * q.s + (1u << 19)) >> 20, 255u);
-  const uint32_t a2 = a2t[z];
-  const uint32_t b2 = ((256u - a2) * __umul24(sum, q.one_over_n) + 2048u) >> 12;
-  return (a2 << 20) | b2;
-}
 
 /* Rows [lo, hi) of n split evenly over nb bands: band b. */
 __device__ __forceinline__ void band_rows(int n, int nb, int b, int &lo, int &hi) {
@@ -188,6 +147,374 @@ __device__ __forceinline__ void copy_rect_wide(T *out, int64_t os, const T *in, 
   for (int it = threadIdx.x; it < rem * rows; it += NT) {
     const int r = it / rem, c = x0 + nch * 8 + (it - r * rem);
     out[(int64_t)(ya + r) * os + c] = in[(int64_t)(ya + r) * is + c];
+  }
+}
+
+/* Self-guided constants of one pass (postfilter.rs:915-935): n = (2r+1)^2,
+ * the scale s, the clamp pmax on p, 1/n in 12-bit fixed point and the
+ * rounding of the sums to 8 bits. z = min((p s + 2^19) >> 20, 255) equals
+ * (min(p, pmax) s + 2^19) >> 20 with pmax the smallest p at which it
+ * reaches 255: below pmax it is at most 254, and pmax s + 2^19 stays below
+ * 256 * 2^20 because s < 2^20. The table's s are 22..3236 and pmax < 2^24,
+ * so z is one 24-bit multiply-add. */
+struct SgrK {
+  uint32_t n, s, pmax, oon;
+};
+struct SgrTable {
+  SgrK k[16][2];
+};
+constexpr SgrK make_sgr_k(int r, int eps) {
+  SgrK q{};
+  if (r == 0) eps = 1; /* pass off: never used */
+  const uint32_t n = (uint32_t)((2 * r + 1) * (2 * r + 1));
+  const uint32_t n2e = n * n * (uint32_t)eps;
+  q.n = n;
+  q.s = ((1u << 20) + n2e / 2) / n2e;
+  q.pmax = ((255u << 20) - (1u << 19) + q.s - 1) / q.s;
+  q.oon = ((1u << 12) + n / 2) / n;
+  return q;
+}
+constexpr SgrTable make_sgr_table() {
+  SgrTable t{};
+  for (int set = 0; set < 16; set++) {
+    t.k[set][0] = make_sgr_k(2 * (AV1_SGR_PARAMS[set][0] != 0), AV1_SGR_PARAMS[set][1]);
+    t.k[set][1] = make_sgr_k(AV1_SGR_PARAMS[set][2] != 0, AV1_SGR_PARAMS[set][3]);
+  }
+  return t;
+}
+constexpr SgrTable SGR_K = make_sgr_table();
+
+/* 256 - SGR_A2 (postfilter.rs:884-898): 0 at 255, 255 at 0, else
+ * 256 - ((z << 8) + z / 2) / (z + 1). A byte per entry. */
+struct SgrT {
+  uint8_t v[256];
+};
+constexpr SgrT make_sgr_t() {
+  SgrT t{};
+  for (int z = 0; z < 256; z++)
+    t.v[z] = (uint8_t)(256 - (z >= 255 ? 256 : (z == 0 ? 1 : ((z << 8) + z / 2) / (z + 1))));
+  return t;
+}
+constexpr SgrT SGR_T = make_sgr_t();
+
+/* Rounding of the box sums to 8 bits: sum >> sh and sq >> 2 sh, rounded. */
+struct SgrRound {
+  uint32_t r1, sh1, r2, sh2;
+};
+
+/* A and B at one position (postfilter.rs:940-965) from the box sum and sum
+ * of squares, packed as (t << 21) | B with t = 256 - A: t <= 255 and
+ * B < 2^20 at 8/10/12 bits, so the sum of two packed values keeps both
+ * fields apart. B = (t (sum / n) + 2048) >> 12 in 12-bit fixed point;
+ * t * oon < 2^17 and the full product is below 2^32. */
+__device__ __forceinline__ uint32_t sgr_ab(uint32_t sum, uint32_t sq, const SgrRound &rd, const SgrK &k,
+                                           const uint8_t *tt) {
+  const uint32_t av = (sq + rd.r2) >> rd.sh2;
+  const uint32_t d = (sum + rd.r1) >> rd.sh1;
+  const uint32_t an = __umul24(av, k.n), dd = __umul24(d, d);
+  const uint32_t p = an > dd ? an - dd : 0;
+  const uint32_t z = (__umul24(min(p, k.pmax), k.s) + (1u << 19)) >> 20;
+  const uint32_t t = tt[z];
+  /* oon < 2^10: the mask lets the compiler keep a 24-bit multiply */
+  const uint32_t b = (__umul24(__umul24(t, k.oon & 0x3FFu), sum) + 2048u) >> 12;
+  return (t << 21) | b;
+}
+
+typedef unsigned short ushort2_l __attribute__((ext_vector_type(2)));
+
+/* acc + a.lo * b.lo + a.hi * b.hi (unsigned 16-bit halves; v_dot2_u32_u16) */
+__device__ __forceinline__ uint32_t udot2(uint32_t a, uint32_t b, uint32_t acc) {
+  return __builtin_amdgcn_udot2(__builtin_bit_cast(ushort2_l, a), __builtin_bit_cast(ushort2_l, b), acc, false);
+}
+
+/* Window samples c .. c + 4 of one row as packed pairs (c, c+1), (c+2, c+3)
+ * and the fifth sample, from three dword-aligned LDS reads: an 8-byte
+ * read at a 2-byte boundary takes the LDS's unaligned path, which stalls
+ * its command queue. sh = 16 (c & 1) selects the half-dword phase. */
+struct Win5 {
+  uint32_t p12, p34, w5;
+};
+__device__ __forceinline__ Win5 win5(const uint16_t *row, int c, uint32_t sh) {
+  const uint32_t *d = reinterpret_cast<const uint32_t *>(row) + (c >> 1);
+  const uint32_t d0 = d[0], d1 = d[1], d2 = d[2];
+  return {__builtin_amdgcn_alignbit(d1, d0, sh), __builtin_amdgcn_alignbit(d2, d1, sh), (d2 >> sh) & 0xFFFFu};
+}
+
+/* Horizontal sums of one window row for one A/B column: over the five
+ * samples (h5, q5: sum and sum of squares) and the middle three (h3, q3). */
+struct HRow {
+  uint32_t h3, h5, q3, q5;
+};
+__device__ __forceinline__ HRow hrow(const Win5 &v) {
+  HRow r;
+  const uint32_t y34 = udot2(v.p34, 0x00010001u, 0);
+  r.h3 = udot2(v.p12, 0x00010000u, y34);
+  r.h5 = udot2(v.p12, 0x00010001u, y34 + v.w5);
+  const uint32_t s34 = udot2(v.p34, v.p34, 0);
+  r.q3 = udot2(v.p12 & 0xFFFF0000u, v.p12, s34);
+  r.q5 = __umul24(v.w5, v.w5) + udot2(v.p12, v.p12, s34);
+  return r;
+}
+/* the sums of window row `row` at LDS column c */
+__device__ __forceinline__ HRow hrow_at(const uint16_t *row, int c, uint32_t sh) { return hrow(win5(row, c, sh)); }
+
+/* Weighting terms of one row of packed A/B, at columns j - 1, j, j + 1
+ * (p[0..2]): E = left + right, C = centre, each split into its t and B
+ * fields. */
+__device__ __forceinline__ void split_row(const uint32_t *p, int &et, int &eb, int &ct, int &cb) {
+  const uint32_t l = p[0], c = p[1], r = p[2];
+  const uint32_t e = l + r;
+  et = (int)(e >> 21);
+  eb = (int)(e & 0x1FFFFFu);
+  ct = (int)(c >> 21);
+  cb = (int)(c & 0x1FFFFFu);
+}
+
+/* Pass 0 (r = 2) on one odd row: a = the sum of 6 A(j) + 5 (A(j-1) +
+ * A(j+1)), as 16 * 256 less the same weights of t, and b = the same sum of
+ * B plus 128, the rounding of an odd output row. */
+struct P0Row {
+  int a, b;
+};
+__device__ __forceinline__ P0Row p0_row(const uint32_t *p) {
+  int et, eb, ct, cb;
+  split_row(p, et, eb, ct, cb);
+  return {__mul24(ct, -6) + (__mul24(et, -5) + 4096), __mul24(cb, 6) + (__mul24(eb, 5) + 128)};
+}
+
+/* Pass 1 (r = 1) on one row: E and E + C of t and of B. */
+struct P1Row {
+  int et, st, eb, sb;
+};
+__device__ __forceinline__ P1Row p1_row(const uint32_t *p) {
+  int et, eb, ct, cb;
+  split_row(p, et, eb, ct, cb);
+  return {et, et + ct, eb, eb + cb};
+}
+
+/* Pass 1 output from rows m, c, n: weights 3 4 3 / 4 4 4 / 3 4 3, that is
+ * 4 (all nine) less the four corners, of A = 256 - t (weights sum to 32)
+ * and of B, rounded by 9 bits. */
+__device__ __forceinline__ int p1_out(const P1Row &m, const P1Row &c, const P1Row &n, int u) {
+  const int a = __mul24(m.st + c.st + n.st, -4) + (m.et + n.et + 8192);
+  const uint32_t b = __umul24((uint32_t)(m.sb + c.sb + n.sb), 4u) + (uint32_t)(256 - m.eb - n.eb);
+  return (int)(((uint32_t)__mul24(a, u) + b) >> 9);
+}
+
+/* A/B of both passes (P0: r = 2 on odd rows, P1: r = 1) at positions
+ * (i, j), i in [r0 - 1, r0 + npos - 2], j in [-1, w]: pass 0 into ab0 row
+ * (i - r0 + 1) >> 1, pass 1 into ab1 row i - r0 + 1, column j + 1. The
+ * stripe starts on an even frame row, so odd i are odd frame rows.
+ * Positions come in pairs (odd i, i + 1): lane group `band` walks pairs
+ * [plo, phi) down column j = col - 1, reading two new window rows per
+ * pair (rows i + 4, i + 5; the pair needs rows i + 1 .. i + 5). Columns
+ * SW - 1 .. w, past the lanes, are computed position by position. */
+template <int SW, bool P0, bool P1>
+__device__ __forceinline__ void sgr_ab_part(LrShared &sm, int band, int col, int w, int r0, int npair,
+                                            const SgrRound &rd, const SgrK &k0, const SgrK &k1) {
+  constexpr int NB = NT / SW;
+  const uint8_t *tt = sm.sgr.t;
+  int plo, phi;
+  band_rows(npair, NB, band, plo, phi);
+  const int j = col - 1;
+  if (j <= w && plo < phi) {
+    const int cw = WOFF + j + 1; /* LDS column of the box's first sample */
+    const uint32_t sh = (cw & 1) << 4;
+    const uint16_t *wp = &sm.win[(r0 + 2 * plo) * WS];
+    const HRow a = hrow_at(wp, cw, sh);
+    HRow b = hrow_at(wp + WS, cw, sh), c = hrow_at(wp + 2 * WS, cw, sh);
+    wp += 3 * WS;
+    uint32_t *p0 = &sm.sgr.ab0[plo * ABW + j + 1];
+    uint32_t *p1 = &sm.sgr.ab1[2 * plo * ABW + j + 1];
+    uint32_t ps = a.h5 + b.h5, pq = a.q5 + b.q5; /* rows i + 1, i + 2 */
+#pragma unroll 2
+    for (int m = plo; m < phi; m++) {
+      const HRow x = hrow_at(wp, cw, sh), y = hrow_at(wp + WS, cw, sh);
+      wp += 2 * WS;
+      if (P0) {
+        *p0 = sgr_ab(ps + c.h5 + x.h5 + y.h5, pq + c.q5 + x.q5 + y.q5, rd, k0, tt);
+        ps = c.h5 + x.h5;
+        pq = c.q5 + x.q5;
+      }
+      if (P1) {
+        p1[0] = sgr_ab(b.h3 + c.h3 + x.h3, b.q3 + c.q3 + x.q3, rd, k1, tt);
+        p1[ABW] = sgr_ab(c.h3 + x.h3 + y.h3, c.q3 + x.q3 + y.q3, rd, k1, tt);
+      }
+      b = x;
+      c = y;
+      p0 += ABW;
+      p1 += 2 * ABW;
+    }
+  }
+  const int npos = 2 * npair, nl = w - SW + 2;
+  for (int t = threadIdx.x; t < nl * npos; t += NT) {
+    const int q = t / npos, i = t - q * npos; /* position r0 - 1 + i */
+    const int jj = SW - 1 + q;
+    const int cw = WOFF + jj + 1;
+    const uint32_t sh = (cw & 1) << 4;
+    const uint16_t *p = &sm.win[(r0 + i) * WS];
+    const HRow h1 = hrow_at(p + WS, cw, sh), h2 = hrow_at(p + 2 * WS, cw, sh), h3 = hrow_at(p + 3 * WS, cw, sh);
+    if (P0 && (i & 1) == 0) { /* odd position: rows 1 .. 5 */
+      const HRow h0 = hrow_at(p, cw, sh), h4 = hrow_at(p + 4 * WS, cw, sh);
+      sm.sgr.ab0[(i >> 1) * ABW + jj + 1] =
+          sgr_ab(h0.h5 + h1.h5 + h2.h5 + h3.h5 + h4.h5, h0.q5 + h1.q5 + h2.q5 + h3.q5 + h4.q5, rd, k0, tt);
+    }
+    if (P1) sm.sgr.ab1[i * ABW + jj + 1] = sgr_ab(h1.h3 + h2.h3 + h3.h3, h1.q3 + h2.q3 + h3.q3, rd, k1, tt);
+  }
+}
+
+/* The 3x3 weighting of both passes and the projection (postfilter.rs:823-851)
+ * for output rows [r0, r1) at column j = col: lane group `band` takes row
+ * pairs (even i, i + 1) [qlo, qhi). acc = cu u + w0 f0 + w2 f1, where cu
+ * folds w1 and the weight of a pass that is off (its f is u << 4). */
+template <typename T, int SW, bool P0, bool P1>
+__device__ __forceinline__ void sgr_weight_part(const LrShared &sm, int band, int col, int w, int r0, int r1,
+                                                T *out, int64_t os, int cu, int w0, int w2, int maxv) {
+  constexpr int NB = NT / SW;
+  const int j = col;
+  int qlo, qhi;
+  band_rows((r1 - r0 + 1) >> 1, NB, band, qlo, qhi);
+  if (j >= w || qlo >= qhi) return;
+  int i = r0 + 2 * qlo;
+  const uint32_t *a0 = &sm.sgr.ab0[qlo * ABW + j];       /* odd row i - 1 */
+  const uint32_t *a1 = &sm.sgr.ab1[2 * qlo * ABW + j];   /* rows i - 1, i */
+  P0Row h0 = {0, 0};
+  P1Row m1 = {0, 0, 0, 0}, c1 = {0, 0, 0, 0};
+  if (P0) h0 = p0_row(a0);
+  if (P1) {
+    m1 = p1_row(a1);
+    c1 = p1_row(a1 + ABW);
+  }
+  a0 += ABW;
+  a1 += 2 * ABW;
+  const uint16_t *up = &sm.win[(i + 3) * WS + WOFF + j + 3];
+  T *o = out + (int64_t)i * os;
+  for (int q = qlo; q < qhi; q++, i += 2) {
+    const int u0 = up[0], u1 = up[WS];
+    up += 2 * WS;
+    int acc0 = __mul24(cu, u0) + 1024, acc1 = __mul24(cu, u1) + 1024;
+    if (P0) {
+      const P0Row n0 = p0_row(a0);
+      a0 += ABW;
+      const int f0e = (int)(((uint32_t)__mul24(h0.a + n0.a, u0) + (uint32_t)(h0.b + n0.b)) >> 9);
+      const int f0o = (int)(((uint32_t)__mul24(n0.a, u1) + (uint32_t)n0.b) >> 8);
+      acc0 += __mul24(w0, f0e);
+      acc1 += __mul24(w0, f0o);
+      h0 = n0;
+    }
+    if (P1) {
+      const P1Row n1 = p1_row(a1), n2 = p1_row(a1 + ABW);
+      a1 += 2 * ABW;
+      acc0 += __mul24(w2, p1_out(m1, c1, n1, u0));
+      acc1 += __mul24(w2, p1_out(c1, n1, n2, u1));
+      m1 = n1;
+      c1 = n2;
+    }
+    o[0] = (T)min(max(acc0 >> 11, 0), maxv);
+    if (i + 1 < r1) o[os] = (T)min(max(acc1 >> 11, 0), maxv);
+    o += 2 * os;
+  }
+}
+
+/* Self-guided restoration of one tile (box_filter_rect,
+ * postfilter.rs:901-991) in parts of SGR_PART output rows: the A/B of the
+ * part's positions, then the weighting. */
+template <typename T, int SW, bool P0, bool P1>
+__device__ __forceinline__ void sgr_tile(LrShared &sm, const rvpf_lr_unit *u, int set, int band, int col, int w,
+                                         int h, int bd, T *out, int64_t os) {
+  const SgrK k0 = SGR_K.k[set][0], k1 = SGR_K.k[set][1];
+  const int sh = bd - 8;
+  const SgrRound rd = {(1u << sh) >> 1, (uint32_t)sh, (1u << (2 * sh)) >> 1, (uint32_t)(2 * sh)};
+  const int w0 = u->sgr_xqd[0], w1 = u->sgr_xqd[1];
+  const int w2 = 128 - w0 - w1;
+  const int cu = (w1 + (P0 ? 0 : w0) + (P1 ? 0 : w2)) << 4;
+  const int maxv = (1 << bd) - 1;
+  for (int r0 = 0; r0 < h; r0 += SGR_PART) {
+    const int r1 = min(h, r0 + SGR_PART);
+    if (r0) __syncthreads(); /* the previous part's weighting has read A/B */
+    sgr_ab_part<SW, P0, P1>(sm, band, col, w, r0, (r1 - r0 + 3) >> 1, rd, k0, k1);
+    __syncthreads();
+    sgr_weight_part<T, SW, P0, P1>(sm, band, col, w, r0, r1, out, os, cu, w0, w2, maxv);
+  }
+}
+
+typedef short short2_l __attribute__((ext_vector_type(2)));
+
+/* acc + a.lo * b.lo + a.hi * b.hi (signed 16-bit halves; v_dot2_i32_i16) */
+__device__ __forceinline__ int sdot2(uint32_t a, uint32_t b, int acc) {
+  return __builtin_amdgcn_sdot2(__builtin_bit_cast(short2_l, a), __builtin_bit_cast(short2_l, b), acc, false);
+}
+
+__device__ __forceinline__ uint32_t pack16(int lo, int hi) { return ((uint32_t)lo & 0xFFFFu) | ((uint32_t)hi << 16); }
+
+/* Wiener restoration of one tile (wiener_coefficient, postfilter.rs:994-1004,
+ * with rounding_variables, predict.rs:1153-1166; symmetric 7-tap filters).
+ * Horizontal pass: a thread filters an aligned pair of output columns (j,
+ * j + 1) of one window row from five aligned dwords of the window: the
+ * taps of column j + 1 are whole sample pairs, those of column j the same
+ * pairs shifted by one sample (v_alignbit), each pair times a packed pair
+ * of taps (v_dot2_i32_i16); the clipped result fits 16 bits and the pair is
+ * stored as one dword. Vertical pass: a thread walks its column down a
+ * band of rows with a 7-row register window. `o` points at the thread's
+ * column in the tile's first row. */
+template <typename T, int SW>
+__device__ __forceinline__ void wiener_tile(LrShared &sm, const rvpf_lr_unit *u, int band, int col, int w, int h,
+                                            int wh, int bd, T *o, int64_t os) {
+  constexpr int NB = NT / SW;
+  constexpr int NPAIR = SW / 2, LPAIR = SW == 64 ? 5 : 4; /* column pairs per row, log2 */
+  const int round0 = bd == 12 ? 5 : 3, round1 = bd == 12 ? 9 : 11;
+  const int v0 = u->wiener[0][0], v1 = u->wiener[0][1], v2 = u->wiener[0][2];
+  const int h0 = u->wiener[1][0], h1 = u->wiener[1][1], h2 = u->wiener[1][2];
+  const int v3 = 128 - 2 * (v0 + v1 + v2), h3 = 128 - 2 * (h0 + h1 + h2);
+  const int offset = 1 << (bd + 7 - round0 - 1);
+  const int limit = (1 << (bd + 1 + 7 - round0)) - 1;
+  const int cmin = -offset, cmax = limit - offset;
+  {
+    const uint32_t t01 = pack16(h0, h1), t23 = pack16(h2, h3), t21 = pack16(h2, h1);
+    const uint32_t t0l = pack16(h0, 0), t0h = pack16(0, h0);
+    const int rnd = (1 << round0) >> 1;
+    const int npairs = (w + 1) >> 1;
+    uint32_t *hb = reinterpret_cast<uint32_t *>(sm.hbuf);
+    for (int it = threadIdx.x; it < wh * NPAIR; it += NT) {
+      const int r = it >> LPAIR, q = it & (NPAIR - 1);
+      if (q >= npairs) continue;
+      /* output column j = 2q: window columns j .. j + 7 at LDS columns
+       * j + 5 .. j + 12, inside dwords (j + 4) / 2 .. (j + 12) / 2 */
+      const uint32_t *d = reinterpret_cast<const uint32_t *>(&sm.win[r * WS]) + q + 2;
+      const uint32_t d2 = d[0], d3 = d[1], d4 = d[2], d5 = d[3], d6 = d[4];
+      int s0 = sdot2(__builtin_amdgcn_alignbit(d3, d2, 16), t01, rnd);
+      s0 = sdot2(__builtin_amdgcn_alignbit(d4, d3, 16), t23, s0);
+      s0 = sdot2(__builtin_amdgcn_alignbit(d5, d4, 16), t21, s0);
+      s0 = sdot2(d5, t0h, s0);
+      int s1 = sdot2(d3, t01, rnd);
+      s1 = sdot2(d4, t23, s1);
+      s1 = sdot2(d5, t21, s1);
+      s1 = sdot2(d6, t0l, s1);
+      hb[r * (MAXW / 2) + q] = pack16(min(max(s0 >> round0, cmin), cmax), min(max(s1 >> round0, cmin), cmax));
+    }
+  }
+  __syncthreads();
+  if (col >= w) return;
+  constexpr int RB = (64 + NB - 1) / NB; /* rows per band at most */
+  int lo, hi;
+  band_rows(h, NB, band, lo, hi);
+  const int16_t *hb = &sm.hbuf[lo * MAXW + col];
+  const int rnd1 = (1 << round1) >> 1, maxv = (1 << bd) - 1;
+  int r0 = hb[0], r1 = hb[MAXW], r2 = hb[2 * MAXW], r3 = hb[3 * MAXW], r4 = hb[4 * MAXW], r5 = hb[5 * MAXW];
+  o += (int64_t)lo * os;
+#pragma unroll
+  for (int k = 0; k < RB; k++) {
+    if (lo + k >= hi) break;
+    const int r6 = hb[(k + 6) * MAXW];
+    const int sum = __mul24(v0, r0 + r6) + __mul24(v1, r1 + r5) + __mul24(v2, r2 + r4) + __mul24(v3, r3) + rnd1;
+    o[0] = (T)min(max(sum >> round1, 0), maxv);
+    o += os;
+    r0 = r1;
+    r1 = r2;
+    r2 = r3;
+    r3 = r4;
+    r4 = r5;
+    r5 = r6;
   }
 }
 
@@ -308,194 +635,23 @@ __device__ __forceinline__ void lr_tile(const LrLaunch &a, int plane, int st, in
     }
   }
   const int bd = g.bd;
-  const int maxv = (1 << bd) - 1;
-
   if (type == RVPF_RESTORE_WIENER) {
     __syncthreads();
-    /* wiener_coefficient (postfilter.rs:994-1004) + rounding_variables
-     * (predict.rs:1153-1166); symmetric taps */
-    const int round0 = bd == 12 ? 5 : 3, round1 = bd == 12 ? 9 : 11;
-    const int v0 = u->wiener[0][0], v1 = u->wiener[0][1], v2 = u->wiener[0][2];
-    const int h0 = u->wiener[1][0], h1 = u->wiener[1][1], h2 = u->wiener[1][2];
-    const int v3 = 128 - 2 * (v0 + v1 + v2), h3 = 128 - 2 * (h0 + h1 + h2);
-    const int offset = 1 << (bd + 7 - round0 - 1);
-    const int limit = (1 << (bd + 1 + 7 - round0)) - 1;
-    if (col < w)
-      for (int r = band; r < wh; r += NB) {
-        const uint16_t *s = &sm.win[r * WS + WOFF + col];
-        const int sum = __mul24(h0, s[0] + s[6]) + __mul24(h1, s[1] + s[5]) + __mul24(h2, s[2] + s[4]) +
-                        __mul24(h3, (int)s[3]);
-        sm.hbuf[r * MAXW + col] = (int16_t)min(max(round2(sum, round0), -offset), limit - offset);
-      }
-    __syncthreads();
-    if (col < w) {
-      int lo, hi;
-      band_rows(h, NB, band, lo, hi);
-      const int16_t *hb = &sm.hbuf[lo * MAXW + col];
-      int r0 = hb[0], r1 = hb[MAXW], r2 = hb[2 * MAXW], r3 = hb[3 * MAXW], r4 = hb[4 * MAXW], r5 = hb[5 * MAXW];
-      T *o = out + (int64_t)(y0 + lo) * os + x0 + col;
-      for (int i = lo; i < hi; i++, o += os) {
-        const int r6 = sm.hbuf[(i + 6) * MAXW + col];
-        const int sum = __mul24(v0, r0 + r6) + __mul24(v1, r1 + r5) + __mul24(v2, r2 + r4) + __mul24(v3, r3);
-        *o = (T)min(max(round2(sum, round1), 0), maxv);
-        r0 = r1;
-        r1 = r2;
-        r2 = r3;
-        r3 = r4;
-        r4 = r5;
-        r5 = r6;
-      }
-    }
+    wiener_tile<T, SW>(sm, u, band, col, w, h, wh, bd, out + (int64_t)y0 * os + x0 + col, os);
     return;
   }
 
-  /* self-guided: box_filter_rect (postfilter.rs:901-991) for both passes */
-  for (int i = threadIdx.x; i < 256; i += NT) sm.sgr.a2[i] = SGR_A2.v[i];
+  /* self-guided */
+  for (int i = threadIdx.x; i < 256; i += NT) sm.sgr.t[i] = SGR_T.v[i];
   __syncthreads();
   const int set = u->sgr_set & 15;
-  const int rad0 = AV1_SGR_PARAMS[set][0], rad1 = AV1_SGR_PARAMS[set][2];
-  const SgrPass q0 = sgr_pass(2, rad0 ? AV1_SGR_PARAMS[set][1] : 1);
-  const SgrPass q1 = sgr_pass(1, rad1 ? AV1_SGR_PARAMS[set][3] : 1);
-  const uint16_t *a2t = sm.sgr.a2;
-  for (int r0 = 0; r0 < h; r0 += SGR_PART) {
-    /* output rows [r0, r1) of this part */
-    const int r1 = min(h, r0 + SGR_PART);
-    if (r0) __syncthreads(); /* the previous part's weighting has read A/B */
-    /* A/B at positions (i, j), i in [r0 - 1, r1], j in [-1, w]; the box of
-     * radius 2 covers window rows i+1..i+5 and columns j+1..j+5. The walk
-     * of thread (band, col) covers column j = col - 1 (the columns past
-     * SW - 2 are done directly below). */
-    {
-      const int j = col - 1;
-      const int np = r1 - r0 + 2;
-      int lo, hi;
-      band_rows(np, NB, band, lo, hi);
-      lo += r0 - 1;
-      hi += r0 - 1;
-      if (j <= w) {
-        uint32_t h5a = 0, h5b = 0, h5c = 0, h5d = 0, q5a = 0, q5b = 0, q5c = 0, q5d = 0;
-        uint32_t h3a = 0, h3b = 0, q3a = 0, q3b = 0;
-        const uint16_t *wp = &sm.win[(lo + 1) * WS + WOFF + j + 1];
-        for (int R = lo + 1; R <= hi + 4; R++, wp += WS) {
-          const uint32_t w1 = wp[0], w2 = wp[1], w3 = wp[2], w4 = wp[3], w5 = wp[4];
-          const uint32_t h3n = w2 + w3 + w4, h5n = h3n + w1 + w5;
-          const uint32_t q3n = __umul24(w2, w2) + __umul24(w3, w3) + __umul24(w4, w4);
-          const uint32_t q5n = q3n + __umul24(w1, w1) + __umul24(w5, w5);
-          const int i5 = R - 5, i3 = R - 4;
-          if (rad0 && i5 >= lo && ((y0 + i5) & 1))
-            sm.sgr.ab0[((i5 - r0 + 1) >> 1) * ABW + j + 1] =
-                sgr_ab(h5a + h5b + h5c + h5d + h5n, q5a + q5b + q5c + q5d + q5n, bd, q0, a2t);
-          if (rad1 && i3 >= lo && i3 < hi) sm.sgr.ab1[(i3 - r0 + 1) * ABW + j + 1] = sgr_ab(h3a + h3b + h3n, q3a + q3b + q3n, bd, q1, a2t);
-          h5a = h5b;
-          h5b = h5c;
-          h5c = h5d;
-          h5d = h5n;
-          q5a = q5b;
-          q5b = q5c;
-          q5c = q5d;
-          q5d = q5n;
-          h3a = h3b;
-          h3b = h3n;
-          q3a = q3b;
-          q3b = q3n;
-        }
-      }
-      /* columns SW - 1 .. w, every row, directly */
-      const int nl = w - SW + 2;
-      for (int t = threadIdx.x; t < nl * np; t += NT) {
-        const int jj = SW - 1 + t / np, i = r0 - 1 + t % np;
-        uint32_t s5 = 0, sq5 = 0, s3 = 0, sq3 = 0;
-        for (int dy = 0; dy < 5; dy++)
-          for (int dx = 0; dx < 5; dx++) {
-            const uint32_t v = sm.win[(i + 1 + dy) * WS + WOFF + jj + 1 + dx];
-            s5 += v;
-            sq5 += __umul24(v, v);
-            if (dy >= 1 && dy <= 3 && dx >= 1 && dx <= 3) {
-              s3 += v;
-              sq3 += __umul24(v, v);
-            }
-          }
-        if (rad0 && ((y0 + i) & 1)) sm.sgr.ab0[((i - r0 + 1) >> 1) * ABW + jj + 1] = sgr_ab(s5, sq5, bd, q0, a2t);
-        if (rad1) sm.sgr.ab1[(i - r0 + 1) * ABW + jj + 1] = sgr_ab(s3, sq3, bd, q1, a2t);
-      }
-    }
-    __syncthreads();
-    /* 3x3 weighting of both passes and the projection (postfilter.rs:823-851) */
-    if (col < w) {
-      const int j = col;
-      const int w0 = u->sgr_xqd[0], w1 = u->sgr_xqd[1];
-      const int w2 = 128 - w0 - w1;
-      int lo, hi;
-      band_rows(r1 - r0, NB, band, lo, hi);
-      lo += r0;
-      hi += r0;
-      /* per A/B row: E = A(j-1) + A(j+1), C = A(j) (and the same for B) */
-      const uint32_t *p1 = &sm.sgr.ab1[(lo - r0) * ABW + j];
-      int ea_m = 0, ca_m = 0, eb_m = 0, cb_m = 0, ea_c = 0, ca_c = 0, eb_c = 0, cb_c = 0;
-      if (rad1) {
-        uint32_t l = p1[0], c = p1[1], r = p1[2];
-        ea_m = (int)((l >> 20) + (r >> 20));
-        ca_m = (int)(c >> 20);
-        eb_m = (int)((l & 0xFFFFF) + (r & 0xFFFFF));
-        cb_m = (int)(c & 0xFFFFF);
-        l = p1[ABW];
-        c = p1[ABW + 1];
-        r = p1[ABW + 2];
-        ea_c = (int)((l >> 20) + (r >> 20));
-        ca_c = (int)(c >> 20);
-        eb_c = (int)((l & 0xFFFFF) + (r & 0xFFFFF));
-        cb_c = (int)(c & 0xFFFFF);
-      }
-      /* pass 0: 6*C + 5*E of the last odd row at or above i */
-      int ra_p = 0, rb_p = 0;
-      if (rad0) {
-        const int io = ((y0 + lo) & 1) ? lo : lo - 1;
-        const uint32_t *p0 = &sm.sgr.ab0[((io - r0 + 1) >> 1) * ABW + j];
-        const uint32_t l = p0[0], c = p0[1], r = p0[2];
-        ra_p = 6 * (int)(c >> 20) + 5 * (int)((l >> 20) + (r >> 20));
-        rb_p = 6 * (int)(c & 0xFFFFF) + 5 * (int)((l & 0xFFFFF) + (r & 0xFFFFF));
-      }
-      T *o = out + (int64_t)(y0 + lo) * os + x0 + j;
-      for (int i = lo; i < hi; i++, o += os) {
-        const int uu = sm.win[(i + 3) * WS + WOFF + j + 3];
-        const int ue = uu << 4;
-        int f0 = ue, f1 = ue;
-        if (rad0) {
-          if ((y0 + i) & 1) {
-            f0 = round2(__mul24(ra_p, uu) + rb_p, 8);
-          } else {
-            const uint32_t *p0 = &sm.sgr.ab0[((i + 2 - r0) >> 1) * ABW + j];
-            const uint32_t l = p0[0], c = p0[1], r = p0[2];
-            const int ra_n = 6 * (int)(c >> 20) + 5 * (int)((l >> 20) + (r >> 20));
-            const int rb_n = 6 * (int)(c & 0xFFFFF) + 5 * (int)((l & 0xFFFFF) + (r & 0xFFFFF));
-            f0 = round2(__mul24(ra_p + ra_n, uu) + rb_p + rb_n, 9);
-            ra_p = ra_n;
-            rb_p = rb_n;
-          }
-        }
-        if (rad1) {
-          const uint32_t *pn = &sm.sgr.ab1[(i + 2 - r0) * ABW + j];
-          const uint32_t l = pn[0], c = pn[1], r = pn[2];
-          const int ea_n = (int)((l >> 20) + (r >> 20)), ca_n = (int)(c >> 20);
-          const int eb_n = (int)((l & 0xFFFFF) + (r & 0xFFFFF)), cb_n = (int)(c & 0xFFFFF);
-          /* weights 3 4 3 / 4 4 4 / 3 4 3 */
-          const int av = 3 * (ea_m + ea_n) + 4 * (ca_m + ca_n + ea_c + ca_c);
-          const int bv = 3 * (eb_m + eb_n) + 4 * (cb_m + cb_n + eb_c + cb_c);
-          f1 = round2(__mul24(av, uu) + bv, 9);
-          ea_m = ea_c;
-          ca_m = ca_c;
-          eb_m = eb_c;
-          cb_m = cb_c;
-          ea_c = ea_n;
-          ca_c = ca_n;
-          eb_c = eb_n;
-          cb_c = cb_n;
-        }
-        const int acc = __mul24(w1, ue) + __mul24(w0, f0) + __mul24(w2, f1);
-        *o = (T)min(max(round2(acc, 11), 0), maxv);
-      }
-    }
-  }
+  T *ob = out + (int64_t)y0 * os + x0 + col;
+  if (AV1_SGR_PARAMS[set][0] == 0)
+    sgr_tile<T, SW, false, true>(sm, u, set, band, col, w, h, bd, ob, os);
+  else if (AV1_SGR_PARAMS[set][2] == 0)
+    sgr_tile<T, SW, true, false>(sm, u, set, band, col, w, h, bd, ob, os);
+  else
+    sgr_tile<T, SW, true, true>(sm, u, set, band, col, w, h, bd, ob, os);
 }
 
 /* Slice lb of grid plane gp (stripe-major). */
