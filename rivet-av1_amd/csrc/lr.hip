@@ -270,6 +270,25 @@ __device__ __forceinline__ void split_row(const uint32_t *p, int &et, int &eb, i
   cb = (int)(c & 0x1FFFFFu);
 }
 
+/* 24-bit signed multiply (-add): values carried around a loop lose the
+ * range the compiler would need to pick the 24-bit instruction itself. */
+__device__ __forceinline__ int mul24i(int a, int b) {
+  int r;
+  asm("v_mul_i32_i24 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+__device__ __forceinline__ int mad24i(int a, int b, int c) {
+  int r;
+  asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+  return r;
+}
+/* clip3(lo, hi, x) as one v_med3_i32 (lo <= hi) */
+__device__ __forceinline__ int med3i(int x, int lo, int hi) {
+  int r;
+  asm("v_med3_i32 %0, %1, %2, %3" : "=v"(r) : "v"(x), "v"(lo), "v"(hi));
+  return r;
+}
+
 /* Pass 0 (r = 2) on one odd row: a = the sum of 6 A(j) + 5 (A(j-1) +
  * A(j+1)), as 16 * 256 less the same weights of t, and b = the same sum of
  * B plus 128, the rounding of an odd output row. */
@@ -279,26 +298,28 @@ struct P0Row {
 __device__ __forceinline__ P0Row p0_row(const uint32_t *p) {
   int et, eb, ct, cb;
   split_row(p, et, eb, ct, cb);
-  return {__mul24(ct, -6) + (__mul24(et, -5) + 4096), __mul24(cb, 6) + (__mul24(eb, 5) + 128)};
+  return {mad24i(ct, -6, mad24i(et, -5, 4096)), mad24i(cb, 6, mad24i(eb, 5, 128))};
 }
 
-/* Pass 1 (r = 1) on one row: E and E + C of t and of B. */
+/* Pass 1 (r = 1) terms of one row: as an outer row (weights 3 4 3) x =
+ * -(3 Et + 4 Ct) of t and z = 3 Eb + 4 Cb + 128 of B, as the centre row
+ * (weights 4 4 4) y = -4 (Et + Ct) and wb = 4 (Eb + Cb). */
 struct P1Row {
-  int et, st, eb, sb;
+  int x, y, z, wb;
 };
 __device__ __forceinline__ P1Row p1_row(const uint32_t *p) {
   int et, eb, ct, cb;
   split_row(p, et, eb, ct, cb);
-  return {et, et + ct, eb, eb + cb};
+  const int x = mad24i(et, -3, mul24i(ct, -4));
+  const int z = mad24i(eb, 3, mad24i(cb, 4, 128));
+  return {x, x - et, z, z + eb - 128};
 }
 
-/* Pass 1 output from rows m, c, n: weights 3 4 3 / 4 4 4 / 3 4 3, that is
- * 4 (all nine) less the four corners, of A = 256 - t (weights sum to 32)
- * and of B, rounded by 9 bits. */
+/* Pass 1 output from rows m, c, n: A = 8192 + xm + xn + yc (the nine
+ * weights of A = 256 - t sum to 32), B = zm + zn + wbc (with the rounding
+ * 256); round2(A u + B, 9). */
 __device__ __forceinline__ int p1_out(const P1Row &m, const P1Row &c, const P1Row &n, int u) {
-  const int a = __mul24(m.st + c.st + n.st, -4) + (m.et + n.et + 8192);
-  const uint32_t b = __umul24((uint32_t)(m.sb + c.sb + n.sb), 4u) + (uint32_t)(256 - m.eb - n.eb);
-  return (int)(((uint32_t)__mul24(a, u) + b) >> 9);
+  return mad24i(m.x + n.x + c.y, u, (u << 13) + (m.z + n.z + c.wb)) >> 9;
 }
 
 /* A/B of both passes (P0: r = 2 on odd rows, P1: r = 1) at positions
@@ -389,31 +410,38 @@ __device__ __forceinline__ void sgr_weight_part(const LrShared &sm, int band, in
   a1 += 2 * ABW;
   const uint16_t *up = &sm.win[(i + 3) * WS + WOFF + j + 3];
   T *o = out + (int64_t)i * os;
-  for (int q = qlo; q < qhi; q++, i += 2) {
+  /* output rows i and i + 1; unrolled by two so that the row terms
+   * carried from one pair to the next need no register moves */
+  auto step = [&]() {
     const int u0 = up[0], u1 = up[WS];
     up += 2 * WS;
-    int acc0 = __mul24(cu, u0) + 1024, acc1 = __mul24(cu, u1) + 1024;
+    int acc0 = mad24i(cu, u0, 1024), acc1 = mad24i(cu, u1, 1024);
     if (P0) {
       const P0Row n0 = p0_row(a0);
       a0 += ABW;
-      const int f0e = (int)(((uint32_t)__mul24(h0.a + n0.a, u0) + (uint32_t)(h0.b + n0.b)) >> 9);
-      const int f0o = (int)(((uint32_t)__mul24(n0.a, u1) + (uint32_t)n0.b) >> 8);
-      acc0 += __mul24(w0, f0e);
-      acc1 += __mul24(w0, f0o);
+      acc0 = mad24i(w0, mad24i(h0.a + n0.a, u0, h0.b + n0.b) >> 9, acc0);
+      acc1 = mad24i(w0, mad24i(n0.a, u1, n0.b) >> 8, acc1);
       h0 = n0;
     }
     if (P1) {
       const P1Row n1 = p1_row(a1), n2 = p1_row(a1 + ABW);
       a1 += 2 * ABW;
-      acc0 += __mul24(w2, p1_out(m1, c1, n1, u0));
-      acc1 += __mul24(w2, p1_out(c1, n1, n2, u1));
+      acc0 = mad24i(w2, p1_out(m1, c1, n1, u0), acc0);
+      acc1 = mad24i(w2, p1_out(c1, n1, n2, u1), acc1);
       m1 = n1;
       c1 = n2;
     }
-    o[0] = (T)min(max(acc0 >> 11, 0), maxv);
-    if (i + 1 < r1) o[os] = (T)min(max(acc1 >> 11, 0), maxv);
+    o[0] = (T)med3i(acc0 >> 11, 0, maxv);
+    if (i + 1 < r1) o[os] = (T)med3i(acc1 >> 11, 0, maxv);
     o += 2 * os;
+    i += 2;
+  };
+  int q = qlo;
+  for (; q + 1 < qhi; q += 2) {
+    step();
+    step();
   }
+  if (q < qhi) step();
 }
 
 /* Self-guided restoration of one tile (box_filter_rect,
