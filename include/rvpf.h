@@ -279,7 +279,7 @@ int rvpf_grain(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_planes *ref
  * is for callers that cannot allocate device memory (code under HIP graph
  * capture, tight serving loops, harnesses that forbid hipMalloc). It takes
  * no context; the film grain scratch (noise templates, scaling LUTs, block
- * offsets, 376 320 bytes) is a static device array of the library, one per
+ * offsets, 399 360 bytes) is a static device array of the library, one per
  * GPU, held until the process ends. Frames up to 8192 x 8192 (upscaled_width
  * x frame_height); larger ones return RVPF_E_INVALID before anything is
  * enqueued. Calls that overlap on one GPU (on different streams or from

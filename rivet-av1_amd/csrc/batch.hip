@@ -493,7 +493,7 @@ int rvpf_batch_create(rvpf_ctx *ctx, const rvpf_batch_frame *frames, int32_t n, 
     if (x.shown && grain_on(p)) {
       int stripes, blocks;
       grain_offsets_dims(p->upscaled_width, p->frame_height, &stripes, &blocks);
-      grain_bytes += align_up(GRAIN_TMPL_BYTES, 256) + align_up(3 * 256 * sizeof(int16_t), 256) +
+      grain_bytes += align_up(GRAIN_TMPL_BYTES, 256) + align_up(GRAIN_LUT_BYTES, 256) +
                      align_up((size_t)stripes * blocks, 256);
     }
   }
@@ -534,7 +534,7 @@ int rvpf_batch_create(rvpf_ctx *ctx, const rvpf_batch_frame *frames, int32_t n, 
         x.gs.tmpl = reinterpret_cast<int16_t *>(base);
         base += align_up(GRAIN_TMPL_BYTES, 256);
         x.gs.lut = reinterpret_cast<int16_t *>(base);
-        base += align_up(3 * 256 * sizeof(int16_t), 256);
+        base += align_up(GRAIN_LUT_BYTES, 256);
         x.gs.offsets = reinterpret_cast<uint8_t *>(base);
         x.gs.max_stripes = stripes;
         x.gs.max_blocks = blocks;

@@ -488,9 +488,9 @@ constexpr int TABLE_ITEMS = 8;
 /* Item j of the tables stage of one frame (256 threads): the shifted
  * copies 1..7 of the templates from copy 0 (copy s at x holds sample
  * x + s, 0 past the row; each thread takes one 8-sample chunk of a row),
- * the scaling LUTs (item 0) and the block offsets of the noise stripes
+ * its share of the scaling functions and the block offsets of the noise stripes
  * j, j + TABLE_ITEMS, ... */
-__device__ __forceinline__ void grain_tables_item(const GrainHdr &h, const GrainScratch &sc, int j) {
+__device__ __forceinline__ void grain_tables_item(const GrainHdr &h, const GrainScratch &sc, int j, int16_t *l256) {
   constexpr int Q = GRAIN_TWS / 8;
   for (int i = j * 256 + threadIdx.x; i < 3 * TH * Q; i += TABLE_ITEMS * 256) {
     const int q = i % Q, r = i / Q;
@@ -511,29 +511,41 @@ __device__ __forceinline__ void grain_tables_item(const GrainHdr &h, const Grain
       *reinterpret_cast<uint4 *>(dst + sft * GRAIN_TCOPY) = make_uint4(c[0], c[1], c[2], c[3]);
     }
   }
-  if (j == 0) {
-  /* scaling LUTs (grain.rs:137-165) */
-    for (int p = 0; p < h.nplanes; p++) {
-      const int np = h.npoints[p];
-      const int *xs = h.points[p][0], *ys = h.points[p][1];
-      for (int x = threadIdx.x; x < 256; x += 256) {
-        int v = 0;
-        if (np > 0) {
-          if (x < xs[0]) {
-            v = ys[0];
-          } else if (x >= xs[np - 1]) {
-            v = ys[np - 1];
-          } else {
-            int i = 0;
-            while (i + 1 < np && xs[i + 1] <= x) i++;
-            const int dy = ys[i + 1] - ys[i], dx = xs[i + 1] - xs[i];
-            const int delta = dy * ((65536 + (dx >> 1)) / dx);
-            v = ys[i] + (((x - xs[i]) * delta + 32768) >> 16);
-          }
-        }
-        sc.lut[p * 256 + x] = (int16_t)v;
+  /* scaling LUTs (grain.rs:137-165) at the 256 8-bit points, then the
+   * scaling function of scale_lut (grain.rs:167-177) at every sample value
+   * 0 .. 2^bd - 1, interpolated between them: item j writes values
+   * j * 256 + t, j * 256 + t + TABLE_ITEMS * 256, ... of every plane */
+  for (int p = 0; p < h.nplanes; p++) {
+    const int np = h.npoints[p];
+    const int *xs = h.points[p][0], *ys = h.points[p][1];
+    const int x = threadIdx.x;
+    int v = 0;
+    if (np > 0) {
+      if (x < xs[0]) {
+        v = ys[0];
+      } else if (x >= xs[np - 1]) {
+        v = ys[np - 1];
+      } else {
+        int i = 0;
+        while (i + 1 < np && xs[i + 1] <= x) i++;
+        const int dy = ys[i + 1] - ys[i], dx = xs[i + 1] - xs[i];
+        const int delta = dy * ((65536 + (dx >> 1)) / dx);
+        v = ys[i] + (((x - xs[i]) * delta + 32768) >> 16);
       }
     }
+    l256[p * 256 + x] = (int16_t)v;
+  }
+  __syncthreads();
+  {
+    const int sh = h.bd - 8, n = 1 << h.bd;
+    for (int p = 0; p < h.nplanes; p++)
+      for (int idx = j * 256 + threadIdx.x; idx < n; idx += TABLE_ITEMS * 256) {
+        const int16_t *lut = l256 + p * 256;
+        const int x = idx >> sh, rem = idx - (x << sh);
+        int v = lut[x];
+        if (sh && x < 255) v += round2((lut[x + 1] - lut[x]) * rem, sh);
+        sc.lut[p * GRAIN_LUT_N + idx] = (int16_t)v;
+      }
   }
   /* block offsets per noise stripe (grain.rs:186-199) */
   const int nstripes = ((h.h + 1) / 2 + 15) / 16;
@@ -556,13 +568,16 @@ __device__ __forceinline__ void grain_prepare_body(const GrainHdr &h, const Grai
   prep_out(h, sc, sm);
 }
 
-__device__ __forceinline__ int scale_lut(const int16_t *lut, int bd, int index) {
-  const int sh = bd - 8;
-  const int x = index >> sh;
-  const int rem = index - (x << sh);
-  if (bd == 8 || x == 255) return lut[x];
-  const int start = lut[x], end = lut[x + 1];
-  return start + round2((end - start) * rem, sh);
+
+/* The scaling functions of the frame's planes into LDS: plane p's value
+ * at sample v at lut[(p << bd) + v], in 16-byte chunks. */
+__device__ __forceinline__ void load_luts(const GrainHdr &h, const GrainScratch &sc, int16_t *lut) {
+  const int lg = h.bd - 3; /* 8-entry chunks per plane, log2 */
+  const int n = h.nplanes << lg;
+  for (int c = threadIdx.x; c < n; c += 256) {
+    const int p = c >> lg, k = c - (p << lg);
+    reinterpret_cast<uint4 *>(lut)[c] = reinterpret_cast<const uint4 *>(sc.lut + p * GRAIN_LUT_N)[k];
+  }
 }
 
 /* Template blend of two noise values (the horizontal and vertical
@@ -714,7 +729,7 @@ __device__ __forceinline__ void grain_apply_tile(const GrainHdr &h, const GrainS
     lon[q] = lx < h.w && ly < h.h;
     if (lon[q]) Seg8<T>::load(lin + (int64_t)ly * ls + lx, h.w - lx, lorig[q]);
   }
-  for (int i = threadIdx.x; i < h.nplanes * 256; i += 256) lut[i] = sc.lut[i];
+  load_luts(h, sc, lut);
   __syncthreads();
   int lz[LS][8];
 #pragma unroll
@@ -784,7 +799,7 @@ __device__ __forceinline__ void grain_apply_tile(const GrainHdr &h, const GrainS
             const int combined = cavg[u][k] * (lm - 128) + corig[u][k] * (mu - 128);
             merged = clip3(0, maxv, (combined >> 6) + (of - 256) * (1 << (bd - 8)));
           }
-          const int sn = round2(scale_lut(lut + p * 256, bd, merged) * cz[u][k], h.scaling_shift);
+          const int sn = round2(lut[(p << bd) + merged] * cz[u][k], h.scaling_shift);
           v[k] = clip3(h.min_value, h.max_chroma, corig[u][k] + sn);
         }
       }
@@ -801,7 +816,7 @@ __device__ __forceinline__ void grain_apply_tile(const GrainHdr &h, const GrainS
     for (int k = 0; k < 8; k++) {
       v[k] = lorig[q][k];
       if (h.num_y > 0) {
-        const int sn = round2(scale_lut(lut, bd, lorig[q][k]) * lz[q][k], h.scaling_shift);
+        const int sn = round2(lut[lorig[q][k]] * lz[q][k], h.scaling_shift);
         v[k] = clip3(h.min_value, h.max_luma, lorig[q][k] + sn);
       }
     }
@@ -817,7 +832,7 @@ __device__ __forceinline__ void grain_apply_tile_unaligned(const GrainHdr &h, co
                                                            const PlanePtr &i0, const PlanePtr &i1, const PlanePtr &i2,
                                                            const PlanePtr &o0, const PlanePtr &o1, const PlanePtr &o2,
                                                            int tile_id, int16_t *lut) {
-  for (int i = threadIdx.x; i < h.nplanes * 256; i += 256) lut[i] = sc.lut[i];
+  load_luts(h, sc, lut);
   __syncthreads();
   const int ntx = (h.w + 63) >> 6;
   const int ln = tile_id / ntx, x0 = (tile_id - ln * ntx) * 64, y0 = ln * 32;
@@ -860,7 +875,7 @@ __device__ __forceinline__ void grain_apply_tile_unaligned(const GrainHdr &h, co
         const int bs = 32 >> ssx;
         int z[8];
         seg_noise(pt, sc.offsets, mb, ln, cx / bs, (cx % bs) & ~7, ci, ssx, ssy, ov, gmin, gmax, z);
-        const int sn = round2(scale_lut(lut + p * 256, bd, merged) * z[cx & 7], h.scaling_shift);
+        const int sn = round2(lut[(p << bd) + merged] * z[cx & 7], h.scaling_shift);
         v = clip3(h.min_value, h.max_chroma, orig + sn);
       }
       gptr(reinterpret_cast<TO *>(op.data))[(int64_t)cy * op.stride + cx] = (TO)v;
@@ -874,7 +889,7 @@ __device__ __forceinline__ void grain_apply_tile_unaligned(const GrainHdr &h, co
     if (h.num_y > 0) {
       int z[8];
       seg_noise(sc.tmpl, sc.offsets, mb, ln, x >> 5, x & 24, i, 0, 0, ov, gmin, gmax, z);
-      const int sn = round2(scale_lut(lut, bd, orig) * z[x & 7], h.scaling_shift);
+      const int sn = round2(lut[orig] * z[x & 7], h.scaling_shift);
       v = clip3(h.min_value, h.max_luma, orig + sn);
     }
     gptr(reinterpret_cast<TO *>(o0.data))[(int64_t)y * o0.stride + x] = (TO)v;
@@ -887,7 +902,8 @@ __global__ __launch_bounds__(PREP_THREADS) void grain_prepare_kernel(GrainHdr h,
 }
 
 __global__ __launch_bounds__(256) void grain_tables_kernel(GrainHdr h, GrainScratch sc) {
-  grain_tables_item(h, sc, blockIdx.x);
+  __shared__ int16_t l256[3 * 256];
+  grain_tables_item(h, sc, blockIdx.x, l256);
 }
 
 /* KIND (apply_kind): 1 (segments, 64 luma columns per tile: 4:2:2, mono),
@@ -899,7 +915,8 @@ __global__ __launch_bounds__(256) void grain_tables_kernel(GrainHdr h, GrainScra
 template <typename T, typename TO, int ALIGNED>
 __global__ __launch_bounds__(256, ALIGNED == 3 ? 5 : 1) void grain_apply_kernel(GrainHdr h, GrainScratch sc, PlanePtr i0, PlanePtr i1,
                                                           PlanePtr i2, PlanePtr o0, PlanePtr o1, PlanePtr o2) {
-  __shared__ int16_t lut[3 * 256];
+  extern __shared__ int16_t lut_dyn[]; /* 2 * (nplanes << bd) bytes */
+  int16_t *lut = lut_dyn;
   const int tile = xcd_remap(blockIdx.x, gridDim.x);
   if constexpr (ALIGNED != 0)
     grain_apply_tile<T, TO, ALIGNED == 2 ? 2 : 1, ALIGNED == 3 ? 2 : 1>(h, sc, i0, i1, i2, o0, o1, o2, tile, lut);
@@ -932,14 +949,15 @@ __global__ __launch_bounds__(PREP_THREADS) void grain_prepare_batch(BatchWork w)
 __global__ __launch_bounds__(256) void grain_tables_batch(BatchWork w) {
   const int f = blockIdx.x / TABLE_ITEMS, j = blockIdx.x - f * TABLE_ITEMS;
   const PrepRecord &r = batch_table<PrepRecord>(w.rec, f);
-  grain_tables_item(r.h, r.s, j);
+  __shared__ int16_t l256[3 * 256];
+  grain_tables_item(r.h, r.s, j, l256);
 }
 
 /* 5 waves per SIMD (96 VGPRs, a few spills; unbounded it takes 108-112,
  * 4 waves, and runs 9 % slower). */
 template <typename T, bool DIRECT>
 __global__ __launch_bounds__(256, 5) void grain_apply_batch(BatchWork w) {
-  __shared__ int16_t lut[3 * 256];
+  __shared__ __align__(16) int16_t lut[3 * GRAIN_LUT_N];
   batch_loop<DIRECT>(w, [&](int f, int j) {
     const ApplyRecord &r = batch_table<ApplyRecord>(w.rec, f);
     if (r.kind == 3)
@@ -1040,14 +1058,15 @@ int launch_grain_apply_batch(const BatchWork &w, const StagePlan &sp, int bps, h
 template <typename T, typename TO>
 void apply_launch(int kind, int grid, const GrainHdr &h, const GrainScratch &s, const PlanePtr in[3],
                   PlanePtr out[3], hipStream_t st) {
+  const size_t lds = (size_t)(h.nplanes << h.bd) * sizeof(int16_t); /* the scaling functions */
   if (kind == 1)
-    grain_apply_kernel<T, TO, 1><<<grid, 256, 0, st>>>(h, s, in[0], in[1], in[2], out[0], out[1], out[2]);
+    grain_apply_kernel<T, TO, 1><<<grid, 256, lds, st>>>(h, s, in[0], in[1], in[2], out[0], out[1], out[2]);
   else if (kind == 3)
-    grain_apply_kernel<T, TO, 3><<<grid, 256, 0, st>>>(h, s, in[0], in[1], in[2], out[0], out[1], out[2]);
+    grain_apply_kernel<T, TO, 3><<<grid, 256, lds, st>>>(h, s, in[0], in[1], in[2], out[0], out[1], out[2]);
   else if (kind == 2)
-    grain_apply_kernel<T, TO, 2><<<grid, 256, 0, st>>>(h, s, in[0], in[1], in[2], out[0], out[1], out[2]);
+    grain_apply_kernel<T, TO, 2><<<grid, 256, lds, st>>>(h, s, in[0], in[1], in[2], out[0], out[1], out[2]);
   else
-    grain_apply_kernel<T, TO, 0><<<grid, 256, 0, st>>>(h, s, in[0], in[1], in[2], out[0], out[1], out[2]);
+    grain_apply_kernel<T, TO, 0><<<grid, 256, lds, st>>>(h, s, in[0], in[1], in[2], out[0], out[1], out[2]);
 }
 
 int launch_grain_apply(const GrainHdr &h, const GrainScratch &s, const PlanePtr in[3], PlanePtr out[3], int bps,

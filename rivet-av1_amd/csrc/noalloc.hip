@@ -7,7 +7,8 @@
  * Its kernels are those of rvpf_grain (grain.hip): the template build
  * (grain_prepare_kernel + grain_tables_kernel) and the noise blend. Their
  * scratch — 3 planes x 8 shifted copies of the 73 x 88 noise templates,
- * the 3 x 256 scaling LUTs and one byte per 32 x 32 noise block — lives in
+ * the 3 scaling functions (an entry per sample value) and one byte per
+ * 32 x 32 noise block — lives in
  * the file-scope array `scratch` below, sized for frames up to
  * NOALLOC_MAX_W x NOALLOC_MAX_H. Every GPU the library runs on holds its
  * own copy until the process ends. The device address of that copy is
@@ -39,7 +40,7 @@ constexpr int NOALLOC_STRIPES = ((NOALLOC_MAX_H + 1) / 2 + 15) / 16 + 1;
 constexpr int NOALLOC_BLOCKS = ((NOALLOC_MAX_W + 1) / 2 + 15) / 16 + 1;
 constexpr size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 constexpr size_t OFF_LUT = align256(GRAIN_TMPL_BYTES);
-constexpr size_t OFF_OFFSETS = OFF_LUT + align256(3 * 256 * sizeof(int16_t));
+constexpr size_t OFF_OFFSETS = OFF_LUT + align256(GRAIN_LUT_BYTES);
 constexpr size_t SCRATCH_BYTES = OFF_OFFSETS + align256((size_t)NOALLOC_STRIPES * NOALLOC_BLOCKS);
 
 __device__ __align__(256) uint8_t scratch[SCRATCH_BYTES];

@@ -482,7 +482,7 @@ int ensure_grain_scratch(rvpf_ctx *ctx, int slot, int w, int h) {
   const int mb = blocks > g.max_blocks ? blocks : g.max_blocks;
   hipError_t e = hipSuccess;
   if (!g.tmpl) e = hipMalloc(&g.tmpl, GRAIN_TMPL_BYTES);
-  if (e == hipSuccess && !g.lut) e = hipMalloc(&g.lut, 3 * 256 * sizeof(int16_t));
+  if (e == hipSuccess && !g.lut) e = hipMalloc(&g.lut, GRAIN_LUT_BYTES);
   if (e == hipSuccess) e = hipMalloc(&g.offsets, (size_t)ms * mb);
   if (e != hipSuccess) {
     (void)hipGetLastError();

@@ -84,11 +84,14 @@ constexpr int GRAIN_TH = 73, GRAIN_TW = 82, GRAIN_TWS = 88;
 constexpr int GRAIN_TCOPY = GRAIN_TH * GRAIN_TWS;  /* samples per shifted copy */
 constexpr int GRAIN_TPLANE = 8 * GRAIN_TCOPY;      /* samples per plane */
 constexpr size_t GRAIN_TMPL_BYTES = 3 * (size_t)GRAIN_TPLANE * sizeof(int16_t);
+/* Scaling function entries per plane: one per sample value up to 12 bits. */
+constexpr int GRAIN_LUT_N = 4096;
+constexpr size_t GRAIN_LUT_BYTES = 3 * (size_t)GRAIN_LUT_N * sizeof(int16_t);
 
 /* Device scratch for the grain stage. */
 struct GrainScratch {
   int16_t *tmpl;    /* [3][8][73][88], see GRAIN_TPLANE */
-  int16_t *lut;     /* [3][256] */
+  int16_t *lut;     /* [3][GRAIN_LUT_N]: the scaling function at every sample value */
   uint8_t *offsets; /* [stripes][blocks]: (offset_x << 4) | offset_y */
   int max_stripes, max_blocks;
 };
