@@ -737,6 +737,22 @@ __device__ __forceinline__ void grain_apply_tile(const GrainHdr &h, const GrainS
     const int lx = lx0 + 64 * q;
     if (lon[q]) seg_noise(sc.tmpl, sc.offsets, mb, ln, lx >> 5, lx & 31, li, 0, 0, ov, gmin, gmax, lz[q]);
   }
+  /* luma out first: its samples and noise are dead before the chroma pass */
+#pragma unroll
+  for (int q = 0; q < LS; q++) {
+    if (!lon[q]) continue;
+    const int lx = lx0 + 64 * q;
+    int v[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      v[k] = lorig[q][k];
+      if (h.num_y > 0) {
+        const int sn = round2(lut[lorig[q][k]] * lz[q][k], h.scaling_shift);
+        v[k] = clip3(h.min_value, h.max_luma, lorig[q][k] + sn);
+      }
+    }
+    Seg8<TO>::store(gptr(reinterpret_cast<TO *>(o0.data)) + (int64_t)ly * o0.stride + lx, h.w - lx, v);
+  }
   /* chroma segments: items threadIdx.x + 256 (u + CI pass), u < CI */
   const bool chroma = h.nplanes > 1;
   const int ssx = chroma ? h.ssx : 0, ssy = chroma ? h.ssy : 0;
@@ -806,21 +822,6 @@ __device__ __forceinline__ void grain_apply_tile(const GrainHdr &h, const GrainS
       const PlanePtr &op = p == 1 ? o1 : o2;
       Seg8<TO>::store(gptr(reinterpret_cast<TO *>(op.data)) + (int64_t)cy[u] * op.stride + cx[u], cw - cx[u], v);
     }
-  }
-#pragma unroll
-  for (int q = 0; q < LS; q++) {
-    if (!lon[q]) continue;
-    const int lx = lx0 + 64 * q;
-    int v[8];
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-      v[k] = lorig[q][k];
-      if (h.num_y > 0) {
-        const int sn = round2(lut[lorig[q][k]] * lz[q][k], h.scaling_shift);
-        v[k] = clip3(h.min_value, h.max_luma, lorig[q][k] + sn);
-      }
-    }
-    Seg8<TO>::store(gptr(reinterpret_cast<TO *>(o0.data)) + (int64_t)ly * o0.stride + lx, h.w - lx, v);
   }
 }
 
@@ -910,10 +911,11 @@ __global__ __launch_bounds__(256) void grain_tables_kernel(GrainHdr h, GrainScra
  * 2 (segments, two chroma segments per thread: 4:4:4), 3 (segments, 128
  * luma columns per tile: 4:2:0), 0 (sample by sample). Input samples T,
  * output samples TO (8-bit output from 16-bit planes for the packed 8-bit
- * frame). Kind 3 is held at 5 waves per SIMD (96 VGPRs and a 12-byte
- * spill; unbounded it takes 101, 4 waves, and runs 5 % slower). */
+ * frame). The luma samples are written before the chroma pass, so kind 3
+ * needs 61 VGPRs (8 waves per SIMD). The scaling functions take
+ * 2 * (nplanes << bd) bytes of dynamic LDS. */
 template <typename T, typename TO, int ALIGNED>
-__global__ __launch_bounds__(256, ALIGNED == 3 ? 5 : 1) void grain_apply_kernel(GrainHdr h, GrainScratch sc, PlanePtr i0, PlanePtr i1,
+__global__ __launch_bounds__(256) void grain_apply_kernel(GrainHdr h, GrainScratch sc, PlanePtr i0, PlanePtr i1,
                                                           PlanePtr i2, PlanePtr o0, PlanePtr o1, PlanePtr o2) {
   extern __shared__ int16_t lut_dyn[]; /* 2 * (nplanes << bd) bytes */
   int16_t *lut = lut_dyn;
@@ -953,8 +955,8 @@ __global__ __launch_bounds__(256) void grain_tables_batch(BatchWork w) {
   grain_tables_item(r.h, r.s, j, l256);
 }
 
-/* 5 waves per SIMD (96 VGPRs, a few spills; unbounded it takes 108-112,
- * 4 waves, and runs 9 % slower). */
+/* At most 5 waves per SIMD in the grab loop (it takes 73 VGPRs, 6 waves;
+ * the bound only caps the persistent form's registers). */
 template <typename T, bool DIRECT>
 __global__ __launch_bounds__(256, 5) void grain_apply_batch(BatchWork w) {
   __shared__ __align__(16) int16_t lut[3 * GRAIN_LUT_N];
