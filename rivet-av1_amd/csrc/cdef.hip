@@ -380,7 +380,7 @@ template <typename T, bool PRI, bool SEC>
 __device__ __forceinline__ void filter_block_pairs(const uint32_t *w, int t0, T *o, int64_t os, int c, int r0,
                                                    int nrows, const Taps &k, bool on) {
   o += (int64_t)r0 * os + c;
-#pragma unroll 1
+#pragma unroll 2
   for (int r = r0; r < r0 + nrows; r++, o += os) {
     const int base = ((t0 + r * TS + c) >> 1);
     const uint32_t f = filter_pair<PRI, SEC>(w, base, k);
@@ -549,11 +549,11 @@ __device__ __forceinline__ void cdef_tile(const CdefLaunch &a, int tile_id, uint
   }
 }
 
-/* 6 waves per SIMD: 80 VGPRs, no spills, with the prefetched 4:2:0 chroma
- * tiles held across the luma filter and the filter's row loop rolled (an
- * unroll by two takes 96 VGPRs, 5 waves, and is 2 % slower per 4K frame). */
+/* 5 waves per SIMD: 96 VGPRs, no spills (at 6 waves, 80 VGPRs, the
+ * prefetched 4:2:0 chroma tiles spill and the kernel is 6 % slower per 4K
+ * frame). */
 template <typename T>
-__global__ __launch_bounds__(256, 6) void cdef_kernel(CdefLaunch a) {
+__global__ __launch_bounds__(256, 5) void cdef_kernel(CdefLaunch a) {
   __shared__ uint32_t words[2 * SOFF];
   __shared__ CdefBlocks bl;
   cdef_tile<T>(a, xcd_remap(blockIdx.x, gridDim.x), words, bl);
