@@ -55,7 +55,7 @@ constexpr uint16_t apply_cols(const uint16_t *cols, uint16_t s) {
 /* LFSR states at the start of every template row: col[w][y][j] = A^(y W) e_j
  * for rows of W = 82 (w = 0) or 44 (w = 1) samples, so the state before
  * sample (y, 0) is one matrix-vector product with the seed. */
-struct RowJump {
+struct alignas(16) RowJump {
   uint16_t col[2][73][16];
 };
 
@@ -91,18 +91,14 @@ constexpr int PREP_THREADS = 192; /* three waves: luma, cb, cr */
 
 /* Row y of a Gaussian template (7.18.3.3, grain.rs:40-56): the LFSR
  * sequence from `seed` runs over the template in raster order, so the row
- * starts at state A^(y cols) seed (ROW_JUMP, wsel = 0 for 82 columns, 1 for
- * 44). The LFSR's bit stream b obeys b[k + 16] = b[k] ^ b[k + 1] ^ b[k + 3]
- * ^ b[k + 12], so 4 new bits at a time come from bits already known: 16
- * steps are 4 such groups, after which a 32-bit window holds b[0 .. 31]
- * and the state after step m is b[m .. m + 15]; its table index (state >> 5)
- * & 2047 is the 11-bit field at m + 5. The 16 table reads are issued before
- * the 16 template writes. Template rows that are off hold 0. */
-__device__ void fill_row(int16_t *t, const int16_t *gauss, uint32_t seed, int y, int cols, int wsel, int shift,
-                         bool on) {
-  uint32_t s = 0;
-#pragma unroll
-  for (int j = 0; j < 16; j++) s ^= (uint32_t)ROW_JUMP.col[wsel][y][j] & (0u - ((seed >> j) & 1u));
+ * starts at state A^(y cols) seed (row_state). The LFSR's bit stream b obeys
+ * b[k + 16] = b[k] ^ b[k + 1] ^ b[k + 3] ^ b[k + 12], so 4 new bits at a
+ * time come from bits already known: 16 steps are 4 such groups, after
+ * which a 32-bit window holds b[0 .. 31] and the state after step m is
+ * b[m .. m + 15]; its table index (state >> 5) & 2047 is the 11-bit field at
+ * m + 5. The 16 table reads are issued before the 16 template writes.
+ * Template rows that are off hold 0. */
+__device__ void fill_row(int16_t *t, const int16_t *gauss, uint32_t s, int y, int cols, int shift, bool on) {
   const int keep = on ? -1 : 0;
   int16_t *row = t + y * TW;
   for (int x0 = 0; x0 < cols; x0 += 16) {
@@ -121,6 +117,22 @@ __device__ void fill_row(int16_t *t, const int16_t *gauss, uint32_t seed, int y,
       if (x0 + k < cols) row[x0 + k] = (int16_t)(round2(v[k], shift) & keep);
     s = w >> 16;
   }
+}
+
+/* The LFSR state before sample (y, 0) of a template of 82 (wsel 0) or 44
+ * (wsel 1) columns: A^(y cols) seed, one GF(2) matrix-vector product with
+ * the row's 16 columns (two 16-byte loads). */
+__device__ __forceinline__ uint32_t row_state(uint32_t seed, int wsel, int y) {
+  const uint4 *c = reinterpret_cast<const uint4 *>(ROW_JUMP.col[wsel][y]);
+  const uint4 a = c[0], b = c[1];
+  const uint32_t w[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+  uint32_t s = 0;
+#pragma unroll
+  for (int j = 0; j < 16; j++) {
+    const uint32_t col = (w[j >> 1] >> ((j & 1) * 16)) & 0xffffu;
+    s ^= col & (0u - ((seed >> j) & 1u));
+  }
+  return s;
 }
 
 /* One chroma AR sample (grain.rs:101-134) with no neighbour taps (lag 0):
@@ -146,22 +158,16 @@ __device__ __forceinline__ int from_lane_below(int v) {
   return __builtin_amdgcn_mov_dpp(v, 0x13C, 0xF, 0xF, false);
 }
 
-/* Diagonals per round of the AR recursion (one workgroup barrier each). */
+/* Diagonals per chunk of the AR recursion. */
 constexpr int AR_ROUND = 16;
 
-/* Schedule of the three AR waves: rounds of AR_ROUND diagonals, a barrier
- * after each. Wave 0 (luma) runs its round r in barrier round r. A chroma
- * round j reads luma samples finished by diagonal 2*(16j+15) - 2K - 2
- * (subsampled) or 16j+15 (4:4:4), so it runs in barrier round 2j+2 or j+1,
- * when every luma round it depends on is behind a barrier. */
-struct ArPlan {
-  int first;  /* barrier round of the first chunk */
-  int halves; /* barrier rounds per chunk of AR_ROUND diagonals (1 or 2) */
-  int rounds; /* barrier rounds in all */
-};
+/* The first filtered diagonal of every template, that of sample (3, 3). */
+constexpr int ar_first(int K) { return 3 * K + 3; }
 
-__device__ __forceinline__ int ar_rounds(int K, int rows, int cols) {
-  return (K * (rows - 1) + cols - 1) / AR_ROUND + 1;
+/* Chunks of AR_ROUND diagonals from the first filtered diagonal through
+ * the last one, that of sample (rows - 1, cols - 4). */
+__device__ __forceinline__ int ar_chunks(int K, int rows, int cols) {
+  return (K * (rows - 1) + cols - 4 - ar_first(K)) / AR_ROUND + 1;
 }
 
 typedef short short2_g __attribute__((ext_vector_type(2)));
@@ -175,6 +181,29 @@ __device__ __forceinline__ int dot2(uint32_t a, uint32_t b, int acc) {
 __device__ __forceinline__ int med3(int lo, int hi, int x) {
   int r;
   asm("v_med3_i32 %0, %1, %2, %3" : "=v"(r) : "v"(x), "v"(lo), "v"(hi));
+  return r;
+}
+
+/* a.lo * b.lo + a.hi * b.hi: the VOP3P form with a zero addend, so no
+ * register is set to 0 first */
+__device__ __forceinline__ int dot2_0(uint32_t a, uint32_t b) {
+  int r;
+  asm("v_dot2_i32_i16 %0, %1, %2, 0" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+
+/* (a & m) | (b & ~m) as one v_bfi_b32 */
+__device__ __forceinline__ int bfi(int m, int a, int b) {
+  int r;
+  asm("v_bfi_b32 %0, %1, %2, %3" : "=v"(r) : "v"(m), "v"(a), "v"(b));
+  return r;
+}
+
+/* (x << sh) + c as one v_lshl_add_u32 (c in a VGPR, so the shift may be an
+ * SGPR) */
+__device__ __forceinline__ int lshl_add(int x, int sh, int c) {
+  int r;
+  asm("v_lshl_add_u32 %0, %1, %2, %3" : "=v"(r) : "v"(x), "s"(sh), "v"(c));
   return r;
 }
 
@@ -220,88 +249,109 @@ struct ArTaps {
   }
 };
 
-/* The auto-regressive filter of 7.18.3.3 (grain.rs:57-73 luma, 101-134
- * chroma) over one template, on one wave. Samples are visited along the
- * diagonals T = K*y + x (K = LAG + 1): every sample (y - d, x + dc) that
- * (y, x) reads lies on an earlier diagonal, T - (d*K - dc).
- *
- * Lane l owns the template rows y = (l + 3) mod 64 and y + 64 and handles,
- * on diagonal T, the sample of its row at x = T - K*y (the two rows of a
- * lane are never both inside the template on one diagonal). Each lane
- * emits one value per diagonal: the filtered sample inside the filtered
- * area, the unfiltered template value elsewhere. It keeps its last 16
- * emitted values as packed pairs of consecutive diagonals (PO[j & 15] =
- * (value of j, value of j + 1)) and the same pairs of the lanes 1, 2 and 3
- * below it (PN1, PN2, PN3, each filled by one lane rotation of the previous
- * array per diagonal), which are exactly the rows above its row; the sum
- * of a sample is then NP dot products of pairs with packed coefficients,
- * in registers indexed by T mod 16 (the rounds of 16 diagonals are
- * unrolled, so every index is a constant). The filtered value replaces the
- * sample only inside the filtered area (a select, no branch).
- *
- * CHROMA adds the tap on the co-located luma average (read from the finished
- * luma template in LDS). coef holds the 2 LAG K neighbour coefficients. */
-/* Diagonals T0 + S0 .. T0 + S1 - 1 of one template's AR filter (see
- * ar_wave). Every sample of these diagonals is read before any is written:
- * each template position is written only on its own diagonal. A lane
- * outside the template on a diagonal reads and writes its spare sample; am
- * is -1 where the sample is filtered, else 0. The luma tap of a chroma
- * sample averages the co-located luma samples (clamped into the template
- * for lanes whose sample is not filtered). */
-template <int LAG, bool CHROMA, int S0, int S1>
-__device__ __forceinline__ void ar_steps(int16_t *t, int rows, int cols, const uint32_t *cp, int luma_coef,
-                                         const int16_t *tl, int lsx, int lsy, bool luma_tap, int ars, int gmin,
-                                         int gmax, int T0, uint32_t (&PO)[AR_ROUND], uint32_t (&PN1)[AR_ROUND],
+/* Where a lane works in the chunk of diagonals T0 .. T0 + 15. Lane l owns
+ * the template rows ya = (l + 3) mod 64 and ya + 64; on diagonal T its
+ * sample is x = T - K y. It takes row ya while the chunk starts inside it
+ * (x0 < cols), then ya + 64 (the two rows are never both inside the
+ * template in one chunk); a row past the template sends the lane to a
+ * 16-sample spare run after it. The chunk's accesses are then base + s for
+ * step s (immediate offsets). A step whose x lies outside the row reads
+ * and writes a neighbouring position (the end of the row above, the start
+ * of the row below, or a padding column): it writes back the value it read
+ * at the chunk's start, and such a position is written by its own lane at
+ * least 82 - K diagonals away, in another chunk, so nothing changes. Bit s
+ * of mask is set where step s is a filtered sample (y >= 3, 3 <= x < cols -
+ * 3); every tap of a filtered sample is a template position, emitted by its
+ * own lane on its own diagonal. */
+struct ArPlace {
+  int base;      /* template index of the sample of step 0 */
+  uint32_t mask; /* filtered steps */
+  int y, x0;     /* row, column of step 0 */
+};
+
+template <int K>
+__device__ __forceinline__ ArPlace ar_place(int T0, int rows, int cols) {
+  const int ya = (threadIdx.x + 3) & 63;
+  int y = ya, x0 = T0 - K * ya;
+  if (x0 >= cols) {
+    y += 64;
+    x0 -= 64 * K;
+  }
+  const bool in = y < rows;
+  const int lo = min(max(3 - x0, 0), AR_ROUND), hi = min(max(cols - 3 - x0, 0), AR_ROUND);
+  ArPlace p;
+  p.mask = (in && y >= 3 && hi > lo) ? (1u << hi) - (1u << lo) : 0u;
+  p.base = in ? y * TW + x0 : TH * TW;
+  p.y = y;
+  p.x0 = x0;
+  return p;
+}
+
+/* Steps S0 .. S1 - 1 of one chunk of a template's AR filter (see ar_wave).
+ * Every sample of these steps is read before any is written. FILTER =
+ * false is the prologue: the history of the 16 diagonals before the first
+ * filtered one, which hold template values. CHROMA adds the tap on the
+ * co-located luma average of subsampling (SSX, SSY), read from the luma
+ * template's finished samples. */
+template <int LAG, bool CHROMA, int SSX, int SSY, bool FILTER, int S0, int S1>
+__device__ __forceinline__ void ar_steps(int16_t *t, const ArPlace &pl, const uint32_t *cp, int luma_coef,
+                                         const int16_t *tl, bool luma_tap, int ars, int gmin, int gmax,
+                                         uint32_t (&PO)[AR_ROUND], uint32_t (&PN1)[AR_ROUND],
                                          uint32_t (&PN2)[AR_ROUND], uint32_t (&PN3)[AR_ROUND], int &prev) {
-  constexpr int K = LAG + 1;
   constexpr ArTaps<LAG> TP{};
   constexpr int NP = ArTaps<LAG>::NP;
   constexpr int N = S1 - S0;
-  const int lane = threadIdx.x & 63;
-  const int ya = (lane + 3) & 63;
+  /* volatile: one 16-bit LDS access per sample; merged into wider
+   * (unaligned) accesses they would cost VALU packing and unpacking */
+  typedef __attribute__((address_space(3))) volatile int16_t lds_i16;
+  lds_i16 *ts = (lds_i16 *)(t + pl.base);
   const int rc = 1 << (ars - 1);
-  const int x0 = T0 - K * ya;
-  int tv[N], lv[N], addr[N], am[N];
+  int tv[N], lv[N];
 #pragma unroll
   for (int k = 0; k < N; k++) {
-    int x = x0 + S0 + k, y = ya;
-    if (x >= cols) {
-      x -= 64 * K;
-      y += 64;
-    }
-    const bool valid = y < rows && x >= 0 && x < cols;
-    am[k] = -(int)(valid && y >= 3 && x >= 3 && x < cols - 3);
-    addr[k] = valid ? y * TW + x : TH * TW + lane;
-    tv[k] = t[addr[k]];
+    tv[k] = ts[S0 + k];
     lv[k] = 0;
-    if (CHROMA && luma_tap) {
-      const int ly = min(max(((y - 3) << lsy) + 3, 0), TH - 1 - lsy), lx = min(max(((x - 3) << lsx) + 3, 0), TW - 1 - lsx);
-      const int16_t *q = tl + ly * TW + lx;
-      int l = q[0];
-      if (lsx) l += q[1];
-      if (lsy) l += q[TW];
-      if (lsx && lsy) l += q[TW + 1];
-      lv[k] = round2(l, lsx + lsy);
+  }
+  if (CHROMA && FILTER && luma_tap) {
+    /* only filtered steps use the tap; with none the reads go to index 0 */
+    const int lb = pl.mask ? (((pl.y - 3) << SSY) + 3) * TW + ((pl.x0 - 3) << SSX) + 3 : 0;
+    const int16_t *q = tl + lb;
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+      const int o = (S0 + k) << SSX;
+      int l = q[o];
+      if (SSX) l += q[o + 1];
+      if (SSY) l += q[o + TW];
+      if (SSX && SSY) l += q[o + TW + 1];
+      lv[k] = round2(l, SSX + SSY);
     }
   }
 #pragma unroll
   for (int k = 0; k < N; k++) {
     const int s = S0 + k;
-    /* three independent sums of the older pairs, then the two latest
-     * pairs (rows 0 and 1, diagonals T - 2 and T - 1) on two of them,
-     * so only one dot product and one add follow the latest values */
-    int acc[3] = {rc, 0, (CHROMA && luma_tap) ? __mul24(lv[k], luma_coef) : 0};
+    int out = tv[k];
+    if constexpr (FILTER) {
+      /* three independent sums of the older pairs, then the two latest
+       * pairs (rows 0 and 1, diagonals T - 2 and T - 1) on two of them,
+       * so only one dot product and one add follow the latest values */
+      /* tv + (sum >> ars) == (sum + (tv << ars)) >> ars: the sample
+       * starts the first sum with the rounding constant; the second sum
+       * (and the third, for luma) starts with a zero-addend dot product */
+      int acc[3] = {lshl_add(tv[k], ars, rc), 0, CHROMA ? __mul24(lv[k], luma_coef) : 0};
+      bool started[3] = {true, false, CHROMA};
 #pragma unroll
-    for (int i = 0; i < NP; i++) {
-      const int idx = (s - TP.m[i]) & (AR_ROUND - 1);
-      const uint32_t pv = TP.d[i] == 0 ? PO[idx] : TP.d[i] == 1 ? PN1[idx] : TP.d[i] == 2 ? PN2[idx] : PN3[idx];
-      const int a = i >= NP - 2 ? i - (NP - 2) : i % 3;
-      acc[a] = dot2(pv, cp[i], acc[a]);
+      for (int i = 0; i < NP; i++) {
+        const int idx = (s - TP.m[i]) & (AR_ROUND - 1);
+        const uint32_t pv = TP.d[i] == 0 ? PO[idx] : TP.d[i] == 1 ? PN1[idx] : TP.d[i] == 2 ? PN2[idx] : PN3[idx];
+        const int a = i >= NP - 2 ? i - (NP - 2) : i % 3;
+        acc[a] = started[a] ? dot2(pv, cp[i], acc[a]) : dot2_0(pv, cp[i]);
+        started[a] = true;
+      }
+      const int val = med3(gmin, gmax, (acc[0] + acc[1] + acc[2]) >> ars);
+      const int am = __builtin_amdgcn_sbfe((int)pl.mask, s, 1); /* -1 where filtered */
+      out = bfi(am, val, tv[k]);
+      ts[s] = (int16_t)out;
     }
-    const int sum = acc[0] + acc[1] + acc[2];
-    const int val = med3(gmin, gmax, tv[k] + (sum >> ars));
-    const int out = (val & am[k]) | (tv[k] & ~am[k]);
-    t[addr[k]] = (int16_t)out;
     const int j = (s - 1) & (AR_ROUND - 1);
     PO[j] = pack16(prev, out);
     prev = out;
@@ -316,63 +366,90 @@ __device__ __forceinline__ void ar_steps(int16_t *t, int rows, int cols, const u
  * diagonals T = K*y + x (K = LAG + 1): every sample (y - d, x + dc) that
  * (y, x) reads lies on an earlier diagonal, T - (d*K - dc).
  *
- * Lane l owns the template rows y = (l + 3) mod 64 and y + 64 and handles,
- * on diagonal T, the sample of its row at x = T - K*y (the two rows of a
- * lane are never both inside the template on one diagonal). Each lane
- * emits one value per diagonal: the filtered sample inside the filtered
- * area, the unfiltered template value elsewhere. It keeps its last 16
- * emitted values as packed pairs of consecutive diagonals (PO[j & 15] =
- * (value of j, value of j + 1)) and the same pairs of the lanes 1, 2 and 3
- * below it (PN1, PN2, PN3, each filled by one lane rotation of the previous
- * array per diagonal), which are exactly the rows above its row; the sum
- * of a sample is then NP dot products of pairs with packed coefficients,
- * in registers indexed by T mod 16 (every index is a constant of the
- * unrolled steps). The filtered value replaces the sample only inside the
- * filtered area (a select, no branch).
+ * Each lane emits one value per diagonal (ar_place): the filtered sample
+ * inside the filtered area, the unfiltered value elsewhere. It keeps its
+ * last 16 emitted values as packed pairs of consecutive diagonals (PO[j] =
+ * (value of j, value of j + 1), j mod 16) and the same pairs of the lanes
+ * 1, 2 and 3 below it (PN1, PN2, PN3, each filled by one lane rotation of
+ * the previous array per diagonal), which are exactly the rows above its
+ * row; the sum of a sample is then NP dot products of pairs with packed
+ * coefficients, in registers whose indices are constants of the unrolled
+ * steps. The filtered value replaces the sample only inside the filtered
+ * area (a bitfield select, no branch).
  *
- * The diagonals run in chunks of AR_ROUND, a chunk over plan.halves barrier
- * rounds (half a chunk per round for subsampled chroma, whose rounds then
- * cost about what a luma round costs). CHROMA adds the tap on the
- * co-located luma average (read from the finished luma template in LDS).
- * coef holds the 2 LAG K neighbour coefficients. */
-template <int LAG, bool CHROMA>
-__device__ void ar_wave(int16_t *t, int rows, int cols, const int *coef, int luma_coef, const int16_t *tl, int lsx,
-                        int lsy, bool luma_tap, int ars, int gmin, int gmax, ArPlan plan) {
+ * The diagonals run from the first filtered one in chunks of AR_ROUND,
+ * after a prologue that loads the 16 diagonals before it; unit u of the
+ * wave (a chunk, or half a chunk for subsampled chroma) runs in barrier
+ * round first + u. coef holds the 2 LAG K neighbour coefficients and, for
+ * chroma, the luma one. */
+template <int LAG, bool CHROMA, int SSX, int SSY>
+__device__ void ar_wave(int16_t *t, int rows, int cols, const int *coef, const int16_t *tl, bool luma_tap, int ars,
+                        int gmin, int gmax, int first, int halves, int rounds) {
   constexpr int K = LAG + 1;
   constexpr ArTaps<LAG> TP{};
   constexpr int NP = ArTaps<LAG>::NP;
+  constexpr int NL = 2 * LAG * K;
+  constexpr int TF = ar_first(K);
   uint32_t cp[NP];
 #pragma unroll
   for (int i = 0; i < NP; i++) cp[i] = pack16(TP.lo[i] >= 0 ? coef[TP.lo[i]] : 0, coef[TP.hi[i]]);
+  const int luma_coef = CHROMA ? coef[NL] : 0;
   uint32_t PO[AR_ROUND], PN1[AR_ROUND], PN2[AR_ROUND], PN3[AR_ROUND];
 #pragma unroll
   for (int i = 0; i < AR_ROUND; i++) PO[i] = PN1[i] = PN2[i] = PN3[i] = 0;
   int prev = 0; /* value this lane emitted on the previous diagonal */
-  const int nchunks = ar_rounds(K, rows, cols);
-  for (int r = 0; r < plan.rounds; r++) {
-    const int q = r - plan.first;
-    if (q >= 0 && q < nchunks * plan.halves) {
-      if constexpr (!CHROMA) {
-        ar_steps<LAG, false, 0, AR_ROUND>(t, rows, cols, cp, luma_coef, tl, lsx, lsy, luma_tap, ars, gmin, gmax,
-                                          q * AR_ROUND, PO, PN1, PN2, PN3, prev);
-      } else if (plan.halves == 1) {
-        ar_steps<LAG, true, 0, AR_ROUND>(t, rows, cols, cp, luma_coef, tl, lsx, lsy, luma_tap, ars, gmin, gmax,
-                                         q * AR_ROUND, PO, PN1, PN2, PN3, prev);
-      } else if ((q & 1) == 0) {
-        ar_steps<LAG, true, 0, AR_ROUND / 2>(t, rows, cols, cp, luma_coef, tl, lsx, lsy, luma_tap, ars, gmin, gmax,
-                                             (q >> 1) * AR_ROUND, PO, PN1, PN2, PN3, prev);
+  {
+    const ArPlace pl = ar_place<K>(TF - AR_ROUND, rows, cols);
+    ar_steps<LAG, CHROMA, SSX, SSY, false, 0, AR_ROUND>(t, pl, cp, luma_coef, tl, luma_tap, ars, gmin, gmax, PO,
+                                                        PN1, PN2, PN3, prev);
+  }
+  const int units = ar_chunks(K, rows, cols) * halves;
+  for (int r = 0; r < rounds; r++) {
+    const int u = r - first;
+    if (u >= 0 && u < units) {
+      if (halves == 1) {
+        const ArPlace pl = ar_place<K>(TF + AR_ROUND * u, rows, cols);
+        ar_steps<LAG, CHROMA, SSX, SSY, true, 0, AR_ROUND>(t, pl, cp, luma_coef, tl, luma_tap, ars, gmin, gmax, PO,
+                                                           PN1, PN2, PN3, prev);
       } else {
-        ar_steps<LAG, true, AR_ROUND / 2, AR_ROUND>(t, rows, cols, cp, luma_coef, tl, lsx, lsy, luma_tap, ars, gmin,
-                                                    gmax, (q >> 1) * AR_ROUND, PO, PN1, PN2, PN3, prev);
+        const ArPlace pl = ar_place<K>(TF + AR_ROUND * (u >> 1), rows, cols);
+        if ((u & 1) == 0)
+          ar_steps<LAG, CHROMA, SSX, SSY, true, 0, AR_ROUND / 2>(t, pl, cp, luma_coef, tl, luma_tap, ars, gmin, gmax,
+                                                                 PO, PN1, PN2, PN3, prev);
+        else
+          ar_steps<LAG, CHROMA, SSX, SSY, true, AR_ROUND / 2, AR_ROUND>(t, pl, cp, luma_coef, tl, luma_tap, ars, gmin,
+                                                                        gmax, PO, PN1, PN2, PN3, prev);
       }
     }
     __syncthreads();
   }
 }
 
+/* The round in which chroma unit 0 can run: unit u (diagonals a .. b of
+ * TF + L u .. TF + L u + L - 1, L = 16 / halves) reads the luma samples
+ * co-located with its filtered samples, the latest of which lies on luma
+ * diagonal 2 b - 2 K - 2 (4:2:0), b + cols - 6 (4:2:2) or b (4:4:4), or on
+ * the last filtered luma diagonal at most; luma chunk i, run in round i,
+ * finishes diagonals through TF + 16 i + 15, so the unit can run in any
+ * round after that chunk's. */
+__device__ __forceinline__ int chroma_first_round(int K, int ssx, int ssy, int cols, int units, int halves) {
+  const int TF = ar_first(K), L = AR_ROUND / halves;
+  const int last_luma = K * (TH - 1) + TW - 4;
+  int first = 0;
+  for (int u = 0; u < units; u++) {
+    const int b = TF + L * u + L - 1;
+    int d = ssy ? 2 * b - 2 * K - 2 : ssx ? b + cols - 6 : b;
+    d = min(d, last_luma);
+    const int dep = d >= TF ? (d - TF) / AR_ROUND : -1;
+    first = max(first, dep + 1 - u);
+  }
+  return first;
+}
+
 /* The AR stage of the three templates: wave 0 luma, waves 1 and 2 cb and
- * cr (on the schedule of ArPlan). Lag 0 has no recursion: luma samples are
- * only clipped and each chroma sample depends on the finished luma alone. */
+ * cr, each chroma unit in a round after the luma chunks it reads. Lag 0 has
+ * no recursion: luma samples are only clipped and each chroma sample
+ * depends on the finished luma alone. */
 template <int LAG>
 __device__ void ar_all(int16_t *tl, int16_t *tcb, int16_t *tcr, const GrainHdr &h, bool chroma, int cw, int ch,
                        int gmin, int gmax) {
@@ -390,58 +467,88 @@ __device__ void ar_all(int16_t *tl, int16_t *tcb, int16_t *tcr, const GrainHdr &
     }
     __syncthreads();
   } else {
-  constexpr int NL = 2 * LAG * K;
-  const int sub = h.ssx | h.ssy;
-  const int lr = ar_rounds(K, TH, TW);
-  const int cr = chroma ? ar_rounds(K, ch, cw) : 0;
-  /* A chroma chunk c reads luma samples finished by diagonal 2 (16 c + 15)
-   * (subsampled) or 16 c + 15 (4:4:4): subsampled, its first half runs in
-   * round 2 c + 1 (luma done through 32 c + 15) and its second half in
-   * round 2 c + 2; 4:4:4, the chunk runs in round c + 1. */
-  const ArPlan pl_luma = {0, 1, 0}, pl_chroma = {1, sub ? 2 : 1, 0};
-  const int rounds = max(lr, chroma ? pl_chroma.first + pl_chroma.halves * cr : 0);
-  if (wave == 0) {
-    ArPlan p = pl_luma;
-    p.rounds = rounds;
-    ar_wave<LAG, false>(tl, TH, TW, h.ar_y, 0, tl, 0, 0, false, h.ar_shift, gmin, gmax, p);
-  } else if (wave == 1 || wave == 2) {
-    ArPlan p = pl_chroma;
-    p.rounds = chroma ? rounds : 0;
-    const int *cf = wave == 1 ? h.ar_cb : h.ar_cr;
-    int16_t *t = wave == 1 ? tcb : tcr;
-    if (chroma)
-      ar_wave<LAG, true>(t, ch, cw, cf, cf[NL], tl, h.ssx, h.ssy, h.num_y > 0, h.ar_shift, gmin, gmax, p);
-    else
-      for (int r = 0; r < rounds; r++) __syncthreads();
-  }
+    /* vertically subsampled chroma reads two luma diagonals per diagonal:
+     * it runs half a chunk per round */
+    const int halves = h.ssy ? 2 : 1;
+    const bool luma_tap = h.num_y > 0;
+    const int lr = ar_chunks(K, TH, TW);
+    const int cu = chroma ? ar_chunks(K, ch, cw) * halves : 0;
+    const int first = chroma && luma_tap ? chroma_first_round(K, h.ssx, h.ssy, cw, cu, halves) : 0;
+    const int rounds = max(lr, first + cu);
+    if (wave == 0) {
+      ar_wave<LAG, false, 0, 0>(tl, TH, TW, h.ar_y, tl, false, h.ar_shift, gmin, gmax, 0, 1, rounds);
+    } else if (wave == 1 || wave == 2) {
+      const int *cf = wave == 1 ? h.ar_cb : h.ar_cr;
+      int16_t *t = wave == 1 ? tcb : tcr;
+      if (!chroma) {
+        for (int r = 0; r < rounds; r++) __syncthreads();
+      } else if (h.ssx && h.ssy) {
+        ar_wave<LAG, true, 1, 1>(t, ch, cw, cf, tl, luma_tap, h.ar_shift, gmin, gmax, first, halves, rounds);
+      } else if (h.ssx) {
+        ar_wave<LAG, true, 1, 0>(t, ch, cw, cf, tl, luma_tap, h.ar_shift, gmin, gmax, first, halves, rounds);
+      } else {
+        ar_wave<LAG, true, 0, 0>(t, ch, cw, cf, tl, luma_tap, h.ar_shift, gmin, gmax, first, halves, rounds);
+      }
+    }
   }
 }
 
-/* Each template is followed by 64 spare samples: the AR filter's lanes that
- * are outside their template on a diagonal store there. */
-constexpr int TPAD = TH * TW + 64;
+/* Each template in LDS has TFRONT samples before it (a lane of the AR
+ * prologue may start up to 3 K - 3 samples before row 0's) and 64 spare
+ * samples after it (the 16-sample run of lanes whose row is past the
+ * template, and the luma-tap reads past the last row of lanes with no
+ * filtered step). */
+constexpr int TFRONT = 16;
+constexpr int TPAD = TFRONT + TH * TW + 64;
+
+struct alignas(16) GaussTable {
+  int16_t v[2048];
+};
+
+constexpr GaussTable make_gauss() {
+  GaussTable g{};
+  for (int i = 0; i < 2048; i++) g.v[i] = AV1_GAUSSIAN_SEQUENCE[i];
+  return g;
+}
+
+constexpr GaussTable GAUSS = make_gauss();
 
 struct PrepShared {
   int16_t tl[TPAD], tcb[TPAD], tcr[TPAD];
-  int16_t gauss[2048];
+  alignas(16) int16_t gauss[2048];
 };
 
 /* Phase 1 of the prepare stage: the Gaussian templates into LDS, one
- * template row per thread. */
+ * template row per thread. The first row's LFSR state is loaded with the
+ * Gaussian table's copy into LDS in flight. */
 __device__ __forceinline__ void prep_fill(const GrainHdr &h, PrepShared &sm) {
-  for (int i = threadIdx.x; i < 2048; i += PREP_THREADS) sm.gauss[i] = AV1_GAUSSIAN_SEQUENCE[i];
-  __syncthreads();
   const int shift = 12 - h.bd + h.grain_scale_shift;
   const bool chroma = h.nplanes > 1;
   const int cw = h.ssx ? 44 : 82, ch = h.ssy ? 38 : 73;
   const int nrows = TH + (chroma ? 2 * ch : 0);
+  auto state = [&](int r) -> uint32_t {
+    if (r < TH) return row_state(h.seed, 0, r);
+    const int pl = (r - TH) / ch, y = (r - TH) - pl * ch;
+    return row_state(h.seed ^ (pl ? 0x49d8u : 0xb524u), cw == 82 ? 0 : 1, y);
+  };
+  const uint32_t s0 = threadIdx.x < nrows ? state(threadIdx.x) : 0u;
+  {
+    const uint4 *g = reinterpret_cast<const uint4 *>(GAUSS.v);
+    uint4 *d = reinterpret_cast<uint4 *>(sm.gauss);
+    const int i0 = threadIdx.x, i1 = threadIdx.x + PREP_THREADS;
+    const uint4 a = g[i0];
+    const uint4 b = i1 < 256 ? g[i1] : make_uint4(0, 0, 0, 0);
+    d[i0] = a;
+    if (i1 < 256) d[i1] = b;
+  }
+  __syncthreads();
   for (int r = threadIdx.x; r < nrows; r += PREP_THREADS) {
+    const uint32_t s = r == (int)threadIdx.x ? s0 : state(r);
     if (r < TH) {
-      fill_row(sm.tl, sm.gauss, h.seed, r, TW, 0, shift, h.num_y > 0);
+      fill_row(sm.tl + TFRONT, sm.gauss, s, r, TW, shift, h.num_y > 0);
     } else {
       const int pl = (r - TH) / ch, y = (r - TH) - pl * ch;
-      fill_row(pl ? sm.tcr : sm.tcb, sm.gauss, h.seed ^ (pl ? 0x49d8u : 0xb524u), y, cw, cw == 82 ? 0 : 1, shift,
-               (pl ? h.num_cr : h.num_cb) > 0 || h.cfl);
+      fill_row((pl ? sm.tcr : sm.tcb) + TFRONT, sm.gauss, s, y, cw, shift, (pl ? h.num_cr : h.num_cb) > 0 || h.cfl);
     }
   }
   __syncthreads();
@@ -453,11 +560,12 @@ __device__ __forceinline__ void prep_ar(const GrainHdr &h, PrepShared &sm) {
   const int gmin = -(128 << (bd - 8)), gmax = (256 << (bd - 8)) - 1 - (128 << (bd - 8));
   const bool chroma = h.nplanes > 1;
   const int cw = h.ssx ? 44 : 82, ch = h.ssy ? 38 : 73;
+  int16_t *tl = sm.tl + TFRONT, *tcb = sm.tcb + TFRONT, *tcr = sm.tcr + TFRONT;
   switch (h.ar_lag) {
-    case 0: ar_all<0>(sm.tl, sm.tcb, sm.tcr, h, chroma, cw, ch, gmin, gmax); break;
-    case 1: ar_all<1>(sm.tl, sm.tcb, sm.tcr, h, chroma, cw, ch, gmin, gmax); break;
-    case 2: ar_all<2>(sm.tl, sm.tcb, sm.tcr, h, chroma, cw, ch, gmin, gmax); break;
-    default: ar_all<3>(sm.tl, sm.tcb, sm.tcr, h, chroma, cw, ch, gmin, gmax); break;
+    case 0: ar_all<0>(tl, tcb, tcr, h, chroma, cw, ch, gmin, gmax); break;
+    case 1: ar_all<1>(tl, tcb, tcr, h, chroma, cw, ch, gmin, gmax); break;
+    case 2: ar_all<2>(tl, tcb, tcr, h, chroma, cw, ch, gmin, gmax); break;
+    default: ar_all<3>(tl, tcb, tcr, h, chroma, cw, ch, gmin, gmax); break;
   }
 }
 
@@ -469,7 +577,7 @@ __device__ __forceinline__ void prep_out(const GrainHdr &h, const GrainScratch &
   for (int i = threadIdx.x; i < 3 * TH * Q; i += PREP_THREADS) {
     const int q = i % Q, r = i / Q;
     const int y = r % TH, p = r / TH;
-    const int16_t *t = p == 0 ? sm.tl : (p == 1 ? sm.tcb : sm.tcr);
+    const int16_t *t = (p == 0 ? sm.tl : (p == 1 ? sm.tcb : sm.tcr)) + TFRONT;
     const bool on = p == 0 || chroma;
     uint32_t w[4];
 #pragma unroll
@@ -563,6 +671,9 @@ __device__ __forceinline__ void grain_tables_item(const GrainHdr &h, const Grain
 
 /* Templates, LUTs and block offsets of one frame (PREP_THREADS threads). */
 __device__ __forceinline__ void grain_prepare_body(const GrainHdr &h, const GrainScratch &sc, PrepShared &sm) {
+  /* the AR filter is a serial chain on three waves: they win issue
+   * arbitration against the filters' waves that share their SIMDs */
+  __builtin_amdgcn_s_setprio(3);
   prep_fill(h, sm);
   prep_ar(h, sm);
   prep_out(h, sc, sm);
