@@ -3,23 +3,25 @@
  *
  * Follows rivet-av1's grain::apply (src/decoder/grain.rs:8-354).
  *
- * grain_prepare_kernel: one workgroup of three waves per frame.
+ * grain_prepare_kernel: per frame, workgroup 0 (four waves) builds the
+ * templates and workgroups 1 .. TABLE_ITEMS the scaling functions
+ * (7.18.3.4, at every sample value) and the per-stripe random block
+ * offsets beside it.
  *   - Gaussian templates (7.18.3.3): luma 73x82, cb/cr 38x44 (4:2:0) or
  *     larger. The 16-bit LFSR is linear over GF(2), so the thread of a
- *     template row starts from A^(y W) seed (a precomputed matrix per row)
- *     and steps along its row.
+ *     template row (or of its second half) starts from A^n seed (a
+ *     precomputed matrix per row start) and steps along its row.
  *   - The auto-regressive filter is a recursion: sample (y, x) reads rows
  *     y-lag..y and columns x-lag..x+lag. Processing it along the diagonals
  *     T = (lag+1)*y + x visits every dependency at an earlier T, so all
  *     samples of one T run in parallel (one lane per template row). The
  *     rows above a lane's row belong to the lanes below it, so the taps
  *     are passed between lanes by DPP rotations and stay in registers.
- *     Luma runs on wave 0, cb and cr on waves 1 and 2, a fixed number of
- *     diagonals behind the luma samples they read.
- *   - The filtered templates into copy 0 of the scratch.
- * grain_tables_kernel: TABLE_ITEMS workgroups per frame: the 7 shifted
- *   copies of the templates, the scaling LUTs (7.18.3.4) and the per-stripe
- *   random block offsets.
+ *     Luma runs on wave 0, cb and cr on waves 1 and 2, in barrier rounds a
+ *     fixed number behind the luma samples they read.
+ *   - The filtered templates go to the scratch in 8 copies, each shifted
+ *     by one more sample (wave 3 copies finished rows during the filter
+ *     where it does not slow it, the rest follow it).
  * grain_apply_kernel: one workgroup per (noise stripe, 64 luma columns).
  *   The noise of a sample is rebuilt directly from the template: its 32x32
  *   block's offset, the horizontal blend with the block to the left, then
@@ -54,9 +56,13 @@ constexpr uint16_t apply_cols(const uint16_t *cols, uint16_t s) {
 
 /* LFSR states at the start of every template row: col[w][y][j] = A^(y W) e_j
  * for rows of W = 82 (w = 0) or 44 (w = 1) samples, so the state before
- * sample (y, 0) is one matrix-vector product with the seed. */
+ * sample (y, 0) is one matrix-vector product with the seed; col[2][y][j] =
+ * A^(82 y + FILL_SEG) e_j, the state before sample (y, FILL_SEG) of an
+ * 82-column template. */
+constexpr int FILL_SEG = 48; /* columns per fill item */
+
 struct alignas(16) RowJump {
-  uint16_t col[2][73][16];
+  uint16_t col[3][73][16];
 };
 
 constexpr RowJump make_row_jump() {
@@ -73,6 +79,14 @@ constexpr RowJump make_row_jump() {
     for (int y = 1; y < 73; y++)
       for (int j = 0; j < 16; j++) t.col[w][y][j] = apply_cols(m1, t.col[w][y - 1][j]);
   }
+  uint16_t ms[16] = {}; /* A^FILL_SEG */
+  for (int j = 0; j < 16; j++) {
+    uint16_t v = (uint16_t)(1u << j);
+    for (int k = 0; k < FILL_SEG; k++) v = lfsr_step(v);
+    ms[j] = v;
+  }
+  for (int y = 0; y < 73; y++)
+    for (int j = 0; j < 16; j++) t.col[2][y][j] = apply_cols(ms, t.col[0][y][j]);
   return t;
 }
 
@@ -87,7 +101,9 @@ __device__ __forceinline__ int round2(int x, int n) { return (x + ((1 << n) >> 1
 __device__ __forceinline__ int clip3(int lo, int hi, int x) { return min(max(x, lo), hi); }
 
 constexpr int TW = 82, TH = 73;
-constexpr int PREP_THREADS = 192; /* three waves: luma, cb, cr */
+/* four waves: the AR filter of luma, cb and cr on waves 0-2, wave 3
+ * writes finished template rows out meanwhile */
+constexpr int PREP_THREADS = 256;
 
 /* Row y of a Gaussian template (7.18.3.3, grain.rs:40-56): the LFSR
  * sequence from `seed` runs over the template in raster order, so the row
@@ -98,10 +114,10 @@ constexpr int PREP_THREADS = 192; /* three waves: luma, cb, cr */
  * b[m .. m + 15]; its table index (state >> 5) & 2047 is the 11-bit field at
  * m + 5. The 16 table reads are issued before the 16 template writes.
  * Template rows that are off hold 0. */
-__device__ void fill_row(int16_t *t, const int16_t *gauss, uint32_t s, int y, int cols, int shift, bool on) {
+__device__ void fill_row(int16_t *t, const int16_t *gauss, uint32_t s, int y, int xb, int cols, int shift, bool on) {
   const int keep = on ? -1 : 0;
   int16_t *row = t + y * TW;
-  for (int x0 = 0; x0 < cols; x0 += 16) {
+  for (int x0 = xb; x0 < cols; x0 += 16) {
     uint32_t w = s;
 #pragma unroll
     for (int g = 0; g < 4; g++) {
@@ -120,8 +136,9 @@ __device__ void fill_row(int16_t *t, const int16_t *gauss, uint32_t s, int y, in
 }
 
 /* The LFSR state before sample (y, 0) of a template of 82 (wsel 0) or 44
- * (wsel 1) columns: A^(y cols) seed, one GF(2) matrix-vector product with
- * the row's 16 columns (two 16-byte loads). */
+ * (wsel 1) columns, or before sample (y, FILL_SEG) of an 82-column one
+ * (wsel 2): one GF(2) matrix-vector product with the seed, from the row's
+ * 16 columns (two 16-byte loads). */
 __device__ __forceinline__ uint32_t row_state(uint32_t seed, int wsel, int y) {
   const uint4 *c = reinterpret_cast<const uint4 *>(ROW_JUMP.col[wsel][y]);
   const uint4 a = c[0], b = c[1];
@@ -287,12 +304,17 @@ __device__ __forceinline__ ArPlace ar_place(int T0, int rows, int cols) {
   return p;
 }
 
+/* volatile: one 16-bit LDS access per sample; merged into wider
+ * (unaligned) accesses they would cost VALU packing and unpacking */
+typedef __attribute__((address_space(3))) volatile int16_t lds_i16;
+
 /* Steps S0 .. S1 - 1 of one chunk of a template's AR filter (see ar_wave).
  * Every sample of these steps is read before any is written. FILTER =
  * false is the prologue: the history of the 16 diagonals before the first
  * filtered one, which hold template values. CHROMA adds the tap on the
  * co-located luma average of subsampling (SSX, SSY), read from the luma
  * template's finished samples. */
+
 template <int LAG, bool CHROMA, int SSX, int SSY, bool FILTER, int S0, int S1>
 __device__ __forceinline__ void ar_steps(int16_t *t, const ArPlace &pl, const uint32_t *cp, int luma_coef,
                                          const int16_t *tl, bool luma_tap, int ars, int gmin, int gmax,
@@ -301,9 +323,6 @@ __device__ __forceinline__ void ar_steps(int16_t *t, const ArPlace &pl, const ui
   constexpr ArTaps<LAG> TP{};
   constexpr int NP = ArTaps<LAG>::NP;
   constexpr int N = S1 - S0;
-  /* volatile: one 16-bit LDS access per sample; merged into wider
-   * (unaligned) accesses they would cost VALU packing and unpacking */
-  typedef __attribute__((address_space(3))) volatile int16_t lds_i16;
   lds_i16 *ts = (lds_i16 *)(t + pl.base);
   const int rc = 1 << (ars - 1);
   int tv[N], lv[N];
@@ -380,11 +399,11 @@ __device__ __forceinline__ void ar_steps(int16_t *t, const ArPlace &pl, const ui
  * The diagonals run from the first filtered one in chunks of AR_ROUND,
  * after a prologue that loads the 16 diagonals before it; unit u of the
  * wave (a chunk, or half a chunk for subsampled chroma) runs in barrier
- * round first + u. coef holds the 2 LAG K neighbour coefficients and, for
- * chroma, the luma one. */
-template <int LAG, bool CHROMA, int SSX, int SSY>
+ * round first + u, then writer(r) copies rows out. coef holds the 2 LAG K
+ * neighbour coefficients and, for chroma, the luma one. */
+template <int LAG, bool CHROMA, int SSX, int SSY, typename Writer>
 __device__ void ar_wave(int16_t *t, int rows, int cols, const int *coef, const int16_t *tl, bool luma_tap, int ars,
-                        int gmin, int gmax, int first, int halves, int rounds) {
+                        int gmin, int gmax, int first, int halves, int rounds, Writer &writer) {
   constexpr int K = LAG + 1;
   constexpr ArTaps<LAG> TP{};
   constexpr int NP = ArTaps<LAG>::NP;
@@ -421,6 +440,7 @@ __device__ void ar_wave(int16_t *t, int rows, int cols, const int *coef, const i
                                                                         gmax, PO, PN1, PN2, PN3, prev);
       }
     }
+    writer(r);
     __syncthreads();
   }
 }
@@ -446,15 +466,53 @@ __device__ __forceinline__ int chroma_first_round(int K, int ssx, int ssy, int c
   return first;
 }
 
+/* Rows y0 .. y1 - 1 of one template (in LDS, row pitch TW) into all 8
+ * shifted copies of its scratch plane: copy s at column x holds sample
+ * x + s, 0 past the row (GRAIN_TWS columns per row). Item (row, 8-column
+ * chunk) per lane, lanes lane, lane + nl, ... */
+__device__ __forceinline__ void write_rows(const int16_t *t, int16_t *dst, int y0, int y1, int lane, int nl) {
+  constexpr int Q = GRAIN_TWS / 8;
+  for (int i = lane; i < (y1 - y0) * Q; i += nl) {
+    const int r = i / Q, q = i - r * Q, y = y0 + r;
+    const uint32_t *row = reinterpret_cast<const uint32_t *>(t + y * TW) + q * 4; /* TW and TFRONT even */
+    uint32_t w[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) w[k] = q * 8 + 2 * k < TW ? row[k] : 0u;
+    int16_t *d = dst + y * GRAIN_TWS + q * 8;
+#pragma unroll
+    for (int sft = 0; sft < 8; sft++) {
+      uint32_t c[4];
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const int a2 = (sft >> 1) + k;
+        c[k] = (sft & 1) ? __builtin_amdgcn_alignbyte(w[a2 + 1], w[a2], 2) : w[a2];
+      }
+      *reinterpret_cast<uint4 *>(d + sft * GRAIN_TCOPY) = make_uint4(c[0], c[1], c[2], c[3]);
+    }
+  }
+}
+
+/* Rows of a template that are final once its diagonals through D are
+ * filtered: rows 0-2 are never filtered, row y >= 3 is final after its
+ * last filtered sample, on diagonal K y + cols - 4. */
+__device__ __forceinline__ int rows_final(int K, int D, int rows, int cols) {
+  const int n = D >= cols - 4 ? (D - (cols - 4)) / K + 1 : 0;
+  return min(max(n, 3), rows);
+}
+
 /* The AR stage of the three templates: wave 0 luma, waves 1 and 2 cb and
- * cr, each chroma unit in a round after the luma chunks it reads. Lag 0 has
- * no recursion: luma samples are only clipped and each chroma sample
- * depends on the finished luma alone. */
+ * cr, each chroma unit in a round after the luma chunks it reads; wave 3
+ * writes the template rows that became final by the previous round into
+ * the scratch (wl luma and wc chroma rows are written when it returns).
+ * Lag 0 has no recursion: luma samples are only clipped and each chroma
+ * sample depends on the finished luma alone; its rows are all written
+ * afterwards. */
 template <int LAG>
-__device__ void ar_all(int16_t *tl, int16_t *tcb, int16_t *tcr, const GrainHdr &h, bool chroma, int cw, int ch,
-                       int gmin, int gmax) {
+__device__ void ar_all(int16_t *tl, int16_t *tcb, int16_t *tcr, const GrainHdr &h, const GrainScratch &sc,
+                       bool chroma, int cw, int ch, int gmin, int gmax, int &wl, int &wc) {
   constexpr int K = LAG + 1;
   const int wave = threadIdx.x >> 6;
+  wl = wc = 0;
   if constexpr (LAG == 0) {
     for (int i = threadIdx.x; i < 70 * 76; i += blockDim.x) {
       const int y = 3 + i / 76, x = 3 + i % 76;
@@ -475,19 +533,49 @@ __device__ void ar_all(int16_t *tl, int16_t *tcb, int16_t *tcr, const GrainHdr &
     const int cu = chroma ? ar_chunks(K, ch, cw) * halves : 0;
     const int first = chroma && luma_tap ? chroma_first_round(K, h.ssx, h.ssy, cw, cu, halves) : 0;
     const int rounds = max(lr, first + cu);
+    /* rows final after the luma chunks and chroma units of rounds 0 .. r - 1 */
+    constexpr int TF = ar_first(K);
+    const int L = AR_ROUND / halves;
+    auto final_l = [&](int r) { return rows_final(K, TF + AR_ROUND * min(r, lr) - 1, TH, TW); };
+    auto final_c = [&](int r) {
+      return chroma ? rows_final(K, TF + L * min(max(r - first, 0), cu) - 1, ch, cw) : 0;
+    };
+    wl = final_l(rounds - 1);
+    wc = final_c(rounds - 1);
+    /* Wave 3 copies the rows final by the previous round out while the
+     * chroma waves set the pace (full-height chroma: 4:4:4, 4:2:2); where
+     * the luma wave does (4:2:0, monochrome) the copying slowed the rounds
+     * by more than it saves, and all rows are copied afterwards. */
+    const bool behind = chroma && !h.ssy;
+    if (!behind) wl = wc = 0;
+    const int lane = threadIdx.x & 63;
+    int dl = 0, dc = 0;
+    auto writer = [&](int r) {
+      if (behind && wave == 3) {
+        const int fl = final_l(r);
+        write_rows(tl, sc.tmpl, dl, fl, lane, 64);
+        dl = fl;
+        const int fc = final_c(r);
+        write_rows(tcb, sc.tmpl + GRAIN_TPLANE, dc, fc, lane, 64);
+        write_rows(tcr, sc.tmpl + 2 * GRAIN_TPLANE, dc, fc, lane, 64);
+        dc = fc;
+      }
+    };
     if (wave == 0) {
-      ar_wave<LAG, false, 0, 0>(tl, TH, TW, h.ar_y, tl, false, h.ar_shift, gmin, gmax, 0, 1, rounds);
-    } else if (wave == 1 || wave == 2) {
+      ar_wave<LAG, false, 0, 0>(tl, TH, TW, h.ar_y, tl, false, h.ar_shift, gmin, gmax, 0, 1, rounds, writer);
+    } else if ((wave == 1 || wave == 2) && chroma) {
       const int *cf = wave == 1 ? h.ar_cb : h.ar_cr;
       int16_t *t = wave == 1 ? tcb : tcr;
-      if (!chroma) {
-        for (int r = 0; r < rounds; r++) __syncthreads();
-      } else if (h.ssx && h.ssy) {
-        ar_wave<LAG, true, 1, 1>(t, ch, cw, cf, tl, luma_tap, h.ar_shift, gmin, gmax, first, halves, rounds);
-      } else if (h.ssx) {
-        ar_wave<LAG, true, 1, 0>(t, ch, cw, cf, tl, luma_tap, h.ar_shift, gmin, gmax, first, halves, rounds);
-      } else {
-        ar_wave<LAG, true, 0, 0>(t, ch, cw, cf, tl, luma_tap, h.ar_shift, gmin, gmax, first, halves, rounds);
+      if (h.ssx && h.ssy)
+        ar_wave<LAG, true, 1, 1>(t, ch, cw, cf, tl, luma_tap, h.ar_shift, gmin, gmax, first, halves, rounds, writer);
+      else if (h.ssx)
+        ar_wave<LAG, true, 1, 0>(t, ch, cw, cf, tl, luma_tap, h.ar_shift, gmin, gmax, first, halves, rounds, writer);
+      else
+        ar_wave<LAG, true, 0, 0>(t, ch, cw, cf, tl, luma_tap, h.ar_shift, gmin, gmax, first, halves, rounds, writer);
+    } else {
+      for (int r = 0; r < rounds; r++) {
+        writer(r);
+        __syncthreads();
       }
     }
   }
@@ -518,107 +606,80 @@ struct PrepShared {
   alignas(16) int16_t gauss[2048];
 };
 
-/* Phase 1 of the prepare stage: the Gaussian templates into LDS, one
- * template row per thread. The first row's LFSR state is loaded with the
- * Gaussian table's copy into LDS in flight. */
+/* Phase 1 of the prepare stage: the Gaussian templates into LDS. An item
+ * is FILL_SEG columns of a template row (two per 82-column row, one per
+ * 44-column row), one per thread; the first item's LFSR state is loaded
+ * with the Gaussian table's copy into LDS in flight. */
 __device__ __forceinline__ void prep_fill(const GrainHdr &h, PrepShared &sm) {
   const int shift = 12 - h.bd + h.grain_scale_shift;
   const bool chroma = h.nplanes > 1;
   const int cw = h.ssx ? 44 : 82, ch = h.ssy ? 38 : 73;
-  const int nrows = TH + (chroma ? 2 * ch : 0);
-  auto state = [&](int r) -> uint32_t {
-    if (r < TH) return row_state(h.seed, 0, r);
-    const int pl = (r - TH) / ch, y = (r - TH) - pl * ch;
-    return row_state(h.seed ^ (pl ? 0x49d8u : 0xb524u), cw == 82 ? 0 : 1, y);
+  const int csegs = cw == 82 ? 2 : 1;
+  const int litems = 2 * TH, nitems = litems + (chroma ? 2 * ch * csegs : 0);
+  struct Item {
+    int16_t *t;
+    uint32_t seed;
+    int y, g, cols;
+    bool on;
   };
-  const uint32_t s0 = threadIdx.x < nrows ? state(threadIdx.x) : 0u;
+  auto item = [&](int i) -> Item {
+    if (i < litems) return {sm.tl + TFRONT, h.seed, i >> 1, i & 1, TW, h.num_y > 0};
+    const int r = i - litems, per = ch * csegs, pl = r >= per, q = r - pl * per;
+    return {(pl ? sm.tcr : sm.tcb) + TFRONT, h.seed ^ (pl ? 0x49d8u : 0xb524u), q / csegs, q % csegs, cw,
+            (pl ? h.num_cr : h.num_cb) > 0 || h.cfl != 0};
+  };
+  auto state = [&](const Item &it) -> uint32_t {
+    return row_state(it.seed, it.g ? 2 : (it.cols == 82 ? 0 : 1), it.y);
+  };
+  const int i0 = threadIdx.x;
+  const uint32_t s0 = i0 < nitems ? state(item(i0)) : 0u;
   {
     const uint4 *g = reinterpret_cast<const uint4 *>(GAUSS.v);
-    uint4 *d = reinterpret_cast<uint4 *>(sm.gauss);
-    const int i0 = threadIdx.x, i1 = threadIdx.x + PREP_THREADS;
-    const uint4 a = g[i0];
-    const uint4 b = i1 < 256 ? g[i1] : make_uint4(0, 0, 0, 0);
-    d[i0] = a;
-    if (i1 < 256) d[i1] = b;
+    reinterpret_cast<uint4 *>(sm.gauss)[threadIdx.x] = g[threadIdx.x]; /* 256 x 16 bytes */
   }
   __syncthreads();
-  for (int r = threadIdx.x; r < nrows; r += PREP_THREADS) {
-    const uint32_t s = r == (int)threadIdx.x ? s0 : state(r);
-    if (r < TH) {
-      fill_row(sm.tl + TFRONT, sm.gauss, s, r, TW, shift, h.num_y > 0);
-    } else {
-      const int pl = (r - TH) / ch, y = (r - TH) - pl * ch;
-      fill_row((pl ? sm.tcr : sm.tcb) + TFRONT, sm.gauss, s, y, cw, shift, (pl ? h.num_cr : h.num_cb) > 0 || h.cfl);
-    }
+  for (int i = i0; i < nitems; i += PREP_THREADS) {
+    const Item it = item(i);
+    const uint32_t st = i == i0 ? s0 : state(it);
+    const int xb = it.g * FILL_SEG, xe = it.g ? it.cols : min(FILL_SEG, it.cols);
+    fill_row(it.t, sm.gauss, st, it.y, xb, xe, shift, it.on);
   }
   __syncthreads();
 }
 
 /* Phase 2: the auto-regressive filter of the three templates, in LDS. */
-__device__ __forceinline__ void prep_ar(const GrainHdr &h, PrepShared &sm) {
+__device__ __forceinline__ void prep_ar(const GrainHdr &h, const GrainScratch &sc, PrepShared &sm, int &wl, int &wc) {
   const int bd = h.bd;
   const int gmin = -(128 << (bd - 8)), gmax = (256 << (bd - 8)) - 1 - (128 << (bd - 8));
   const bool chroma = h.nplanes > 1;
   const int cw = h.ssx ? 44 : 82, ch = h.ssy ? 38 : 73;
   int16_t *tl = sm.tl + TFRONT, *tcb = sm.tcb + TFRONT, *tcr = sm.tcr + TFRONT;
   switch (h.ar_lag) {
-    case 0: ar_all<0>(tl, tcb, tcr, h, chroma, cw, ch, gmin, gmax); break;
-    case 1: ar_all<1>(tl, tcb, tcr, h, chroma, cw, ch, gmin, gmax); break;
-    case 2: ar_all<2>(tl, tcb, tcr, h, chroma, cw, ch, gmin, gmax); break;
-    default: ar_all<3>(tl, tcb, tcr, h, chroma, cw, ch, gmin, gmax); break;
+    case 0: ar_all<0>(tl, tcb, tcr, h, sc, chroma, cw, ch, gmin, gmax, wl, wc); break;
+    case 1: ar_all<1>(tl, tcb, tcr, h, sc, chroma, cw, ch, gmin, gmax, wl, wc); break;
+    case 2: ar_all<2>(tl, tcb, tcr, h, sc, chroma, cw, ch, gmin, gmax, wl, wc); break;
+    default: ar_all<3>(tl, tcb, tcr, h, sc, chroma, cw, ch, gmin, gmax, wl, wc); break;
   }
 }
 
-/* Phase 3: the filtered templates into copy 0 of the scratch (the
- * template itself, rows padded to GRAIN_TWS with zeros). */
-__device__ __forceinline__ void prep_out(const GrainHdr &h, const GrainScratch &sc, PrepShared &sm) {
-  const bool chroma = h.nplanes > 1;
-  constexpr int Q = GRAIN_TWS / 8;
-  for (int i = threadIdx.x; i < 3 * TH * Q; i += PREP_THREADS) {
-    const int q = i % Q, r = i / Q;
-    const int y = r % TH, p = r / TH;
-    const int16_t *t = (p == 0 ? sm.tl : (p == 1 ? sm.tcb : sm.tcr)) + TFRONT;
-    const bool on = p == 0 || chroma;
-    uint32_t w[4];
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-      const int x = q * 8 + 2 * k;
-      w[k] = (on && x < TW) ? *reinterpret_cast<const uint32_t *>(t + y * TW + x) : 0u;
-    }
-    *reinterpret_cast<uint4 *>(sc.tmpl + (size_t)p * GRAIN_TPLANE + y * GRAIN_TWS + q * 8) =
-        make_uint4(w[0], w[1], w[2], w[3]);
+/* Phase 3: the template rows the AR stage left (rows wl .. 72 of luma, wc
+ * .. ch - 1 of chroma) into the 8 shifted copies, on every thread. */
+__device__ __forceinline__ void prep_out(const GrainHdr &h, const GrainScratch &sc, PrepShared &sm, int wl, int wc) {
+  write_rows(sm.tl + TFRONT, sc.tmpl, wl, TH, threadIdx.x, PREP_THREADS);
+  if (h.nplanes > 1) {
+    const int ch = h.ssy ? 38 : 73;
+    write_rows(sm.tcb + TFRONT, sc.tmpl + GRAIN_TPLANE, wc, ch, threadIdx.x, PREP_THREADS);
+    write_rows(sm.tcr + TFRONT, sc.tmpl + 2 * GRAIN_TPLANE, wc, ch, threadIdx.x, PREP_THREADS);
   }
 }
 
 /* Workgroups per frame of the tables stage. */
 constexpr int TABLE_ITEMS = 8;
 
-/* Item j of the tables stage of one frame (256 threads): the shifted
- * copies 1..7 of the templates from copy 0 (copy s at x holds sample
- * x + s, 0 past the row; each thread takes one 8-sample chunk of a row),
- * its share of the scaling functions and the block offsets of the noise stripes
- * j, j + TABLE_ITEMS, ... */
+/* Item j of the tables stage of one frame (256 threads): its share of
+ * the scaling functions and the block offsets of the noise stripes j,
+ * j + TABLE_ITEMS, ... */
 __device__ __forceinline__ void grain_tables_item(const GrainHdr &h, const GrainScratch &sc, int j, int16_t *l256) {
-  constexpr int Q = GRAIN_TWS / 8;
-  for (int i = j * 256 + threadIdx.x; i < 3 * TH * Q; i += TABLE_ITEMS * 256) {
-    const int q = i % Q, r = i / Q;
-    const int y = r % TH, p = r / TH;
-    const int16_t *row = sc.tmpl + (size_t)p * GRAIN_TPLANE + y * GRAIN_TWS;
-    const uint4 a = *reinterpret_cast<const uint4 *>(row + q * 8);
-    const uint4 b = q + 1 < Q ? *reinterpret_cast<const uint4 *>(row + q * 8 + 8) : make_uint4(0, 0, 0, 0);
-    const uint32_t w[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-    int16_t *dst = sc.tmpl + (size_t)p * GRAIN_TPLANE + y * GRAIN_TWS + q * 8;
-#pragma unroll
-    for (int sft = 1; sft < 8; sft++) {
-      uint32_t c[4];
-#pragma unroll
-      for (int k = 0; k < 4; k++) {
-        const int a2 = (sft >> 1) + k;
-        c[k] = (sft & 1) ? __builtin_amdgcn_alignbyte(w[a2 + 1], w[a2], 2) : w[a2];
-      }
-      *reinterpret_cast<uint4 *>(dst + sft * GRAIN_TCOPY) = make_uint4(c[0], c[1], c[2], c[3]);
-    }
-  }
   /* scaling LUTs (grain.rs:137-165) at the 256 8-bit points, then the
    * scaling function of scale_lut (grain.rs:167-177) at every sample value
    * 0 .. 2^bd - 1, interpolated between them: item j writes values
@@ -675,8 +736,10 @@ __device__ __forceinline__ void grain_prepare_body(const GrainHdr &h, const Grai
    * arbitration against the filters' waves that share their SIMDs */
   __builtin_amdgcn_s_setprio(3);
   prep_fill(h, sm);
-  prep_ar(h, sm);
-  prep_out(h, sc, sm);
+  int wl, wc;
+  prep_ar(h, sc, sm, wl, wc);
+  __syncthreads();
+  prep_out(h, sc, sm, wl, wc);
 }
 
 
@@ -1008,14 +1071,15 @@ __device__ __forceinline__ void grain_apply_tile_unaligned(const GrainHdr &h, co
   }
 }
 
+/* Workgroup 0: the templates (grain_prepare_body); workgroups 1 ..
+ * TABLE_ITEMS: the scaling functions and block offsets beside it. */
 __global__ __launch_bounds__(PREP_THREADS) void grain_prepare_kernel(GrainHdr h, GrainScratch sc) {
   __shared__ PrepShared sm;
-  grain_prepare_body(h, sc, sm);
-}
-
-__global__ __launch_bounds__(256) void grain_tables_kernel(GrainHdr h, GrainScratch sc) {
-  __shared__ int16_t l256[3 * 256];
-  grain_tables_item(h, sc, blockIdx.x, l256);
+  static_assert(PREP_THREADS == 256, "the tables items and the Gaussian copy take 256 threads");
+  if (blockIdx.x == 0)
+    grain_prepare_body(h, sc, sm);
+  else
+    grain_tables_item(h, sc, blockIdx.x - 1, sm.gauss);
 }
 
 /* KIND (apply_kind): 1 (segments, 64 luma columns per tile: 4:2:2, mono),
@@ -1110,8 +1174,7 @@ int apply_kind(const GrainHdr &h, const PlanePtr in[3], const PlanePtr out[3], i
 }  // namespace
 
 int launch_grain_prepare(const GrainHdr &h, const GrainScratch &s, hipStream_t st) {
-  grain_prepare_kernel<<<1, PREP_THREADS, 0, st>>>(h, s);
-  grain_tables_kernel<<<TABLE_ITEMS, 256, 0, st>>>(h, s);
+  grain_prepare_kernel<<<1 + TABLE_ITEMS, PREP_THREADS, 0, st>>>(h, s);
   return (int)hipGetLastError();
 }
 

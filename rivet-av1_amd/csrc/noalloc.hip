@@ -5,7 +5,8 @@
  * hipMalloc).
  *
  * Its kernels are those of rvpf_grain (grain.hip): the template build
- * (grain_prepare_kernel + grain_tables_kernel) and the noise blend. Their
+ * (grain_prepare_kernel, with the scaling functions and block offsets) and
+ * the noise blend. Their
  * scratch — 3 planes x 8 shifted copies of the 73 x 88 noise templates,
  * the 3 scaling functions (an entry per sample value) and one byte per
  * 32 x 32 noise block — lives in

@@ -16,6 +16,8 @@ padding and spare samples filled with random values), and checks:
   - every LDS index stays inside the template's allocation;
   - a chroma sample reads co-located luma samples only after the round in
     which the luma wave filtered them;
+  - the rows the fourth wave copies out in a round (rows_final) already hold
+    their final values, whatever the other waves do in that round;
   - the resulting templates equal the oracle's (orc_grain_templates) at
     every lag, chroma layout and bit depth, with and without luma points
     (the chroma luma tap) and with chroma scaling from luma.
@@ -129,6 +131,11 @@ def ar_place(K, T0, rows, cols):
     return base, mask, y, x0
 
 
+def rows_final(K, D, rows, cols):
+    n = (D - (cols - 4)) // K + 1 if D >= cols - 4 else 0
+    return min(max(n, 3), rows)
+
+
 def chroma_first_round(K, ssx, ssy, cols, units, halves):
     TF, L = ar_first(K), AR_ROUND // halves
     last_luma = K * (TH - 1) + TW - 4
@@ -159,9 +166,14 @@ class Wave:
         self.hi = [z() for _ in range(4)]
         self.prev = np.zeros(64, dtype=np.int64)
 
-    def steps(self, T0, s0, s1, filt, rnd, luma_round=None):
+    def load(self, T0):
+        """The chunk's 16 samples (load_chunk), read when the previous chunk starts."""
+        base = ar_place(self.K, T0, self.rows, self.cols)[0]
+        return [self.lds.read(base + s) for s in range(AR_ROUND)]
+
+    def steps(self, T0, s0, s1, filt, rnd, tvc, luma_round=None):
         base, mask, y, x0 = ar_place(self.K, T0, self.rows, self.cols)
-        tv = [self.lds.read(base + s) for s in range(s0, s1)]
+        tv = tvc[s0:s1]
         lv = None
         if self.chroma is not None and filt and self.chroma[3]:
             luma, ssx, ssy, _, done = self.chroma
@@ -233,21 +245,47 @@ def replay(P, seed=0):
     wc = [Wave(lag, cl[i], ch, cw, (ccb, ccr)[i], ars, gmin, gmax, chroma=(luma, P.ssx, P.ssy, luma_tap, done))
           for i in range(2)] if chroma else []
     TF = ar_first(K)
+    chunks = {id(wl): nl}
+    for w in wc:
+        chunks[id(w)] = cu // halves
     for w in [wl] + wc:
-        w.steps(TF - AR_ROUND, 0, AR_ROUND, False, -1)
+        w.steps(TF - AR_ROUND, 0, AR_ROUND, False, -1, w.load(TF - AR_ROUND))
+        w.tvn = w.load(TF)
+
+    def run_unit(w, u, hv, r, luma_round=None):
+        c = u if hv == 1 else u >> 1
+        if hv == 1 or u % 2 == 0:
+            w.tvc = w.tvn
+            if c + 1 < chunks[id(w)]:
+                w.tvn = w.load(TF + AR_ROUND * (c + 1))
+        s0 = 0 if hv == 1 or u % 2 == 0 else AR_ROUND // 2
+        w.steps(TF + AR_ROUND * c, s0, s0 + AR_ROUND // hv, True, r, w.tvc, luma_round)
+    L = AR_ROUND // halves
+    # with full-height chroma (4:4:4, 4:2:2) the fourth wave copies rows
+    # out during the rounds; otherwise every row is copied afterwards
+    behind = chroma and not P.ssy
+    copied = []  # (Lds, rows, cols, y, row as copied)
+    dl = dc = 0
     for r in range(rounds):
+        # the fourth wave copies rows final by the previous round
+        fl = rows_final(K, TF + AR_ROUND * min(r, nl) - 1, TH, TW)
+        fc = rows_final(K, TF + L * min(max(r - first, 0), cu) - 1, ch, cw) if chroma else 0
+        if behind:
+            copied += [(luma, TW, y, luma.a[TFRONT + y * TW:TFRONT + y * TW + TW].copy()) for y in range(dl, fl)]
+            copied += [(c, cw, y, c.a[TFRONT + y * TW:TFRONT + y * TW + cw].copy()) for c in cl for y in range(dc, fc)]
+            dl, dc = fl, fc
         # the chroma waves run beside the luma wave: they may only rely on
         # luma chunks of earlier rounds (checked through `done`)
         for w in wc:
             u = r - first
             if 0 <= u < cu:
-                if halves == 1:
-                    w.steps(TF + AR_ROUND * u, 0, AR_ROUND, True, r)
-                else:
-                    s0 = 0 if u % 2 == 0 else AR_ROUND // 2
-                    w.steps(TF + AR_ROUND * (u >> 1), s0, s0 + AR_ROUND // 2, True, r)
+                run_unit(w, u, halves, r)
         if r < nl:
-            wl.steps(TF + AR_ROUND * r, 0, AR_ROUND, True, r, luma_round=done)
+            run_unit(wl, r, 1, r, luma_round=done)
+    for lds, cols, y, row in copied:
+        assert np.array_equal(row, lds.a[TFRONT + y * TW:TFRONT + y * TW + cols]), "row copied before it was final"
+    if behind:
+        assert dl == rows_final(K, TF + AR_ROUND * min(rounds - 1, nl) - 1, TH, TW)
     out = [luma.template(TH, TW)]
     out += [c.template(ch, cw) for c in cl] if chroma else []
     return out, (cw, ch), rounds
