@@ -514,6 +514,244 @@ __global__ __launch_bounds__(256) void lf_pass_kernel(LfLaunch a) {
   lf_block<T, PASS, EX>(a, gp, lb, &tsm);
 }
 
+/* ------------------------------------------------------------------ */
+/* Both passes in one launch, out of place (the whole chain's deblock).
+ *
+ * A workgroup produces one 64 x 64 block of one plane's deblocked frame at
+ * (X0, Y0). Its samples after both passes depend on the vertical edges at
+ * X0, X0 + 4, .., X0 + 64 of rows Y0 - 8 .. Y0 + 71 (the horizontal edges
+ * read 7 rows either side of theirs) and on the horizontal edges at Y0,
+ * Y0 + 4, .., Y0 + 64 of its columns: the edges at X0 + 64 and Y0 + 64 are
+ * the next blocks' first ones, whose p side reaches into this block, and no
+ * edge left of X0 or above Y0 reaches into it (transform blocks do not
+ * cross a 64 boundary, so their q-side footprint ends before it). Edges on
+ * the block boundaries are filtered by both blocks alike.
+ *
+ * The block stages its 80 x 80 input samples (rows and columns 8 either
+ * side) in LDS, decides every edge (both passes at once: the decision
+ * reads side info only), filters the vertical class lists in LDS in place,
+ * then the horizontal ones, and writes its 64 x 64 samples, clipped to the
+ * mode-info area, to the output planes. Reading the input and writing
+ * another buffer keeps blocks from seeing each other's results; in LDS a
+ * line never reads another line's footprint (legal side info, as above). */
+constexpr int FT = 80;                 /* staged samples per row and rows */
+constexpr int FV_COLS = 17, FV_ROWS = 20; /* vertical edges: columns x rows of 4 lines */
+constexpr int FH_ROWS = 17, FH_COLS = 16; /* horizontal edges */
+constexpr int FV_N = FV_COLS * FV_ROWS, FH_N = FH_ROWS * FH_COLS;
+constexpr int F_ITEMS = (FV_N + FH_N + 255) / 256; /* decisions per thread */
+
+struct LfFusedShared {
+  __align__(16) uint16_t tile[FT * FT];
+  LfTables t;
+  int count[8];             /* [pass * 4 + class] */
+  uint32_t lv[4][FV_N];     /* vertical edges by class: index | limit << 10 | blimit << 16 | thresh << 24 */
+  uint32_t lh[4][FH_N];     /* horizontal edges */
+};
+
+__device__ __forceinline__ void lds_st4(uint16_t *p, const int *o) {
+  *reinterpret_cast<uint2 *>(p) = make_uint2((uint32_t)o[0] | ((uint32_t)o[1] << 16), (uint32_t)o[2] | ((uint32_t)o[3] << 16));
+}
+__device__ __forceinline__ void lds_st2(uint16_t *p, int a, int b) {
+  *reinterpret_cast<uint32_t *>(p) = (uint32_t)a | ((uint32_t)b << 16);
+}
+
+/* One vertical-edge class: a row of an edge per item, in the staged tile. */
+template <int C>
+__device__ __forceinline__ void fused_v(LfFusedShared &sm, int bd) {
+  constexpr int N = LfClass<C>::N, H = N / 2;
+  const int sh = bd - 8, n = sm.count[C];
+  for (int it = threadIdx.x; it < 4 * n; it += 256) {
+    const uint32_t e = sm.lv[C][it >> 2];
+    const int li = e & 1023;
+    const int col = 8 + 4 * (li % FV_COLS), row = 4 * (li / FV_COLS) + (it & 3);
+    uint16_t *trow = sm.tile + row * FT + col;
+    int s[16];
+    if (C == 3) {
+      lds4(trow - 8, s);
+      lds4(trow + 4, s + 12);
+    }
+    lds4(trow - 4, s + 4);
+    lds4(trow, s + 8);
+    int v[N];
+#pragma unroll
+    for (int k = 0; k < N; k++) v[k] = s[8 - H + k];
+    if (!filter_class<C>(v, ((e >> 10) & 63) << sh, ((e >> 16) & 255) << sh, (e >> 24) << sh, bd)) continue;
+#pragma unroll
+    for (int k = 0; k < N; k++) s[8 - H + k] = v[k];
+    if (C <= 1) {
+      lds_st2(trow - 2, s[6], s[7]);
+      lds_st2(trow, s[8], s[9]);
+    } else {
+      if (C == 3) {
+        lds_st4(trow - 8, s);
+        lds_st4(trow + 4, s + 12);
+      }
+      lds_st4(trow - 4, s + 4);
+      lds_st4(trow, s + 8);
+    }
+  }
+}
+
+/* One horizontal-edge class: a column of an edge per item. */
+template <int C>
+__device__ __forceinline__ void fused_h(LfFusedShared &sm, int bd) {
+  constexpr int N = LfClass<C>::N, H = N / 2;
+  constexpr int R = C <= 1 ? 2 : (C == 2 ? 3 : 6); /* rows the filter modifies either side */
+  const int sh = bd - 8, n = sm.count[4 + C];
+  for (int it = threadIdx.x; it < 4 * n; it += 256) {
+    const uint32_t e = sm.lh[C][it >> 2];
+    const int li = e & 1023;
+    const int row = 8 + 4 * (li / FH_COLS), col = 8 + 4 * (li % FH_COLS) + (it & 3);
+    uint16_t *tcol = sm.tile + row * FT + col;
+    int v[N];
+#pragma unroll
+    for (int k = 0; k < N; k++) v[k] = tcol[(k - H) * FT];
+    if (!filter_class<C>(v, ((e >> 10) & 63) << sh, ((e >> 16) & 255) << sh, (e >> 24) << sh, bd)) continue;
+#pragma unroll
+    for (int k = H - R; k < H + R; k++) tcol[(k - H) * FT] = (uint16_t)v[k];
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void lf_fused_block(const LfLaunch &a, int gp, int lb, const PlanePtr &o0,
+                                               const PlanePtr &o1, const PlanePtr &o2, LfFusedShared *sm) {
+  const int plane = a.plane_id[gp];
+  const int bx = lb % a.blk_x[gp], by = lb / a.blk_x[gp];
+  const int X0 = 64 * bx, Y0 = 64 * by;
+  const Geom &g = a.g;
+  const int ssx = plane ? g.ssx : 0, ssy = plane ? g.ssy : 0;
+  const uint32_t tword = threadIdx.x < LF_TWORDS ? reinterpret_cast<const uint32_t *>(&a.t)[threadIdx.x] : 0u;
+  /* the side info of this thread's decisions: item k < FV_N is vertical
+   * edge (column i, row group j), the rest horizontal */
+  EdgeSide side[F_ITEMS];
+  bool valid[F_ITEMS];
+  int pass_of[F_ITEMS], ex_of[F_ITEMS], ey_of[F_ITEMS];
+#pragma unroll
+  for (int k = 0; k < F_ITEMS; k++) {
+    const int item = threadIdx.x + 256 * k;
+    int pass = 0, ex = 0, ey = 0;
+    if (item < FV_N) {
+      ex = X0 / 4 + item % FV_COLS;
+      ey = Y0 / 4 - 2 + item / FV_COLS;
+    } else {
+      const int h = item - FV_N;
+      pass = 1;
+      ex = X0 / 4 + h % FH_COLS;
+      ey = Y0 / 4 + h / FH_COLS;
+    }
+    pass_of[k] = pass;
+    ex_of[k] = ex;
+    ey_of[k] = ey;
+    valid[k] = item < FV_N + FH_N && ey >= 0 && (ey << ssy) < g.mi_rows && (ex << ssx) < g.mi_cols &&
+               (pass == 0 ? ex != 0 : ey != 0);
+    if (valid[k]) side[k] = load_side(a, plane, pass, ex, ey);
+  }
+  const T *in = gptr(reinterpret_cast<const T *>(a.planes[plane].data));
+  const int64_t stride = a.planes[plane].stride;
+  const int area_w = (g.mi_cols * 4) >> ssx, area_h = (g.mi_rows * 4) >> ssy;
+  /* staged rows Y0 - 8 .. Y0 + 71, columns X0 - 8 .. X0 + 71; a filtered
+   * edge reads rows below area_h + 8 and columns below area_w + 8 (the
+   * margin every deblocked input plane has); chunks past them are loaded
+   * from a clamped position and never read */
+  constexpr int CW = FT / 4, CHUNKS = FT * CW, PER = (CHUNKS + 255) / 256;
+  typename Io<T>::W v[PER];
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    const int ch = min((int)threadIdx.x + 256 * k, CHUNKS - 1);
+    const int r = ch / CW, c = (ch - r * CW) * 4;
+    const int y = min(max(Y0 - 8 + r, 0), area_h + 7), x = min(max(X0 - 8 + c, 0), area_w + 4);
+    v[k] = Io<T>::ldw(in + (int64_t)y * stride + x);
+  }
+  if (threadIdx.x < LF_TWORDS) reinterpret_cast<uint32_t *>(&sm->t)[threadIdx.x] = tword;
+  if (threadIdx.x < 8) sm->count[threadIdx.x] = 0;
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    const int ch = threadIdx.x + 256 * k;
+    if (ch < CHUNKS) {
+      int o[4];
+      Io<T>::unpack(v[k], o);
+      lds_st4(&sm->tile[ch * 4], o);
+    }
+  }
+  __syncthreads();
+  /* decisions, appended wave by wave to the 8 lists (pass, class) */
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int k = 0; k < F_ITEMS; k++) {
+    int cls = -1;
+    uint32_t word = 0;
+    if (valid[k]) {
+      int limit, blimit, thresh;
+      const int fs = edge_decision(a, sm->t, side[k], plane, pass_of[k], ex_of[k], ey_of[k], &limit, &blimit, &thresh);
+      if (fs) {
+        cls = 4 * pass_of[k] + (fs == 4 ? 0 : (fs == 16 ? 3 : (plane ? 1 : 2)));
+        const int item = threadIdx.x + 256 * k - (pass_of[k] ? FV_N : 0);
+        word = item | (limit << 10) | (blimit << 16) | (thresh << 24);
+      }
+    }
+    uint64_t m[8];
+#pragma unroll
+    for (int c = 0; c < 8; c++) m[c] = __ballot(cls == c);
+    uint64_t mk = 0;
+#pragma unroll
+    for (int c = 0; c < 8; c++) mk = lane == c ? m[c] : mk;
+    int base = 0;
+    if (lane < 8 && mk != 0) base = atomicAdd(&sm->count[lane], __popcll(mk));
+    uint64_t mc = 0;
+#pragma unroll
+    for (int c = 0; c < 8; c++) mc = cls == c ? m[c] : mc;
+    const int below = __builtin_amdgcn_mbcnt_hi((uint32_t)(mc >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mc, 0u));
+    base = __shfl(base, cls < 0 ? 0 : cls, 64);
+    if (cls >= 0) {
+      if (cls < 4)
+        sm->lv[cls][base + below] = word;
+      else
+        sm->lh[cls - 4][base + below] = word;
+    }
+  }
+  __syncthreads();
+  fused_v<0>(*sm, g.bd);
+  if (plane) {
+    fused_v<1>(*sm, g.bd);
+  } else {
+    fused_v<2>(*sm, g.bd);
+    fused_v<3>(*sm, g.bd);
+  }
+  __syncthreads();
+  fused_h<0>(*sm, g.bd);
+  if (plane) {
+    fused_h<1>(*sm, g.bd);
+  } else {
+    fused_h<2>(*sm, g.bd);
+    fused_h<3>(*sm, g.bd);
+  }
+  __syncthreads();
+  /* the block's 64 x 64 samples, clipped to the mode-info area (a chunk
+   * that starts inside it is written whole: output rows are padded) */
+  const PlanePtr &op = plane == 0 ? o0 : (plane == 1 ? o1 : o2);
+  T *o = gptr(reinterpret_cast<T *>(op.data));
+  const int64_t ostride = op.stride;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int ch = threadIdx.x + 256 * k;
+    const int r = ch >> 4, c = (ch & 15) * 4;
+    if (Y0 + r < area_h && X0 + c < area_w) {
+      int s4[4];
+      lds4(&sm->tile[(8 + r) * FT + 8 + c], s4);
+      Io<T>::st4(o + (int64_t)(Y0 + r) * ostride + X0 + c, s4);
+    }
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void lf_fused_kernel(LfLaunch a, PlanePtr o0, PlanePtr o1, PlanePtr o2) {
+  __shared__ LfFusedShared sm;
+  int lb;
+  const int gp = xcd_segment(blockIdx.x, a.blk_start, a.nseg, &lb);
+  if (gp >= a.nseg || lb >= a.blk_n[gp]) return; /* padding */
+  lf_fused_block<T>(a, gp, lb, o0, o1, o2, &sm);
+}
+
 /* Batched: item j of a frame is block j of its (unpadded) grid. */
 template <typename T, int PASS, int EX, bool DIRECT>
 __global__ __launch_bounds__(256) void lf_pass_batch(BatchWork w) {
@@ -635,6 +873,18 @@ int launch_deblock_batch(const BatchWork &wv, const BatchWork &wh, const StagePl
     launch_batch_shape<uint8_t>(wv, wh, sp, st);
   else
     launch_batch_shape<uint16_t>(wv, wh, sp, st);
+  return (int)hipGetLastError();
+}
+
+int launch_deblock_fused(const Geom &g, const LfHdr &h, const LfSide &s, const PlanePtr in[3], const PlanePtr out[3],
+                         int bps, hipStream_t st) {
+  LfLaunch a;
+  const int total = plan(g, h, s, in, LF_EX, true, a);
+  if (total == 0) return 0;
+  if (bps == 1)
+    lf_fused_kernel<uint8_t><<<total, 256, 0, st>>>(a, out[0], out[1], out[2]);
+  else
+    lf_fused_kernel<uint16_t><<<total, 256, 0, st>>>(a, out[0], out[1], out[2]);
   return (int)hipGetLastError();
 }
 

@@ -43,11 +43,13 @@ struct rvpf_ctx {
   hipStream_t side;
   hipEvent_t ev_ready[2], ev_done[2];
   int grain_slot;
+  void *deb_buf; /* the chain's deblocked frame */
+  size_t deb_bytes;
   void *cdef_buf;
   size_t cdef_bytes;
   void *up_buf; /* upscaled CDEF output and deblocked frame (super-resolution) */
   size_t up_bytes;
-  ScratchGate chain_gate; /* cdef_buf and up_buf */
+  ScratchGate chain_gate; /* deb_buf, cdef_buf and up_buf */
   void *search_buf; /* encoder searches: work planes, error tables */
   size_t search_bytes;
   ScratchGate search_gate;

@@ -622,8 +622,6 @@ int superres_tail(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_side_inf
   const Geom g = make_geom(p);
   const int np = g.nplanes;
   int rc;
-  /* cdef_buf and up_buf are written below: wait for the call that read them last */
-  if ((rc = scratch_acquire(ctx, &ctx->chain_gate, st))) return rc;
   PlanePtr cdef[3];
   for (int i = 0; i < 3; i++) cdef[i] = rec[i];
   if (cdef_on(p)) {
@@ -685,7 +683,7 @@ int superres_tail(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_side_inf
     if ((rc = check_launch(ctx, launch_lr(make_lr_geom(p), make_lr(p, s), up_deb, up_cdef, ref, bps, st), "lr")))
       return rc;
   }
-  return scratch_release(ctx, &ctx->chain_gate, st);
+  return RVPF_OK;
 }
 
 }  // namespace host
@@ -725,6 +723,7 @@ int rvpf_ctx_create(rvpf_ctx **out) {
 void rvpf_ctx_destroy(rvpf_ctx *ctx) {
   if (!ctx) return;
   if (ctx->side) (void)hipStreamSynchronize(ctx->side);
+  if (ctx->deb_buf) (void)hipFree(ctx->deb_buf);
   if (ctx->cdef_buf) (void)hipFree(ctx->cdef_buf);
   if (ctx->up_buf) (void)hipFree(ctx->up_buf);
   if (ctx->search_buf) (void)hipFree(ctx->search_buf);
@@ -960,27 +959,42 @@ int rvpf_postfilter_frame(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_
   /* film grain templates on the side stream, overlapping the filters */
   int slot = 0;
   if (show_grain && (rc = grain_prepare(ctx, p, ctx->side, &slot))) return rc;
-  PlanePtr rec[3], ref[3];
+  PlanePtr rec[3], ref[3], deb[3];
   to_ptrs(recon, rec);
   to_ptrs(ref_out, ref);
-  /* 1. deblock in place */
+  const bool any_lr = lr_on(p);
+  /* context scratch this call writes (the deblocked frame, the CDEF output
+   * of the restored planes, the upscaled frames) waits for the call that
+   * read it last */
+  const bool gate = lf_on(p) || p->use_superres || (cdef_on(p) && any_lr);
+  if (gate && (rc = scratch_acquire(ctx, &ctx->chain_gate, st))) return rc;
+  /* 1. deblock, both passes in one launch, from recon (not modified) into
+   * scratch planes */
+  for (int i = 0; i < 3; i++) deb[i] = rec[i];
   if (lf_on(p)) {
     LfSide side;
     side.mi = s->mi;
     side.mi_stride = s->mi_stride;
     for (int i = 0; i < 3; i++) side.tx[i] = s->tx_size[i];
+    const LfHdr lh = make_lf(p);
+    int w[3] = {0, 0, 0}, h[3] = {0, 0, 0};
+    for (int i = 0; i < g.nplanes; i++)
+      if (i == 0 || lh.level[1 + i] != 0) area(p, i, &w[i], &h[i]);
+    if ((rc = ensure_buf(ctx, &ctx->deb_buf, &ctx->deb_bytes, layout_planes(w, h, bps, nullptr, nullptr),
+                         "deblocked scratch planes")))
+      return rc;
+    PlanePtr tmp[3];
+    layout_planes(w, h, bps, ctx->deb_buf, tmp);
+    for (int i = 0; i < g.nplanes; i++)
+      if (w[i]) deb[i] = tmp[i];
     StageMark m(ctx, RVPF_STAGE_DEBLOCK, st);
-    if ((rc = check_launch(ctx, launch_deblock(g, make_lf(p), side, rec, bps, st), "deblock"))) return rc;
+    if ((rc = check_launch(ctx, launch_deblock_fused(g, lh, side, rec, deb, bps, st), "deblock"))) return rc;
   }
-  const bool any_lr = lr_on(p);
   if (p->use_superres) {
-    if ((rc = superres_tail(ctx, p, s, rec, ref, bps, st))) return rc;
+    if ((rc = superres_tail(ctx, p, s, deb, ref, bps, st))) return rc;
   } else {
     /* 2. CDEF: planes restored next go to scratch, the others straight to ref_out */
     PlanePtr cdef[3];
-    /* CDEF writes cdef_buf for the restored planes and LR reads it */
-    const bool scratch = cdef_on(p) && any_lr;
-    if (scratch && (rc = scratch_acquire(ctx, &ctx->chain_gate, st))) return rc;
     if (cdef_on(p)) {
       for (int i = 0; i < 3; i++) cdef[i] = ref[i];
       if (any_lr) {
@@ -994,7 +1008,7 @@ int rvpf_postfilter_frame(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_
       }
       StageMark m(ctx, RVPF_STAGE_CDEF, st);
       if ((rc = check_launch(ctx,
-                             launch_cdef(g, make_cdef(p), s->mi, s->mi_stride, s->cdef_idx, s->cdef_stride, 1, rec,
+                             launch_cdef(g, make_cdef(p), s->mi, s->mi_stride, s->cdef_idx, s->cdef_stride, 1, deb,
                                          cdef, bps, st),
                              "cdef")))
         return rc;
@@ -1002,19 +1016,19 @@ int rvpf_postfilter_frame(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_
       /* CDEF is the identity: planes not restored are copied to ref_out */
       int w[3] = {0, 0, 0}, h[3] = {0, 0, 0};
       for (int i = 0; i < g.nplanes; i++) {
-        cdef[i] = rec[i];
+        cdef[i] = deb[i];
         if (!lr_plane_on(p, i)) area(p, i, &w[i], &h[i]);
       }
       StageMark m(ctx, RVPF_STAGE_COPY, st);
-      if ((rc = check_launch(ctx, launch_copy_planes(g, rec, ref, w, h, bps, st), "copy"))) return rc;
+      if ((rc = check_launch(ctx, launch_copy_planes(g, deb, ref, w, h, bps, st), "copy"))) return rc;
     }
     /* 3. loop restoration of the restored planes into ref_out */
     if (any_lr) {
       StageMark m(ctx, RVPF_STAGE_LR, st);
-      if ((rc = check_launch(ctx, launch_lr(g, make_lr(p, s), rec, cdef, ref, bps, st), "lr"))) return rc;
+      if ((rc = check_launch(ctx, launch_lr(g, make_lr(p, s), deb, cdef, ref, bps, st), "lr"))) return rc;
     }
-    if (scratch && (rc = scratch_release(ctx, &ctx->chain_gate, st))) return rc;
   }
+  if (gate && (rc = scratch_release(ctx, &ctx->chain_gate, st))) return rc;
   /* 4. display */
   if (!display) return RVPF_OK;
   return run_output(ctx, p, ref_out, display, st, slot);

@@ -291,6 +291,10 @@ int launch_lrd_rect_sse(const rvpf_lrd_item *items, int n, int nunits, const int
 /* Launch entry points (each returns a hipError_t cast to int). */
 int launch_deblock(const Geom &g, const LfHdr &h, const LfSide &s, PlanePtr planes[3], int bps,
                    hipStream_t st);
+/* Both passes in one launch from `in` into `out` (the mode-info area of
+ * each filtered plane; `in` keeps its samples). */
+int launch_deblock_fused(const Geom &g, const LfHdr &h, const LfSide &s, const PlanePtr in[3], const PlanePtr out[3],
+                         int bps, hipStream_t st);
 int launch_cdef(const Geom &g, const CdefHdr &h, const rvpf_mi *mi, int mi_stride,
                 const int8_t *cdef_idx, int cdef_stride, int enabled, const PlanePtr in[3],
                 PlanePtr out[3], int bps, hipStream_t st);

@@ -43,6 +43,13 @@ def first_diff(a, b, rows, cols):
     return "n=%d first at (y=%d, x=%d): got %d want %d" % (len(d), y, x, a[y, x], b[y, x])
 
 
+def assert_recon_kept(got, frame, what):
+    """The whole chain deblocks into context scratch: `recon` keeps its samples."""
+    for p in range(frame.num_planes):
+        msg = first_diff(got[p], frame.rec[p], *frame.rec[p].shape)
+        assert msg is None, "%s plane %d: recon modified: %s" % (what, p, msg)
+
+
 def assert_planes(got, want, frame, region, what):
     for p in range(frame.num_planes):
         if region == "full":

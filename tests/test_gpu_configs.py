@@ -1,9 +1,10 @@
 """GPU parity at the measurement configurations' own sizes.
 
 Every frame of each workload goes through the HIP library (C ABI) and is
-compared with the oracle sample for sample (deblocked frame, reference
-frame, display frame), as the reference decoder runs its chain per frame
-(src/decoder/mod.rs:747-917):
+compared with the oracle sample for sample (reference frame, display
+frame; the batched path's in-place deblocked frame; the per-frame chain
+leaves the reconstruction unmodified), as the reference decoder runs its
+chain per frame (src/decoder/mod.rs:747-917):
 
 - 352x288 8-bit 4:2:0 key frame: SB 64, qindex 128, deblock + CDEF (8
   strengths) + Wiener restoration on luma (64-sample units);
@@ -75,7 +76,7 @@ def _run_chain_and_check(ctx, cfgs, name, bps=2):
     for i, ((f, df, rec, ref, disp), (deb_w, ref_w, disp_w)) in enumerate(zip(jobs, wants)):
         what = "%s frame %d (qindex %d)" % (name, i, f.cfg.qindex)
         _say("check", what)
-        common.assert_planes(api.to_host(rec), deb_w, f, "full", what + " deblocked")
+        common.assert_recon_kept(api.to_host(rec), f, what)
         common.assert_planes(api.to_host(ref), ref_w, f, "ref", what + " reference")
         common.assert_planes(api.to_host(disp), disp_w, f, "crop", what + " display")
 
