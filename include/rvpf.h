@@ -319,7 +319,8 @@ int rvpf_output_packed(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_pla
 
 /*
  * The whole chain of Decoder::decode_frame_wrapup (src/decoder/mod.rs:747-816)
- * and Decoder::output: deblock in place on `recon`, CDEF, super-resolution
+ * and Decoder::output: deblock (`recon` is read, not modified: the deblocked
+ * frame is kept in the context's scratch), CDEF, super-resolution
  * upscaling when use_superres, loop restoration into `ref_out` (the
  * reference frame; upscaled with super-resolution, when only its
  * upscaled_width x frame_height crop is defined), then crop + grain into
