@@ -21,19 +21,36 @@ struct TuneEntry {
   rvpf_tile_config cfg[RVPF_TUNE_COUNT];
 };
 
-/* A scratch buffer shared by calls that may run on different streams:
- * `free` is recorded on the stream of the call that used the buffer last,
- * after its last kernel that reads it, and the next call's stream waits for
- * it before its first kernel writes the buffer. */
+/* A scratch buffer shared by calls that may run on different streams.
+ * While every call comes from one stream, stream order alone keeps them
+ * apart and nothing is recorded (an event on the stream costs the queue a
+ * few microseconds per call). The first call from another stream waits for
+ * the device once; from then on `free` is recorded on the stream of each
+ * call after its last kernel that reads the buffer, and a call from another
+ * stream waits for it before its first kernel writes the buffer. */
 struct ScratchGate {
   hipEvent_t free;
-  int recorded;
+  int used;           /* a call has used the buffer */
+  int multi;          /* calls have come from more than one stream */
+  int recorded;       /* `free` holds a record */
+  hipStream_t stream; /* the stream of the last call */
 };
 
-/* Film grain scratch is double-buffered: the templates of frame n+1 are
- * built on the side stream while frame n still filters. Slot s is rebuilt
- * only after the apply that last read it (ev_done[s]); the apply waits for
- * its templates (ev_ready[s]). */
+/* Film grain scratch slots: the templates of a frame are built on the side
+ * stream while the previous frames still filter, into slot n mod
+ * GRAIN_SLOTS for the n-th frame; its apply waits for them (ev_ready[slot]).
+ * A slot is rebuilt only after the apply that last read it: the apply of
+ * every GRAIN_DONE_EVERY-th frame records ev_done[] (alternately), and the
+ * build of frame n waits for the record that follows frame n - GRAIN_SLOTS,
+ * so the filter stream records an event once per GRAIN_DONE_EVERY frames. */
+constexpr int GRAIN_SLOTS = 8, GRAIN_DONE_EVERY = 4;
+
+/* The template slot of one frame's film grain. */
+struct GrainTicket {
+  int slot;
+  int64_t frame;
+  hipStream_t built_on;
+};
 struct rvpf_ctx {
   int device;
   int cus;                          /* compute units of the device */
@@ -41,8 +58,9 @@ struct rvpf_ctx {
   std::vector<ProfMark> marks;      /* recorded, not yet read */
   std::vector<hipEvent_t> free_evs; /* timing events for reuse */
   hipStream_t side;
-  hipEvent_t ev_ready[2], ev_done[2];
-  int grain_slot;
+  hipEvent_t ev_ready[GRAIN_SLOTS], ev_done[2];
+  int64_t grain_frames; /* templates built so far */
+  int64_t done_frame[2]; /* the frame whose apply the latest record of ev_done[i] follows; -1: none */
   void *deb_buf; /* the chain's deblocked frame */
   size_t deb_bytes;
   void *cdef_buf;
@@ -53,7 +71,7 @@ struct rvpf_ctx {
   void *search_buf; /* encoder searches: work planes, error tables */
   size_t search_bytes;
   ScratchGate search_gate;
-  rvpf::GrainScratch grain[2];
+  rvpf::GrainScratch grain[GRAIN_SLOTS];
   std::vector<TuneEntry> tune_cache;
   char err[256];
 };

@@ -431,12 +431,26 @@ int ensure_buf(rvpf_ctx *ctx, void **buf, size_t *cap, size_t bytes, const char 
 }
 
 int scratch_acquire(rvpf_ctx *ctx, ScratchGate *g, hipStream_t st) {
-  if (!g->recorded) return RVPF_OK;
-  hipError_t e = hipStreamWaitEvent(st, g->free, 0);
+  if (!g->used || g->stream == st) {
+    g->used = 1;
+    g->stream = st;
+    return RVPF_OK;
+  }
+  hipError_t e = hipSuccess;
+  if (!g->multi) {
+    /* the first call from another stream: the earlier calls recorded
+     * nothing (and their stream may be gone), so wait for the device */
+    e = hipDeviceSynchronize();
+    g->multi = 1;
+  } else if (g->recorded) {
+    e = hipStreamWaitEvent(st, g->free, 0);
+  }
+  g->stream = st;
   return e == hipSuccess ? RVPF_OK : hip_fail(ctx, e, "stream wait (scratch)");
 }
 
 int scratch_release(rvpf_ctx *ctx, ScratchGate *g, hipStream_t st) {
+  if (!g->multi) return RVPF_OK;
   hipError_t e = hipEventRecord(g->free, st);
   if (e != hipSuccess) return hip_fail(ctx, e, "event record (scratch)");
   g->recorded = 1;
@@ -543,21 +557,36 @@ int check_launch(rvpf_ctx *ctx, int rc, const char *what) {
 }
 
 /* Builds the film grain templates of `p` into a scratch slot on stream
- * `prep` and returns the slot; the apply must wait for ev_ready[slot]. */
-int grain_prepare(rvpf_ctx *ctx, const rvpf_frame_params *p, hipStream_t prep, int *slot_out) {
-  const int slot = ctx->grain_slot;
-  ctx->grain_slot ^= 1;
+ * `prep`; the apply (run_output) waits for ev_ready[slot]. */
+int grain_prepare(rvpf_ctx *ctx, const rvpf_frame_params *p, hipStream_t prep, GrainTicket *t) {
+  const int64_t n = ctx->grain_frames;
+  const int slot = (int)(n % GRAIN_SLOTS);
   int rc;
   hipError_t e;
   if ((rc = ensure_grain_scratch(ctx, slot, p->upscaled_width, p->frame_height))) return rc;
-  if ((e = hipStreamWaitEvent(prep, ctx->ev_done[slot], 0)) != hipSuccess) return hip_fail(ctx, e, "stream wait");
+  if (n >= GRAIN_SLOTS) {
+    /* the slot's last reader is the apply of frame n - GRAIN_SLOTS: wait for
+     * the earliest ev_done record after it (one was made within
+     * GRAIN_DONE_EVERY frames of it unless a call failed in between) */
+    int k = -1;
+    for (int i = 0; i < 2; i++)
+      if (ctx->done_frame[i] >= n - GRAIN_SLOTS && (k < 0 || ctx->done_frame[i] < ctx->done_frame[k])) k = i;
+    if (k >= 0)
+      e = hipStreamWaitEvent(prep, ctx->ev_done[k], 0);
+    else
+      e = hipDeviceSynchronize();
+    if (e != hipSuccess) return hip_fail(ctx, e, "stream wait");
+  }
+  ctx->grain_frames = n + 1;
   {
     StageMark m(ctx, RVPF_STAGE_GRAIN_PREPARE, prep);
     if ((rc = check_launch(ctx, launch_grain_prepare(make_grain(p), ctx->grain[slot], prep), "grain prepare")))
       return rc;
   }
   if ((e = hipEventRecord(ctx->ev_ready[slot], prep)) != hipSuccess) return hip_fail(ctx, e, "event record");
-  *slot_out = slot;
+  t->slot = slot;
+  t->frame = n;
+  t->built_on = prep;
   return RVPF_OK;
 }
 
@@ -583,7 +612,8 @@ void packed_layout(const rvpf_frame_params *p, rvpf_packed_layout *l) {
 
 /* display output: crop (+ grain with the templates of `slot`) of `ref` */
 int run_output(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_planes *ref, rvpf_planes *disp,
-               hipStream_t st, int slot) {
+               hipStream_t st, const GrainTicket &t) {
+  const int slot = t.slot;
   const Geom g = make_geom(p);
   PlanePtr in[3], out[3];
   to_ptrs(ref, in);
@@ -591,7 +621,8 @@ int run_output(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_planes *ref
   if (grain_on(p)) {
     int rc;
     hipError_t e;
-    if ((e = hipStreamWaitEvent(st, ctx->ev_ready[slot], 0)) != hipSuccess) return hip_fail(ctx, e, "stream wait");
+    if (t.built_on != st && (e = hipStreamWaitEvent(st, ctx->ev_ready[slot], 0)) != hipSuccess)
+      return hip_fail(ctx, e, "stream wait");
     {
       StageMark m(ctx, RVPF_STAGE_GRAIN_APPLY, st);
       if ((rc = check_launch(ctx,
@@ -600,7 +631,11 @@ int run_output(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_planes *ref
                              "grain apply")))
         return rc;
     }
-    if ((e = hipEventRecord(ctx->ev_done[slot], st)) != hipSuccess) return hip_fail(ctx, e, "event record");
+    if (t.frame % GRAIN_DONE_EVERY == GRAIN_DONE_EVERY - 1) {
+      const int k = (int)((t.frame / GRAIN_DONE_EVERY) & 1);
+      if ((e = hipEventRecord(ctx->ev_done[k], st)) != hipSuccess) return hip_fail(ctx, e, "event record");
+      ctx->done_frame[k] = t.frame;
+    }
     return RVPF_OK;
   }
   int w[3] = {0, 0, 0}, h[3] = {0, 0, 0};
@@ -702,13 +737,18 @@ int rvpf_ctx_create(rvpf_ctx **out) {
   hipError_t e = hipGetDevice(&ctx->device);
   if (e == hipSuccess) e = hipDeviceGetAttribute(&ctx->cus, hipDeviceAttributeMultiprocessorCount, ctx->device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking);
-  if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->chain_gate.free, hipEventDisableTiming);
-  if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->search_gate.free, hipEventDisableTiming);
-  for (int i = 0; i < 2; i++) {
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_ready[i], hipEventDisableTiming);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_done[i], hipEventDisableTiming);
-    if (e == hipSuccess) e = hipEventRecord(ctx->ev_done[i], ctx->side);
+  /* ordering events between the device's streams only: no system-scope
+   * fence (a cache writeback and invalidation) when they complete */
+  const unsigned evf = hipEventDisableTiming | hipEventDisableSystemFence;
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->chain_gate.free, evf);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->search_gate.free, evf);
+  for (int i = 0; i < GRAIN_SLOTS; i++) {
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_ready[i], evf);
     if (e == hipSuccess) e = hipEventRecord(ctx->ev_ready[i], ctx->side);
+  }
+  for (int i = 0; i < 2; i++) {
+    ctx->done_frame[i] = -1;
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_done[i], evf);
   }
   if (e != hipSuccess) {
     int rc = hip_fail(nullptr, e, "rvpf_ctx_create");
@@ -733,13 +773,14 @@ void rvpf_ctx_destroy(rvpf_ctx *ctx) {
     (void)hipEventDestroy(m.b);
   }
   for (auto e : ctx->free_evs) (void)hipEventDestroy(e);
-  for (int i = 0; i < 2; i++) {
+  for (int i = 0; i < GRAIN_SLOTS; i++) {
     if (ctx->grain[i].tmpl) (void)hipFree(ctx->grain[i].tmpl);
     if (ctx->grain[i].lut) (void)hipFree(ctx->grain[i].lut);
     if (ctx->grain[i].offsets) (void)hipFree(ctx->grain[i].offsets);
     if (ctx->ev_ready[i]) (void)hipEventDestroy(ctx->ev_ready[i]);
-    if (ctx->ev_done[i]) (void)hipEventDestroy(ctx->ev_done[i]);
   }
+  for (int i = 0; i < 2; i++)
+    if (ctx->ev_done[i]) (void)hipEventDestroy(ctx->ev_done[i]);
   if (ctx->chain_gate.free) (void)hipEventDestroy(ctx->chain_gate.free);
   if (ctx->search_gate.free) (void)hipEventDestroy(ctx->search_gate.free);
   if (ctx->side) (void)hipStreamDestroy(ctx->side);
@@ -897,9 +938,9 @@ int rvpf_grain(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_planes *ref
   if (!display_bps_ok(p, ref->bytes_per_sample, display->bytes_per_sample))
     return fail(ctx, RVPF_E_INVALID, "ref and display sample sizes differ (bytes from 16-bit samples only at 8 bits)");
   hipStream_t st = (hipStream_t)stream;
-  int slot = 0;
-  if (grain_on(p) && (rc = grain_prepare(ctx, p, st, &slot))) return rc;
-  return run_output(ctx, p, ref, display, st, slot);
+  GrainTicket t = {0, 0, st};
+  if (grain_on(p) && (rc = grain_prepare(ctx, p, st, &t))) return rc;
+  return run_output(ctx, p, ref, display, st, t);
 }
 
 int rvpf_packed_layout_of(const rvpf_frame_params *p, rvpf_packed_layout *out) {
@@ -936,9 +977,9 @@ int rvpf_output_packed(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_pla
     disp.height[i] = l.height[i];
   }
   hipStream_t st = (hipStream_t)stream;
-  int slot = 0;
-  if (grain_on(p) && (rc = grain_prepare(ctx, p, ctx->side, &slot))) return rc;
-  return run_output(ctx, p, ref, &disp, st, slot);
+  GrainTicket t = {0, 0, st};
+  if (grain_on(p) && (rc = grain_prepare(ctx, p, ctx->side, &t))) return rc;
+  return run_output(ctx, p, ref, &disp, st, t);
 }
 
 int rvpf_postfilter_frame(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_side_info *s, rvpf_planes *recon,
@@ -957,8 +998,8 @@ int rvpf_postfilter_frame(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_
   const Geom g = make_geom(p);
   const bool show_grain = display && grain_on(p);
   /* film grain templates on the side stream, overlapping the filters */
-  int slot = 0;
-  if (show_grain && (rc = grain_prepare(ctx, p, ctx->side, &slot))) return rc;
+  GrainTicket t = {0, 0, st};
+  if (show_grain && (rc = grain_prepare(ctx, p, ctx->side, &t))) return rc;
   PlanePtr rec[3], ref[3], deb[3];
   to_ptrs(recon, rec);
   to_ptrs(ref_out, ref);
@@ -1031,7 +1072,7 @@ int rvpf_postfilter_frame(rvpf_ctx *ctx, const rvpf_frame_params *p, const rvpf_
   if (gate && (rc = scratch_release(ctx, &ctx->chain_gate, st))) return rc;
   /* 4. display */
   if (!display) return RVPF_OK;
-  return run_output(ctx, p, ref_out, display, st, slot);
+  return run_output(ctx, p, ref_out, display, st, t);
 }
 
 }  // extern "C"
