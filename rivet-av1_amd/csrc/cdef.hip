@@ -226,19 +226,13 @@ __device__ __forceinline__ Taps make_taps(int pri, int sec, int damping, int dir
  * the update stays inside by itself; with no taps it is the identity.
  * Every tap is loaded before the arithmetic, and the taps are
  * independent, so the packed operations interleave. */
+/* The filtered pair xw from its loaded taps pw (primary) and sw
+ * (secondary). */
 template <bool PRI, bool SEC>
-__device__ __forceinline__ uint32_t filter_pair(const uint32_t *w, int base, const Taps &k) {
-  const s16x2 x = as_s2(w[base]);
-  const u16x2 xu = __builtin_bit_cast(u16x2, w[base]);
-  uint32_t pw[4], sw[8];
-  if (PRI) {
-#pragma unroll
-    for (int t = 0; t < 4; t++) pw[t] = w[base + k.po[t]];
-  }
-  if (SEC) {
-#pragma unroll
-    for (int t = 0; t < 8; t++) sw[t] = w[base + k.so[t]];
-  }
+__device__ __forceinline__ uint32_t filter_pair_taps(uint32_t xw, const uint32_t *pw, const uint32_t *sw,
+                                                     const Taps &k) {
+  const s16x2 x = as_s2(xw);
+  const u16x2 xu = __builtin_bit_cast(u16x2, xw);
   constexpr bool CLAMP = PRI && SEC;
   s16x2 sum = splat(0), mx = x;
   u16x2 mn = __builtin_bit_cast(u16x2, x);
@@ -274,6 +268,20 @@ __device__ __forceinline__ uint32_t filter_pair(const uint32_t *w, int base, con
   s16x2 v = x + ((sum + splat(8) + (sum >> splat(15))) >> splat(4));
   if (CLAMP) v = __builtin_elementwise_min(__builtin_elementwise_max(v, __builtin_bit_cast(s16x2, mn)), mx);
   return __builtin_bit_cast(uint32_t, v);
+}
+
+template <bool PRI, bool SEC>
+__device__ __forceinline__ uint32_t filter_pair(const uint32_t *w, int base, const Taps &k) {
+  uint32_t pw[4], sw[8];
+  if (PRI) {
+#pragma unroll
+    for (int t = 0; t < 4; t++) pw[t] = w[base + k.po[t]];
+  }
+  if (SEC) {
+#pragma unroll
+    for (int t = 0; t < 8; t++) sw[t] = w[base + k.so[t]];
+  }
+  return filter_pair_taps<PRI, SEC>(w[base], pw, sw, k);
 }
 
 /* Three of the direction costs of 7.15.2 (postfilter.rs:345-390) for an
@@ -372,24 +380,55 @@ __device__ __forceinline__ int adjust_pri(int pri, int32_t var) {
   return var != 0 ? (pri * (4 + var_str) + 8) >> 4 : 0;
 }
 
-/* Filters this lane's pairs of one block: rows r0 .. r0 + nrows - 1, the
+/* Filters this lane's pairs of one block: rows r0 .. r0 + NROWS - 1, the
  * column pair starting at block column c (even); t0 = the block's top-left
  * tile sample (even), o = the block's top-left output sample. Lanes whose
- * block is not filtered (on == false) store the input pair. */
-template <typename T, bool PRI, bool SEC>
-__device__ __forceinline__ void filter_block_pairs(const uint32_t *w, int t0, T *o, int64_t os, int c, int r0,
-                                                   int nrows, const Taps &k, bool on) {
+ * block is not filtered (on == false) store the input pair. The words of
+ * the 12 taps on the first row are addressed once; the rows follow at
+ * constant offsets (immediates of the LDS reads). */
+template <typename T, bool PRI, bool SEC, int NROWS>
+__device__ __forceinline__ void filter_block_rows(const uint32_t *w, int t0, T *o, int64_t os, int c, int r0,
+                                                  const Taps &k, bool on) {
+  constexpr int RW = TS / 2; /* words per tile row */
   o += (int64_t)r0 * os + c;
-#pragma unroll 2
-  for (int r = r0; r < r0 + nrows; r++, o += os) {
-    const int base = ((t0 + r * TS + c) >> 1);
-    const uint32_t f = filter_pair<PRI, SEC>(w, base, k);
-    const uint32_t v = on ? f : w[base];
+  const uint32_t *b = w + ((t0 + r0 * TS + c) >> 1);
+  const uint32_t *pt[4], *st[8];
+#pragma unroll
+  for (int t = 0; t < 4; t++) pt[t] = b + k.po[t];
+#pragma unroll
+  for (int t = 0; t < 8; t++) st[t] = b + k.so[t];
+#pragma clang loop unroll(full)
+  for (int r = 0; r < NROWS; r++, o += os) {
+    uint32_t pw[4], sw[8];
+    if (PRI) {
+#pragma unroll
+      for (int t = 0; t < 4; t++) pw[t] = pt[t][r * RW];
+    }
+    if (SEC) {
+#pragma unroll
+      for (int t = 0; t < 8; t++) sw[t] = st[t][r * RW];
+    }
+    const uint32_t x = b[r * RW];
+    uint32_t v = x;
+    if (PRI || SEC) {
+      const uint32_t f = filter_pair_taps<PRI, SEC>(x, pw, sw, k); /* branch-free: every lane filters */
+      v = on ? f : x;
+    }
     if (sizeof(T) == 2) {
       *reinterpret_cast<uint32_t *>(o) = v;
     } else {
       *reinterpret_cast<uint16_t *>(o) = (uint16_t)((v & 0xff) | ((v >> 8) & 0xff00));
     }
+  }
+}
+
+template <typename T, bool PRI, bool SEC>
+__device__ __forceinline__ void filter_block_pairs(const uint32_t *w, int t0, T *o, int64_t os, int c, int r0,
+                                                   int nrows, const Taps &k, bool on) {
+  switch (nrows) {
+    case 8: filter_block_rows<T, PRI, SEC, 8>(w, t0, o, os, c, r0, k, on); break;
+    case 4: filter_block_rows<T, PRI, SEC, 4>(w, t0, o, os, c, r0, k, on); break;
+    default: filter_block_rows<T, PRI, SEC, 2>(w, t0, o, os, c, r0, k, on); break;
   }
 }
 
@@ -529,6 +568,7 @@ __device__ __forceinline__ void cdef_tile(const CdefLaunch &a, int tile_id, uint
   const int nrows = (bhs * ncp) >> 2;     /* rows per lane */
   const int cpair = q % ncp, r0 = (q / ncp) * nrows;
   const Taps k = make_taps(upri, usec, a.h.damping + cs - 1, upri ? AV1_CDEF_UV_DIR[ssx][ssy][ydir] : 0, cs);
+#pragma unroll
   for (int p = 1; p < 3; p++) {
     __syncthreads();
     if (pre)
