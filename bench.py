@@ -50,7 +50,7 @@ def make_frames(n, w, h, bd, seed, superres_denom=0, chroma="420"):
 # HBM bytes per launch of each stage's kernels on the default workload,
 # from the calibrated FETCH_SIZE / WRITE_SIZE counter passes summarised here
 PMC_SUMMARY = "profiles/chain_4k10b_pmc.md"
-STAGE_KERNELS = {"deblock": ["lf_pass_kernel<u16,V>", "lf_pass_kernel<u16,H>"], "cdef": ["cdef_kernel<u16>"],
+STAGE_KERNELS = {"deblock": ["lf_fused_kernel<u16>"], "cdef": ["cdef_kernel<u16>"],
                  "lr": ["lr_kernel<u16>"], "grain_apply": ["grain_apply_kernel<u16>"],
                  "upscale": ["upscale_kernel<u16>"]}
 
@@ -431,6 +431,44 @@ def bench_lrdiff(args):
     ctx.close()
 
 
+def bench_grain(args):
+    """The film grain output stage alone (rvpf_grain: template build, then
+    the noise blend into the display planes, both on the caller's stream)
+    on the GOP's 4K frames; a kernel trace of this run gives the template
+    build's own duration, with nothing else on the GPU."""
+    import torch
+    from rvpf import api
+    torch.cuda.set_device(0)
+    frames = make_frames(args.frames, args.width, args.height, args.bit_depth, args.seed, chroma=args.chroma)
+    ctx = api.Context()
+    stream = torch.cuda.current_stream()
+    items = []
+    for f in frames:
+        df = api.DeviceFrame(f)
+        items.append((df, df.upload_planes(f.rec), df.alloc_display()))
+    torch.cuda.synchronize()
+    for _ in range(args.warmup):
+        for df, ref, disp in items:
+            api.grain(ctx, df, ref, disp, stream)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(args.steps):
+        for df, ref, disp in items:
+            api.grain(ctx, df, ref, disp, stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    mpix = sum(f.luma_pixels() for f in frames) / 1e6
+    ctx.close()
+    print(json.dumps({"metric": "film grain output stage Mpix/s (template build + noise blend, serial)",
+                      "value": round(mpix / (ms / 1e3), 1), "unit": "Mpix/s", "n_gpus": 1, "steps": args.steps,
+                      "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True,
+                      "dtype": "u16", "data": "synthetic",
+                      "config": {"workload": "%dx%d %d-bit %s, %d frames, film grain lag 3" % (
+                          args.width, args.height, args.bit_depth, CHROMA_NAME[args.chroma], args.frames)}}))
+
+
 def bench_encsearch(args):
     """The encoder's loop filter level search and CDEF strength search on
     the GPU (search_lf + cdef::search, normal effort) on 4K 10-bit frames:
@@ -547,11 +585,12 @@ def ac_q(bd, q):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=["gop4k", "batch32", "lrdiff", "encsearch"], default="gop4k",
+    ap.add_argument("--workload", choices=["gop4k", "batch32", "lrdiff", "encsearch", "grain"], default="gop4k",
                     help="gop4k: the per-frame chain on the 4K 10-bit GOP (default); "
                          "batch32: the batched path against the per-frame drop-in on 32 mixed streams; "
                          "lrdiff: the encoder's restoration search on 1080p 10-bit frames; "
-                         "encsearch: the encoder's loop filter and CDEF searches on the GOP's frames")
+                         "encsearch: the encoder's loop filter and CDEF searches on the GOP's frames; "
+                         "grain: the film grain output stage alone on the GOP's frames")
     ap.add_argument("--lr-frames", type=int, default=16)
     ap.add_argument("--streams", type=int, default=32)
     ap.add_argument("--stream-frames", type=int, default=4)
@@ -588,6 +627,9 @@ def main():
         return
     if args.workload == "encsearch":
         bench_encsearch(args)
+        return
+    if args.workload == "grain":
+        bench_grain(args)
         return
 
     import numpy as np
@@ -698,8 +740,7 @@ def main():
             stages[k] = st
 
     roofline = None
-    # the roofline names one kernel: the deblock stage is two (one per pass,
-    # each shorter than the CDEF and LR kernels), so it is not a candidate
+    # the roofline names one kernel: that of the longest main-stream stage
     main = [k for k in stages if "GB_s" in stages[k] and len(STAGE_KERNELS.get(k, [k])) == 1]
     if main:
         dom = max(main, key=lambda k: stages[k]["ms"])
