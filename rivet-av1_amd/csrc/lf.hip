@@ -539,14 +539,61 @@ constexpr int FV_COLS = 17, FV_ROWS = 20; /* vertical edges: columns x rows of 4
 constexpr int FH_ROWS = 17, FH_COLS = 16; /* horizontal edges */
 constexpr int FV_N = FV_COLS * FV_ROWS, FH_N = FH_ROWS * FH_COLS;
 constexpr int F_ITEMS = (FV_N + FH_N + 255) / 256; /* decisions per thread */
+/* side-info positions (4x4 units of the plane) the decisions read: columns
+ * X0 / 4 - 1 .. X0 / 4 + 16, rows Y0 / 4 - 2 .. Y0 / 4 + 17 */
+constexpr int FM_COLS = 18, FM_ROWS = 20, FM_N = FM_COLS * FM_ROWS;
+constexpr int FM_ITEMS = (FM_N + 255) / 256;
 
 struct LfFusedShared {
   __align__(16) uint16_t tile[FT * FT];
   LfTables t;
   int count[8];             /* [pass * 4 + class] */
-  uint32_t lv[4][FV_N];     /* vertical edges by class: index | limit << 10 | blimit << 16 | thresh << 24 */
-  uint32_t lh[4][FH_N];     /* horizontal edges */
+  uint16_t lv[4][FV_N];     /* vertical edges by class (edge index) */
+  uint16_t lh[4][FH_N];     /* horizontal edges */
+  uint8_t elv[FV_N], elh[FH_N]; /* the edge's level */
+  rvpf_mi mi[FM_N];         /* mode info at each side-info position */
+  uint8_t tx[FM_N];         /* transform size */
+  uint8_t lvl[2][FM_N];     /* level of the position's block in the vertical / horizontal pass */
 };
+
+/* limit, blimit and thresh of a level (unscaled; filter_strength,
+ * postfilter.rs:242-252) */
+__device__ __forceinline__ void level_limits(int lvl, int sharp, int *limit, int *blimit, int *thresh) {
+  const int shift = sharp > 4 ? 2 : (sharp > 0 ? 1 : 0);
+  const int l = sharp > 0 ? clip3(1, 9 - sharp, lvl >> shift) : max(lvl >> shift, 1);
+  *limit = l;
+  *blimit = 2 * (lvl + 2) + l;
+  *thresh = lvl >> 4;
+}
+
+/* Edge decision of edge_loop_filter (postfilter.rs:115-172) from the
+ * staged side info: m / tx of the edge's block, ptx of the block before
+ * it, and the level of each (lvl, plvl). Returns the filter size (0: no
+ * filter) and, in *level, the level it filters with. */
+__device__ __forceinline__ int edge_decision_staged(const Geom &g, const LfTables &t, const rvpf_mi &m, int tx,
+                                                    int ptx, int lvl, int plvl, int plane, int pass, int ex, int ey,
+                                                    int *level) {
+  const int ssx = plane ? g.ssx : 0, ssy = plane ? g.ssy : 0;
+  const int x = (ex << ssx) * 4, y = (ey << ssy) * 4;
+  if (x >= g.width || y >= g.height) return 0;
+  const int pos = pass == 0 ? x >> ssx : y >> ssy;
+  const int tx_sz = min(tx, 18), prev_tx_sz = min(ptx, 18);
+  const int bl = t.blk_log2[plane != 0][pass][min((int)m.mi_size, 21)];
+  const bool is_block_edge = bl == 0xff || (pos & ((1 << bl) - 1)) == 0;
+  const int tl = t.tx_log2[pass][tx_sz], ptl = t.tx_log2[pass][prev_tx_sz];
+  const bool is_tx_edge = (pos & ((1 << tl) - 1)) == 0;
+  const bool skip = m.seg_skip & 1;
+  const bool is_intra = m.ref_frame0 <= 0;
+  if (!(is_tx_edge && (is_block_edge || !skip || is_intra))) return 0;
+  const int base_size = 1 << min(tl, ptl);
+  int filter_size = min(plane == 0 ? 16 : 8, base_size);
+  /* A 16-wide edge of legal side info is at least 16 samples from the
+   * plane's edge; keep the reads in bounds for side info that is not. */
+  if (filter_size == 16 && pos < 8) filter_size = 8;
+  const int l = lvl ? lvl : plvl;
+  *level = l;
+  return l ? filter_size : 0;
+}
 
 __device__ __forceinline__ void lds_st4(uint16_t *p, const int *o) {
   *reinterpret_cast<uint2 *>(p) = make_uint2((uint32_t)o[0] | ((uint32_t)o[1] << 16), (uint32_t)o[2] | ((uint32_t)o[3] << 16));
@@ -557,12 +604,13 @@ __device__ __forceinline__ void lds_st2(uint16_t *p, int a, int b) {
 
 /* One vertical-edge class: a row of an edge per item, in the staged tile. */
 template <int C>
-__device__ __forceinline__ void fused_v(LfFusedShared &sm, int bd) {
+__device__ __forceinline__ void fused_v(LfFusedShared &sm, int bd, int sharp) {
   constexpr int N = LfClass<C>::N, H = N / 2;
   const int sh = bd - 8, n = sm.count[C];
   for (int it = threadIdx.x; it < 4 * n; it += 256) {
-    const uint32_t e = sm.lv[C][it >> 2];
-    const int li = e & 1023;
+    const int li = sm.lv[C][it >> 2];
+    int limit, blimit, thresh;
+    level_limits(sm.elv[li], sharp, &limit, &blimit, &thresh);
     const int col = 8 + 4 * (li % FV_COLS), row = 4 * (li / FV_COLS) + (it & 3);
     uint16_t *trow = sm.tile + row * FT + col;
     int s[16];
@@ -575,7 +623,7 @@ __device__ __forceinline__ void fused_v(LfFusedShared &sm, int bd) {
     int v[N];
 #pragma unroll
     for (int k = 0; k < N; k++) v[k] = s[8 - H + k];
-    if (!filter_class<C>(v, ((e >> 10) & 63) << sh, ((e >> 16) & 255) << sh, (e >> 24) << sh, bd)) continue;
+    if (!filter_class<C>(v, limit << sh, blimit << sh, thresh << sh, bd)) continue;
 #pragma unroll
     for (int k = 0; k < N; k++) s[8 - H + k] = v[k];
     if (C <= 1) {
@@ -594,19 +642,20 @@ __device__ __forceinline__ void fused_v(LfFusedShared &sm, int bd) {
 
 /* One horizontal-edge class: a column of an edge per item. */
 template <int C>
-__device__ __forceinline__ void fused_h(LfFusedShared &sm, int bd) {
+__device__ __forceinline__ void fused_h(LfFusedShared &sm, int bd, int sharp) {
   constexpr int N = LfClass<C>::N, H = N / 2;
   constexpr int R = C <= 1 ? 2 : (C == 2 ? 3 : 6); /* rows the filter modifies either side */
   const int sh = bd - 8, n = sm.count[4 + C];
   for (int it = threadIdx.x; it < 4 * n; it += 256) {
-    const uint32_t e = sm.lh[C][it >> 2];
-    const int li = e & 1023;
+    const int li = sm.lh[C][it >> 2];
+    int limit, blimit, thresh;
+    level_limits(sm.elh[li], sharp, &limit, &blimit, &thresh);
     const int row = 8 + 4 * (li / FH_COLS), col = 8 + 4 * (li % FH_COLS) + (it & 3);
     uint16_t *tcol = sm.tile + row * FT + col;
     int v[N];
 #pragma unroll
     for (int k = 0; k < N; k++) v[k] = tcol[(k - H) * FT];
-    if (!filter_class<C>(v, ((e >> 10) & 63) << sh, ((e >> 16) & 255) << sh, (e >> 24) << sh, bd)) continue;
+    if (!filter_class<C>(v, limit << sh, blimit << sh, thresh << sh, bd)) continue;
 #pragma unroll
     for (int k = H - R; k < H + R; k++) tcol[(k - H) * FT] = (uint16_t)v[k];
   }
@@ -621,30 +670,24 @@ __device__ __forceinline__ void lf_fused_block(const LfLaunch &a, int gp, int lb
   const Geom &g = a.g;
   const int ssx = plane ? g.ssx : 0, ssy = plane ? g.ssy : 0;
   const uint32_t tword = threadIdx.x < LF_TWORDS ? reinterpret_cast<const uint32_t *>(&a.t)[threadIdx.x] : 0u;
-  /* the side info of this thread's decisions: item k < FV_N is vertical
-   * edge (column i, row group j), the rest horizontal */
-  EdgeSide side[F_ITEMS];
-  bool valid[F_ITEMS];
-  int pass_of[F_ITEMS], ex_of[F_ITEMS], ey_of[F_ITEMS];
+  /* the side info at this thread's positions (mode info of the position's
+   * 4x4 block, its transform size), loaded with the samples */
+  const int ex0 = X0 / 4 - 1, ey0 = Y0 / 4 - 2;
+  const int ms = a.s.mi_stride;
+  uint2 pmi[FM_ITEMS];
+  int ptx[FM_ITEMS];
 #pragma unroll
-  for (int k = 0; k < F_ITEMS; k++) {
-    const int item = threadIdx.x + 256 * k;
-    int pass = 0, ex = 0, ey = 0;
-    if (item < FV_N) {
-      ex = X0 / 4 + item % FV_COLS;
-      ey = Y0 / 4 - 2 + item / FV_COLS;
-    } else {
-      const int h = item - FV_N;
-      pass = 1;
-      ex = X0 / 4 + h % FH_COLS;
-      ey = Y0 / 4 + h / FH_COLS;
+  for (int k = 0; k < FM_ITEMS; k++) {
+    const int i = threadIdx.x + 256 * k;
+    const int ex = ex0 + i % FM_COLS, ey = ey0 + i / FM_COLS;
+    pmi[k] = make_uint2(0, 0);
+    ptx[k] = 0;
+    /* positions outside the mode-info area are read by no filtered edge */
+    if (i < FM_N && ex >= 0 && ey >= 0 && (ex << ssx) < g.mi_cols && (ey << ssy) < g.mi_rows) {
+      const int rowo = (ey << ssy) | ssy, colo = (ex << ssx) | ssx;
+      pmi[k] = gptr(reinterpret_cast<const uint2 *>(a.s.mi))[rowo * ms + colo];
+      ptx[k] = gptr(a.s.tx[plane])[ey * ms + ex];
     }
-    pass_of[k] = pass;
-    ex_of[k] = ex;
-    ey_of[k] = ey;
-    valid[k] = item < FV_N + FH_N && ey >= 0 && (ey << ssy) < g.mi_rows && (ex << ssx) < g.mi_cols &&
-               (pass == 0 ? ex != 0 : ey != 0);
-    if (valid[k]) side[k] = load_side(a, plane, pass, ex, ey);
   }
   const T *in = gptr(reinterpret_cast<const T *>(a.planes[plane].data));
   const int64_t stride = a.planes[plane].stride;
@@ -674,19 +717,50 @@ __device__ __forceinline__ void lf_fused_block(const LfLaunch &a, int gp, int lb
     }
   }
   __syncthreads();
-  /* decisions, appended wave by wave to the 8 lists (pass, class) */
+  /* the level of each position's block, per pass (luma: the vertical and
+   * horizontal levels; chroma: the plane's) */
+#pragma unroll
+  for (int k = 0; k < FM_ITEMS; k++) {
+    const int i = threadIdx.x + 256 * k;
+    if (i < FM_N) {
+      const rvpf_mi m = __builtin_bit_cast(rvpf_mi, pmi[k]);
+      const int l0 = lf_level(a.h, sm->t, m, plane == 0 ? 0 : plane + 1);
+      sm->mi[i] = m;
+      sm->tx[i] = (uint8_t)ptx[k];
+      sm->lvl[0][i] = (uint8_t)l0;
+      sm->lvl[1][i] = (uint8_t)(plane == 0 ? lf_level(a.h, sm->t, m, 1) : l0);
+    }
+  }
+  __syncthreads();
+  /* decisions (item < FV_N: vertical edge (column i, row group j); else
+   * horizontal), appended wave by wave to the 8 lists (pass, class) */
   const int lane = threadIdx.x & 63;
 #pragma unroll
   for (int k = 0; k < F_ITEMS; k++) {
-    int cls = -1;
-    uint32_t word = 0;
-    if (valid[k]) {
-      int limit, blimit, thresh;
-      const int fs = edge_decision(a, sm->t, side[k], plane, pass_of[k], ex_of[k], ey_of[k], &limit, &blimit, &thresh);
-      if (fs) {
-        cls = 4 * pass_of[k] + (fs == 4 ? 0 : (fs == 16 ? 3 : (plane ? 1 : 2)));
-        const int item = threadIdx.x + 256 * k - (pass_of[k] ? FV_N : 0);
-        word = item | (limit << 10) | (blimit << 16) | (thresh << 24);
+    const int item = threadIdx.x + 256 * k;
+    int cls = -1, idx = 0, lvl = 0;
+    if (item < FV_N + FH_N) {
+      const int pass = item < FV_N ? 0 : 1;
+      idx = pass ? item - FV_N : item;
+      int ex, ey, pos, ppos;
+      if (pass == 0) {
+        const int i = idx % FV_COLS, j = idx / FV_COLS;
+        ex = X0 / 4 + i;
+        ey = Y0 / 4 - 2 + j;
+        pos = j * FM_COLS + i + 1;
+        ppos = pos - 1;
+      } else {
+        const int i = idx % FH_COLS, j = idx / FH_COLS;
+        ex = X0 / 4 + i;
+        ey = Y0 / 4 + j;
+        pos = (j + 2) * FM_COLS + i + 1;
+        ppos = pos - FM_COLS;
+      }
+      const bool valid = ey >= 0 && (ey << ssy) < g.mi_rows && (ex << ssx) < g.mi_cols && (pass == 0 ? ex != 0 : ey != 0);
+      if (valid) {
+        const int fs = edge_decision_staged(g, sm->t, sm->mi[pos], sm->tx[pos], sm->tx[ppos], sm->lvl[pass][pos],
+                                            sm->lvl[pass][ppos], plane, pass, ex, ey, &lvl);
+        if (fs) cls = 4 * pass + (fs == 4 ? 0 : (fs == 16 ? 3 : (plane ? 1 : 2)));
       }
     }
     uint64_t m[8];
@@ -703,27 +777,31 @@ __device__ __forceinline__ void lf_fused_block(const LfLaunch &a, int gp, int lb
     const int below = __builtin_amdgcn_mbcnt_hi((uint32_t)(mc >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mc, 0u));
     base = __shfl(base, cls < 0 ? 0 : cls, 64);
     if (cls >= 0) {
-      if (cls < 4)
-        sm->lv[cls][base + below] = word;
-      else
-        sm->lh[cls - 4][base + below] = word;
+      if (cls < 4) {
+        sm->lv[cls][base + below] = (uint16_t)idx;
+        sm->elv[idx] = (uint8_t)lvl;
+      } else {
+        sm->lh[cls - 4][base + below] = (uint16_t)idx;
+        sm->elh[idx] = (uint8_t)lvl;
+      }
     }
   }
   __syncthreads();
-  fused_v<0>(*sm, g.bd);
+  const int sharp = a.h.sharpness;
+  fused_v<0>(*sm, g.bd, sharp);
   if (plane) {
-    fused_v<1>(*sm, g.bd);
+    fused_v<1>(*sm, g.bd, sharp);
   } else {
-    fused_v<2>(*sm, g.bd);
-    fused_v<3>(*sm, g.bd);
+    fused_v<2>(*sm, g.bd, sharp);
+    fused_v<3>(*sm, g.bd, sharp);
   }
   __syncthreads();
-  fused_h<0>(*sm, g.bd);
+  fused_h<0>(*sm, g.bd, sharp);
   if (plane) {
-    fused_h<1>(*sm, g.bd);
+    fused_h<1>(*sm, g.bd, sharp);
   } else {
-    fused_h<2>(*sm, g.bd);
-    fused_h<3>(*sm, g.bd);
+    fused_h<2>(*sm, g.bd, sharp);
+    fused_h<3>(*sm, g.bd, sharp);
   }
   __syncthreads();
   /* the block's 64 x 64 samples, clipped to the mode-info area (a chunk
