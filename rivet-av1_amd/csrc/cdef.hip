@@ -37,8 +37,11 @@ namespace rvpf {
 namespace {
 
 constexpr int NA_S = 0xA000;
-constexpr int TS = 68;                /* tile stride in samples: 64 + 2 * 2 */
-constexpr int SOFF = 68 * (TS / 2);   /* word offset of the shifted tile S */
+constexpr int TS = 68; /* tile stride in samples: 64 + 2 * 2 */
+/* word offset of the shifted tile S behind a tile P of `rows` rows plus
+ * the 2-row borders */
+constexpr int soff_rows(int rows) { return (rows + 4) * (TS / 2); }
+constexpr int SOFF = soff_rows(64);
 
 struct CdefLaunch {
   Geom g;
@@ -63,7 +66,7 @@ __device__ __forceinline__ T *pp(const PlanePtr &p) {
  * samples shifted left by one go to S (S[r][c] = P[r][c + 1]), so that any
  * horizontal pair (c + dx, c + dx + 1) with c even is one aligned 32-bit
  * word of P (dx even) or of S (dx odd). w <= 64. */
-template <typename T>
+template <typename T, int NW>
 __device__ void stage_tile(uint16_t *t, uint16_t *sh, const T *src, int64_t stride, int x0, int y0, int w,
                            int h, int xlim, int ylim) {
   const int tw = w + 4, th = h + 4;
@@ -74,11 +77,11 @@ __device__ void stage_tile(uint16_t *t, uint16_t *sh, const T *src, int64_t stri
   /* ROWS rows per batch: every load (clamped into the plane) is issued
    * before the LDS stores */
   constexpr int ROWS = 6;
-  for (int r0 = ty; r0 < th; r0 += 4 * ROWS) {
+  for (int r0 = ty; r0 < th; r0 += NW * ROWS) {
     uint16_t va[ROWS], vb[ROWS];
 #pragma unroll
     for (int k = 0; k < ROWS; k++) {
-      const int y = y0 + r0 + 4 * k - 2;
+      const int y = y0 + r0 + NW * k - 2;
       const bool yok = y >= 0 && y < ylim;
       const T *row = src + (int64_t)min(max(y, 0), ylim - 1) * stride;
       const int a = row[xac], b = row[xbc];
@@ -87,7 +90,7 @@ __device__ void stage_tile(uint16_t *t, uint16_t *sh, const T *src, int64_t stri
     }
 #pragma unroll
     for (int k = 0; k < ROWS; k++) {
-      const int r = r0 + 4 * k;
+      const int r = r0 + NW * k;
       if (r < th) {
         if (tx < tw) {
           t[r * TS + tx] = va[k];
@@ -103,39 +106,47 @@ __device__ void stage_tile(uint16_t *t, uint16_t *sh, const T *src, int64_t stri
   }
 }
 
-/* A chroma tile of a 4:2:0 frame ((32 + 4) x (32 + 4) samples) loaded into
- * registers, 9 rows per thread (rows ty, ty + 4, ...; column tx < 36),
- * with the unavailable samples already NA_S, so that its loads can be in
- * flight while the luma tile filters; stage_store writes it to P and S. */
+/* A chroma tile of a 4:2:0 frame ((32 + 4) x (8 NW + 4) samples, the
+ * chroma of an NW-wave tile) loaded into registers, CR = 8 + 4 / NW rows
+ * per thread (rows ty, ty + NW, ...; column tx < 36), with the unavailable
+ * samples already NA_S, so that its loads can be in flight while the luma
+ * tile filters; chroma_store writes it to P and S. */
 struct ChromaRegs {
   uint32_t v[5]; /* row k in half k & 1 of word k >> 1 */
 };
 
-template <typename T>
+template <int NW>
+constexpr int chroma_rows() { return 8 + 4 / NW; }
+
+template <typename T, int NW>
 __device__ __forceinline__ void chroma_load(ChromaRegs &cr, const T *src, int64_t stride, int x0, int y0, int xlim,
                                             int ylim) {
+  constexpr int CR = chroma_rows<NW>();
+  static_assert(CR <= 10, "ChromaRegs holds 10 rows");
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
   const int xa = x0 + tx - 2;
   const bool xa_ok = tx < 36 && xa >= 0 && xa < xlim;
   const int xac = min(max(xa, 0), xlim - 1);
-  uint32_t v[9];
+  uint32_t v[CR];
 #pragma unroll
-  for (int k = 0; k < 9; k++) {
-    const int y = y0 + ty + 4 * k - 2;
+  for (int k = 0; k < CR; k++) {
+    const int y = y0 + ty + NW * k - 2;
     const bool yok = y >= 0 && y < ylim;
     const int a = src[(int64_t)min(max(y, 0), ylim - 1) * stride + xac];
     v[k] = yok && xa_ok ? (uint32_t)a : (uint32_t)NA_S;
   }
 #pragma unroll
-  for (int k = 0; k < 5; k++) cr.v[k] = v[2 * k] | (2 * k + 1 < 9 ? v[2 * k + 1] << 16 : 0u);
+  for (int k = 0; k < (CR + 1) / 2; k++) cr.v[k] = v[2 * k] | (2 * k + 1 < CR ? v[2 * k + 1] << 16 : 0u);
 }
 
+template <int NW>
 __device__ __forceinline__ void chroma_store(const ChromaRegs &cr, uint16_t *t, uint16_t *sh) {
+  constexpr int CR = chroma_rows<NW>();
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
   if (tx >= 36) return;
 #pragma unroll
-  for (int k = 0; k < 9; k++) {
-    const int r = ty + 4 * k;
+  for (int k = 0; k < CR; k++) {
+    const int r = ty + NW * k;
     const uint16_t v = (uint16_t)(cr.v[k >> 1] >> (16 * (k & 1)));
     t[r * TS + tx] = v;
     if (tx > 0) sh[r * TS + tx - 1] = v;
@@ -182,12 +193,13 @@ struct Taps {
 
 __device__ __forceinline__ int floor_log2(int x) { return 31 - __clz(x); }
 
-/* word offset of tap (dy, dx) for a pair at an even column */
-__device__ __forceinline__ int word_off(int dy, int dx) {
-  return (dx & 1) ? SOFF + dy * (TS / 2) + ((dx - 1) >> 1) : dy * (TS / 2) + (dx >> 1);
+/* word offset of tap (dy, dx) for a pair at an even column; soff = the
+ * word offset of S behind P */
+__device__ __forceinline__ int word_off(int dy, int dx, int soff) {
+  return (dx & 1) ? soff + dy * (TS / 2) + ((dx - 1) >> 1) : dy * (TS / 2) + (dx >> 1);
 }
 
-__device__ __forceinline__ Taps make_taps(int pri, int sec, int damping, int dir, int coeff_shift) {
+__device__ __forceinline__ Taps make_taps(int pri, int sec, int damping, int dir, int coeff_shift, int soff) {
   Taps k;
   k.has_pri = pri != 0;
   k.has_sec = sec != 0;
@@ -205,14 +217,14 @@ __device__ __forceinline__ Taps make_taps(int pri, int sec, int damping, int dir
 #pragma unroll
   for (int t = 0; t < 2; t++) {
     const int dy = AV1_CDEF_DIRECTIONS[dir][t][0], dx = AV1_CDEF_DIRECTIONS[dir][t][1];
-    k.po[2 * t] = word_off(dy, dx);
-    k.po[2 * t + 1] = word_off(-dy, -dx);
+    k.po[2 * t] = word_off(dy, dx, soff);
+    k.po[2 * t + 1] = word_off(-dy, -dx, soff);
     const int ly = AV1_CDEF_DIRECTIONS[dlo][t][0], lx = AV1_CDEF_DIRECTIONS[dlo][t][1];
     const int hy = AV1_CDEF_DIRECTIONS[dhi][t][0], hx = AV1_CDEF_DIRECTIONS[dhi][t][1];
-    k.so[4 * t + 0] = word_off(ly, lx);
-    k.so[4 * t + 1] = word_off(-ly, -lx);
-    k.so[4 * t + 2] = word_off(hy, hx);
-    k.so[4 * t + 3] = word_off(-hy, -hx);
+    k.so[4 * t + 0] = word_off(ly, lx, soff);
+    k.so[4 * t + 1] = word_off(-ly, -lx, soff);
+    k.so[4 * t + 2] = word_off(hy, hx, soff);
+    k.so[4 * t + 3] = word_off(-hy, -hx, soff);
   }
   return k;
 }
@@ -455,26 +467,32 @@ struct CdefBlocks {
   int nfilt, ncopy;
 };
 
-/* One 64x64 filter block (tile_id in raster order); words holds the
- * tile P, then the shifted tile S. The four lanes of slot s = tid / 4 take
- * block list[s]: only the waves whose slots hold filtered blocks run the
- * direction search and the filters, the rest copy their blocks from the
- * staged tile. */
-template <typename T>
+/* One tile of NW waves: the 64 x 16 NW luma samples (and their chroma)
+ * of part tile_id % (4 / NW) of the 64x64 filter block tile_id / (4 / NW)
+ * (raster order); words holds the tile P, then the shifted tile S. The
+ * four lanes of slot s = tid / 4 take block list[s]: only the waves whose
+ * slots hold filtered blocks run the direction search and the filters,
+ * the rest copy their blocks from the staged tile. */
+template <typename T, int NW>
 __device__ __forceinline__ void cdef_tile(const CdefLaunch &a, int tile_id, uint32_t *words, CdefBlocks &bl) {
+  constexpr int TH = 16 * NW;      /* luma rows of the tile */
+  constexpr int NPART = 64 / TH;   /* tiles per filter block */
+  constexpr int SO = soff_rows(TH);
   uint16_t *tile = reinterpret_cast<uint16_t *>(words);
-  uint16_t *shifted = reinterpret_cast<uint16_t *>(words + SOFF);
+  uint16_t *shifted = reinterpret_cast<uint16_t *>(words + SO);
   const Geom &g = a.g;
   const int tid = threadIdx.x;
   const int tx = tid & 63, ty = tid >> 6;
-  const int fbx = tile_id % a.fb_cols, fby = tile_id / a.fb_cols;
+  const int fb = tile_id / NPART, part = tile_id % NPART;
+  const int fbx = fb % a.fb_cols, fby = fb / a.fb_cols;
   const int cs = g.bd - 8;
+  const int xlim = g.mi_cols * 4, ylim = g.mi_rows * 4;
+  const int x0 = fbx * 64, y0 = fby * 64 + part * TH;
+  if (y0 >= ylim) return; /* the lower part of a filter block below the area */
   int idx = -1;
   if (a.enabled) idx = gptr(a.cdef_idx)[(int64_t)fby * a.cdef_stride + fbx];
   if (idx >= 0) idx &= 7; /* cdef_bits <= 3 */
-  const int xlim = g.mi_cols * 4, ylim = g.mi_rows * 4;
-  const int x0 = fbx * 64, y0 = fby * 64;
-  const int bw = min(64, xlim - x0), bh = min(64, ylim - y0); /* multiples of 8 */
+  const int bw = min(64, xlim - x0), bh = min(TH, ylim - y0); /* multiples of 8 */
 
   if (idx < 0) {
     /* not filtered: copy the block area of every plane */
@@ -486,13 +504,13 @@ __device__ __forceinline__ void cdef_tile(const CdefLaunch &a, int tile_id, uint
         T *dst = pp<T>(a.out[p]) + (int64_t)py * a.out[p].stride + px + tx;
         /* all loads of a batch (rows clamped to the block) before the stores */
         constexpr int ROWS = 8;
-        for (int r0 = ty; r0 < h; r0 += 4 * ROWS) {
+        for (int r0 = ty; r0 < h; r0 += NW * ROWS) {
           T v[ROWS];
 #pragma unroll
-          for (int k = 0; k < ROWS; k++) v[k] = src[(int64_t)min(r0 + 4 * k, h - 1) * a.in[p].stride];
+          for (int k = 0; k < ROWS; k++) v[k] = src[(int64_t)min(r0 + NW * k, h - 1) * a.in[p].stride];
 #pragma unroll
           for (int k = 0; k < ROWS; k++)
-            if (r0 + 4 * k < h) dst[(int64_t)(r0 + 4 * k) * a.out[p].stride] = v[k];
+            if (r0 + NW * k < h) dst[(int64_t)(r0 + NW * k) * a.out[p].stride] = v[k];
         }
       }
     }
@@ -523,13 +541,13 @@ __device__ __forceinline__ void cdef_tile(const CdefLaunch &a, int tile_id, uint
       bl.ncopy = __popcll(mc);
     }
   }
-  stage_tile<T>(tile, shifted, pp<T>(a.in[0]), a.in[0].stride, x0, y0, 64, 64, xlim, ylim);
+  stage_tile<T, NW>(tile, shifted, pp<T>(a.in[0]), a.in[0].stride, x0, y0, 64, TH, xlim, ylim);
   /* 4:2:0: both chroma tiles' loads go out now and land while luma filters */
   const bool pre = g.nplanes > 1 && g.ssx && g.ssy;
   ChromaRegs cpre[2];
   if (pre)
     for (int p = 1; p < 3; p++)
-      chroma_load<T>(cpre[p - 1], pp<T>(a.in[p]), a.in[p].stride, x0 >> 1, y0 >> 1, xlim >> 1, ylim >> 1);
+      chroma_load<T, NW>(cpre[p - 1], pp<T>(a.in[p]), a.in[p].stride, x0 >> 1, y0 >> 1, xlim >> 1, ylim >> 1);
   __syncthreads();
 
   const int nf = bl.nfilt, nact = nf + bl.ncopy;
@@ -547,7 +565,7 @@ __device__ __forceinline__ void cdef_tile(const CdefLaunch &a, int tile_id, uint
   /* luma: lane q filters (or copies) the column pair 2q, 2q + 1 of its block */
   if (active) {
     const int sec = a.h.y_sec[idx] << cs;
-    const Taps k = make_taps(pri, sec, a.h.damping + cs, a.h.y_pri[idx] ? ydir : 0, cs);
+    const Taps k = make_taps(pri, sec, a.h.damping + cs, a.h.y_pri[idx] ? ydir : 0, cs, SO);
     T *o = pp<T>(a.out[0]) + (int64_t)(y0 + by * 8) * a.out[0].stride + x0 + bx * 8;
     const int t0 = (by * 8 + 2) * TS + bx * 8 + 2;
     if (wave_filters)
@@ -561,20 +579,20 @@ __device__ __forceinline__ void cdef_tile(const CdefLaunch &a, int tile_id, uint
   /* chroma planes: block of (8 >> ssx) x (8 >> ssy) = ncp column pairs;
    * lane q takes pair q % ncp and its share of the rows */
   const int ssx = g.ssx, ssy = g.ssy;
-  const int cw = 64 >> ssx, ch = 64 >> ssy;
+  const int cw = 64 >> ssx, ch = TH >> ssy;
   const int upri = a.h.uv_pri[idx] << cs, usec = a.h.uv_sec[idx] << cs;
   const int bws = 8 >> ssx, bhs = 8 >> ssy;
   const int ncp = bws >> 1;               /* 4 (4:4:4) or 2 */
   const int nrows = (bhs * ncp) >> 2;     /* rows per lane */
   const int cpair = q % ncp, r0 = (q / ncp) * nrows;
-  const Taps k = make_taps(upri, usec, a.h.damping + cs - 1, upri ? AV1_CDEF_UV_DIR[ssx][ssy][ydir] : 0, cs);
+  const Taps k = make_taps(upri, usec, a.h.damping + cs - 1, upri ? AV1_CDEF_UV_DIR[ssx][ssy][ydir] : 0, cs, SO);
 #pragma unroll
   for (int p = 1; p < 3; p++) {
     __syncthreads();
     if (pre)
-      chroma_store(cpre[p - 1], tile, shifted);
+      chroma_store<NW>(cpre[p - 1], tile, shifted);
     else
-      stage_tile<T>(tile, shifted, pp<T>(a.in[p]), a.in[p].stride, x0 >> ssx, y0 >> ssy, cw, ch, xlim >> ssx,
+      stage_tile<T, NW>(tile, shifted, pp<T>(a.in[p]), a.in[p].stride, x0 >> ssx, y0 >> ssy, cw, ch, xlim >> ssx,
                     ylim >> ssy);
     __syncthreads();
     if (active) {
@@ -589,14 +607,18 @@ __device__ __forceinline__ void cdef_tile(const CdefLaunch &a, int tile_id, uint
   }
 }
 
+/* Tiles of the per-frame kernel: CDEF_WAVES waves, 64 x 16 CDEF_WAVES
+ * luma samples each. */
+constexpr int CDEF_WAVES = 4;
+
 /* 5 waves per SIMD: 96 VGPRs, no spills (at 6 waves, 80 VGPRs, the
  * prefetched 4:2:0 chroma tiles spill and the kernel is 6 % slower per 4K
  * frame). */
 template <typename T>
-__global__ __launch_bounds__(256, 5) void cdef_kernel(CdefLaunch a) {
-  __shared__ uint32_t words[2 * SOFF];
+__global__ __launch_bounds__(64 * CDEF_WAVES, 5) void cdef_kernel(CdefLaunch a) {
+  __shared__ uint32_t words[2 * soff_rows(16 * CDEF_WAVES)];
   __shared__ CdefBlocks bl;
-  cdef_tile<T>(a, xcd_remap(blockIdx.x, gridDim.x), words, bl);
+  cdef_tile<T, CDEF_WAVES>(a, xcd_remap(blockIdx.x, gridDim.x), words, bl);
 }
 
 /* Batched. The persistent form is held at 5 waves per SIMD (inside the
@@ -607,7 +629,7 @@ template <typename T, bool DIRECT>
 __global__ __launch_bounds__(256, DIRECT ? 6 : 5) void cdef_batch(BatchWork w) {
   __shared__ uint32_t words[2 * SOFF];
   __shared__ CdefBlocks bl;
-  batch_loop<DIRECT>(w, [&](int f, int j) { cdef_tile<T>(batch_table<CdefLaunch>(w.rec, f), j, words, bl); });
+  batch_loop<DIRECT>(w, [&](int f, int j) { cdef_tile<T, 4>(batch_table<CdefLaunch>(w.rec, f), j, words, bl); });
 }
 
 /* ---- the encoder's strength search (src/encoder/cdef.rs:52-197) ---- */
@@ -695,7 +717,7 @@ __device__ void cdef_search_tile(const CdefSearchLaunch &a, int tile_id, uint32_
   const int xlim = g.mi_cols * 4, ylim = g.mi_rows * 4;
   const int x0 = fbx * 64, y0 = fby * 64;
   const int bw = min(64, xlim - x0), bh = min(64, ylim - y0);
-  stage_tile<T>(tile, shifted, pp<T>(a.in[0]), a.in[0].stride, x0, y0, 64, 64, xlim, ylim);
+  stage_tile<T, 4>(tile, shifted, pp<T>(a.in[0]), a.in[0].stride, x0, y0, 64, 64, xlim, ylim);
   __syncthreads();
 
   const int b = tid >> 2, q = tid & 3;
@@ -731,7 +753,7 @@ __device__ void cdef_search_tile(const CdefSearchLaunch &a, int tile_id, uint32_
   const int t0 = (by * 8 + 2) * TS + bx * 8 + 2;
   for (int k = 0; k < a.ny; k++) {
     const int pri0 = a.ypri[k] << cs, sec = a.ysec[k] << cs;
-    const Taps tk = make_taps(filt ? adjust_pri(pri0, var) : 0, sec, a.damping + cs, pri0 ? ydir : 0, cs);
+    const Taps tk = make_taps(filt ? adjust_pri(pri0, var) : 0, sec, a.damping + cs, pri0 ? ydir : 0, cs, SOFF);
     wave_add(&acc[k], (uint32_t)pairs_sse_any(pri0 != 0, sec != 0, words, t0, 2 * q, 0, 8, tk, sp, mask));
   }
 
@@ -748,7 +770,7 @@ __device__ void cdef_search_tile(const CdefSearchLaunch &a, int tile_id, uint32_
     const int ct0 = (by * bhs + 2) * TS + bx * bws + 2;
     for (int p = 1; p < 3; p++) {
       __syncthreads();
-      stage_tile<T>(tile, shifted, pp<T>(a.in[p]), a.in[p].stride, x0 >> ssx, y0 >> ssy, cw, ch, cxl, cyl);
+      stage_tile<T, 4>(tile, shifted, pp<T>(a.in[p]), a.in[p].stride, x0 >> ssx, y0 >> ssy, cw, ch, cxl, cyl);
       __syncthreads();
       const int px = (x0 >> ssx) + bx * bws + 2 * cpair, py = (y0 >> ssy) + by * bhs + r0;
       const T *src = pp<T>(a.src[p]);
@@ -760,7 +782,7 @@ __device__ void cdef_search_tile(const CdefSearchLaunch &a, int tile_id, uint32_
       }
       for (int k = 0; k < a.nuv; k++) {
         const int pri = a.uvpri[k] << cs, sec = a.uvsec[k] << cs;
-        const Taps tk = make_taps(pri, sec, a.damping + cs - 1, pri ? cdir : 0, cs);
+        const Taps tk = make_taps(pri, sec, a.damping + cs - 1, pri ? cdir : 0, cs, SOFF);
         wave_add(&acc[a.ny + k], (uint32_t)pairs_sse_any(pri != 0, sec != 0, words, ct0, 2 * cpair, r0, nrows, tk,
                                                          sp, mask));
       }
@@ -826,11 +848,11 @@ int launch_cdef(const Geom &g, const CdefHdr &h, const rvpf_mi *mi, int mi_strid
                 const int8_t *cdef_idx, int cdef_stride, int enabled, const PlanePtr in[3],
                 PlanePtr out[3], int bps, hipStream_t st) {
   CdefLaunch a;
-  const int n = plan(g, h, mi, mi_stride, cdef_idx, cdef_stride, enabled, in, out, a);
+  const int n = plan(g, h, mi, mi_stride, cdef_idx, cdef_stride, enabled, in, out, a) * (4 / CDEF_WAVES);
   if (bps == 1)
-    cdef_kernel<uint8_t><<<n, 256, 0, st>>>(a);
+    cdef_kernel<uint8_t><<<n, 64 * CDEF_WAVES, 0, st>>>(a);
   else
-    cdef_kernel<uint16_t><<<n, 256, 0, st>>>(a);
+    cdef_kernel<uint16_t><<<n, 64 * CDEF_WAVES, 0, st>>>(a);
   return (int)hipGetLastError();
 }
 

@@ -911,19 +911,29 @@ __device__ __forceinline__ void grain_apply_tile(const GrainHdr &h, const GrainS
     const int lx = lx0 + 64 * q;
     if (lon[q]) seg_noise(sc.tmpl, sc.offsets, mb, ln, lx >> 5, lx & 31, li, 0, 0, ov, gmin, gmax, lz[q]);
   }
-  /* luma out first: its samples and noise are dead before the chroma pass */
+  /* noised sample: orig + round2(scale * noise, shift), clipped; the
+   * scale is at most 255 and |noise| <= 2048, so the product is one 24-bit
+   * multiply-add */
+  const int sh = h.scaling_shift, rnd = (1 << sh) >> 1;
+  /* luma out first: its samples and noise are dead before the chroma pass.
+   * Every branch below is on frame parameters (uniform), outside the
+   * per-sample loops, so the 8 scaling reads of a segment go out together. */
+  const bool noise_y = h.num_y > 0;
 #pragma unroll
   for (int q = 0; q < LS; q++) {
     if (!lon[q]) continue;
     const int lx = lx0 + 64 * q;
     int v[8];
+    if (noise_y) {
+      int sv[8];
 #pragma unroll
-    for (int k = 0; k < 8; k++) {
-      v[k] = lorig[q][k];
-      if (h.num_y > 0) {
-        const int sn = round2(lut[lorig[q][k]] * lz[q][k], h.scaling_shift);
-        v[k] = clip3(h.min_value, h.max_luma, lorig[q][k] + sn);
-      }
+      for (int k = 0; k < 8; k++) sv[k] = lut[lorig[q][k]];
+#pragma unroll
+      for (int k = 0; k < 8; k++)
+        v[k] = med3(h.min_value, h.max_luma, lorig[q][k] + ((__mul24(sv[k], lz[q][k]) + rnd) >> sh));
+    } else {
+#pragma unroll
+      for (int k = 0; k < 8; k++) v[k] = lorig[q][k];
     }
     Seg8<TO>::store(gptr(reinterpret_cast<TO *>(o0.data)) + (int64_t)ly * o0.stride + lx, h.w - lx, v);
   }
@@ -931,7 +941,10 @@ __device__ __forceinline__ void grain_apply_tile(const GrainHdr &h, const GrainS
   const bool chroma = h.nplanes > 1;
   const int ssx = chroma ? h.ssx : 0, ssy = chroma ? h.ssy : 0;
   const int cw = (h.w + ssx) >> ssx, ch = (h.h + ssy) >> ssy;
-  const int segs = (8 * LS) >> ssx, per = segs * (32 >> ssy);
+  /* segments per chroma row of the tile (4, 8 or 16) and per plane
+   * (segs * (32 >> ssy) >= 64, a power of two: a wave's items all lie in
+   * one plane, so the plane and its parameters are uniform) */
+  const int lsegs = 3 + (LS == 2) - ssx, segs = 1 << lsegs, per = segs * (32 >> ssy);
   const int maxv = (1 << bd) - 1;
   for (int pass = 0; chroma && pass * 256 * CI < 2 * per; pass++) {
     int cp[CI], cx[CI], cy[CI], ci[CI], corig[CI][8], cavg[CI][8], cz[CI][8];
@@ -939,10 +952,10 @@ __device__ __forceinline__ void grain_apply_tile(const GrainHdr &h, const GrainS
 #pragma unroll
     for (int u = 0; u < CI; u++) {
       const int it = threadIdx.x + 256 * (u + CI * pass);
-      cp[u] = 1 + (it >= per);
+      cp[u] = __builtin_amdgcn_readfirstlane(1 + (it >= per));
       const int r = it - (cp[u] - 1) * per;
-      ci[u] = r / segs;
-      cx[u] = (x0 >> ssx) + (r - ci[u] * segs) * 8;
+      ci[u] = r >> lsegs;
+      cx[u] = (x0 >> ssx) + ((r & (segs - 1)) << 3);
       cy[u] = (y0 >> ssy) + ci[u];
       con[u] = it < 2 * per && cx[u] < cw && cy[u] < ch;
       if (con[u]) {
@@ -955,16 +968,21 @@ __device__ __forceinline__ void grain_apply_tile(const GrainHdr &h, const GrainS
           const int nl = h.w - (cx[u] << 1);
           Seg8<T>::load(lrow, nl, l);
           Seg8<T>::load(lrow + 8, nl - 8, l + 8);
+          if (nl >= 16) {
 #pragma unroll
-          for (int k = 0; k < 8; k++) {
-            /* the right neighbour of the last luma column is itself */
-            const int b = 2 * k + 1 < nl ? l[2 * k + 1] : l[2 * k];
-            cavg[u][k] = round2(l[2 * k] + b, 1);
+            for (int k = 0; k < 8; k++) cavg[u][k] = (l[2 * k] + l[2 * k + 1] + 1) >> 1;
+          } else {
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+              /* the right neighbour of the last luma column is itself */
+              const int b = 2 * k + 1 < nl ? l[2 * k + 1] : l[2 * k];
+              cavg[u][k] = (l[2 * k] + b + 1) >> 1;
+            }
           }
         } else {
           Seg8<T>::load(lrow, n, cavg[u]);
         }
-        seg_noise(sc.tmpl + (size_t)cp[u] * GRAIN_TPLANE, sc.offsets, mb, ln, cx[u] / (32 >> ssx),
+        seg_noise(sc.tmpl + (size_t)cp[u] * GRAIN_TPLANE, sc.offsets, mb, ln, cx[u] >> (5 - ssx),
                   cx[u] & ((32 >> ssx) - 1), ci[u], ssx, ssy, ov, gmin, gmax, cz[u]);
       }
     }
@@ -978,20 +996,27 @@ __device__ __forceinline__ void grain_apply_tile(const GrainHdr &h, const GrainS
       const int mu = p == 1 ? h.cb_mult : h.cr_mult;
       const int of = p == 1 ? h.cb_offset : h.cr_offset;
       int v[8];
+      if (on) {
+        int m[8], sv[8];
+        if (h.cfl) {
 #pragma unroll
-      for (int k = 0; k < 8; k++) {
-        v[k] = corig[u][k];
-        if (on) {
-          int merged;
-          if (h.cfl) {
-            merged = cavg[u][k];
-          } else {
-            const int combined = cavg[u][k] * (lm - 128) + corig[u][k] * (mu - 128);
-            merged = clip3(0, maxv, (combined >> 6) + (of - 256) * (1 << (bd - 8)));
-          }
-          const int sn = round2(lut[(p << bd) + merged] * cz[u][k], h.scaling_shift);
-          v[k] = clip3(h.min_value, h.max_chroma, corig[u][k] + sn);
+          for (int k = 0; k < 8; k++) m[k] = cavg[u][k];
+        } else {
+          /* |weights| <= 128, samples <= 4095: 24-bit products */
+          const int wl = lm - 128, wc = mu - 128, off = (of - 256) * (1 << (bd - 8));
+#pragma unroll
+          for (int k = 0; k < 8; k++)
+            m[k] = med3(0, maxv, ((__mul24(cavg[u][k], wl) + __mul24(corig[u][k], wc)) >> 6) + off);
         }
+        const int16_t *pl = lut + (p << bd);
+#pragma unroll
+        for (int k = 0; k < 8; k++) sv[k] = pl[m[k]];
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+          v[k] = med3(h.min_value, h.max_chroma, corig[u][k] + ((__mul24(sv[k], cz[u][k]) + rnd) >> sh));
+      } else {
+#pragma unroll
+        for (int k = 0; k < 8; k++) v[k] = corig[u][k];
       }
       const PlanePtr &op = p == 1 ? o1 : o2;
       Seg8<TO>::store(gptr(reinterpret_cast<TO *>(op.data)) + (int64_t)cy[u] * op.stride + cx[u], cw - cx[u], v);
