@@ -758,8 +758,9 @@ __device__ __forceinline__ void load_luts(const GrainHdr &h, const GrainScratch 
  * overlaps of 7.18.3.5): weights 23/22 for a one-sample overlap, 27/17
  * and 17/27 for the two samples of a two-sample overlap. */
 __device__ __forceinline__ int overlap_blend(int old, int gv, bool one, int k, int gmin, int gmax) {
-  const int v = one ? old * 23 + gv * 22 : (k == 0 ? old * 27 + gv * 17 : old * 17 + gv * 27);
-  return clip3(gmin, gmax, round2(v, 5));
+  /* |noise| <= 2048: 24-bit products */
+  const int wo = one ? 23 : (k == 0 ? 27 : 17), wn = one ? 22 : (k == 0 ? 17 : 27);
+  return clip3(gmin, gmax, round2(__mul24(old, wo) + __mul24(gv, wn), 5));
 }
 
 /* Template sample (y, x) of a block whose offset byte is o: the block's
@@ -868,19 +869,291 @@ struct Seg8<uint8_t> {
   }
 };
 
+/* 8 consecutive samples of a plane row as loaded, unpacked later so that
+ * every load of a tile is in flight before the first is used: one 16-byte
+ * (u16) or 8-byte (u8) access when the whole segment lies inside the row,
+ * else the n samples present and zeros. */
+template <typename T>
+struct Raw8;
+template <>
+struct Raw8<uint16_t> {
+  uint4 v;
+  __device__ __forceinline__ void load(const uint16_t *p, int n) {
+    if (n >= 8) {
+      v = *reinterpret_cast<const uint4 *>(p);
+    } else {
+      uint32_t w[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+      for (int k = 0; k < 8; k++)
+        if (k < n) w[k >> 1] |= (uint32_t)p[k] << (16 * (k & 1));
+      v = make_uint4(w[0], w[1], w[2], w[3]);
+    }
+  }
+  __device__ __forceinline__ void unpack(int *o) const {
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      o[2 * k] = w[k] & 0xffff;
+      o[2 * k + 1] = w[k] >> 16;
+    }
+  }
+};
+template <>
+struct Raw8<uint8_t> {
+  uint2 v;
+  __device__ __forceinline__ void load(const uint8_t *p, int n) {
+    if (n >= 8) {
+      v = *reinterpret_cast<const uint2 *>(p);
+    } else {
+      uint32_t w[2] = {0u, 0u};
+#pragma unroll
+      for (int k = 0; k < 8; k++)
+        if (k < n) w[k >> 2] |= (uint32_t)p[k] << (8 * (k & 3));
+      v = make_uint2(w[0], w[1]);
+    }
+  }
+  __device__ __forceinline__ void unpack(int *o) const {
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      o[k] = (v.x >> (8 * k)) & 0xff;
+      o[4 + k] = (v.y >> (8 * k)) & 0xff;
+    }
+  }
+};
+
+/* The block offsets a segment's noise reads (grain.rs:186-199 draws them),
+ * one byte each in one word: its own block's (o), and in the overlaps
+ * those of the block to the left (l), above (u) and above-left (ul). */
+struct SegOffs {
+  uint32_t w;
+  __device__ __forceinline__ int o() const { return w & 0xff; }
+  __device__ __forceinline__ int l() const { return (w >> 8) & 0xff; }
+  __device__ __forceinline__ int u() const { return (w >> 16) & 0xff; }
+  __device__ __forceinline__ int ul() const { return w >> 24; }
+};
+
+__device__ __forceinline__ SegOffs seg_offs(const uint8_t *offs, int mb, int ln, int bx, bool hov, bool vov) {
+  const uint8_t *row = offs + ln * mb + bx;
+  const uint32_t o = row[0], l = hov ? row[-1] : 0u, u = vov ? row[-mb] : 0u, ul = hov && vov ? row[-mb - 1] : 0u;
+  return SegOffs{o | (l << 8) | (u << 16) | (ul << 24)};
+}
+
+/* Template sample (y, x) .. (y, x + 7) of plane template pt: the copy
+ * shifted by x & 7 makes them one aligned 16-byte word. */
+__device__ __forceinline__ const int16_t *tmpl_at(const int16_t *pt, int y, int x) {
+  return pt + (x & 7) * GRAIN_TCOPY + y * GRAIN_TWS + (x & ~7);
+}
+
+/* The template words of a segment (block row i, block columns j0 ..
+ * j0 + 7): its own 8 samples and, in the horizontal overlap, the left
+ * block's samples just past its width (the two columns it blends with). */
+struct SegTaps {
+  uint4 m;
+  uint32_t l;
+};
+
+__device__ __forceinline__ SegTaps seg_taps(const int16_t *pt, const SegOffs &s, int i, int j0, int psx, int psy,
+                                            bool hov) {
+  SegTaps t;
+  t.m = *reinterpret_cast<const uint4 *>(tmpl_at(pt, origin_y(s.o(), psy) + i, origin_x(s.o(), psx) + j0));
+  t.l = hov ? *reinterpret_cast<const uint32_t *>(
+                  tmpl_at(pt, origin_y(s.l(), psy) + i, origin_x(s.l(), psx) + (32 >> psx)))
+            : 0u;
+  return t;
+}
+
+__device__ __forceinline__ void unpack_s16(const uint4 &v, int *g) {
+  const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    g[2 * k] = (int)(int16_t)(w[k] & 0xffff);
+    g[2 * k + 1] = (int)w[k] >> 16;
+  }
+}
+
+__device__ __forceinline__ int lo16(uint32_t w) { return (int)(int16_t)(w & 0xffff); }
+__device__ __forceinline__ int hi16(uint32_t w) { return (int)w >> 16; }
+
+/* The horizontal overlap (7.18.3.5) of the first word of a row segment
+ * (its first two samples, or one when subsampled) with the two samples
+ * of the left block in word l. */
+__device__ __forceinline__ uint32_t hblend_word(uint32_t w, uint32_t l, int psx, int gmin, int gmax) {
+  const int b0 = overlap_blend(lo16(l), lo16(w), psx, 0, gmin, gmax);
+  const int b1 = psx ? hi16(w) : overlap_blend(hi16(l), hi16(w), psx, 1, gmin, gmax);
+  return pack16(b0, b1);
+}
+
+/* Noise of a segment from its template words (seg_noise's blends,
+ * 7.18.3.5), as 4 words of two samples: the horizontal overlap with the
+ * left block, then, in the first rows of a stripe, the vertical one with
+ * the stripe above (its words are read here: only the lanes of those rows
+ * get there). The words stay packed, so the blends hold few registers. */
+__device__ __forceinline__ uint4 seg_blend(const int16_t *pt, const SegOffs &s, const SegTaps &t, int i, int j0,
+                                           int psx, int psy, bool hov, bool vov, int gmin, int gmax) {
+  const int bs = 32 >> psx, bsy = 32 >> psy;
+  uint32_t g[4] = {t.m.x, t.m.y, t.m.z, t.m.w};
+  if (hov) g[0] = hblend_word(g[0], t.l, psx, gmin, gmax);
+  if (vov) {
+    const uint4 uv =
+        *reinterpret_cast<const uint4 *>(tmpl_at(pt, origin_y(s.u(), psy) + i + bsy, origin_x(s.u(), psx) + j0));
+    uint32_t u[4] = {uv.x, uv.y, uv.z, uv.w};
+    if (hov)
+      u[0] = hblend_word(
+          u[0],
+          *reinterpret_cast<const uint32_t *>(tmpl_at(pt, origin_y(s.ul(), psy) + i + bsy, origin_x(s.ul(), psx) + bs)),
+          psx, gmin, gmax);
+#pragma unroll
+    for (int w = 0; w < 4; w++)
+      g[w] = pack16(overlap_blend(lo16(u[w]), lo16(g[w]), psy, i, gmin, gmax),
+                    overlap_blend(hi16(u[w]), hi16(g[w]), psy, i, gmin, gmax));
+  }
+  return make_uint4(g[0], g[1], g[2], g[3]);
+}
+
+/* A chroma segment of the noise-applying stage through its three rounds:
+ * chroma_load (its samples, the luma it averages, its block offsets),
+ * chroma_index once they have landed (the packed index into the scaling
+ * function, 7.18.3.5, so the luma is dead before the next round),
+ * chroma_taps (its template words) and chroma_out. */
+template <typename T>
+struct ChromaSeg {
+  int p, x, y, i; /* plane, column, row, row in the noise stripe */
+  bool on, hov, vov, noise;
+  Raw8<T> raw, l0, l1;
+  SegOffs so;
+  uint32_t m[4];
+  SegTaps t;
+};
+
+/* Item it of the tile (columns x0 .. of noise stripe ln) of an LS tile:
+ * per items per plane, segs = 1 << lsegs of them per row. per is a power
+ * of two >= 64, so a wave's items all lie in one plane and the plane and
+ * its parameters are uniform. */
+template <typename T>
+__device__ __forceinline__ void chroma_load(ChromaSeg<T> &c, const GrainHdr &h, const GrainScratch &sc,
+                                            const PlanePtr &i0, const PlanePtr &i1, const PlanePtr &i2, int it,
+                                            int x0, int y0, int ln, int lsegs, int per) {
+  const int ssx = h.ssx, ssy = h.ssy;
+  const int cw = (h.w + ssx) >> ssx, ch = (h.h + ssy) >> ssy, cbs = 32 >> ssx;
+  c.p = __builtin_amdgcn_readfirstlane(1 + (it >= per));
+  const int r = it - (c.p - 1) * per;
+  c.i = r >> lsegs;
+  c.x = (x0 >> ssx) + ((r & ((1 << lsegs) - 1)) << 3);
+  c.y = (y0 >> ssy) + c.i;
+  c.on = it < 2 * per && c.x < cw && c.y < ch;
+  c.hov = h.overlap && c.x >= cbs && (c.x & (cbs - 1)) == 0;
+  c.vov = h.overlap && ln > 0 && c.i < (ssy ? 1 : 2);
+  c.noise = (c.p == 1 ? h.num_cb : h.num_cr) > 0 || h.cfl;
+  if (!c.on) return;
+  const PlanePtr &ip = c.p == 1 ? i1 : i2;
+  c.raw.load(gptr(reinterpret_cast<const T *>(ip.data)) + (int64_t)c.y * ip.stride + c.x, cw - c.x);
+  if (!c.noise) return;
+  const T *lrow = gptr(reinterpret_cast<const T *>(i0.data)) + (int64_t)(c.y << ssy) * i0.stride + (c.x << ssx);
+  const int nl = h.w - (c.x << ssx);
+  c.l0.load(lrow, nl);
+  if (ssx) c.l1.load(lrow + 8, nl - 8);
+  c.so = seg_offs(sc.offsets, sc.max_blocks, ln, c.x >> (5 - ssx), c.hov, c.vov);
+}
+
+template <typename T>
+__device__ __forceinline__ void chroma_index(ChromaSeg<T> &c, const GrainHdr &h) {
+  if (!c.on || !c.noise) return;
+  const int ssx = h.ssx, bd = h.bd, maxv = (1 << bd) - 1;
+  int orig[8], avg[8], m[8];
+  c.raw.unpack(orig);
+  if (ssx) {
+    int l[16];
+    c.l0.unpack(l);
+    c.l1.unpack(l + 8);
+    const int nl = h.w - (c.x << 1);
+    if (nl >= 16) {
+#pragma unroll
+      for (int k = 0; k < 8; k++) avg[k] = (l[2 * k] + l[2 * k + 1] + 1) >> 1;
+    } else {
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        /* the right neighbour of the last luma column is itself */
+        const int b = 2 * k + 1 < nl ? l[2 * k + 1] : l[2 * k];
+        avg[k] = (l[2 * k] + b + 1) >> 1;
+      }
+    }
+  } else {
+    c.l0.unpack(avg);
+  }
+  if (h.cfl) {
+#pragma unroll
+    for (int k = 0; k < 8; k++) m[k] = avg[k];
+  } else {
+    /* |weights| <= 128, samples <= 4095: 24-bit products */
+    const int wl = (c.p == 1 ? h.cb_luma_mult : h.cr_luma_mult) - 128;
+    const int wc = (c.p == 1 ? h.cb_mult : h.cr_mult) - 128;
+    const int off = ((c.p == 1 ? h.cb_offset : h.cr_offset) - 256) * (1 << (bd - 8));
+#pragma unroll
+    for (int k = 0; k < 8; k++) m[k] = med3(0, maxv, ((__mul24(avg[k], wl) + __mul24(orig[k], wc)) >> 6) + off);
+  }
+#pragma unroll
+  for (int k = 0; k < 4; k++) c.m[k] = (uint32_t)m[2 * k] | ((uint32_t)m[2 * k + 1] << 16);
+}
+
+template <typename T>
+__device__ __forceinline__ void chroma_taps(ChromaSeg<T> &c, const GrainHdr &h, const GrainScratch &sc) {
+  if (c.on && c.noise)
+    c.t = seg_taps(sc.tmpl + (size_t)c.p * GRAIN_TPLANE, c.so, c.i, c.x & ((32 >> h.ssx) - 1), h.ssx, h.ssy, c.hov);
+}
+
+/* noised sample: orig + round2(scale * noise, shift), clipped; the scale
+ * is at most 255 and |noise| <= 2048, so the product is one 24-bit
+ * multiply-add; the 8 scaling reads go out together */
+__device__ __forceinline__ void add_noise8(const int16_t *pl, const int *idx, const int *orig, const int *z, int rnd,
+                                           int sh, int lo, int hi, int *v) {
+  int sv[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) sv[k] = pl[idx[k]];
+#pragma unroll
+  for (int k = 0; k < 8; k++) v[k] = med3(lo, hi, orig[k] + ((__mul24(sv[k], z[k]) + rnd) >> sh));
+}
+
+template <typename T, typename TO>
+__device__ __forceinline__ void chroma_out(const ChromaSeg<T> &c, const GrainHdr &h, const GrainScratch &sc,
+                                           const PlanePtr &o1, const PlanePtr &o2, int16_t *lut) {
+  if (!c.on) return;
+  const int ssx = h.ssx, ssy = h.ssy, bd = h.bd;
+  const int cw = (h.w + ssx) >> ssx;
+  int orig[8], v[8];
+  c.raw.unpack(orig);
+  if (c.noise) {
+    const int gmin = -(128 << (bd - 8)), gmax = (256 << (bd - 8)) - 1 - (128 << (bd - 8));
+    int z[8], idx[8];
+    unpack_s16(seg_blend(sc.tmpl + (size_t)c.p * GRAIN_TPLANE, c.so, c.t, c.i, c.x & ((32 >> ssx) - 1), ssx, ssy,
+                         c.hov, c.vov, gmin, gmax),
+               z);
+#pragma unroll
+    for (int k = 0; k < 8; k++) idx[k] = (c.m[k >> 1] >> (16 * (k & 1))) & 0xffff;
+    const int sh = h.scaling_shift;
+    add_noise8(lut + (c.p << bd), idx, orig, z, (1 << sh) >> 1, sh, h.min_value, h.max_chroma, v);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 8; k++) v[k] = orig[k];
+  }
+  const PlanePtr &op = c.p == 1 ? o1 : o2;
+  Seg8<TO>::store(gptr(reinterpret_cast<TO *>(op.data)) + (int64_t)c.y * op.stride + c.x, cw - c.x, v);
+}
+
 /* One tile of the noise-applying stage: noise stripe ln (32 luma rows) and
  * luma columns [x0, x0 + 64 LS), with the chroma samples they cover. A
  * thread takes LS 8-sample row segments of luma, 64 columns apart (8 per
- * row and 64 columns, 32 rows), and CI chroma segments per pass (one pass,
- * or two for 4:4:4 with CI = 1). LS = 2 for 4:2:0 gives every thread one
- * chroma segment (with LS = 1 only half of them would have one, and their
- * waves would carry all of the chroma work);
- * every segment lies in one 32-column noise block, so its noise is 8
- * consecutive template samples. The luma plane load is issued before the
- * scaling tables are staged. The planes' rows must be 16-byte (u16) or
- * 8-byte (u8) aligned. Chroma reads the un-noised luma of the input, as
- * the reference blends chroma first. */
-template <typename T, typename TO, int CI, int LS>
+ * row and 64 columns, 32 rows), and CI chroma segments per pass (items
+ * threadIdx.x + 256 (u + CI pass) of the 2 per items of both planes:
+ * one pass, or two for 4:4:4 with CI = 1). LS = 2 for 4:2:0 gives every
+ * thread one chroma segment. Every segment lies in one 32-column noise
+ * block, so its noise is 8 consecutive template samples. The first
+ * pass's accesses go out in three rounds, each before anything waits on
+ * it: the plane samples and block offsets (with the scaling tables' copy
+ * into LDS), then the template words the offsets address, then, in the
+ * first rows of a stripe only, the stripe above's. The planes' rows must
+ * be 16-byte (u16) or 8-byte (u8) aligned. Chroma reads the un-noised luma
+ * of the input, as the reference blends chroma first. */
+template <typename T, typename TO, int CI, int LS, bool TWO_PASS = false>
 __device__ __forceinline__ void grain_apply_tile(const GrainHdr &h, const GrainScratch &sc, const PlanePtr &i0,
                                                  const PlanePtr &i1, const PlanePtr &i2, const PlanePtr &o0,
                                                  const PlanePtr &o1, const PlanePtr &o2, int tile_id, int16_t *lut) {
@@ -890,136 +1163,80 @@ __device__ __forceinline__ void grain_apply_tile(const GrainHdr &h, const GrainS
   const int bd = h.bd;
   const int gmin = -(128 << (bd - 8)), gmax = (256 << (bd - 8)) - 1 - (128 << (bd - 8));
   const bool ov = h.overlap;
-  const int mb = sc.max_blocks;
   const T *lin = gptr(reinterpret_cast<const T *>(i0.data));
   const int64_t ls = i0.stride;
-  /* luma segments of this thread */
+
+  /* round 1 */
   const int lx0 = x0 + (threadIdx.x & 7) * 8, li = threadIdx.x >> 3, ly = y0 + li;
-  bool lon[LS];
-  int lorig[LS][8];
+  const bool lvov = ov && ln > 0 && li < 2;
+  const bool noise_y = h.num_y > 0;
+  bool lon[LS], lhov[LS];
+  Raw8<T> lraw[LS];
+  SegOffs lso[LS];
 #pragma unroll
   for (int q = 0; q < LS; q++) {
     const int lx = lx0 + 64 * q;
     lon[q] = lx < h.w && ly < h.h;
-    if (lon[q]) Seg8<T>::load(lin + (int64_t)ly * ls + lx, h.w - lx, lorig[q]);
+    lhov[q] = ov && lx >= 32 && (lx & 31) == 0;
+    if (lon[q]) {
+      lraw[q].load(lin + (int64_t)ly * ls + lx, h.w - lx);
+      if (noise_y) lso[q] = seg_offs(sc.offsets, sc.max_blocks, ln, lx >> 5, lhov[q], lvov);
+    }
+  }
+  const bool chroma = h.nplanes > 1;
+  /* chroma segments per tile row (4, 8 or 16) and per plane */
+  const int lsegs = chroma ? 3 + (LS == 2) - h.ssx : 0, per = (1 << lsegs) * (32 >> (chroma ? h.ssy : 0));
+  ChromaSeg<T> cs[CI];
+#pragma unroll
+  for (int u = 0; u < CI; u++) {
+    cs[u].on = false;
+    if (chroma) chroma_load(cs[u], h, sc, i0, i1, i2, threadIdx.x + 256 * u, x0, y0, ln, lsegs, per);
   }
   load_luts(h, sc, lut);
-  __syncthreads();
-  int lz[LS][8];
+  __syncthreads(); /* round 1 has landed */
 #pragma unroll
-  for (int q = 0; q < LS; q++) {
-    const int lx = lx0 + 64 * q;
-    if (lon[q]) seg_noise(sc.tmpl, sc.offsets, mb, ln, lx >> 5, lx & 31, li, 0, 0, ov, gmin, gmax, lz[q]);
-  }
-  /* noised sample: orig + round2(scale * noise, shift), clipped; the
-   * scale is at most 255 and |noise| <= 2048, so the product is one 24-bit
-   * multiply-add */
+  for (int u = 0; u < CI; u++) chroma_index(cs[u], h);
+
+  /* round 2 */
+  SegTaps lt[LS];
+#pragma unroll
+  for (int q = 0; q < LS; q++)
+    if (lon[q] && noise_y) lt[q] = seg_taps(sc.tmpl, lso[q], li, (lx0 + 64 * q) & 31, 0, 0, lhov[q]);
+#pragma unroll
+  for (int u = 0; u < CI; u++) chroma_taps(cs[u], h, sc);
+
+  /* luma out first: its samples and noise are dead before the chroma */
   const int sh = h.scaling_shift, rnd = (1 << sh) >> 1;
-  /* luma out first: its samples and noise are dead before the chroma pass.
-   * Every branch below is on frame parameters (uniform), outside the
-   * per-sample loops, so the 8 scaling reads of a segment go out together. */
-  const bool noise_y = h.num_y > 0;
 #pragma unroll
   for (int q = 0; q < LS; q++) {
     if (!lon[q]) continue;
     const int lx = lx0 + 64 * q;
-    int v[8];
+    int orig[8], v[8];
+    lraw[q].unpack(orig);
     if (noise_y) {
-      int sv[8];
-#pragma unroll
-      for (int k = 0; k < 8; k++) sv[k] = lut[lorig[q][k]];
-#pragma unroll
-      for (int k = 0; k < 8; k++)
-        v[k] = med3(h.min_value, h.max_luma, lorig[q][k] + ((__mul24(sv[k], lz[q][k]) + rnd) >> sh));
+      int z[8];
+      unpack_s16(seg_blend(sc.tmpl, lso[q], lt[q], li, lx & 31, 0, 0, lhov[q], lvov, gmin, gmax), z);
+      add_noise8(lut, orig, orig, z, rnd, sh, h.min_value, h.max_luma, v);
     } else {
 #pragma unroll
-      for (int k = 0; k < 8; k++) v[k] = lorig[q][k];
+      for (int k = 0; k < 8; k++) v[k] = orig[k];
     }
     Seg8<TO>::store(gptr(reinterpret_cast<TO *>(o0.data)) + (int64_t)ly * o0.stride + lx, h.w - lx, v);
   }
-  /* chroma segments: items threadIdx.x + 256 (u + CI pass), u < CI */
-  const bool chroma = h.nplanes > 1;
-  const int ssx = chroma ? h.ssx : 0, ssy = chroma ? h.ssy : 0;
-  const int cw = (h.w + ssx) >> ssx, ch = (h.h + ssy) >> ssy;
-  /* segments per chroma row of the tile (4, 8 or 16) and per plane
-   * (segs * (32 >> ssy) >= 64, a power of two: a wave's items all lie in
-   * one plane, so the plane and its parameters are uniform) */
-  const int lsegs = 3 + (LS == 2) - ssx, segs = 1 << lsegs, per = segs * (32 >> ssy);
-  const int maxv = (1 << bd) - 1;
-  for (int pass = 0; chroma && pass * 256 * CI < 2 * per; pass++) {
-    int cp[CI], cx[CI], cy[CI], ci[CI], corig[CI][8], cavg[CI][8], cz[CI][8];
-    bool con[CI];
+#pragma unroll
+  for (int u = 0; u < CI; u++) chroma_out<T, TO>(cs[u], h, sc, o1, o2, lut);
+
+  /* a second pass (TWO_PASS: 4:4:4 with CI = 1, which only the batched
+   * stage runs; every other layout has at most 256 CI items): the same
+   * rounds, one after another */
+  if constexpr (TWO_PASS)
+  for (int it0 = 256 * CI; chroma && it0 < 2 * per; it0 += 256 * CI) {
 #pragma unroll
     for (int u = 0; u < CI; u++) {
-      const int it = threadIdx.x + 256 * (u + CI * pass);
-      cp[u] = __builtin_amdgcn_readfirstlane(1 + (it >= per));
-      const int r = it - (cp[u] - 1) * per;
-      ci[u] = r >> lsegs;
-      cx[u] = (x0 >> ssx) + ((r & (segs - 1)) << 3);
-      cy[u] = (y0 >> ssy) + ci[u];
-      con[u] = it < 2 * per && cx[u] < cw && cy[u] < ch;
-      if (con[u]) {
-        const int n = cw - cx[u];
-        const PlanePtr &ip = cp[u] == 1 ? i1 : i2;
-        Seg8<T>::load(gptr(reinterpret_cast<const T *>(ip.data)) + (int64_t)cy[u] * ip.stride + cx[u], n, corig[u]);
-        const T *lrow = lin + (int64_t)(cy[u] << ssy) * ls + (cx[u] << ssx);
-        if (ssx) {
-          int l[16];
-          const int nl = h.w - (cx[u] << 1);
-          Seg8<T>::load(lrow, nl, l);
-          Seg8<T>::load(lrow + 8, nl - 8, l + 8);
-          if (nl >= 16) {
-#pragma unroll
-            for (int k = 0; k < 8; k++) cavg[u][k] = (l[2 * k] + l[2 * k + 1] + 1) >> 1;
-          } else {
-#pragma unroll
-            for (int k = 0; k < 8; k++) {
-              /* the right neighbour of the last luma column is itself */
-              const int b = 2 * k + 1 < nl ? l[2 * k + 1] : l[2 * k];
-              cavg[u][k] = (l[2 * k] + b + 1) >> 1;
-            }
-          }
-        } else {
-          Seg8<T>::load(lrow, n, cavg[u]);
-        }
-        seg_noise(sc.tmpl + (size_t)cp[u] * GRAIN_TPLANE, sc.offsets, mb, ln, cx[u] >> (5 - ssx),
-                  cx[u] & ((32 >> ssx) - 1), ci[u], ssx, ssy, ov, gmin, gmax, cz[u]);
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < CI; u++) {
-      if (!con[u]) continue;
-      const int p = cp[u];
-      const int npts = p == 1 ? h.num_cb : h.num_cr;
-      const bool on = npts > 0 || h.cfl;
-      const int lm = p == 1 ? h.cb_luma_mult : h.cr_luma_mult;
-      const int mu = p == 1 ? h.cb_mult : h.cr_mult;
-      const int of = p == 1 ? h.cb_offset : h.cr_offset;
-      int v[8];
-      if (on) {
-        int m[8], sv[8];
-        if (h.cfl) {
-#pragma unroll
-          for (int k = 0; k < 8; k++) m[k] = cavg[u][k];
-        } else {
-          /* |weights| <= 128, samples <= 4095: 24-bit products */
-          const int wl = lm - 128, wc = mu - 128, off = (of - 256) * (1 << (bd - 8));
-#pragma unroll
-          for (int k = 0; k < 8; k++)
-            m[k] = med3(0, maxv, ((__mul24(cavg[u][k], wl) + __mul24(corig[u][k], wc)) >> 6) + off);
-        }
-        const int16_t *pl = lut + (p << bd);
-#pragma unroll
-        for (int k = 0; k < 8; k++) sv[k] = pl[m[k]];
-#pragma unroll
-        for (int k = 0; k < 8; k++)
-          v[k] = med3(h.min_value, h.max_chroma, corig[u][k] + ((__mul24(sv[k], cz[u][k]) + rnd) >> sh));
-      } else {
-#pragma unroll
-        for (int k = 0; k < 8; k++) v[k] = corig[u][k];
-      }
-      const PlanePtr &op = p == 1 ? o1 : o2;
-      Seg8<TO>::store(gptr(reinterpret_cast<TO *>(op.data)) + (int64_t)cy[u] * op.stride + cx[u], cw - cx[u], v);
+      chroma_load(cs[u], h, sc, i0, i1, i2, it0 + threadIdx.x + 256 * u, x0, y0, ln, lsegs, per);
+      chroma_index(cs[u], h);
+      chroma_taps(cs[u], h, sc);
+      chroma_out<T, TO>(cs[u], h, sc, o1, o2, lut);
     }
   }
 }
@@ -1165,7 +1382,7 @@ __global__ __launch_bounds__(256, 5) void grain_apply_batch(BatchWork w) {
     if (r.kind == 3)
       grain_apply_tile<T, T, 1, 2>(r.h, r.s, r.in[0], r.in[1], r.in[2], r.out[0], r.out[1], r.out[2], j, lut);
     else if (r.kind != 0)
-      grain_apply_tile<T, T, 1, 1>(r.h, r.s, r.in[0], r.in[1], r.in[2], r.out[0], r.out[1], r.out[2], j, lut);
+      grain_apply_tile<T, T, 1, 1, true>(r.h, r.s, r.in[0], r.in[1], r.in[2], r.out[0], r.out[1], r.out[2], j, lut);
     else
       grain_apply_tile_unaligned<T, T>(r.h, r.s, r.in[0], r.in[1], r.in[2], r.out[0], r.out[1], r.out[2], j, lut);
   });
