@@ -53,6 +53,7 @@ struct CdefLaunch {
   PlanePtr in[3], out[3];
   int enabled;
   int fb_cols;
+  int vec; /* every input row is 16-byte (u16) or 8-byte (u8) aligned: chunked staging */
 };
 
 template <typename T>
@@ -103,6 +104,102 @@ __device__ void stage_tile(uint16_t *t, uint16_t *sh, const T *src, int64_t stri
         if (tx == 0) sh[r * TS + tw - 1] = (uint16_t)NA_S;
       }
     }
+  }
+}
+
+/* Chunked staging, for planes whose rows are 16-byte (u16) or 8-byte (u8)
+ * aligned. Item (r, k) of a tile with NCH chunks per row covers the 8
+ * samples x0 - 8 + 8k .. x0 - 1 + 8k of tile row r: P words 4k - 3 .. 4k
+ * (tile columns 2m, 2m + 1 for word m), loaded as one access, and the
+ * next chunk's first word, from which the S words (S[m] = samples 2m + 1,
+ * 2m + 2) are aligned shifts. Loads read clamped addresses inside
+ * [0, xlim) (xlim >= 8) and unavailable words become NA_S pairs. x0 is a
+ * multiple of 8 and xlim of 4 (8 for luma): a chunk lies wholly inside,
+ * wholly outside, or (chroma) its first half inside, read from the access
+ * at xlim - 8. */
+constexpr uint32_t NA_W = (uint32_t)NA_S * 0x10001u;
+
+template <typename T>
+__device__ __forceinline__ uint4 load_chunk(const T *p);
+template <>
+__device__ __forceinline__ uint4 load_chunk<uint16_t>(const uint16_t *p) {
+  return *reinterpret_cast<const uint4 *>(p);
+}
+template <>
+__device__ __forceinline__ uint4 load_chunk<uint8_t>(const uint8_t *p) {
+  const uint2 v = *reinterpret_cast<const uint2 *>(p);
+  return make_uint4(__builtin_amdgcn_perm(0u, v.x, 0x0c010c00u), __builtin_amdgcn_perm(0u, v.x, 0x0c030c02u),
+                    __builtin_amdgcn_perm(0u, v.y, 0x0c010c00u), __builtin_amdgcn_perm(0u, v.y, 0x0c030c02u));
+}
+/* the first two samples at p as one word */
+template <typename T>
+__device__ __forceinline__ uint32_t load_word(const T *p);
+template <>
+__device__ __forceinline__ uint32_t load_word<uint16_t>(const uint16_t *p) {
+  return *reinterpret_cast<const uint32_t *>(p);
+}
+template <>
+__device__ __forceinline__ uint32_t load_word<uint8_t>(const uint8_t *p) {
+  return __builtin_amdgcn_perm(0u, (uint32_t)*reinterpret_cast<const uint16_t *>(p), 0x0c010c00u);
+}
+
+/* A chunk as loaded (every address clamped into the plane, no branch, so
+ * that nothing waits for it before its store); chunk_store applies the
+ * availability. */
+struct Chunk {
+  uint4 c;    /* P words 4k - 3 .. 4k */
+  uint32_t n; /* word 4k + 1 */
+};
+
+template <typename T, int NCH>
+__device__ __forceinline__ Chunk chunk_load(const T *src, int64_t stride, int x0, int y0, int it, int xlim,
+                                            int ylim) {
+  const int r = it / NCH, k = it - r * NCH;
+  const int y = y0 - 2 + r, x = x0 - 8 + 8 * k;
+  const T *row = src + (int64_t)min(max(y, 0), ylim - 1) * stride;
+  Chunk ch;
+  ch.c = load_chunk<T>(row + min(max(x, 0), xlim - 8));
+  ch.n = load_word<T>(row + min(x + 8, xlim - 2));
+  return ch;
+}
+
+template <int NCH, int RW = TS / 2>
+__device__ __forceinline__ void chunk_store(const Chunk &ch, uint32_t *pw, uint32_t *sw, int it, int x0, int y0,
+                                            int xlim, int ylim) {
+  const int r = it / NCH, k = it - r * NCH;
+  const int y = y0 - 2 + r, x = x0 - 8 + 8 * k;
+  const bool yok = y >= 0 && y < ylim;
+  const bool ok = yok && x >= 0 && x + 8 <= xlim, half = yok && x + 4 == xlim, ok1 = yok && x + 8 < xlim;
+  const uint32_t w[5] = {ok ? ch.c.x : (half ? ch.c.z : NA_W), ok ? ch.c.y : (half ? ch.c.w : NA_W),
+                         ok ? ch.c.z : NA_W, ok ? ch.c.w : NA_W, ok1 ? ch.n : NA_W};
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const int m = 4 * k - 3 + j;
+    if (m >= 0 && m < RW) {
+      pw[r * RW + m] = w[j];
+      sw[r * RW + m] = __builtin_amdgcn_alignbit(w[j + 1], w[j], 16);
+    }
+  }
+}
+
+/* A 4:2:0 chroma tile held beside the luma tile (chunked staging): 32 + 4
+ * samples per row. */
+constexpr int TSC = 36;
+constexpr int csoff_rows(int rows) { return (rows + 4) * (TSC / 2); }
+
+/* stage_tile (w = 64, or 32 with NCH = 6) from chunks: every load of a
+ * thread's items before its LDS stores */
+template <typename T, int NW, int NCH>
+__device__ __forceinline__ void stage_chunks(uint32_t *pw, uint32_t *sw, const T *src, int64_t stride, int x0,
+                                             int y0, int th, int xlim, int ylim) {
+  constexpr int NT = 64 * NW, MAXI = ((16 * NW + 4) * NCH + NT - 1) / NT;
+  Chunk c[MAXI];
+#pragma unroll
+  for (int i = 0; i < MAXI; i++) c[i] = chunk_load<T, NCH>(src, stride, x0, y0, threadIdx.x + i * NT, xlim, ylim);
+#pragma unroll
+  for (int i = 0; i < MAXI; i++) {
+    const int it = threadIdx.x + i * NT;
+    if (it < th * NCH) chunk_store<NCH>(c[i], pw, sw, it, x0, y0, xlim, ylim);
   }
 }
 
@@ -195,11 +292,12 @@ __device__ __forceinline__ int floor_log2(int x) { return 31 - __clz(x); }
 
 /* word offset of tap (dy, dx) for a pair at an even column; soff = the
  * word offset of S behind P */
-__device__ __forceinline__ int word_off(int dy, int dx, int soff) {
-  return (dx & 1) ? soff + dy * (TS / 2) + ((dx - 1) >> 1) : dy * (TS / 2) + (dx >> 1);
+__device__ __forceinline__ int word_off(int dy, int dx, int soff, int rw) {
+  return (dx & 1) ? soff + dy * rw + ((dx - 1) >> 1) : dy * rw + (dx >> 1);
 }
 
-__device__ __forceinline__ Taps make_taps(int pri, int sec, int damping, int dir, int coeff_shift, int soff) {
+__device__ __forceinline__ Taps make_taps(int pri, int sec, int damping, int dir, int coeff_shift, int soff,
+                                          int rw = TS / 2) {
   Taps k;
   k.has_pri = pri != 0;
   k.has_sec = sec != 0;
@@ -217,14 +315,14 @@ __device__ __forceinline__ Taps make_taps(int pri, int sec, int damping, int dir
 #pragma unroll
   for (int t = 0; t < 2; t++) {
     const int dy = AV1_CDEF_DIRECTIONS[dir][t][0], dx = AV1_CDEF_DIRECTIONS[dir][t][1];
-    k.po[2 * t] = word_off(dy, dx, soff);
-    k.po[2 * t + 1] = word_off(-dy, -dx, soff);
+    k.po[2 * t] = word_off(dy, dx, soff, rw);
+    k.po[2 * t + 1] = word_off(-dy, -dx, soff, rw);
     const int ly = AV1_CDEF_DIRECTIONS[dlo][t][0], lx = AV1_CDEF_DIRECTIONS[dlo][t][1];
     const int hy = AV1_CDEF_DIRECTIONS[dhi][t][0], hx = AV1_CDEF_DIRECTIONS[dhi][t][1];
-    k.so[4 * t + 0] = word_off(ly, lx, soff);
-    k.so[4 * t + 1] = word_off(-ly, -lx, soff);
-    k.so[4 * t + 2] = word_off(hy, hx, soff);
-    k.so[4 * t + 3] = word_off(-hy, -hx, soff);
+    k.so[4 * t + 0] = word_off(ly, lx, soff, rw);
+    k.so[4 * t + 1] = word_off(-ly, -lx, soff, rw);
+    k.so[4 * t + 2] = word_off(hy, hx, soff, rw);
+    k.so[4 * t + 3] = word_off(-hy, -hx, soff, rw);
   }
   return k;
 }
@@ -398,12 +496,12 @@ __device__ __forceinline__ int adjust_pri(int pri, int32_t var) {
  * block is not filtered (on == false) store the input pair. The words of
  * the 12 taps on the first row are addressed once; the rows follow at
  * constant offsets (immediates of the LDS reads). */
-template <typename T, bool PRI, bool SEC, int NROWS>
+template <typename T, bool PRI, bool SEC, int NROWS, int TSX>
 __device__ __forceinline__ void filter_block_rows(const uint32_t *w, int t0, T *o, int64_t os, int c, int r0,
                                                   const Taps &k, bool on) {
-  constexpr int RW = TS / 2; /* words per tile row */
+  constexpr int RW = TSX / 2; /* words per tile row */
   o += (int64_t)r0 * os + c;
-  const uint32_t *b = w + ((t0 + r0 * TS + c) >> 1);
+  const uint32_t *b = w + ((t0 + r0 * TSX + c) >> 1);
   const uint32_t *pt[4], *st[8];
 #pragma unroll
   for (int t = 0; t < 4; t++) pt[t] = b + k.po[t];
@@ -431,31 +529,35 @@ __device__ __forceinline__ void filter_block_rows(const uint32_t *w, int t0, T *
     } else {
       *reinterpret_cast<uint16_t *>(o) = (uint16_t)((v & 0xff) | ((v >> 8) & 0xff00));
     }
+    /* the chroma tiles beside the luma tile: one row at a time (the
+     * scheduler would interleave the two rows of a chroma lane and take
+     * 105 registers) */
+    if (TSX == TSC) __builtin_amdgcn_sched_barrier(0);
   }
 }
 
-template <typename T, bool PRI, bool SEC>
+template <typename T, bool PRI, bool SEC, int TSX>
 __device__ __forceinline__ void filter_block_pairs(const uint32_t *w, int t0, T *o, int64_t os, int c, int r0,
                                                    int nrows, const Taps &k, bool on) {
   switch (nrows) {
-    case 8: filter_block_rows<T, PRI, SEC, 8>(w, t0, o, os, c, r0, k, on); break;
-    case 4: filter_block_rows<T, PRI, SEC, 4>(w, t0, o, os, c, r0, k, on); break;
-    default: filter_block_rows<T, PRI, SEC, 2>(w, t0, o, os, c, r0, k, on); break;
+    case 8: filter_block_rows<T, PRI, SEC, 8, TSX>(w, t0, o, os, c, r0, k, on); break;
+    case 4: filter_block_rows<T, PRI, SEC, 4, TSX>(w, t0, o, os, c, r0, k, on); break;
+    default: filter_block_rows<T, PRI, SEC, 2, TSX>(w, t0, o, os, c, r0, k, on); break;
   }
 }
 
 /* filter_block_pairs with the tile's tap kinds as template arguments. */
-template <typename T>
+template <typename T, int TSX = TS>
 __device__ __forceinline__ void filter_block_any(bool pri, bool sec, const uint32_t *w, int t0, T *o, int64_t os,
                                                  int c, int r0, int nrows, const Taps &k, bool on) {
   if (pri && sec)
-    filter_block_pairs<T, true, true>(w, t0, o, os, c, r0, nrows, k, on);
+    filter_block_pairs<T, true, true, TSX>(w, t0, o, os, c, r0, nrows, k, on);
   else if (pri)
-    filter_block_pairs<T, true, false>(w, t0, o, os, c, r0, nrows, k, on);
+    filter_block_pairs<T, true, false, TSX>(w, t0, o, os, c, r0, nrows, k, on);
   else if (sec)
-    filter_block_pairs<T, false, true>(w, t0, o, os, c, r0, nrows, k, on);
+    filter_block_pairs<T, false, true, TSX>(w, t0, o, os, c, r0, nrows, k, on);
   else
-    filter_block_pairs<T, false, false>(w, t0, o, os, c, r0, nrows, k, false);
+    filter_block_pairs<T, false, false, TSX>(w, t0, o, os, c, r0, nrows, k, false);
 }
 
 /* The 8x8 blocks of a filter block in the order lanes take them: the
@@ -473,7 +575,7 @@ struct CdefBlocks {
  * four lanes of slot s = tid / 4 take block list[s]: only the waves whose
  * slots hold filtered blocks run the direction search and the filters,
  * the rest copy their blocks from the staged tile. */
-template <typename T, int NW>
+template <typename T, int NW, bool VEC>
 __device__ __forceinline__ void cdef_tile(const CdefLaunch &a, int tile_id, uint32_t *words, CdefBlocks &bl) {
   constexpr int TH = 16 * NW;      /* luma rows of the tile */
   constexpr int NPART = 64 / TH;   /* tiles per filter block */
@@ -541,13 +643,40 @@ __device__ __forceinline__ void cdef_tile(const CdefLaunch &a, int tile_id, uint
       bl.ncopy = __popcll(mc);
     }
   }
-  stage_tile<T, NW>(tile, shifted, pp<T>(a.in[0]), a.in[0].stride, x0, y0, 64, TH, xlim, ylim);
-  /* 4:2:0: both chroma tiles' loads go out now and land while luma filters */
-  const bool pre = g.nplanes > 1 && g.ssx && g.ssy;
-  ChromaRegs cpre[2];
-  if (pre)
-    for (int p = 1; p < 3; p++)
-      chroma_load<T, NW>(cpre[p - 1], pp<T>(a.in[p]), a.in[p].stride, x0 >> 1, y0 >> 1, xlim >> 1, ylim >> 1);
+  /* 4:2:0 chroma: with chunked staging both chroma tiles are staged with
+   * the luma tile, into their own LDS beside it (no barrier or restaging
+   * between planes); else their loads go out now into registers and land
+   * while luma filters */
+  const bool c420 = g.nplanes > 1 && g.ssx && g.ssy;
+  constexpr int CSO = csoff_rows(8 * NW);
+  uint32_t *cwords = words + 2 * SO; /* VEC: Cb P, Cb S, Cr P, Cr S */
+  ChromaRegs cpre[VEC ? 1 : 2];
+  if constexpr (VEC) {
+    /* luma (bh + 4) x 10 chunks, chroma ((bh >> 1) + 4) x 6 per plane (at
+     * most 64 NW: one per thread); every load before the first store */
+    constexpr int NT = 64 * NW, MAXI = ((16 * NW + 4) * 10 + NT - 1) / NT;
+    const T *ysrc = pp<T>(a.in[0]);
+    Chunk cy[MAXI], cc[2];
+#pragma unroll
+    for (int i = 0; i < MAXI; i++) cy[i] = chunk_load<T, 10>(ysrc, a.in[0].stride, x0, y0, tid + i * NT, xlim, ylim);
+    if (c420)
+#pragma unroll
+      for (int p = 1; p < 3; p++)
+        cc[p - 1] = chunk_load<T, 6>(pp<T>(a.in[p]), a.in[p].stride, x0 >> 1, y0 >> 1, tid, xlim >> 1, ylim >> 1);
+#pragma unroll
+    for (int i = 0; i < MAXI; i++)
+      if (tid + i * NT < (bh + 4) * 10) chunk_store<10>(cy[i], words, words + SO, tid + i * NT, x0, y0, xlim, ylim);
+    if (c420 && tid < ((bh >> 1) + 4) * 6)
+#pragma unroll
+      for (int p = 1; p < 3; p++)
+        chunk_store<6, TSC / 2>(cc[p - 1], cwords + (p - 1) * 2 * CSO, cwords + (p - 1) * 2 * CSO + CSO, tid,
+                                x0 >> 1, y0 >> 1, xlim >> 1, ylim >> 1);
+  } else {
+    stage_tile<T, NW>(tile, shifted, pp<T>(a.in[0]), a.in[0].stride, x0, y0, 64, TH, xlim, ylim);
+    if (c420)
+      for (int p = 1; p < 3; p++)
+        chroma_load<T, NW>(cpre[p - 1], pp<T>(a.in[p]), a.in[p].stride, x0 >> 1, y0 >> 1, xlim >> 1, ylim >> 1);
+  }
   __syncthreads();
 
   const int nf = bl.nfilt, nact = nf + bl.ncopy;
@@ -585,15 +714,35 @@ __device__ __forceinline__ void cdef_tile(const CdefLaunch &a, int tile_id, uint
   const int ncp = bws >> 1;               /* 4 (4:4:4) or 2 */
   const int nrows = (bhs * ncp) >> 2;     /* rows per lane */
   const int cpair = q % ncp, r0 = (q / ncp) * nrows;
-  const Taps k = make_taps(upri, usec, a.h.damping + cs - 1, upri ? AV1_CDEF_UV_DIR[ssx][ssy][ydir] : 0, cs, SO);
+  const int udir = upri ? AV1_CDEF_UV_DIR[ssx][ssy][ydir] : 0;
+  if (VEC && c420) {
+    const Taps k = make_taps(upri, usec, a.h.damping + cs - 1, udir, cs, CSO, TSC / 2);
+    if (active) {
+      const int t0 = (by * 4 + 2) * TSC + bx * 4 + 2;
+#pragma unroll
+      for (int p = 1; p < 3; p++) {
+        T *o = pp<T>(a.out[p]) + (int64_t)((y0 >> 1) + by * 4) * a.out[p].stride + (x0 >> 1) + bx * 4;
+        const uint32_t *pw = cwords + (p - 1) * 2 * CSO;
+        if (wave_filters)
+          filter_block_any<T, TSC>(upri != 0, usec != 0, pw, t0, o, a.out[p].stride, 2 * cpair, r0, nrows, k,
+                                   filt && (upri | usec));
+        else
+          filter_block_any<T, TSC>(false, false, pw, t0, o, a.out[p].stride, 2 * cpair, r0, nrows, k, false);
+      }
+    }
+    return;
+  }
+  const Taps k = make_taps(upri, usec, a.h.damping + cs - 1, udir, cs, SO);
 #pragma unroll
   for (int p = 1; p < 3; p++) {
     __syncthreads();
-    if (pre)
+    /* (with chunked staging only 4:2:2 and 4:4:4 get here) */
+    if (!VEC && c420) {
       chroma_store<NW>(cpre[p - 1], tile, shifted);
-    else
+    } else {
       stage_tile<T, NW>(tile, shifted, pp<T>(a.in[p]), a.in[p].stride, x0 >> ssx, y0 >> ssy, cw, ch, xlim >> ssx,
-                    ylim >> ssy);
+                        ylim >> ssy);
+    }
     __syncthreads();
     if (active) {
       T *o = pp<T>(a.out[p]) + (int64_t)((y0 >> ssy) + by * bhs) * a.out[p].stride + (x0 >> ssx) + bx * bws;
@@ -614,11 +763,12 @@ constexpr int CDEF_WAVES = 4;
 /* 5 waves per SIMD: 96 VGPRs, no spills (at 6 waves, 80 VGPRs, the
  * prefetched 4:2:0 chroma tiles spill and the kernel is 6 % slower per 4K
  * frame). */
-template <typename T>
+template <typename T, bool VEC>
 __global__ __launch_bounds__(64 * CDEF_WAVES, 5) void cdef_kernel(CdefLaunch a) {
-  __shared__ uint32_t words[2 * soff_rows(16 * CDEF_WAVES)];
+  /* the tile P and S; chunked staging adds both 4:2:0 chroma tiles' */
+  __shared__ uint32_t words[2 * soff_rows(16 * CDEF_WAVES) + (VEC ? 4 * csoff_rows(8 * CDEF_WAVES) : 0)];
   __shared__ CdefBlocks bl;
-  cdef_tile<T, CDEF_WAVES>(a, xcd_remap(blockIdx.x, gridDim.x), words, bl);
+  cdef_tile<T, CDEF_WAVES, VEC>(a, xcd_remap(blockIdx.x, gridDim.x), words, bl);
 }
 
 /* Batched. The persistent form is held at 5 waves per SIMD (inside the
@@ -629,7 +779,7 @@ template <typename T, bool DIRECT>
 __global__ __launch_bounds__(256, DIRECT ? 6 : 5) void cdef_batch(BatchWork w) {
   __shared__ uint32_t words[2 * SOFF];
   __shared__ CdefBlocks bl;
-  batch_loop<DIRECT>(w, [&](int f, int j) { cdef_tile<T, 4>(batch_table<CdefLaunch>(w.rec, f), j, words, bl); });
+  batch_loop<DIRECT>(w, [&](int f, int j) { cdef_tile<T, 4, false>(batch_table<CdefLaunch>(w.rec, f), j, words, bl); });
 }
 
 /* ---- the encoder's strength search (src/encoder/cdef.rs:52-197) ---- */
@@ -817,6 +967,11 @@ int plan(const Geom &g, const CdefHdr &h, const rvpf_mi *mi, int mi_stride, cons
   }
   a.enabled = enabled;
   a.fb_cols = (g.mi_cols + 15) / 16;
+  /* chunked staging: 8 samples per access, 16 bytes (u16) or 8 (u8);
+   * 16-byte aligned planes with strides of 8k samples suit both */
+  a.vec = (g.mi_cols * 4) >> g.ssx >= 8; /* a chunk's access stays inside the mode-info width */
+  for (int p = 0; p < g.nplanes; p++)
+    if ((uintptr_t)in[p].data % 16 != 0 || in[p].stride % 8 != 0) a.vec = 0;
   return a.fb_cols * ((g.mi_rows + 15) / 16);
 }
 
@@ -849,10 +1004,14 @@ int launch_cdef(const Geom &g, const CdefHdr &h, const rvpf_mi *mi, int mi_strid
                 PlanePtr out[3], int bps, hipStream_t st) {
   CdefLaunch a;
   const int n = plan(g, h, mi, mi_stride, cdef_idx, cdef_stride, enabled, in, out, a) * (4 / CDEF_WAVES);
-  if (bps == 1)
-    cdef_kernel<uint8_t><<<n, 64 * CDEF_WAVES, 0, st>>>(a);
+  if (bps == 1 && a.vec)
+    cdef_kernel<uint8_t, true><<<n, 64 * CDEF_WAVES, 0, st>>>(a);
+  else if (bps == 1)
+    cdef_kernel<uint8_t, false><<<n, 64 * CDEF_WAVES, 0, st>>>(a);
+  else if (a.vec)
+    cdef_kernel<uint16_t, true><<<n, 64 * CDEF_WAVES, 0, st>>>(a);
   else
-    cdef_kernel<uint16_t><<<n, 64 * CDEF_WAVES, 0, st>>>(a);
+    cdef_kernel<uint16_t, false><<<n, 64 * CDEF_WAVES, 0, st>>>(a);
   return (int)hipGetLastError();
 }
 
