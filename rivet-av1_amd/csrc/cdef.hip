@@ -270,11 +270,23 @@ __device__ __forceinline__ s16x2 splat(int v) { return (s16x2){(short)v, (short)
  * 0, so their or is |d|), m = max(0, t - (|d| >> adj)) is a saturating
  * difference too, and the result is min(pos, m) - min(neg, m). An
  * unavailable tap (NA_S = 40960) has |d| >= 36865, which keeps m at 0. */
-__device__ __forceinline__ s16x2 constrain2(u16x2 p, u16x2 x, u16x2 t, u16x2 adj) {
-  const u16x2 pos = __builtin_elementwise_sub_sat(p, x), neg = __builtin_elementwise_sub_sat(x, p);
+__device__ __forceinline__ s16x2 constrain2(u16x2 p, u16x2 x, u16x2 t, u16x2 adj, u16x2 &neg) {
+  const u16x2 pos = __builtin_elementwise_sub_sat(p, x);
+  neg = __builtin_elementwise_sub_sat(x, p);
   const u16x2 m = __builtin_elementwise_sub_sat(t, (pos | neg) >> adj);
   return __builtin_bit_cast(s16x2, __builtin_elementwise_min(pos, m)) -
          __builtin_bit_cast(s16x2, __builtin_elementwise_min(neg, m));
+}
+
+/* Packed maximum of pairs of samples (0 .. 4095) or NA_S, read as half
+ * floats (v_pk_maximum3_f16): a sample's bit pattern is a non-negative
+ * half (normal or subnormal; the kernels keep subnormal halves) whose
+ * order is the integer order, and NA_S is a negative half, below every
+ * sample. */
+typedef _Float16 h16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t max3_h(uint32_t a, uint32_t b, uint32_t c) {
+  const h16x2 m = __builtin_elementwise_maximum(__builtin_bit_cast(h16x2, a), __builtin_bit_cast(h16x2, b));
+  return __builtin_bit_cast(uint32_t, __builtin_elementwise_maximum(m, __builtin_bit_cast(h16x2, c)));
 }
 
 /* Per-lane filter set-up: strengths, damping shifts, primary tap weights
@@ -337,46 +349,56 @@ __device__ __forceinline__ Taps make_taps(int pri, int sec, int damping, int dir
  * Every tap is loaded before the arithmetic, and the taps are
  * independent, so the packed operations interleave. */
 /* The filtered pair xw from its loaded taps pw (primary) and sw
- * (secondary). */
+ * (secondary). The clamp's bounds: max = the largest of x and the
+ * available taps (three at a time, max3_h), min = x - the largest
+ * saturating difference x - tap (0 for NA_S), which constrain() computes
+ * anyway. */
 template <bool PRI, bool SEC>
 __device__ __forceinline__ uint32_t filter_pair_taps(uint32_t xw, const uint32_t *pw, const uint32_t *sw,
                                                      const Taps &k) {
   const s16x2 x = as_s2(xw);
   const u16x2 xu = __builtin_bit_cast(u16x2, xw);
   constexpr bool CLAMP = PRI && SEC;
-  s16x2 sum = splat(0), mx = x;
-  u16x2 mn = __builtin_bit_cast(u16x2, x);
+  s16x2 sum = splat(0);
+  uint32_t mx = 0, mneg = 0, n3 = 0; /* CLAMP: running maxima, folded in as the taps go */
+  const auto w32 = [](u16x2 v) { return __builtin_bit_cast(uint32_t, v); };
   if (PRI) {
     s16x2 c[4];
+    u16x2 n[4];
 #pragma unroll
-    for (int t = 0; t < 4; t++) c[t] = constrain2(__builtin_bit_cast(u16x2, pw[t]), xu, k.pri, k.pri_adj);
+    for (int t = 0; t < 4; t++) c[t] = constrain2(__builtin_bit_cast(u16x2, pw[t]), xu, k.pri, k.pri_adj, n[t]);
     if (CLAMP) {
-#pragma unroll
-      for (int t = 0; t < 4; t++) {
-        mn = __builtin_elementwise_min(mn, __builtin_bit_cast(u16x2, pw[t]));
-        mx = __builtin_elementwise_max(mx, as_s2(pw[t]));
-      }
+      mx = max3_h(max3_h(xw, pw[0], pw[1]), pw[2], pw[3]);
+      mneg = max3_h(w32(n[0]), w32(n[1]), w32(n[2]));
+      n3 = w32(n[3]);
     }
     sum = k.pt0 * (c[0] + c[1]) + k.pt1 * (c[2] + c[3]);
   }
   if (SEC) {
     /* Cdef_Sec_Taps is {2, 1} in both rows */
     s16x2 c[8];
+    u16x2 n[8];
 #pragma unroll
-    for (int t = 0; t < 8; t++) c[t] = constrain2(__builtin_bit_cast(u16x2, sw[t]), xu, k.sec, k.sec_adj);
+    for (int t = 0; t < 8; t++) {
+      c[t] = constrain2(__builtin_bit_cast(u16x2, sw[t]), xu, k.sec, k.sec_adj, n[t]);
+      if (CLAMP && (t & 1)) mx = max3_h(mx, sw[t - 1], sw[t]);
+    }
     if (CLAMP) {
-#pragma unroll
-      for (int t = 0; t < 8; t++) {
-        mn = __builtin_elementwise_min(mn, __builtin_bit_cast(u16x2, sw[t]));
-        mx = __builtin_elementwise_max(mx, as_s2(sw[t]));
-      }
+      mneg = max3_h(mneg, n3, w32(n[0]));
+      mneg = max3_h(mneg, w32(n[1]), w32(n[2]));
+      mneg = max3_h(mneg, w32(n[3]), w32(n[4]));
+      mneg = max3_h(mneg, w32(n[5]), w32(n[6]));
+      mneg = max3_h(mneg, w32(n[7]), w32(n[7]));
     }
     const s16x2 s0 = (c[0] + c[1]) + (c[2] + c[3]), s1 = (c[4] + c[5]) + (c[6] + c[7]);
     sum += s0 + s0 + s1;
   }
   /* x + ((8 + sum - (sum < 0)) >> 4), clamped to [min, max] of the taps */
   s16x2 v = x + ((sum + splat(8) + (sum >> splat(15))) >> splat(4));
-  if (CLAMP) v = __builtin_elementwise_min(__builtin_elementwise_max(v, __builtin_bit_cast(s16x2, mn)), mx);
+  if (CLAMP) {
+    const s16x2 mn = __builtin_bit_cast(s16x2, xu - __builtin_bit_cast(u16x2, mneg));
+    v = __builtin_elementwise_min(__builtin_elementwise_max(v, mn), as_s2(mx));
+  }
   return __builtin_bit_cast(uint32_t, v);
 }
 
