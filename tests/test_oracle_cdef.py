@@ -211,3 +211,29 @@ def test_frame_cdef_matches_independent_walk():
         for p in range(f.num_planes):
             rows, cols = f.mi_area(p)
             assert np.array_equal(got[p][:rows, :cols], want[p][:rows, :cols]), (kw, p)
+
+
+def test_sample_bit_patterns_order_as_halves():
+    """The CDEF kernel takes the largest of samples and of saturating
+    differences (0 .. 4095 at 12 bits) with a packed half-float maximum,
+    and relies on the unavailable-tap sentinel 0xA000 being below all of
+    them: the bit patterns 0 .. 4095 read as IEEE halves (subnormals kept)
+    must be strictly increasing and finite, and 0xA000 negative."""
+    pats = np.arange(4096, dtype=np.uint16)
+    h = pats.view(np.float16).astype(np.float64)
+    assert np.all(np.isfinite(h))
+    assert np.all(np.diff(h) > 0)
+    assert h[0] == 0.0 and not np.signbit(h[0])
+    na = np.array([0xA000], dtype=np.uint16).view(np.float16)[0]
+    assert float(na) < 0.0
+    # the maximum over a random set of patterns (with sentinels) is the
+    # integer maximum of its non-sentinel members
+    rng = np.random.default_rng(7)
+    for _ in range(200):
+        n = int(rng.integers(1, 14))
+        v = rng.integers(0, 4096, size=n).astype(np.uint16)
+        v[rng.random(n) < 0.3] = 0xA000
+        v[0] = rng.integers(0, 4096)  # the centre sample is always available
+        got = v.view(np.float16).astype(np.float64).max()
+        want = int(v[v != 0xA000].max())
+        assert np.array([want], dtype=np.uint16).view(np.float16)[0] == got
