@@ -50,7 +50,7 @@ def make_frames(n, w, h, bd, seed, superres_denom=0, chroma="420"):
 # HBM bytes per launch of each stage's kernels on the default workload,
 # from the calibrated FETCH_SIZE / WRITE_SIZE counter passes summarised here
 PMC_SUMMARY = "profiles/chain_4k10b_pmc.md"
-STAGE_KERNELS = {"deblock": ["lf_fused_kernel<u16>"], "cdef": ["cdef_kernel<u16>"],
+STAGE_KERNELS = {"deblock": ["lf_fused_kernel<u16>"], "cdef": ["cdef_kernel<u16, true>"],
                  "lr": ["lr_kernel<u16>"], "grain_apply": ["grain_apply_kernel<u16>"],
                  "upscale": ["upscale_kernel<u16>"]}
 
