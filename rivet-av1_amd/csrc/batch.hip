@@ -52,6 +52,7 @@ struct Work {
   size_t start_off = 0;  /* byte offset of the n + 1 prefix sums */
   int n = 0;             /* frames */
   int queue = 0;         /* index of the kernel's 2-int queue */
+  int vec = 1;           /* CDEF: every frame's planes take the chunked staging */
 };
 
 struct Group {
@@ -184,9 +185,11 @@ int build_tables(rvpf_batch *b) {
       const size_t rs = record_size(k);
       std::vector<char> recs(rs * nf);
       std::vector<int> start(nf + 1, 0);
+      w.vec = 1;
       for (int i = 0; i < nf; i++) {
         const int items = plan_frame(b, k, b->f[gr.frames[i]], recs.data() + rs * i);
         start[i + 1] = start[i] + items;
+        if (k == K_CDEF && items > 0 && !cdef_record_vec(recs.data() + rs * i)) w.vec = 0;
       }
       w.total = start[nf];
       place(t, w, rs, recs, start);
@@ -277,7 +280,7 @@ int launch_kernel(rvpf_batch *b, const Group &gr, int k, hipStream_t st) {
     }
     case K_CDEF: {
       const StagePlan sp = stage_plan(b, RVPF_TUNE_CDEF, w.total);
-      return launch_cdef_batch(batch_work(b, w, sp.chunk), sp, gr.bps, st);
+      return launch_cdef_batch(batch_work(b, w, sp.chunk), sp, gr.bps, w.vec, st);
     }
     case K_LR: {
       const StagePlan sp = stage_plan(b, RVPF_TUNE_LR, w.total);

@@ -797,11 +797,11 @@ __global__ __launch_bounds__(64 * CDEF_WAVES, 5) void cdef_kernel(CdefLaunch a) 
  * grab loop the compiler otherwise takes 192 VGPRs, 2 waves; the few
  * spills the bound costs are cheaper than the lost latency hiding); the
  * direct form at 6, as the per-frame kernel. */
-template <typename T, bool DIRECT>
+template <typename T, bool DIRECT, bool VEC>
 __global__ __launch_bounds__(256, DIRECT ? 6 : 5) void cdef_batch(BatchWork w) {
-  __shared__ uint32_t words[2 * SOFF];
+  __shared__ uint32_t words[2 * SOFF + (VEC ? 4 * csoff_rows(32) : 0)];
   __shared__ CdefBlocks bl;
-  batch_loop<DIRECT>(w, [&](int f, int j) { cdef_tile<T, 4, false>(batch_table<CdefLaunch>(w.rec, f), j, words, bl); });
+  batch_loop<DIRECT>(w, [&](int f, int j) { cdef_tile<T, 4, VEC>(batch_table<CdefLaunch>(w.rec, f), j, words, bl); });
 }
 
 /* ---- the encoder's strength search (src/encoder/cdef.rs:52-197) ---- */
@@ -1006,18 +1006,26 @@ int cdef_plan(const Geom &g, const CdefHdr &h, const rvpf_mi *mi, int mi_stride,
   return plan(g, h, mi, mi_stride, cdef_idx, cdef_stride, enabled, in, out, *static_cast<CdefLaunch *>(rec));
 }
 
-int launch_cdef_batch(const BatchWork &w, const StagePlan &sp, int bps, hipStream_t st) {
-  if (w.total == 0) return 0;
-  if (bps == 1)
-    if (batch_direct(w, sp.grid))
-      cdef_batch<uint8_t, true><<<sp.grid, 256, 0, st>>>(w);
-    else
-      cdef_batch<uint8_t, false><<<sp.grid, 256, 0, st>>>(w);
+int cdef_record_vec(const void *rec) { return static_cast<const CdefLaunch *>(rec)->vec; }
+
+template <typename T, bool VEC>
+void cdef_batch_launch(const BatchWork &w, const StagePlan &sp, hipStream_t st) {
+  if (batch_direct(w, sp.grid))
+    cdef_batch<T, true, VEC><<<sp.grid, 256, 0, st>>>(w);
   else
-    if (batch_direct(w, sp.grid))
-      cdef_batch<uint16_t, true><<<sp.grid, 256, 0, st>>>(w);
-    else
-      cdef_batch<uint16_t, false><<<sp.grid, 256, 0, st>>>(w);
+    cdef_batch<T, false, VEC><<<sp.grid, 256, 0, st>>>(w);
+}
+
+int launch_cdef_batch(const BatchWork &w, const StagePlan &sp, int bps, int vec, hipStream_t st) {
+  if (w.total == 0) return 0;
+  if (bps == 1 && vec)
+    cdef_batch_launch<uint8_t, true>(w, sp, st);
+  else if (bps == 1)
+    cdef_batch_launch<uint8_t, false>(w, sp, st);
+  else if (vec)
+    cdef_batch_launch<uint16_t, true>(w, sp, st);
+  else
+    cdef_batch_launch<uint16_t, false>(w, sp, st);
   return (int)hipGetLastError();
 }
 

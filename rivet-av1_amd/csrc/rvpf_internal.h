@@ -249,7 +249,9 @@ int launch_deblock_batch(const BatchWork &wv, const BatchWork &wh, const StagePl
 size_t cdef_record_size();
 int cdef_plan(const Geom &g, const CdefHdr &h, const rvpf_mi *mi, int mi_stride, const int8_t *cdef_idx,
               int cdef_stride, int enabled, const PlanePtr in[3], const PlanePtr out[3], void *rec);
-int launch_cdef_batch(const BatchWork &w, const StagePlan &sp, int bps, hipStream_t st);
+/* 1 when the record's planes take CDEF's chunked staging */
+int cdef_record_vec(const void *rec);
+int launch_cdef_batch(const BatchWork &w, const StagePlan &sp, int bps, int vec, hipStream_t st);
 
 size_t lr_record_size();
 int lr_plan(const Geom &g, const LrHdr &h, const PlanePtr deblocked[3], const PlanePtr cdef[3],
