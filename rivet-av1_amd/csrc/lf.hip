@@ -548,8 +548,13 @@ struct LfFusedShared {
   __align__(16) uint16_t tile[FT * FT];
   LfTables t;
   int count[8];             /* [pass * 4 + class] */
-  uint16_t lv[4][FV_N];     /* vertical edges by class (edge index) */
-  uint16_t lh[4][FH_N];     /* horizontal edges */
+  /* Edge indices by class, two lists per pass. A plane produces at most
+   * three classes (luma 4/8/16-wide: 0, 2, 3; chroma 4/6-wide: 0, 1) and
+   * each edge is in one: class 0 fills list 0 from the front, class 1 or 2
+   * (never both in one plane) from the back, class 3 fills list 1. Sized
+   * this way the block's LDS stays below 20 KB, 8 workgroups per CU. */
+  uint16_t lv[2][FV_N];     /* vertical edges */
+  uint16_t lh[2][FH_N];     /* horizontal edges */
   uint8_t elv[FV_N], elh[FH_N]; /* the edge's level */
   rvpf_mi mi[FM_N];         /* mode info at each side-info position */
   uint8_t tx[FM_N];         /* transform size */
@@ -602,13 +607,23 @@ __device__ __forceinline__ void lds_st2(uint16_t *p, int a, int b) {
   *reinterpret_cast<uint32_t *>(p) = (uint32_t)a | ((uint32_t)b << 16);
 }
 
+/* Entry i of class c's list in the two lists of a pass (N entries each). */
+template <int N>
+__device__ __forceinline__ uint16_t &class_slot(uint16_t (*l)[N], int c, int i) {
+  return c == 3 ? l[1][i] : (c == 0 ? l[0][i] : l[0][N - 1 - i]);
+}
+template <int C, int N>
+__device__ __forceinline__ int class_item(const uint16_t (*l)[N], int i) {
+  return C == 3 ? l[1][i] : (C == 0 ? l[0][i] : l[0][N - 1 - i]);
+}
+
 /* One vertical-edge class: a row of an edge per item, in the staged tile. */
 template <int C>
 __device__ __forceinline__ void fused_v(LfFusedShared &sm, int bd, int sharp) {
   constexpr int N = LfClass<C>::N, H = N / 2;
   const int sh = bd - 8, n = sm.count[C];
   for (int it = threadIdx.x; it < 4 * n; it += 256) {
-    const int li = sm.lv[C][it >> 2];
+    const int li = class_item<C, FV_N>(sm.lv, it >> 2);
     int limit, blimit, thresh;
     level_limits(sm.elv[li], sharp, &limit, &blimit, &thresh);
     const int col = 8 + 4 * (li % FV_COLS), row = 4 * (li / FV_COLS) + (it & 3);
@@ -647,7 +662,7 @@ __device__ __forceinline__ void fused_h(LfFusedShared &sm, int bd, int sharp) {
   constexpr int R = C <= 1 ? 2 : (C == 2 ? 3 : 6); /* rows the filter modifies either side */
   const int sh = bd - 8, n = sm.count[4 + C];
   for (int it = threadIdx.x; it < 4 * n; it += 256) {
-    const int li = sm.lh[C][it >> 2];
+    const int li = class_item<C, FH_N>(sm.lh, it >> 2);
     int limit, blimit, thresh;
     level_limits(sm.elh[li], sharp, &limit, &blimit, &thresh);
     const int row = 8 + 4 * (li / FH_COLS), col = 8 + 4 * (li % FH_COLS) + (it & 3);
@@ -778,10 +793,10 @@ __device__ __forceinline__ void lf_fused_block(const LfLaunch &a, int gp, int lb
     base = __shfl(base, cls < 0 ? 0 : cls, 64);
     if (cls >= 0) {
       if (cls < 4) {
-        sm->lv[cls][base + below] = (uint16_t)idx;
+        class_slot<FV_N>(sm->lv, cls, base + below) = (uint16_t)idx;
         sm->elv[idx] = (uint8_t)lvl;
       } else {
-        sm->lh[cls - 4][base + below] = (uint16_t)idx;
+        class_slot<FH_N>(sm->lh, cls - 4, base + below) = (uint16_t)idx;
         sm->elh[idx] = (uint8_t)lvl;
       }
     }
