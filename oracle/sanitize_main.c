@@ -165,6 +165,8 @@ int main(void) {
       {170, 90, 10, 0, 0, 0, 64, 64, 2, 64, 3, 0, 0},    {257, 129, 10, 1, 1, 0, 128, 90, 2, 256, 3, 1, 0},
       {17, 9, 8, 1, 1, 0, 64, 200, 2, 64, 3, 0, 0},      {200, 96, 8, 1, 1, 0, 64, 120, 2, 64, 3, 0, 16},
       {170, 90, 12, 0, 0, 0, 64, 64, 2, 64, 2, 0, 12},   {130, 70, 8, 1, 1, 1, 64, 120, 2, 64, 3, 0, 11},
+      {1, 1, 8, 1, 1, 0, 64, 60, 2, 64, 3, 1, 0},        {7, 5, 10, 0, 0, 0, 64, 60, 2, 64, 3, 1, 0},
+      {16, 200, 12, 1, 0, 0, 64, 60, 2, 64, 3, 1, 0},    {9, 1, 8, 1, 1, 1, 64, 60, 2, 64, 3, 1, 0},
   };
   int bad = 0;
   for (size_t k = 0; k < sizeof cases / sizeof cases[0]; k++) {

@@ -54,7 +54,7 @@ static int count_units(int unit, int size) {
 
 int synth_geometry(const synth_cfg *c, synth_geom *g) {
   memset(g, 0, sizeof *g);
-  if (c->width < 8 || c->height < 8 || c->width > 16384 || c->height > 16384) return -1;
+  if (c->width < 1 || c->height < 1 || c->width > 16384 || c->height > 16384) return -1;
   if (c->bit_depth != 8 && c->bit_depth != 10 && c->bit_depth != 12) return -1;
   if (c->sb_size != 64 && c->sb_size != 128) return -1;
   if (c->superres_denom && (c->superres_denom < 9 || c->superres_denom > 16)) return -1;

@@ -26,6 +26,17 @@ SMALL_CASES = [
                                 grain=True, grain_lag=1, seed=12, mc_identity=True)),
 ]
 
+# The frame sizes of the reference's odd_sizes test (tests/encode.rs:119-126:
+# 1x1, 7x5, 66x66, 130x34, 16x200, decoded with every post-filter on; 66x66
+# is in SMALL_CASES), plus 2x3 and 9x1: a frame smaller than one 8x8 block, a
+# stripe one row high, one or two samples per chroma row.
+ODD_SIZES = [(1, 1), (7, 5), (130, 34), (16, 200), (2, 3), (9, 1)]
+ODD_LAYOUTS = [("8b_420", dict(bit_depth=8)), ("10b_444", dict(bit_depth=10, ssx=0, ssy=0)),
+               ("12b_422", dict(bit_depth=12, ssx=1, ssy=0)), ("8b_mono", dict(bit_depth=8, mono=True))]
+ODD_CASES = [("%dx%d_%s" % (w, h, name), dict(width=w, height=h, qindex=60, lr=2, grain=True, lf_extras=True,
+                                              seed=7 * w + h, **kw))
+             for (w, h) in ODD_SIZES for name, kw in ODD_LAYOUTS]
+
 
 def make_frame(**kw):
     return workload.Frame(**kw)

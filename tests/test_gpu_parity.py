@@ -79,6 +79,26 @@ def test_chain(ctx, name, kw):
     common.assert_planes(api.to_host(disp_dev), disp, f, "crop", name + " display")
 
 
+@pytest.mark.parametrize("name,kw", common.ODD_CASES, ids=[c[0] for c in common.ODD_CASES])
+def test_chain_odd_sizes(ctx, name, kw):
+    """The reference's odd frame sizes (down to 1x1) through the whole chain,
+    in 16-bit planes and, at 8 bits, in 8-bit planes."""
+    _say("case odd", name)
+    f = common.make_frame(**kw)
+    deb, ref, disp = pyoracle.postfilter_frame(f, threads=4)
+    for bps in ((2, 1) if kw["bit_depth"] == 8 else (2,)):
+        df = api.DeviceFrame(f, bytes_per_sample=bps)
+        rec = df.upload_planes(f.rec)
+        ref_dev = df.alloc_like_recon()
+        disp_dev = df.alloc_display()
+        api.postfilter_frame(ctx, df, rec, ref_dev, disp_dev)
+        ctx.sync()
+        what = "%s bps %d" % (name, bps)
+        common.assert_recon_kept(api.to_host(rec), f, what)
+        common.assert_planes(api.to_host(ref_dev), ref, f, "mi", what + " reference")
+        common.assert_planes(api.to_host(disp_dev), disp, f, "crop", what + " display")
+
+
 U8_CASES = [c for c in common.SMALL_CASES if c[1]["bit_depth"] == 8]
 
 

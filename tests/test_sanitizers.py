@@ -1,7 +1,8 @@
 """Host sanitizers: the oracle (oracle/rvpf_oracle.c) and the synthetic
 workload generator (rivet-av1_amd/csrc/synth.c) built with AddressSanitizer
 and UndefinedBehaviorSanitizer and run over small frames of every layout
-(8/10/12 bits; 4:2:0, 4:2:2, 4:4:4, mono; super-resolution; odd sizes):
+(8/10/12 bits; 4:2:0, 4:2:2, 4:4:4, mono; super-resolution; odd sizes down
+to 1x1):
 the chain, each stage, and the encoder searches. Any out-of-bounds access,
 leak, signed overflow or invalid shift fails the run."""
 import os
@@ -22,5 +23,5 @@ def test_oracle_and_generator_under_asan_ubsan(tmp_path):
     env.pop("LD_PRELOAD", None)
     r = subprocess.run([os.path.join(out, "sanitize_oracle")], env=env, capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
-    assert r.stdout.count("rc 0") == 10, r.stdout
+    assert r.stdout.count("rc 0") == 14, r.stdout
     assert "runtime error" not in r.stderr and "ERROR: AddressSanitizer" not in r.stderr, r.stderr
