@@ -140,3 +140,19 @@ def test_batch_autotune_and_cache(ctx, entries):
         _run_and_check(ctx, b2, entries, "cached tiles")
     finally:
         b2.close()
+
+
+def test_batch_odd_sizes(ctx):
+    """The reference's odd frame sizes (1x1 to 130x34, four layouts, 8-bit
+    ones in 8-bit planes) in one batch: every frame against the oracle, on
+    two replays, with the default tiles and after autotuning."""
+    odd = [Entry(kw, 1 if kw["bit_depth"] == 8 else 2, True) for _, kw in common.ODD_CASES]
+    _say("batch of", len(odd), "odd-size frames")
+    b = api.Batch(ctx, [(e.df, e.rec, e.ref, e.disp) for e in odd])
+    try:
+        for rep in range(2):
+            _run_and_check(ctx, b, odd, "odd replay %d" % rep)
+        b.autotune(force=True)
+        _run_and_check(ctx, b, odd, "odd autotuned")
+    finally:
+        b.close()
