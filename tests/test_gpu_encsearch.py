@@ -20,6 +20,11 @@ CASES = [
     ("264x136_8b_420_u8", dict(width=264, height=136, bit_depth=8, qindex=120, lr=0, seed=35, lf_extras=True), 1),
     ("330x200_10b_420_extras", dict(width=330, height=200, bit_depth=10, qindex=180, lr=0, seed=36,
                                     lf_extras=True, frame_index=2, pan_px=1.5), 2),
+    # frame sizes of the reference's odd_sizes test, whose encoder runs these searches (tests/encode.rs:119)
+    ("1x1_8b_420_u8", dict(width=1, height=1, bit_depth=8, qindex=60, lr=0, seed=37), 1),
+    ("7x5_10b_444", dict(width=7, height=5, bit_depth=10, ssx=0, ssy=0, qindex=60, lr=0, seed=38), 2),
+    ("16x200_12b_422", dict(width=16, height=200, bit_depth=12, ssx=1, ssy=0, qindex=60, lr=0, seed=39), 2),
+    ("130x34_8b_mono", dict(width=130, height=34, bit_depth=8, mono=True, qindex=60, lr=0, seed=40), 2),
 ]
 IDS = [c[0] for c in CASES]
 

@@ -260,3 +260,38 @@ def test_multi_frame_search_equals_per_frame(ctx, cases):
         u0, u1 = multi.frame_units[k]
         assert np.array_equal(choice[u0:u1], ch1)
         assert ftypes[k] == ft1
+
+
+ODD_FRAMES = [  # frame sizes of the reference's odd_sizes test (tests/encode.rs:119), whose encoder runs this search
+    dict(width=1, height=1, bit_depth=8, qindex=160, lr=2, seed=41),
+    dict(width=7, height=5, bit_depth=10, qindex=160, lr=2, seed=42, ssx=0, ssy=0),
+    dict(width=16, height=200, bit_depth=12, qindex=160, lr=2, seed=43, ssx=1, ssy=0),
+    dict(width=130, height=34, bit_depth=8, qindex=160, lr=2, seed=44),
+]
+
+
+def test_odd_sizes_exact_judge_and_search(ctx):
+    """Units smaller than one 8x8 block: the exact judge of the unrestored,
+    Wiener and self-guided records equals the oracle's, and the search
+    chooses a valid type for every unit at no more than the unrestored
+    unit's cost, SSE + lambda (the unrestored unit is one of its
+    candidates)."""
+    for kw in ODD_FRAMES:
+        c = Case(ctx, kw)
+        _say("odd judge", kw["width"], kw["height"], "units", c.s.U)
+        rw = c.s.unit_records("wiener", c.s.quantise_wiener(c.s.fit_wiener(3)))
+        w, _ = c.s.fit_sgr(3)
+        rs = c.s.unit_records("sgr", c.s.quantise_sgr(w, 3), 3)
+        for recs in (None, rw, rs):
+            got = c.s.exact_sse(recs).cpu().numpy()
+            for u, (p, ur, uc) in enumerate(c.s.units):
+                want = c.host.unit_sse(int(p), int(ur), int(uc), None if recs is None else recs[u])
+                assert got[u] == want, (kw, u, got[u], want)
+        lam = lrdiff.rd_lambda(c.f.bit_depth, c.f.cfg.qindex)
+        choice, cost, ftype = c.s.search(lam)
+        cost = cost.cpu().numpy()
+        none_sse = c.s.exact_sse(None).cpu().numpy()
+        for u in range(c.s.U):
+            assert int(choice[u, 0]) in (0, 1, 2)  # none, Wiener, self-guided (include/rvpf.h RVPF_RESTORE_*)
+            none_cost = float(none_sse[u]) + lam  # cost of the unrestored unit (lr.rs:154-186)
+            assert cost[u] <= none_cost * (1 + 1e-9), (kw, u, cost[u], none_cost)
