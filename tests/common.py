@@ -37,6 +37,18 @@ ODD_CASES = [("%dx%d_%s" % (w, h, name), dict(width=w, height=h, qindex=60, lr=2
                                               seed=7 * w + h, **kw))
              for (w, h) in ODD_SIZES for name, kw in ODD_LAYOUTS]
 
+# The widest and tallest frames the synthesiser makes (16384 samples): rows
+# of 16384 samples (512 noise blocks per row of film grain blocks), 257
+# restoration stripes, and an 8200-wide 4:2:2 frame with 256-sample units.
+EXTREME_CASES = [
+    ("16384x64_10b_420", dict(width=16384, height=64, bit_depth=10, qindex=120, lr=2, grain=True, lf_extras=True,
+                              seed=50)),
+    ("64x16384_8b_444_sb128", dict(width=64, height=16384, bit_depth=8, ssx=0, ssy=0, qindex=140, lr=2, grain=True,
+                                   seed=51, sb_size=128)),
+    ("8200x136_12b_422_u256", dict(width=8200, height=136, bit_depth=12, ssx=1, ssy=0, qindex=90, lr=2, grain=True,
+                                   seed=52, lr_unit_size=256)),
+]
+
 
 def make_frame(**kw):
     return workload.Frame(**kw)

@@ -82,10 +82,12 @@ def test_chain(ctx, name, kw):
     common.assert_planes(api.to_host(disp_dev), disp, f, "crop", name + " display")
 
 
-@pytest.mark.parametrize("name,kw", common.ODD_CASES, ids=[c[0] for c in common.ODD_CASES])
+@pytest.mark.parametrize("name,kw", common.ODD_CASES + common.EXTREME_CASES,
+                         ids=[c[0] for c in common.ODD_CASES + common.EXTREME_CASES])
 def test_chain_odd_sizes(ctx, name, kw):
-    """The reference's odd frame sizes (down to 1x1) through the whole chain,
-    in 16-bit planes and, at 8 bits, in 8-bit planes."""
+    """The reference's odd frame sizes (down to 1x1) and the widest and
+    tallest synthetic frames through the whole chain, in 16-bit planes and,
+    at 8 bits, in 8-bit planes."""
     _say("case odd", name)
     f = common.make_frame(**kw)
     deb, ref, disp = pyoracle.postfilter_frame(f, threads=4)
